@@ -1,0 +1,149 @@
+"""ctypes binding of libust.so (the C ABI in include/ust.h).
+
+The library must have been built (``python -m earthbyte_underworld_amd.build``); loading fails loudly
+otherwise -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libust.so")
+
+UST_OK = 0
+PREC_FP64, PREC_MIXED = 0, 1
+SCHUR_UW, SCHUR_LEARNED = 0, 1
+RH_MATERIALVISCOSITY, RH_ARRHENIUS, RH_VISCOSITYLIMITER, RH_VONMISES, RH_DRUCKERPRAGER = 1, 2, 3, 4, 5
+SHAPE_EVERYWHERE, SHAPE_BELOWCOSINEPLANE, SHAPE_SPHERE, SHAPE_BOX = 0, 1, 2, 3
+TERM_CONSTITUTIVE, TERM_GRADIENT, TERM_UZAWA_PC, TERM_BUOYANCY = 1, 2, 3, 4
+MAX_MATERIALS = 64
+
+
+class Rheology(C.Structure):
+    _fields_ = [("type", C.c_int), ("p", C.c_double * 6)]
+
+
+class Material(C.Structure):
+    _fields_ = [("nrh", C.c_int), ("rh", Rheology * 4), ("density", C.c_double)]
+
+
+class Shape(C.Structure):
+    _fields_ = [("type", C.c_int), ("material", C.c_int), ("p", C.c_double * 6)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("dim", C.c_int), ("nel", C.c_int * 3), ("h", C.c_double * 3), ("xmin", C.c_double * 3),
+                ("ymax", C.c_double), ("xmax", C.c_double * 3)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("schur_pc", C.c_int), ("mg_levels", C.c_int),
+                ("smoother_degree", C.c_int), ("cheby_ratio", C.c_double), ("outer_rtol", C.c_double),
+                ("inner_rtol", C.c_double), ("mixed_inner_rtol", C.c_double), ("max_outer", C.c_int),
+                ("max_inner", C.c_int), ("remove_nullspace", C.c_int), ("stall_window", C.c_int),
+                ("verbose", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("outer_its", C.c_int), ("inner_its", C.c_int), ("inner_solves", C.c_int),
+                ("refinements", C.c_int), ("fallback_used", C.c_int), ("mg_levels", C.c_int),
+                ("stokes_residual", C.c_double), ("momentum_residual", C.c_double),
+                ("continuity_residual", C.c_double), ("t_assemble_ms", C.c_double), ("t_setup_ms", C.c_double),
+                ("t_solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+vp = C.c_void_p
+I = C.c_int
+D = C.c_double
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "ust_create": (I, [C.POINTER(vp), I]),
+    "ust_destroy": (None, [vp]),
+    "ust_error": (C.c_char_p, [vp]),
+    "ust_get_stream": (vp, [vp]),
+    "ust_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
+    "ust_set_mesh": (I, [vp, I, vp, vp, vp]),
+    "ust_set_materials": (I, [vp, I, vp]),
+    "ust_rheology_from_xml": (I, [C.c_char_p, I, vp, vp, C.POINTER(Rheology)]),
+    "ust_swarm_upload": (I, [vp, I, vp, vp, vp, vp, vp]),
+    "ust_swarm_upload_dev": (I, [vp, I, vp, vp, vp, vp, vp, vp, vp, I]),
+    "ust_swarm_random": (I, [vp, vp, I, C.c_long]),
+    "ust_swarm_assign_materials": (I, [vp, I, vp, vp]),
+    "ust_swarm_cohesion_noise": (I, [vp, C.c_float, C.c_ulonglong]),
+    "ust_swarm_download": (I, [vp, vp, vp, vp, vp]),
+    "ust_swarm_count": (I, [vp]),
+    "ust_set_temperature": (I, [vp, vp]),
+    "ust_set_solution": (I, [vp, vp, vp]),
+    "ust_set_previous_solution_exists": (I, [vp, I]),
+    "ust_set_bcs": (I, [vp, vp, vp]),
+    "ust_assemble": (I, [vp, D]),
+    "ust_get_element_matrices": (I, [vp, I, I, vp]),
+    "ust_get_element_matrices_dev": (I, [vp, I, I, vp]),
+    "ust_get_element_G": (I, [vp, I, I, vp]),
+    "ust_get_element_f": (I, [vp, I, I, vp]),
+    "ust_get_approxS": (I, [vp, vp]),
+    "ust_get_particle_viscosity": (I, [vp, vp, vp]),
+    "ust_get_weights": (I, [vp, vp, vp, vp]),
+    "ust_get_rhs": (I, [vp, vp, vp]),
+    "ust_apply_K": (I, [vp, vp, vp]),
+    "ust_apply_K_dev": (I, [vp, vp, vp]),
+    "ust_apply_K_f32_dev": (I, [vp, vp, vp]),
+    "ust_apply_G": (I, [vp, vp, vp]),
+    "ust_apply_Gt": (I, [vp, vp, vp]),
+    "ust_solve": (I, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "ust_get_solution": (I, [vp, vp, vp]),
+    "ust_get_solution_dev": (I, [vp, vp, vp]),
+    "ust_velocity_solve": (I, [vp, C.POINTER(SolveOpts), vp, vp, C.POINTER(I)]),
+    "ust_picard": (I, [vp, D, D, I, C.POINTER(SolveOpts), C.POINTER(I), C.POINTER(D), C.POINTER(SolveStats)]),
+    "ust_mlp_init": (I, [vp, I, C.c_ulonglong]),
+    "ust_mlp_set_params": (I, [vp, I, vp, vp]),
+    "ust_mlp_get_params": (I, [vp, vp, vp]),
+    "ust_mlp_param_counts": (I, [I, I, C.POINTER(I), C.POINTER(I)]),
+    "ust_mlp_features": (I, [vp, vp]),
+    "ust_mlp_probe_target": (I, [vp, C.POINTER(SolveOpts), I, C.c_ulonglong, vp]),
+    "ust_mlp_train": (I, [vp, I, vp, vp, I, I, C.c_float, C.c_ulonglong, C.POINTER(C.c_float)]),
+    "ust_mlp_predict": (I, [vp, I, vp, vp]),
+    "ust_mlp_gradients": (I, [vp, I, vp, vp, vp, vp, vp]),
+    "ust_stg_bind_term": (I, [vp, vp, I]),
+    "ust_ConstitutiveMatrixCartesian_AssembleElement": (None, [vp, vp, I, vp, vp, vp]),
+    "ust_GradientStiffnessMatrixTerm_AssembleElement": (None, [vp, vp, I, vp, vp, vp]),
+    "ust_UzawaPreconditionerTerm_AssembleElement": (None, [vp, vp, I, vp, vp, vp]),
+    "ust_BuoyancyForceTerm_AssembleElement": (None, [vp, vp, I, vp]),
+    # thin kernel ABI used directly by tests and the benchmark timers
+    "ust_k_apply_K_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp]),
+    "ust_k_apply_K_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp]),
+    "ust_k_particle_weights": (I, [C.POINTER(Grid), vp, I, vp, vp, vp, vp, vp, vp, vp, I, vp, vp, vp, I, D, vp,
+                                   vp, vp, vp, vp, vp]),
+    "ust_event_create": (I, [C.POINTER(vp)]),
+    "ust_event_destroy": (I, [vp]),
+    "ust_event_record": (I, [vp, vp]),
+    "ust_event_elapsed_ms": (I, [vp, vp, C.POINTER(C.c_float)]),
+    "ust_stream_sync": (I, [vp]),
+    "ust_dev_error_string": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def load():
+    """Load libust.so; raises if it is missing (the HIP path is the only path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m earthbyte_underworld_amd.build` "
+                           "(the gfx950 HIP library is the only implementation; there is no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_SIGS)

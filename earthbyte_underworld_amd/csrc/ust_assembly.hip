@@ -1,0 +1,464 @@
+// PIC element assembly for gfx950: cell-sorted particles -> quadrature weights with the
+// per-particle rheology fused in, and the batched B^T D B element contraction on FP64 MFMA.
+//
+// Reference loops restated here:
+//   particle loop of _ConstitutiveMatrixCartesian_AssembleElement  ConstitutiveMatrixCartesian.c:277-410
+//   rheology chain                                                   RheologyMaterial.c:240-270,
+//       MaterialViscosity.c:119-131, Arrhenius.c:156-200, ViscosityLimiter.c:106-125,
+//       YieldRheology.c:198-243, VonMises.c:182-269, DruckerPrager.c:236-345, TensorMath.c:430-460
+//   G_e: GradientStiffnessMatrixTerm.c:220-249 ; f_e: BuoyancyForceTerm.c:370-448 ;
+//   approxS_e: UzawaPreconditionerTerm.c:195-212
+#include <math.h>
+
+#include "ust_common.h"
+
+// ------------------------------------------------------------------------------------------
+// per-particle rheology (isotropic D carried as eta)
+// ------------------------------------------------------------------------------------------
+template <int DIM>
+__device__ double particle_viscosity(const GridDev &g, const ust_material &M, const int *nodes, const double *xi,
+                                     const double *T, const double *vel, const double *pres, int e, float coh,
+                                     int prev, int *yflag) {
+    constexpr int NV = 1 << DIM;
+    double eta = 0.0;
+    for (int r = 0; r < M.nrh; r++) {
+        const ust_rheology &rh = M.rh[r];
+        if (rh.type == UST_RH_MATERIALVISCOSITY) {
+            eta = rh.p[0];
+        } else if (rh.type == UST_RH_ARRHENIUS) {
+            double Tp = 0.0, y = 0.0;
+#pragma unroll
+            for (int a = 0; a < NV; a++) {
+                double N = 1.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) N *= 0.5 * (1.0 + sgn((a >> d) & 1) * xi[d]);
+                Tp += N * T[nodes[a]];
+                y += N * ((a >> 1) & 1);
+            }
+            if (rh.p[2] > (0.0 + 1e-12)) {
+                // y(xi) on a rectilinear element: ymin + (j + local fraction) * hy
+                int j = (e / g.nx) % g.ny;
+                double yc = g.ymin + ((double)j + y) * g.hy;
+                double depth = g.ymax - yc;
+                eta = rh.p[0] * exp((rh.p[1] + rh.p[2] * depth) / (Tp + rh.p[3]));
+            } else {
+                eta = rh.p[0] * exp((rh.p[1]) / (Tp + rh.p[3]));
+            }
+        } else if (rh.type == UST_RH_VISCOSITYLIMITER) {
+            if (eta > rh.p[0]) eta = rh.p[0];
+            if (eta < rh.p[1]) eta = rh.p[1];
+        } else if (rh.type == UST_RH_VONMISES || rh.type == UST_RH_DRUCKERPRAGER) {
+            const bool dp = rh.type == UST_RH_DRUCKERPRAGER;
+            const double minVisc = dp ? rh.p[3] : rh.p[1];
+            const int srSoft = (int)(dp ? rh.p[4] : rh.p[2]);
+            if (!prev) continue;
+            // velocity gradient at xi: L[i][k] = sum_a GNx[k][a] u[a][i]
+            double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            const double s[3] = {2.0 / g.hx, 2.0 / g.hy, 2.0 / g.hz};
+#pragma unroll
+            for (int a = 0; a < NV; a++) {
+                double dN[3];
+#pragma unroll
+                for (int k = 0; k < DIM; k++) {
+                    double v = 0.5 * sgn((a >> k) & 1);
+#pragma unroll
+                    for (int d = 0; d < DIM; d++)
+                        if (d != k) v *= 0.5 * (1.0 + sgn((a >> d) & 1) * xi[d]);
+                    dN[k] = s[k] * v;
+                }
+#pragma unroll
+                for (int i = 0; i < DIM; i++) {
+                    double ui = vel[(size_t)nodes[a] * DIM + i];
+#pragma unroll
+                    for (int k = 0; k < DIM; k++) L[i][k] += dN[k] * ui;
+                }
+            }
+            double inv;
+            if (DIM == 3) {
+                double sxx = 2.0 * eta * L[0][0], syy = 2.0 * eta * L[1][1], szz = 2.0 * eta * L[2][2];
+                double sxy = eta * 2.0 * (0.5 * (L[0][1] + L[1][0]));
+                double sxz = eta * 2.0 * (0.5 * (L[0][2] + L[2][0]));
+                double syz = eta * 2.0 * (0.5 * (L[1][2] + L[2][1]));
+                inv = 0.5 * (sxx * sxx + syy * syy + szz * szz) + sxy * sxy + sxz * sxz + syz * syz;
+            } else {
+                double sxx = 2.0 * eta * L[0][0], syy = 2.0 * eta * L[1][1];
+                double sxy = eta * 2.0 * (0.5 * (L[0][1] + L[1][0]));
+                inv = 0.5 * (sxx * sxx + syy * syy) + sxy * sxy;
+            }
+            double indicator = sqrt(inv);
+            double cohesion = rh.p[0] * (double)coh, criterion;
+            if (!dp) {
+                criterion = cohesion;
+            } else {
+                double pressure = pres[e], phi = atan(rh.p[1]), k, c;
+                if (DIM == 3) {
+                    double sinphi = sin(phi), oneOver = 1.0 / (sqrt(3.0) * (3.0 - sinphi));
+                    k = 6.0 * sinphi * oneOver;
+                    c = 6.0 * cohesion * cos(phi) * oneOver;
+                } else {
+                    c = cohesion * cos(phi);
+                    k = sin(phi);
+                }
+                criterion = k * pressure + c;
+                if (criterion < rh.p[2]) criterion = rh.p[2];
+            }
+            *yflag = criterion < indicator;
+            if (criterion < indicator) {
+                if (srSoft) {
+                    eta = 2.0 * criterion * criterion * eta / (criterion * criterion + indicator * indicator);
+                } else {
+                    double beta = 1.0 - criterion / indicator, corr = -eta * beta;
+                    if ((eta + corr) < minVisc) corr = minVisc - eta;
+                    eta += corr;
+                }
+            }
+            if (eta < minVisc) eta = minVisc;
+        }
+    }
+    return eta;
+}
+
+// ------------------------------------------------------------------------------------------
+// particles -> weights.  A wavefront covers 64/S consecutive elements; lane k of a segment takes
+// particles k, k+S, ... of its element (coalesced: particles are cell-sorted), then the segment
+// reduces its partial sums with cross-lane shuffles and one lane stores.
+// ------------------------------------------------------------------------------------------
+template <int DIM, int S>
+__global__ __launch_bounds__(256) void k_particle_weights(GridDev g, const int *__restrict__ cell_off,
+                                                          const double *__restrict__ xi0,
+                                                          const double *__restrict__ xi1,
+                                                          const double *__restrict__ xi2,
+                                                          const double *__restrict__ w,
+                                                          const unsigned char *__restrict__ mat,
+                                                          const float *__restrict__ coh,
+                                                          const ust_material *__restrict__ mats, int nmat,
+                                                          const double *__restrict__ T,
+                                                          const double *__restrict__ vel,
+                                                          const double *__restrict__ pres, int prev,
+                                                          double *__restrict__ eta_out,
+                                                          unsigned char *__restrict__ yield_out,
+                                                          double *__restrict__ WK, double *__restrict__ WG,
+                                                          double *__restrict__ WF) {
+    constexpr int NQ = DIM == 3 ? 27 : 9;
+    constexpr int NV = 1 << DIM;
+    const int lane = threadIdx.x & 63;
+    const int seg = lane / S, k = lane % S;
+    const int e = (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / S) + seg);
+    const bool valid = e < g.nel;
+    double aK[NQ], aG[NV], aF[NV];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) aK[q] = 0.0;
+#pragma unroll
+    for (int b = 0; b < NV; b++) {
+        aG[b] = 0.0;
+        aF[b] = 0.0;
+    }
+    if (valid) {
+        const int ie = e % g.nx, je = (e / g.nx) % g.ny, ke = e / (g.nx * g.ny);
+        const int n0 = ie + g.nnx * (je + g.nny * ke);
+        int nodes[NV];
+#pragma unroll
+        for (int a = 0; a < NV; a++)
+            nodes[a] = n0 + (a & 1) + g.nnx * ((a >> 1) & 1) + (DIM == 3 ? g.nnx * g.nny * ((a >> 2) & 1) : 0);
+        const double detJ = (DIM == 3) ? 0.125 * g.hx * g.hy * g.hz : 0.25 * g.hx * g.hy;
+        const int pend = cell_off[e + 1];
+        for (int pi = cell_off[e] + k; pi < pend; pi += S) {
+            double xi[3];
+            xi[0] = xi0[pi];
+            xi[1] = xi1[pi];
+            xi[2] = DIM == 3 ? xi2[pi] : 0.0;
+            const int m = mat ? (int)mat[pi] : 0;
+            const ust_material &M = mats[m < nmat ? m : 0];
+            int yf = -1;
+            const double eta =
+                particle_viscosity<DIM>(g, M, nodes, xi, T, vel, pres, e, coh ? coh[pi] : 1.0f, prev, &yf);
+            if (eta_out) eta_out[pi] = eta;
+            if (yield_out && yf >= 0) yield_out[pi] = (unsigned char)yf;
+            const double wd = w[pi] * detJ;
+            const double et = wd * eta;
+            const double rho = M.density;
+            double Lx[3], Ly[3], Lz[3];
+            lagrange3(xi[0], Lx);
+            lagrange3(xi[1], Ly);
+            if (DIM == 3) lagrange3(xi[2], Lz);
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int qx = q % 3, qy = (q / 3) % 3, qz = q / 9;
+                double v = et * Lx[qx] * Ly[qy];
+                if (DIM == 3) v *= Lz[qz];
+                aK[q] += v;
+            }
+#pragma unroll
+            for (int b = 0; b < NV; b++) {
+                double N = 1.0;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) N *= 0.5 * (1.0 + sgn((b >> d) & 1) * xi[d]);
+                aG[b] += wd * N;
+                aF[b] += wd * rho * N;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = S / 2; off > 0; off >>= 1) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) aK[q] += __shfl_xor(aK[q], off, S);
+#pragma unroll
+        for (int b = 0; b < NV; b++) {
+            aG[b] += __shfl_xor(aG[b], off, S);
+            aF[b] += __shfl_xor(aF[b], off, S);
+        }
+    }
+    if (valid && k == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) WK[(size_t)q * g.nel + e] = aK[q];
+#pragma unroll
+        for (int b = 0; b < NV; b++) {
+            WG[(size_t)b * g.nel + e] = aG[b];
+            WF[(size_t)b * g.nel + e] = aF[b];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// B(xi_q)^T D0 B(xi_q) entry (row (a,i), col (b,j)) on the Lagrange point q:
+//   delta_ij sum_k GNx_k(a) GNx_k(b) + GNx_j(a) GNx_i(b)
+// ------------------------------------------------------------------------------------------
+template <int DIM>
+__device__ __forceinline__ double gnx_at(int a, int k, int q, const double *s) {
+    // s_k * dN_a/dxi_k at Lagrange point q:  sgn(a_k) prod_{d != k} cfac(a_d, q_d) / 2^DIM
+    int qd[3] = {q % 3, (q / 3) % 3, q / 9};
+    int v = sgn((a >> k) & 1);
+#pragma unroll
+    for (int d = 0; d < DIM; d++)
+        if (d != k) v *= cfac((a >> d) & 1, qd[d]);
+    return s[k] * (double)v / (double)(1 << DIM);
+}
+
+template <int DIM>
+__device__ __forceinline__ double bdb_entry(int q, int row, int col, const double *s) {
+    const int a = row / DIM, i = row % DIM, b = col / DIM, j = col % DIM;
+    double v = 0.0;
+    if (i == j) {
+#pragma unroll
+        for (int k = 0; k < DIM; k++) v += gnx_at<DIM>(a, k, q, s) * gnx_at<DIM>(b, k, q, s);
+    }
+    v += gnx_at<DIM>(a, j, q, s) * gnx_at<DIM>(b, i, q, s);
+    return v;
+}
+
+typedef double ust_double4 __attribute__((ext_vector_type(4)));
+
+// K_e = W (count x NQ) * M (NQ x nd^2) on v_mfma_f64_16x16x4f64.  One wavefront per 16 elements;
+// A fragment: lane l holds A[l%16][l/16], B fragment: B[l/16][l%16], D: acc[r] = D[4(l/16)+r][l%16].
+template <int DIM>
+__global__ __launch_bounds__(256) void k_element_matrices(GridDev g, const double *__restrict__ WK, int first,
+                                                          int count, double *__restrict__ Ke) {
+    constexpr int NQ = DIM == 3 ? 27 : 9;
+    constexpr int ND = (1 << DIM) * DIM;
+    constexpr int NCOL = ND * ND;
+    constexpr int KSTEPS = (NQ + 3) / 4;
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int e0 = tile * 16;
+    if (e0 >= count) return;
+    const double s[3] = {2.0 / g.hx, 2.0 / g.hy, 2.0 / g.hz};
+    const int li = lane & 15, lk = lane >> 4;
+    double afrag[KSTEPS];
+#pragma unroll
+    for (int ks = 0; ks < KSTEPS; ks++) {
+        const int q = ks * 4 + lk, el = e0 + li;
+        afrag[ks] = (q < NQ && el < count) ? WK[(size_t)q * g.nel + first + el] : 0.0;
+    }
+    for (int ct = 0; ct < NCOL / 16; ct++) {
+        ust_double4 acc = {0.0, 0.0, 0.0, 0.0};
+        const int col = ct * 16 + li;
+#pragma unroll
+        for (int ks = 0; ks < KSTEPS; ks++) {
+            const int q = ks * 4 + lk;
+            const double b = (q < NQ) ? bdb_entry<DIM>(q, col / ND, col % ND, s) : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(afrag[ks], b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int el = e0 + 4 * lk + r;
+            if (el < count) Ke[(size_t)el * NCOL + col] = acc[r];
+        }
+    }
+}
+
+// G_e[a,i] = -(sgn(a_i)/h_i) (WG[b_i=0,b_perp=a_perp] + WG[b_i=1,b_perp=a_perp])
+template <int DIM>
+__device__ __forceinline__ void element_G(const GridDev &g, const double *WG, int e, double *Ge) {
+    constexpr int NV = 1 << DIM;
+    double wv[NV];
+#pragma unroll
+    for (int b = 0; b < NV; b++) wv[b] = WG[(size_t)b * g.nel + e];
+    const double h[3] = {g.hx, g.hy, g.hz};
+#pragma unroll
+    for (int a = 0; a < NV; a++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            const int b0 = a & ~(1 << i), b1 = a | (1 << i);
+            Ge[a * DIM + i] = -(double)sgn((a >> i) & 1) / h[i] * (wv[b0] + wv[b1]);
+        }
+}
+
+template <int DIM>
+__global__ void k_element_G(GridDev g, const double *__restrict__ WG, int first, int count, double *__restrict__ Ge) {
+    constexpr int ND = (1 << DIM) * DIM;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    double ge[ND];
+    element_G<DIM>(g, WG, first + t, ge);
+#pragma unroll
+    for (int v = 0; v < ND; v++) Ge[(size_t)t * ND + v] = ge[v];
+}
+
+template <int DIM>
+__global__ void k_element_f(GridDev g, const double *__restrict__ WF, double gravity, int first, int count,
+                            double *__restrict__ fe) {
+    constexpr int NV = 1 << DIM, ND = NV * DIM;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+#pragma unroll
+    for (int a = 0; a < NV; a++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++)
+            fe[(size_t)t * ND + a * DIM + i] = (i == 1) ? -gravity * WF[(size_t)a * g.nel + first + t] : 0.0;
+}
+
+// diag of K_e: sum_q W_q (sum_k GNx_k(a)^2 + GNx_i(a)^2)
+template <int DIM>
+__device__ __forceinline__ void element_K_diag(const GridDev &g, const double *WK, int e, double *dg) {
+    constexpr int NQ = DIM == 3 ? 27 : 9, NV = 1 << DIM;
+    const double s[3] = {2.0 / g.hx, 2.0 / g.hy, 2.0 / g.hz};
+#pragma unroll
+    for (int v = 0; v < NV * DIM; v++) dg[v] = 0.0;
+    for (int q = 0; q < NQ; q++) {
+        const double wq = WK[(size_t)q * g.nel + e];
+#pragma unroll
+        for (int a = 0; a < NV; a++) {
+            double gk[3], s2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < DIM; k++) {
+                gk[k] = gnx_at<DIM>(a, k, q, s);
+                s2 += gk[k] * gk[k];
+            }
+#pragma unroll
+            for (int i = 0; i < DIM; i++) dg[a * DIM + i] += wq * (s2 + gk[i] * gk[i]);
+        }
+    }
+}
+
+template <int DIM>
+__global__ void k_uzawa_diag(GridDev g, const double *__restrict__ WK, const double *__restrict__ WG,
+                             double *__restrict__ approxS) {
+    constexpr int ND = (1 << DIM) * DIM;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    double dg[ND], ge[ND];
+    element_K_diag<DIM>(g, WK, e, dg);
+    element_G<DIM>(g, WG, e, ge);
+    double s = 0.0;
+#pragma unroll
+    for (int v = 0; v < ND; v++) s += ge[v] * (ge[v] / dg[v]);
+    approxS[e] = s;
+}
+
+// ------------------------------------------------------------------------------------------
+extern "C" int ust_k_particle_weights(const ust_grid *gr, const int *cell_off, int max_ppc, const double *xi0,
+                                      const double *xi1, const double *xi2, const double *w,
+                                      const unsigned char *mat, const float *coh_scale,
+                                      const ust_material *materials_dev, int nmat, const double *temperature,
+                                      const double *velocity, const double *pressure, int prev_solution,
+                                      double gravity, double *eta_out, unsigned char *yield_out, double *WK,
+                                      double *WG, double *WF, void *stream) {
+    (void)gravity;
+    if (!gr || !cell_off || !xi0 || !xi1 || !w || !materials_dev || !WK || !WG || !WF) return UST_EINVAL;
+    if (gr->dim == 3 && !xi2) return UST_EINVAL;
+    if (nmat < 1 || nmat > UST_MAX_MATERIALS) return UST_EINVAL;
+    GridDev g = grid_dev(gr);
+    if (g.nel <= 0) return UST_OK;
+    // segment width: the smallest of 16/32/64 that covers the typical cell in one pass
+    const int S = max_ppc <= 16 ? 16 : (max_ppc <= 32 ? 32 : 64);
+    const int elems_per_block = (UST_BLOCK / 64) * (64 / S);
+    const unsigned grid = ust_grid_size(g.nel, elems_per_block);
+    hipStream_t st = ust_stream(stream);
+#define LAUNCH_PW(D, SS)                                                                                        \
+    hipLaunchKernelGGL((k_particle_weights<D, SS>), dim3(grid), dim3(UST_BLOCK), 0, st, g, cell_off, xi0, xi1, \
+                       xi2, w, mat, coh_scale, materials_dev, nmat, temperature, velocity, pressure,            \
+                       prev_solution, eta_out, yield_out, WK, WG, WF)
+    if (g.dim == 3) {
+        if (S == 16) LAUNCH_PW(3, 16);
+        else if (S == 32) LAUNCH_PW(3, 32);
+        else LAUNCH_PW(3, 64);
+    } else {
+        if (S == 16) LAUNCH_PW(2, 16);
+        else if (S == 32) LAUNCH_PW(2, 32);
+        else LAUNCH_PW(2, 64);
+    }
+#undef LAUNCH_PW
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+extern "C" int ust_k_element_matrices(const ust_grid *gr, const double *WK, int first, int count, double *Ke,
+                                      void *stream) {
+    GridDev g = grid_dev(gr);
+    if (first < 0 || count < 0 || first + count > g.nel || !WK || !Ke) return UST_EINVAL;
+    if (count == 0) return UST_OK;
+    const int tiles = (count + 15) / 16;
+    const unsigned grid = ust_grid_size(tiles, UST_BLOCK / 64);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_element_matrices<3>, dim3(grid), dim3(UST_BLOCK), 0, ust_stream(stream), g, WK, first,
+                           count, Ke);
+    else
+        hipLaunchKernelGGL(k_element_matrices<2>, dim3(grid), dim3(UST_BLOCK), 0, ust_stream(stream), g, WK, first,
+                           count, Ke);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+extern "C" int ust_k_element_G(const ust_grid *gr, const double *WG, int first, int count, double *Ge,
+                               void *stream) {
+    GridDev g = grid_dev(gr);
+    if (first < 0 || count < 0 || first + count > g.nel || !WG || !Ge) return UST_EINVAL;
+    if (count == 0) return UST_OK;
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_element_G<3>, dim3(ust_grid_size(count, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           ust_stream(stream), g, WG, first, count, Ge);
+    else
+        hipLaunchKernelGGL(k_element_G<2>, dim3(ust_grid_size(count, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           ust_stream(stream), g, WG, first, count, Ge);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+extern "C" int ust_k_element_f(const ust_grid *gr, const double *WF, double gravity, int first, int count,
+                               double *fe, void *stream) {
+    GridDev g = grid_dev(gr);
+    if (first < 0 || count < 0 || first + count > g.nel || !WF || !fe) return UST_EINVAL;
+    if (count == 0) return UST_OK;
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_element_f<3>, dim3(ust_grid_size(count, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           ust_stream(stream), g, WF, gravity, first, count, fe);
+    else
+        hipLaunchKernelGGL(k_element_f<2>, dim3(ust_grid_size(count, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           ust_stream(stream), g, WF, gravity, first, count, fe);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+extern "C" int ust_k_uzawa_diag(const ust_grid *gr, const double *WK, const double *WG, double *approxS,
+                                void *stream) {
+    GridDev g = grid_dev(gr);
+    if (!WK || !WG || !approxS) return UST_EINVAL;
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_uzawa_diag<3>, dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           ust_stream(stream), g, WK, WG, approxS);
+    else
+        hipLaunchKernelGGL(k_uzawa_diag<2>, dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           ust_stream(stream), g, WK, WG, approxS);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}

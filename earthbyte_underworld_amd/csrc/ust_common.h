@@ -1,0 +1,68 @@
+// Device-side helpers shared by the gfx950 kernels of the PIC Stokes hot path.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ust_kernels.h"
+
+#define UST_BLOCK 256
+
+#define UST_LAUNCH_CHECK()                                   \
+    do {                                                     \
+        hipError_t _e = hipGetLastError();                   \
+        if (_e != hipSuccess) {                              \
+            ust_set_hip_error(_e, __FILE__, __LINE__);       \
+            return UST_EDEVICE;                              \
+        }                                                    \
+    } while (0)
+
+void ust_set_hip_error(hipError_t e, const char *file, int line);
+
+static inline unsigned ust_grid_size(size_t n, unsigned block) {
+    size_t g = (n + block - 1) / block;
+    return (unsigned)(g == 0 ? 1 : g);
+}
+
+static inline hipStream_t ust_stream(void *s) { return (hipStream_t)s; }
+
+// Quadratic Lagrange basis on the points {-1, 0, 1}.
+__device__ __forceinline__ void lagrange3(double x, double *L) {
+    L[0] = 0.5 * x * (x - 1.0);
+    L[1] = 1.0 - x * x;
+    L[2] = 0.5 * x * (x + 1.0);
+}
+
+// c(a, q) = 1 + s_a * xi_q for xi_q = q - 1 and s_a = 2a - 1: the factor of the Q1 shape
+// function of vertex bit a evaluated at Lagrange point q (values 0, 1, 2).
+__host__ __device__ constexpr int cfac(int a, int q) { return a ? q : 2 - q; }
+__host__ __device__ constexpr int sgn(int a) { return 2 * a - 1; }
+
+struct GridDev {
+    int dim;
+    int nx, ny, nz;       // elements per axis (nz = 1 in 2D)
+    int nnx, nny, nnz;    // nodes per axis
+    int nel;
+    double hx, hy, hz;
+    double xmin, ymin, zmin;
+    double ymax;
+};
+
+static inline GridDev grid_dev(const ust_grid *g) {
+    GridDev d;
+    d.dim = g->dim;
+    d.nx = g->nel[0];
+    d.ny = g->nel[1];
+    d.nz = g->dim == 3 ? g->nel[2] : 1;
+    d.nnx = d.nx + 1;
+    d.nny = d.ny + 1;
+    d.nnz = g->dim == 3 ? d.nz + 1 : 1;
+    d.nel = d.nx * d.ny * d.nz;
+    d.hx = g->h[0];
+    d.hy = g->h[1];
+    d.hz = g->dim == 3 ? g->h[2] : 1.0;
+    d.xmin = g->xmin[0];
+    d.ymin = g->xmin[1];
+    d.zmin = g->dim == 3 ? g->xmin[2] : 0.0;
+    d.ymax = g->ymax;
+    return d;
+}

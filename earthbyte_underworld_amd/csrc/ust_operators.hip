@@ -1,0 +1,649 @@
+// Matrix-free Stokes operators for gfx950 on the quadrature-weight representation:
+// K u (the MatMult of the A11 block inside BSSCR and its multigrid smoothers,
+// auglag-driver-DGTGD.c:191-403, PETScMGSolver.c:641-716), G p and G^T u
+// (MatSchurComplement's A01/A10, StokesBlockKSPInterface.c:242), BC lifting
+// (StiffnessMatrix.c:900-951), Galerkin coarse operators (PETScMGSolver.c:518-527) and the
+// SROpGenerator transfers (SROpGenerator.c:1000-1041).
+#include <math.h>
+
+#include "ust_common.h"
+
+// ------------------------------------------------------------------------------------------
+// Element action y_e = K_e u_e with K_e = sum_q W_q B_q^T D0 B_q.  Gradients are taken at the
+// 3^DIM Lagrange points, tau = W_q (grad u + grad u^T), y_a,i = sum_q sum_k GNx_k(a) tau_ik.
+// All coefficients are compile-time constants after unrolling (values 0, +-1, +-2 / 2^DIM).
+// ------------------------------------------------------------------------------------------
+template <int DIM, typename T>
+__device__ __forceinline__ void element_apply(const T *wq, const T s[3], const T u[1 << DIM][DIM],
+                                              T y[1 << DIM][DIM]) {
+    constexpr int NV = 1 << DIM;
+    constexpr int NQ = DIM == 3 ? 27 : 9;
+    const T inv = (T)1.0 / (T)(1 << DIM);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int qd[3] = {q % 3, (q / 3) % 3, q / 9};
+        T gr[DIM][DIM];  // gr[i][k] = d u_i / d x_k
+#pragma unroll
+        for (int i = 0; i < DIM; i++)
+#pragma unroll
+            for (int k = 0; k < DIM; k++) gr[i][k] = (T)0;
+#pragma unroll
+        for (int a = 0; a < NV; a++) {
+#pragma unroll
+            for (int k = 0; k < DIM; k++) {
+                int c = sgn((a >> k) & 1);
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+                    if (d != k) c *= cfac((a >> d) & 1, qd[d]);
+                if (c != 0) {
+#pragma unroll
+                    for (int i = 0; i < DIM; i++) gr[i][k] += (T)c * u[a][i];
+                }
+            }
+        }
+        T tau[DIM][DIM];
+        const T wv = wq[q] * inv;
+#pragma unroll
+        for (int i = 0; i < DIM; i++)
+#pragma unroll
+            for (int k = 0; k < DIM; k++) tau[i][k] = wv * (s[k] * gr[i][k] + s[i] * gr[k][i]);
+#pragma unroll
+        for (int a = 0; a < NV; a++) {
+#pragma unroll
+            for (int k = 0; k < DIM; k++) {
+                int c = sgn((a >> k) & 1);
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+                    if (d != k) c *= cfac((a >> d) & 1, qd[d]);
+                if (c != 0) {
+                    const T cs = (T)c * s[k] * inv;
+#pragma unroll
+                    for (int i = 0; i < DIM; i++) y[a][i] += cs * tau[i][k];
+                }
+            }
+        }
+    }
+}
+
+// INMODE 0: input on free dofs (normal apply);  1: input on BC dofs only (lifting).
+// SIGN: +1 y += K u ; -1 y -= K u.  Output on free dofs only, accumulated with atomics.
+template <int DIM, typename T, int INMODE, int SIGN>
+__global__ __launch_bounds__(256) void k_apply_K(GridDev g, const T *__restrict__ W,
+                                                 const unsigned char *__restrict__ mask, const T *__restrict__ u,
+                                                 T *__restrict__ y) {
+    constexpr int NV = 1 << DIM;
+    constexpr int NQ = DIM == 3 ? 27 : 9;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    const int ie = e % g.nx, je = (e / g.nx) % g.ny, ke = e / (g.nx * g.ny);
+    const int n0 = ie + g.nnx * (je + g.nny * ke);
+    int nodes[NV];
+#pragma unroll
+    for (int a = 0; a < NV; a++)
+        nodes[a] = n0 + (a & 1) + g.nnx * ((a >> 1) & 1) + (DIM == 3 ? g.nnx * g.nny * ((a >> 2) & 1) : 0);
+    T ue[NV][DIM], ye[NV][DIM], wq[NQ];
+#pragma unroll
+    for (int a = 0; a < NV; a++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            const size_t dof = (size_t)nodes[a] * DIM + i;
+            const bool bc = mask[dof] != 0;
+            ue[a][i] = ((INMODE == 0) ? !bc : bc) ? u[dof] : (T)0;
+            ye[a][i] = (T)0;
+        }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) wq[q] = W[(size_t)q * g.nel + e];
+    const T s[3] = {(T)(2.0 / g.hx), (T)(2.0 / g.hy), (T)(2.0 / g.hz)};
+    element_apply<DIM, T>(wq, s, ue, ye);
+#pragma unroll
+    for (int a = 0; a < NV; a++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            const size_t dof = (size_t)nodes[a] * DIM + i;
+            if (!mask[dof]) unsafeAtomicAdd(&y[dof], SIGN > 0 ? ye[a][i] : -ye[a][i]);
+        }
+}
+
+template <typename T>
+__global__ void k_fill(T *x, T v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+template <int DIM>
+__device__ __forceinline__ void elem_nodes(const GridDev &g, int e, int *nodes) {
+    const int ie = e % g.nx, je = (e / g.nx) % g.ny, ke = e / (g.nx * g.ny);
+    const int n0 = ie + g.nnx * (je + g.nny * ke);
+#pragma unroll
+    for (int a = 0; a < (1 << DIM); a++)
+        nodes[a] = n0 + (a & 1) + g.nnx * ((a >> 1) & 1) + (DIM == 3 ? g.nnx * g.nny * ((a >> 2) & 1) : 0);
+}
+
+// G_e[a,i] = -(sgn(a_i)/h_i)(WG[a with bit i = 0] + WG[a with bit i = 1])
+template <int DIM>
+__device__ __forceinline__ double g_entry(const GridDev &g, const double *wv, int a, int i) {
+    const double h = i == 0 ? g.hx : (i == 1 ? g.hy : g.hz);
+    return -(double)sgn((a >> i) & 1) / h * (wv[a & ~(1 << i)] + wv[a | (1 << i)]);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_apply_G(GridDev g, const double *__restrict__ WG,
+                                                 const unsigned char *__restrict__ mask, const double *__restrict__ p,
+                                                 double *__restrict__ y) {
+    constexpr int NV = 1 << DIM;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    int nodes[NV];
+    elem_nodes<DIM>(g, e, nodes);
+    double wv[NV];
+#pragma unroll
+    for (int b = 0; b < NV; b++) wv[b] = WG[(size_t)b * g.nel + e];
+    const double pe = p[e];
+#pragma unroll
+    for (int a = 0; a < NV; a++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            const size_t dof = (size_t)nodes[a] * DIM + i;
+            if (!mask[dof]) unsafeAtomicAdd(&y[dof], g_entry<DIM>(g, wv, a, i) * pe);
+        }
+}
+
+// MODE 0: q_e = sum_free G_e u ;  MODE 1: q_e -= sum_bc G_e u  (continuity-RHS lifting)
+template <int DIM, int MODE>
+__global__ __launch_bounds__(256) void k_apply_Gt(GridDev g, const double *__restrict__ WG,
+                                                  const unsigned char *__restrict__ mask,
+                                                  const double *__restrict__ u, double *__restrict__ q) {
+    constexpr int NV = 1 << DIM;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    int nodes[NV];
+    elem_nodes<DIM>(g, e, nodes);
+    double wv[NV];
+#pragma unroll
+    for (int b = 0; b < NV; b++) wv[b] = WG[(size_t)b * g.nel + e];
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < NV; a++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            const size_t dof = (size_t)nodes[a] * DIM + i;
+            const bool bc = mask[dof] != 0;
+            if (MODE == 2) {
+                if (!bc) s += fabs(g_entry<DIM>(g, wv, a, i));
+            } else if (MODE == 0 ? !bc : bc) {
+                s += g_entry<DIM>(g, wv, a, i) * u[dof];
+            }
+        }
+    if (MODE == 1) q[e] -= s;
+    else q[e] = s;
+}
+
+template <int DIM>
+__global__ void k_force(GridDev g, const double *__restrict__ WF, double gravity,
+                        const unsigned char *__restrict__ mask, double *__restrict__ f) {
+    constexpr int NV = 1 << DIM;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    int nodes[NV];
+    elem_nodes<DIM>(g, e, nodes);
+#pragma unroll
+    for (int a = 0; a < NV; a++) {
+        const size_t dof = (size_t)nodes[a] * DIM + 1;
+        if (!mask[dof]) unsafeAtomicAdd(&f[dof], -gravity * WF[(size_t)a * g.nel + e]);
+    }
+}
+
+template <int DIM>
+__global__ void k_diag_acc(GridDev g, const double *__restrict__ WK, const unsigned char *__restrict__ mask,
+                           double *__restrict__ dg) {
+    constexpr int NV = 1 << DIM, NQ = DIM == 3 ? 27 : 9;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    int nodes[NV];
+    elem_nodes<DIM>(g, e, nodes);
+    const double s[3] = {2.0 / g.hx, 2.0 / g.hy, 2.0 / g.hz};
+    double acc[NV][DIM];
+#pragma unroll
+    for (int a = 0; a < NV; a++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++) acc[a][i] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int qd[3] = {q % 3, (q / 3) % 3, q / 9};
+        const double wq = WK[(size_t)q * g.nel + e] / (double)(1 << (2 * DIM));
+#pragma unroll
+        for (int a = 0; a < NV; a++) {
+            double gk[3], s2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < DIM; k++) {
+                int c = sgn((a >> k) & 1);
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+                    if (d != k) c *= cfac((a >> d) & 1, qd[d]);
+                gk[k] = s[k] * c;
+                s2 += gk[k] * gk[k];
+            }
+#pragma unroll
+            for (int i = 0; i < DIM; i++) acc[a][i] += wq * (s2 + gk[i] * gk[i]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NV; a++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            const size_t dof = (size_t)nodes[a] * DIM + i;
+            if (!mask[dof]) unsafeAtomicAdd(&dg[dof], acc[a][i]);
+        }
+}
+
+__global__ void k_diag_invert(const unsigned char *__restrict__ mask, double *__restrict__ d64,
+                              float *__restrict__ d32, size_t n) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = mask[i] ? 1.0 : 1.0 / d64[i];
+    d64[i] = v;
+    if (d32) d32[i] = (float)v;
+}
+
+// ------------------------------------------------------------------------------------------
+// Galerkin coarsening of the weights: W^E_{q'} = sum_children sum_q W_{q,c} L_{q'}(xi^E(xi_q)).
+// Per axis a child half cb maps fine Lagrange point qd to coarse coordinate (qd + 2 cb - 2)/2.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ double lag1(int qp, double x) {
+    return qp == 0 ? 0.5 * x * (x - 1.0) : (qp == 1 ? 1.0 - x * x : 0.5 * x * (x + 1.0));
+}
+
+template <int DIM>
+__global__ void k_coarsen_weights(GridDev gf, GridDev gc, const double *__restrict__ Wf, double *__restrict__ Wc) {
+    constexpr int NQ = DIM == 3 ? 27 : 9;
+    const int E = blockIdx.x * blockDim.x + threadIdx.x;
+    if (E >= gc.nel) return;
+    const int I = E % gc.nx, J = (E / gc.nx) % gc.ny, K = E / (gc.nx * gc.ny);
+    double T1[2][3][3];  // [cb][qd][q'd]
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+        for (int qd = 0; qd < 3; qd++)
+#pragma unroll
+            for (int qp = 0; qp < 3; qp++) T1[cb][qd][qp] = lag1(qp, 0.5 * (double)(qd + 2 * cb - 2));
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) acc[q] = 0.0;
+    for (int c = 0; c < (1 << DIM); c++) {
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+        const int ef = (2 * I + cx) + gf.nx * ((2 * J + cy) + gf.ny * (DIM == 3 ? (2 * K + cz) : 0));
+        for (int q = 0; q < NQ; q++) {
+            const double wf = Wf[(size_t)q * gf.nel + ef];
+            const int qx = q % 3, qy = (q / 3) % 3, qz = q / 9;
+#pragma unroll
+            for (int qp = 0; qp < NQ; qp++) {
+                const int px = qp % 3, py = (qp / 3) % 3, pz = qp / 9;
+                double t = T1[cx][qx][px] * T1[cy][qy][py];
+                if (DIM == 3) t *= T1[cz][qz][pz];
+                acc[qp] += wf * t;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) Wc[(size_t)q * gc.nel + E] = acc[q];
+}
+
+// restriction b_c = P^T r_f : coarse node gathers fine nodes 2I+{-1,0,1} (weights 1/2, 1, 1/2)
+template <int DIM, typename T>
+__global__ void k_restrict(GridDev gf, GridDev gc, const unsigned char *__restrict__ maskf,
+                           const unsigned char *__restrict__ maskc, const T *__restrict__ rf, T *__restrict__ bc) {
+    const int nnc = gc.nnx * gc.nny * gc.nnz;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnc) return;
+    const int I = n % gc.nnx, J = (n / gc.nnx) % gc.nny, K = n / (gc.nnx * gc.nny);
+    T acc[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++) acc[i] = (T)0;
+    for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); dz++) {
+        const int k = 2 * K + dz;
+        if (k < 0 || k >= gf.nnz) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            const int j = 2 * J + dy;
+            if (j < 0 || j >= gf.nny) continue;
+            for (int dx = -1; dx <= 1; dx++) {
+                const int i = 2 * I + dx;
+                if (i < 0 || i >= gf.nnx) continue;
+                const T wt = (T)((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0));
+                const size_t fn = (size_t)i + gf.nnx * ((size_t)j + gf.nny * k);
+#pragma unroll
+                for (int c = 0; c < DIM; c++)
+                    if (!maskf[fn * DIM + c]) acc[c] += wt * rf[fn * DIM + c];
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < DIM; c++) bc[(size_t)n * DIM + c] = maskc[(size_t)n * DIM + c] ? (T)0 : acc[c];
+}
+
+// prolongation x_f += P x_c : even index injects, odd index averages its two coarse neighbours
+template <int DIM, typename T>
+__global__ void k_prolong_add(GridDev gf, GridDev gc, const unsigned char *__restrict__ maskf,
+                              const T *__restrict__ xc, T *__restrict__ xf) {
+    const int nnf = gf.nnx * gf.nny * gf.nnz;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnf) return;
+    const int i = n % gf.nnx, j = (n / gf.nnx) % gf.nny, k = n / (gf.nnx * gf.nny);
+    const int ix[2] = {i >> 1, (i >> 1) + (i & 1)}, jy[2] = {j >> 1, (j >> 1) + (j & 1)},
+              kz[2] = {k >> 1, (k >> 1) + (k & 1)};
+    const int nxo = (i & 1) + 1, nyo = (j & 1) + 1, nzo = (DIM == 3) ? (k & 1) + 1 : 1;
+    const T wt = (T)1 / (T)(nxo * nyo * nzo);
+    T acc[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; c++) acc[c] = (T)0;
+    for (int oz = 0; oz < nzo; oz++)
+        for (int oy = 0; oy < nyo; oy++)
+            for (int ox = 0; ox < nxo; ox++) {
+                const size_t cn = (size_t)ix[ox] + gc.nnx * ((size_t)jy[oy] + gc.nny * kz[oz]);
+#pragma unroll
+                for (int c = 0; c < DIM; c++) acc[c] += xc[cn * DIM + c];
+            }
+#pragma unroll
+    for (int c = 0; c < DIM; c++)
+        if (!maskf[(size_t)n * DIM + c]) xf[(size_t)n * DIM + c] += wt * acc[c];
+}
+
+// ------------------------------------------------------------------------------------------
+// dense coarse operator (FP64): element matrices scattered into A, identity on BC dofs
+// ------------------------------------------------------------------------------------------
+template <int DIM>
+__global__ void k_dense_assemble(GridDev g, const double *__restrict__ WK, const unsigned char *__restrict__ mask,
+                                 double *__restrict__ A, int n) {
+    constexpr int NV = 1 << DIM, ND = NV * DIM, NQ = DIM == 3 ? 27 : 9;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nel * ND * ND) return;
+    const int e = t / (ND * ND), rc = t % (ND * ND), r = rc / ND, c = rc % ND;
+    int nodes[NV];
+    elem_nodes<DIM>(g, e, nodes);
+    const int R = nodes[r / DIM] * DIM + r % DIM, C = nodes[c / DIM] * DIM + c % DIM;
+    if (mask[R] || mask[C]) return;
+    const double s[3] = {2.0 / g.hx, 2.0 / g.hy, 2.0 / g.hz};
+    const int a = r / DIM, i = r % DIM, b = c / DIM, j = c % DIM;
+    double v = 0.0;
+    for (int q = 0; q < NQ; q++) {
+        const int qd[3] = {q % 3, (q / 3) % 3, q / 9};
+        double ga[3], gb[3];
+        for (int k = 0; k < DIM; k++) {
+            int ca = sgn((a >> k) & 1), cb = sgn((b >> k) & 1);
+            for (int d = 0; d < DIM; d++)
+                if (d != k) {
+                    ca *= cfac((a >> d) & 1, qd[d]);
+                    cb *= cfac((b >> d) & 1, qd[d]);
+                }
+            ga[k] = s[k] * ca / (double)NV;
+            gb[k] = s[k] * cb / (double)NV;
+        }
+        double m = 0.0;
+        if (i == j)
+            for (int k = 0; k < DIM; k++) m += ga[k] * gb[k];
+        m += ga[j] * gb[i];
+        v += WK[(size_t)q * g.nel + e] * m;
+    }
+    unsafeAtomicAdd(&A[(size_t)R * n + C], v);
+}
+
+__global__ void k_dense_bc_identity(const unsigned char *__restrict__ mask, double *__restrict__ A, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mask[i]) A[(size_t)i * n + i] = 1.0;
+}
+
+// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one workgroup.
+__global__ __launch_bounds__(1024) void k_dense_spd_inverse(double *__restrict__ A, int n) {
+    extern __shared__ double colk[];
+    for (int k = 0; k < n; k++) {
+        __syncthreads();
+        const double piv = A[(size_t)k * n + k];
+        for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = A[(size_t)i * n + k];
+        __syncthreads();
+        // scale row k: A[k][j] /= piv (j != k), A[k][k] = 1/piv
+        for (int j = threadIdx.x; j < n; j += blockDim.x)
+            A[(size_t)k * n + j] = (j == k) ? 1.0 / piv : A[(size_t)k * n + j] / piv;
+        __syncthreads();
+        for (size_t t = threadIdx.x; t < (size_t)n * n; t += blockDim.x) {
+            const int i = (int)(t / n), j = (int)(t % n);
+            if (i == k) continue;
+            const double aij = (j == k) ? 0.0 : A[t];
+            A[t] = aij - colk[i] * A[(size_t)k * n + j];
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_dense_matvec(const double *__restrict__ Ainv, int n, const T *__restrict__ x, T *__restrict__ y) {
+    // one wavefront per row
+    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double s = 0.0;
+    for (int j = lane; j < n; j += 64) s += Ainv[(size_t)row * n + j] * (double)x[j];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) y[row] = (T)s;
+}
+
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+int ust_k_apply_K_f64(const ust_grid *gr, const double *WK, const unsigned char *mask, const double *u, double *y,
+                      void *stream) {
+    GridDev g = grid_dev(gr);
+    if (!WK || !mask || !u || !y) return UST_EINVAL;
+    const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * g.dim;
+    hipStream_t st = ust_stream(stream);
+    hipLaunchKernelGGL(k_fill<double>, dim3(ust_grid_size(ndof, UST_BLOCK)), dim3(UST_BLOCK), 0, st, y, 0.0, ndof);
+    if (g.dim == 3)
+        hipLaunchKernelGGL((k_apply_K<3, double, 0, 1>), dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           st, g, WK, mask, u, y);
+    else
+        hipLaunchKernelGGL((k_apply_K<2, double, 0, 1>), dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           st, g, WK, mask, u, y);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_apply_K_f32(const ust_grid *gr, const float *WK, const unsigned char *mask, const float *u, float *y,
+                      void *stream) {
+    GridDev g = grid_dev(gr);
+    if (!WK || !mask || !u || !y) return UST_EINVAL;
+    const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * g.dim;
+    hipStream_t st = ust_stream(stream);
+    hipLaunchKernelGGL(k_fill<float>, dim3(ust_grid_size(ndof, UST_BLOCK)), dim3(UST_BLOCK), 0, st, y, 0.0f, ndof);
+    if (g.dim == 3)
+        hipLaunchKernelGGL((k_apply_K<3, float, 0, 1>), dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           st, g, WK, mask, u, y);
+    else
+        hipLaunchKernelGGL((k_apply_K<2, float, 0, 1>), dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           st, g, WK, mask, u, y);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_lift_K(const ust_grid *gr, const double *WK, const unsigned char *mask, const double *ubc, double *y,
+                 void *stream) {
+    GridDev g = grid_dev(gr);
+    if (!WK || !mask || !ubc || !y) return UST_EINVAL;
+    if (g.dim == 3)
+        hipLaunchKernelGGL((k_apply_K<3, double, 1, -1>), dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           ust_stream(stream), g, WK, mask, ubc, y);
+    else
+        hipLaunchKernelGGL((k_apply_K<2, double, 1, -1>), dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           ust_stream(stream), g, WK, mask, ubc, y);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_apply_G(const ust_grid *gr, const double *WG, const unsigned char *mask, const double *p, double *y,
+                  void *stream) {
+    GridDev g = grid_dev(gr);
+    if (!WG || !mask || !p || !y) return UST_EINVAL;
+    const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * g.dim;
+    hipStream_t st = ust_stream(stream);
+    hipLaunchKernelGGL(k_fill<double>, dim3(ust_grid_size(ndof, UST_BLOCK)), dim3(UST_BLOCK), 0, st, y, 0.0, ndof);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_apply_G<3>, dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WG, mask,
+                           p, y);
+    else
+        hipLaunchKernelGGL(k_apply_G<2>, dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WG, mask,
+                           p, y);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_apply_Gt(const ust_grid *gr, const double *WG, const unsigned char *mask, const double *u, double *q,
+                   int bc_only, void *stream) {
+    GridDev g = grid_dev(gr);
+    if (!WG || !mask || !u || !q) return UST_EINVAL;
+    hipStream_t st = ust_stream(stream);
+    const unsigned grid = ust_grid_size(g.nel, UST_BLOCK);
+    if (g.dim == 3) {
+        if (bc_only == 1) hipLaunchKernelGGL((k_apply_Gt<3, 1>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
+        else if (bc_only == 2) hipLaunchKernelGGL((k_apply_Gt<3, 2>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
+        else hipLaunchKernelGGL((k_apply_Gt<3, 0>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
+    } else {
+        if (bc_only == 1) hipLaunchKernelGGL((k_apply_Gt<2, 1>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
+        else if (bc_only == 2) hipLaunchKernelGGL((k_apply_Gt<2, 2>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
+        else hipLaunchKernelGGL((k_apply_Gt<2, 0>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
+    }
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_force(const ust_grid *gr, const double *WF, double gravity, const unsigned char *mask, double *f,
+                void *stream) {
+    GridDev g = grid_dev(gr);
+    if (!WF || !mask || !f) return UST_EINVAL;
+    const unsigned grid = ust_grid_size(g.nel, UST_BLOCK);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_force<3>, dim3(grid), dim3(UST_BLOCK), 0, ust_stream(stream), g, WF, gravity, mask, f);
+    else
+        hipLaunchKernelGGL(k_force<2>, dim3(grid), dim3(UST_BLOCK), 0, ust_stream(stream), g, WF, gravity, mask, f);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_inv_diag(const ust_grid *gr, const double *WK, const unsigned char *mask, double *invd64, float *invd32,
+                   void *stream) {
+    GridDev g = grid_dev(gr);
+    if (!WK || !mask || !invd64) return UST_EINVAL;
+    const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * g.dim;
+    hipStream_t st = ust_stream(stream);
+    hipLaunchKernelGGL(k_fill<double>, dim3(ust_grid_size(ndof, UST_BLOCK)), dim3(UST_BLOCK), 0, st, invd64, 0.0,
+                       ndof);
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_diag_acc<3>, dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WK, mask,
+                           invd64);
+    else
+        hipLaunchKernelGGL(k_diag_acc<2>, dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WK, mask,
+                           invd64);
+    hipLaunchKernelGGL(k_diag_invert, dim3(ust_grid_size(ndof, UST_BLOCK)), dim3(UST_BLOCK), 0, st, mask, invd64,
+                       invd32, ndof);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+static int coarse_grid(const ust_grid *gf, ust_grid *gc) {
+    *gc = *gf;
+    for (int d = 0; d < gf->dim; d++) {
+        if (gf->nel[d] % 2) return UST_EINVAL;
+        gc->nel[d] = gf->nel[d] / 2;
+        gc->h[d] = gf->h[d] * 2.0;
+    }
+    return UST_OK;
+}
+
+int ust_k_coarsen_weights(const ust_grid *gf, const double *WKf, double *WKc, void *stream) {
+    ust_grid gcr;
+    if (coarse_grid(gf, &gcr) != UST_OK || !WKf || !WKc) return UST_EINVAL;
+    GridDev f = grid_dev(gf), c = grid_dev(&gcr);
+    if (f.dim == 3)
+        hipLaunchKernelGGL(k_coarsen_weights<3>, dim3(ust_grid_size(c.nel, 128)), dim3(128), 0, ust_stream(stream), f,
+                           c, WKf, WKc);
+    else
+        hipLaunchKernelGGL(k_coarsen_weights<2>, dim3(ust_grid_size(c.nel, 128)), dim3(128), 0, ust_stream(stream), f,
+                           c, WKf, WKc);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+#define TRANSFER_IMPL(T, SUF)                                                                                    \
+    int ust_k_restrict_##SUF(const ust_grid *gf, const unsigned char *maskf, const unsigned char *maskc,       \
+                             const T *rf, T *bc, void *stream) {                                               \
+        ust_grid gcr;                                                                                          \
+        if (coarse_grid(gf, &gcr) != UST_OK || !maskf || !maskc || !rf || !bc) return UST_EINVAL;              \
+        GridDev f = grid_dev(gf), c = grid_dev(&gcr);                                                          \
+        const int nnc = c.nnx * c.nny * c.nnz;                                                                 \
+        if (f.dim == 3)                                                                                        \
+            hipLaunchKernelGGL((k_restrict<3, T>), dim3(ust_grid_size(nnc, UST_BLOCK)), dim3(UST_BLOCK), 0,    \
+                               ust_stream(stream), f, c, maskf, maskc, rf, bc);                               \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_restrict<2, T>), dim3(ust_grid_size(nnc, UST_BLOCK)), dim3(UST_BLOCK), 0,    \
+                               ust_stream(stream), f, c, maskf, maskc, rf, bc);                               \
+        UST_LAUNCH_CHECK();                                                                                    \
+        return UST_OK;                                                                                         \
+    }                                                                                                          \
+    int ust_k_prolong_add_##SUF(const ust_grid *gf, const unsigned char *maskf, const T *xc, T *xf,           \
+                                void *stream) {                                                                \
+        ust_grid gcr;                                                                                          \
+        if (coarse_grid(gf, &gcr) != UST_OK || !maskf || !xc || !xf) return UST_EINVAL;                        \
+        GridDev f = grid_dev(gf), c = grid_dev(&gcr);                                                          \
+        const int nnf = f.nnx * f.nny * f.nnz;                                                                 \
+        if (f.dim == 3)                                                                                        \
+            hipLaunchKernelGGL((k_prolong_add<3, T>), dim3(ust_grid_size(nnf, UST_BLOCK)), dim3(UST_BLOCK), 0, \
+                               ust_stream(stream), f, c, maskf, xc, xf);                                       \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_prolong_add<2, T>), dim3(ust_grid_size(nnf, UST_BLOCK)), dim3(UST_BLOCK), 0, \
+                               ust_stream(stream), f, c, maskf, xc, xf);                                       \
+        UST_LAUNCH_CHECK();                                                                                    \
+        return UST_OK;                                                                                         \
+    }
+TRANSFER_IMPL(double, f64)
+TRANSFER_IMPL(float, f32)
+
+int ust_k_dense_assemble(const ust_grid *gr, const double *WK, const unsigned char *mask, double *A, void *stream) {
+    GridDev g = grid_dev(gr);
+    const int n = g.nnx * g.nny * g.nnz * g.dim;
+    if (!WK || !mask || !A || n > 4096) return UST_EINVAL;
+    hipStream_t st = ust_stream(stream);
+    hipLaunchKernelGGL(k_fill<double>, dim3(ust_grid_size((size_t)n * n, UST_BLOCK)), dim3(UST_BLOCK), 0, st, A, 0.0,
+                       (size_t)n * n);
+    const int nd = (1 << g.dim) * g.dim;
+    const size_t work = (size_t)g.nel * nd * nd;
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_dense_assemble<3>, dim3(ust_grid_size(work, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WK,
+                           mask, A, n);
+    else
+        hipLaunchKernelGGL(k_dense_assemble<2>, dim3(ust_grid_size(work, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WK,
+                           mask, A, n);
+    hipLaunchKernelGGL(k_dense_bc_identity, dim3(ust_grid_size(n, UST_BLOCK)), dim3(UST_BLOCK), 0, st, mask, A, n);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_dense_spd_inverse(double *A, int n, void *stream) {
+    if (!A || n <= 0 || n > 4096) return UST_EINVAL;
+    hipLaunchKernelGGL(k_dense_spd_inverse, dim3(1), dim3(1024), (size_t)n * sizeof(double), ust_stream(stream), A, n);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_dense_matvec_f64(const double *Ainv, int n, const double *x, double *y, void *stream) {
+    if (!Ainv || !x || !y || n <= 0) return UST_EINVAL;
+    hipLaunchKernelGGL(k_dense_matvec<double>, dim3(ust_grid_size(n, 4)), dim3(256), 0, ust_stream(stream), Ainv, n,
+                       x, y);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_dense_matvec_f32(const double *Ainv, int n, const float *x, float *y, void *stream) {
+    if (!Ainv || !x || !y || n <= 0) return UST_EINVAL;
+    hipLaunchKernelGGL(k_dense_matvec<float>, dim3(ust_grid_size(n, 4)), dim3(256), 0, ust_stream(stream), Ainv, n, x,
+                       y);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+}  // extern "C"

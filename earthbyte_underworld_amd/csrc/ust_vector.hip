@@ -1,0 +1,253 @@
+// Krylov / smoother vector kernels (the PETSc Vec operations inside the BSSCR CG and the
+// multigrid smoothers).  Scalars that the device computes (dots, CG ratios) stay in device
+// memory, so a Krylov iteration issues no host round-trip except its convergence check.
+#include "ust_common.h"
+
+#define GRID_STRIDE(i, n) \
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); i += (size_t)gridDim.x * blockDim.x)
+
+static inline unsigned vgrid(size_t n) {
+    size_t g = (n + UST_BLOCK - 1) / UST_BLOCK;
+    if (g > 8192) g = 8192;
+    return (unsigned)(g == 0 ? 1 : g);
+}
+
+template <typename T>
+__global__ void kv_fill(T *x, T v, size_t n) {
+    GRID_STRIDE(i, n) x[i] = v;
+}
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    // splitmix64 finaliser
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__global__ void kv_hash(double *x, size_t n, unsigned long long seed) {
+    GRID_STRIDE(i, n) {
+        const unsigned long long h = mix64(seed * 0x100000001B3ULL + i);
+        x[i] = 2.0 * ((double)(h >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+    }
+}
+__global__ void kv_sign(double *x, size_t n, unsigned long long seed) {
+    GRID_STRIDE(i, n) x[i] = (mix64(seed * 0x100000001B3ULL + i) >> 63) ? 1.0 : -1.0;
+}
+__global__ void kv_d2f(const double *x, float *y, size_t n) {
+    GRID_STRIDE(i, n) y[i] = (float)x[i];
+}
+__global__ void kv_f2d(const float *x, double *y, size_t n) {
+    GRID_STRIDE(i, n) y[i] = (double)x[i];
+}
+__global__ void kv_axpy(double a, const double *x, double *y, size_t n) {
+    GRID_STRIDE(i, n) y[i] += a * x[i];
+}
+__global__ void kv_axpby(double a, const double *x, double b, double *y, size_t n) {
+    GRID_STRIDE(i, n) y[i] = a * x[i] + b * y[i];
+}
+__global__ void kv_addf(const float *x, double *y, size_t n) {
+    GRID_STRIDE(i, n) y[i] += (double)x[i];
+}
+__global__ void kv_sub(const double *a, const double *b, double *y, size_t n) {
+    GRID_STRIDE(i, n) y[i] = a[i] - b[i];
+}
+__global__ void kv_mul(const double *a, const double *b, double *y, size_t n) {
+    GRID_STRIDE(i, n) y[i] = a[i] * b[i];
+}
+__global__ void kv_div(const double *a, const double *b, double *y, size_t n) {
+    GRID_STRIDE(i, n) y[i] = a[i] / b[i];
+}
+
+// two-stage deterministic reductions: stage 1 writes one partial per block (<= 1024 blocks)
+template <typename T, bool SQ>
+__global__ __launch_bounds__(256) void kv_dot_stage1(const T *x, const T *y, size_t n, double *partial) {
+    __shared__ double sh[256 / 64];
+    double s = 0.0;
+    GRID_STRIDE(i, n) s += SQ ? (double)x[i] * (double)y[i] : (double)x[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void kv_reduce_stage2(const double *partial, int np, double *out) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s += partial[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ __launch_bounds__(256) void kv_absmax_stage1(const double *x, size_t n, double *partial) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    GRID_STRIDE(i, n) s = fmax(s, fabs(x[i]));
+    for (int off = 32; off > 0; off >>= 1) s = fmax(s, __shfl_xor(s, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+__global__ __launch_bounds__(256) void kv_max_stage2(const double *partial, int np, double *out) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s = fmax(s, partial[i]);
+    for (int off = 32; off > 0; off >>= 1) s = fmax(s, __shfl_xor(s, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+static inline unsigned rgrid(size_t n) {
+    size_t g = (n + UST_BLOCK * 4 - 1) / (UST_BLOCK * 4);
+    if (g > 1024) g = 1024;
+    return (unsigned)(g == 0 ? 1 : g);
+}
+
+__global__ void kv_mask_zero(double *x, const unsigned char *mask, size_t n) {
+    GRID_STRIDE(i, n) if (mask[i]) x[i] = 0.0;
+}
+
+__global__ void kv_remove_mean(double *x, size_t n, const double *sum) {
+    const double m = sum[0] / (double)n;
+    GRID_STRIDE(i, n) x[i] -= m;
+}
+
+template <typename T>
+__global__ void kv_cg_p(const T *z, T *p, const double *rz_new, const double *rz_old, int first, size_t n) {
+    const T beta = first ? (T)0 : (T)(rz_new[0] / rz_old[0]);
+    GRID_STRIDE(i, n) p[i] = z[i] + (first ? (T)0 : beta * p[i]);
+}
+
+template <typename T>
+__global__ void kv_cg_xr(T *x, T *r, const T *p, const T *q, const double *num, const double *den, size_t n) {
+    const T a = (T)(num[0] / den[0]);
+    GRID_STRIDE(i, n) {
+        x[i] += a * p[i];
+        r[i] -= a * q[i];
+    }
+}
+
+template <typename T>
+__global__ void kv_cheby(const T *b, const T *Ax, const T *invd, T *d, T *x, T c1, T c2, int first, size_t n) {
+    GRID_STRIDE(i, n) {
+        const T r = b[i] - Ax[i];
+        const T dn = (first ? (T)0 : c1 * d[i]) + c2 * invd[i] * r;
+        d[i] = dn;
+        x[i] += dn;
+    }
+}
+
+template <typename T>
+__global__ void kv_residual(const T *b, const T *Ax, T *r, size_t n) {
+    GRID_STRIDE(i, n) r[i] = b[i] - Ax[i];
+}
+
+#define LV(kern, n, ...)                                                                               \
+    do {                                                                                               \
+        hipLaunchKernelGGL(kern, dim3(vgrid(n)), dim3(UST_BLOCK), 0, ust_stream(stream), __VA_ARGS__); \
+        UST_LAUNCH_CHECK();                                                                            \
+        return UST_OK;                                                                                 \
+    } while (0)
+
+extern "C" {
+
+int ust_k_fill_f64(double *x, double v, size_t n, void *stream) { LV(kv_fill<double>, n, x, v, n); }
+int ust_k_fill_f32(float *x, float v, size_t n, void *stream) { LV(kv_fill<float>, n, x, v, n); }
+int ust_k_fill_hash_f64(double *x, size_t n, unsigned long long seed, void *stream) {
+    LV(kv_hash, n, x, n, seed);
+}
+int ust_k_fill_sign_f64(double *x, size_t n, unsigned long long seed, void *stream) {
+    LV(kv_sign, n, x, n, seed);
+}
+int ust_k_copy_f64_to_f32(const double *x, float *y, size_t n, void *stream) { LV(kv_d2f, n, x, y, n); }
+int ust_k_copy_f32_to_f64(const float *x, double *y, size_t n, void *stream) { LV(kv_f2d, n, x, y, n); }
+int ust_k_axpy_f64(double a, const double *x, double *y, size_t n, void *stream) { LV(kv_axpy, n, a, x, y, n); }
+int ust_k_axpby_f64(double a, const double *x, double b, double *y, size_t n, void *stream) {
+    LV(kv_axpby, n, a, x, b, y, n);
+}
+int ust_k_add_f32_to_f64(const float *x, double *y, size_t n, void *stream) { LV(kv_addf, n, x, y, n); }
+int ust_k_sub_f64(const double *a, const double *b, double *y, size_t n, void *stream) { LV(kv_sub, n, a, b, y, n); }
+int ust_k_scale_mul_f64(const double *a, const double *b, double *y, size_t n, void *stream) {
+    LV(kv_mul, n, a, b, y, n);
+}
+int ust_k_div_f64(const double *a, const double *b, double *y, size_t n, void *stream) { LV(kv_div, n, a, b, y, n); }
+
+int ust_k_dot_f64(const double *x, const double *y, size_t n, double *partial, double *out, void *stream) {
+    const unsigned g = rgrid(n);
+    hipLaunchKernelGGL((kv_dot_stage1<double, true>), dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
+    hipLaunchKernelGGL(kv_reduce_stage2, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, out);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_dot_f32(const float *x, const float *y, size_t n, double *partial, double *out, void *stream) {
+    const unsigned g = rgrid(n);
+    hipLaunchKernelGGL((kv_dot_stage1<float, true>), dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
+    hipLaunchKernelGGL(kv_reduce_stage2, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, out);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_sum_f64(const double *x, size_t n, double *partial, double *out, void *stream) {
+    const unsigned g = rgrid(n);
+    hipLaunchKernelGGL((kv_dot_stage1<double, false>), dim3(g), dim3(256), 0, ust_stream(stream), x, x, n, partial);
+    hipLaunchKernelGGL(kv_reduce_stage2, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, out);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_absmax_f64(const double *x, size_t n, double *partial, double *out, void *stream) {
+    const unsigned g = rgrid(n);
+    hipLaunchKernelGGL(kv_absmax_stage1, dim3(g), dim3(256), 0, ust_stream(stream), x, n, partial);
+    hipLaunchKernelGGL(kv_max_stage2, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, out);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_mask_zero_f64(double *x, const unsigned char *mask, size_t n, void *stream) {
+    LV(kv_mask_zero, n, x, mask, n);
+}
+
+int ust_k_remove_mean_f64(double *x, size_t n, const double *sum, void *stream) {
+    LV(kv_remove_mean, n, x, n, sum);
+}
+
+int ust_k_cg_update_p_f64(const double *z, double *p, const double *rz_new, const double *rz_old, int first, size_t n,
+                          void *stream) {
+    LV(kv_cg_p<double>, n, z, p, rz_new, rz_old, first, n);
+}
+int ust_k_cg_update_p_f32(const float *z, float *p, const double *rz_new, const double *rz_old, int first, size_t n,
+                          void *stream) {
+    LV(kv_cg_p<float>, n, z, p, rz_new, rz_old, first, n);
+}
+int ust_k_cg_update_xr_f64(double *x, double *r, const double *p, const double *q, const double *num,
+                           const double *den, size_t n, void *stream) {
+    LV(kv_cg_xr<double>, n, x, r, p, q, num, den, n);
+}
+int ust_k_cg_update_xr_f32(float *x, float *r, const float *p, const float *q, const double *num, const double *den,
+                           size_t n, void *stream) {
+    LV(kv_cg_xr<float>, n, x, r, p, q, num, den, n);
+}
+
+int ust_k_cheby_step_f64(const double *b, const double *Ax, const double *invd, double *d, double *x, double c1,
+                         double c2, size_t n, void *stream) {
+    LV(kv_cheby<double>, n, b, Ax, invd, d, x, c1, c2, c1 == 0.0 ? 1 : 0, n);
+}
+int ust_k_cheby_step_f32(const float *b, const float *Ax, const float *invd, float *d, float *x, float c1, float c2,
+                         size_t n, void *stream) {
+    LV(kv_cheby<float>, n, b, Ax, invd, d, x, c1, c2, c1 == 0.0f ? 1 : 0, n);
+}
+int ust_k_residual_f64(const double *b, const double *Ax, double *r, size_t n, void *stream) {
+    LV(kv_residual<double>, n, b, Ax, r, n);
+}
+int ust_k_residual_f32(const float *b, const float *Ax, float *r, size_t n, void *stream) {
+    LV(kv_residual<float>, n, b, Ax, r, n);
+}
+
+}  // extern "C"

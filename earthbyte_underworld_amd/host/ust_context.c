@@ -1,0 +1,683 @@
+/*
+ * ust_context.c -- C host layer: context, mesh, materials, swarm, fields, BCs, assembly
+ * driver and the operator entry points.  Calls only the thin kernel ABI (ust_kernels.h).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ust_internal.h"
+
+static void dfree(void **p) {
+    if (*p) ust_dev_free(*p);
+    *p = NULL;
+}
+#define DFREE(x) dfree((void **)&(x))
+
+int ust_dev_alloc_zero(ust_context *ctx, void **p, size_t bytes) {
+    /* round up to whole doubles so that the zero fill stays inside the allocation */
+    const size_t ab = (bytes + 7) & ~(size_t)7;
+    int rc = ust_dev_malloc(p, ab);
+    if (rc != UST_OK) UST_FAIL(ctx, rc, "device allocation of %zu bytes failed: %s", ab, ust_dev_error_string());
+    CKD(ctx, ust_k_fill_f64((double *)*p, 0.0, ab / 8, ctx->stream));
+    return UST_OK;
+}
+#define DALLOC(ctx, ptr, bytes) CKD(ctx, ust_dev_alloc_zero(ctx, (void **)&(ptr), (bytes)))
+
+void ust_solve_opts_default(ust_solve_opts *o) {
+    memset(o, 0, sizeof(*o));
+    o->precision = UST_PREC_FP64;
+    o->schur_pc = UST_SCHUR_UW;
+    o->mg_levels = 0;
+    o->smoother_degree = 2;
+    o->cheby_ratio = 12.0;
+    o->outer_rtol = 1e-7;
+    o->inner_rtol = 1e-9;
+    o->mixed_inner_rtol = 1e-4;
+    o->max_outer = 1000;
+    o->max_inner = 500;
+    o->remove_nullspace = -1;
+    o->stall_window = 30;
+    o->verbose = 0;
+}
+
+int ust_create(ust_context **out, int device) {
+    *out = NULL;
+    ust_context *ctx = (ust_context *)calloc(1, sizeof(ust_context));
+    if (!ctx) return UST_ENOMEM;
+    ctx->device = device;
+    int rc = ust_dev_set(device);
+    if (rc == UST_OK) rc = ust_stream_create(&ctx->stream);
+    if (rc != UST_OK) {
+        free(ctx);
+        return rc;
+    }
+    rc = ust_dev_malloc((void **)&ctx->partial, 1024 * sizeof(double));
+    if (rc == UST_OK) rc = ust_dev_malloc((void **)&ctx->scal, SC_COUNT * sizeof(double));
+    if (rc == UST_OK) rc = ust_host_malloc_pinned((void **)&ctx->scal_h, SC_COUNT * sizeof(double));
+    if (rc == UST_OK) rc = ust_dev_malloc((void **)&ctx->mats_d, UST_MAX_MATERIALS * sizeof(ust_material));
+    if (rc != UST_OK) {
+        ust_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return UST_OK;
+}
+
+const char *ust_error(const ust_context *ctx) { return ctx ? ctx->err : "null context"; }
+void *ust_get_stream(ust_context *ctx) { return ctx->stream; }
+
+static void free_mesh_data(ust_context *ctx) {
+    DFREE(ctx->T);
+    DFREE(ctx->u);
+    DFREE(ctx->p);
+    DFREE(ctx->mask_d);
+    DFREE(ctx->bcval_d);
+    free(ctx->mask_h);
+    free(ctx->bcval_h);
+    ctx->mask_h = NULL;
+    ctx->bcval_h = NULL;
+    DFREE(ctx->WK);
+    DFREE(ctx->WG);
+    DFREE(ctx->WF);
+    DFREE(ctx->approxS);
+    DFREE(ctx->f);
+    DFREE(ctx->h);
+    DFREE(ctx->tmpv);
+    DFREE(ctx->tmpp);
+    DFREE(ctx->vr); DFREE(ctx->vz); DFREE(ctx->vp); DFREE(ctx->vq); DFREE(ctx->vt);
+    DFREE(ctx->va); DFREE(ctx->vb);
+    DFREE(ctx->wr); DFREE(ctx->wz); DFREE(ctx->wp); DFREE(ctx->wq); DFREE(ctx->wx);
+    DFREE(ctx->pr); DFREE(ctx->pz); DFREE(ctx->pd); DFREE(ctx->pq); DFREE(ctx->ph); DFREE(ctx->ppc);
+    DFREE(ctx->pprev); DFREE(ctx->uprev);
+    DFREE(ctx->feat); DFREE(ctx->mlp_out); DFREE(ctx->act1); DFREE(ctx->act2);
+    ctx->feat_cap = 0;
+    free(ctx->ke_cache); free(ctx->ge_cache); free(ctx->fe_cache); free(ctx->as_cache);
+    ctx->ke_cache = ctx->ge_cache = ctx->fe_cache = ctx->as_cache = NULL;
+    ctx->cache_valid = 0;
+    ust_levels_free(ctx);
+    ctx->has_bcs = 0;
+    ctx->assembled = 0;
+}
+
+static void free_swarm(ust_context *ctx) {
+    DFREE(ctx->cell_off);
+    DFREE(ctx->xi0); DFREE(ctx->xi1); DFREE(ctx->xi2); DFREE(ctx->w);
+    DFREE(ctx->mat); DFREE(ctx->coh); DFREE(ctx->eta); DFREE(ctx->yielded);
+    ctx->np = 0;
+}
+
+void ust_destroy(ust_context *ctx) {
+    if (!ctx) return;
+    if (ctx->stream) ust_stream_sync(ctx->stream);
+    free_mesh_data(ctx);
+    free_swarm(ctx);
+    ust_mlp_free(ctx);
+    DFREE(ctx->partial);
+    DFREE(ctx->scal);
+    DFREE(ctx->mats_d);
+    if (ctx->scal_h) ust_host_free_pinned(ctx->scal_h);
+    if (ctx->stream) ust_stream_destroy(ctx->stream);
+    free(ctx);
+}
+
+int ust_read_scalar(ust_context *ctx, int slot, double *v) {
+    CKD(ctx, ust_memcpy_d2h(ctx->scal_h + slot, ctx->scal + slot, sizeof(double), ctx->stream));
+    *v = ctx->scal_h[slot];
+    return UST_OK;
+}
+
+int ust_set_mesh(ust_context *ctx, int dim, const int nel[3], const double xmin[3], const double xmax[3]) {
+    if (dim != 2 && dim != 3) UST_FAIL(ctx, UST_EINVAL, "dim must be 2 or 3 (got %d)", dim);
+    for (int d = 0; d < dim; d++) {
+        if (nel[d] < 1) UST_FAIL(ctx, UST_EINVAL, "elementRes[%d] must be >= 1", d);
+        if (!(xmax[d] > xmin[d])) UST_FAIL(ctx, UST_EINVAL, "max <= min on axis %d", d);
+    }
+    free_mesh_data(ctx);
+    memset(&ctx->grid, 0, sizeof(ctx->grid));
+    ctx->grid.dim = dim;
+    ctx->nnodes = 1;
+    ctx->nel = 1;
+    for (int d = 0; d < 3; d++) {
+        ctx->grid.nel[d] = d < dim ? nel[d] : 1;
+        ctx->grid.xmin[d] = d < dim ? xmin[d] : 0.0;
+        ctx->grid.h[d] = d < dim ? (xmax[d] - xmin[d]) / nel[d] : 1.0;
+        if (d < dim) {
+            ctx->nnodes *= nel[d] + 1;
+            ctx->nel *= nel[d];
+        }
+    }
+    ctx->grid.ymax = xmax[1];
+    for (int d = 0; d < 3; d++) ctx->grid.xmax[d] = d < dim ? xmax[d] : 1.0;
+    ctx->ndof = ctx->nnodes * dim;
+    size_t nd = (size_t)ctx->ndof, ne = (size_t)ctx->nel;
+    DALLOC(ctx, ctx->T, (size_t)ctx->nnodes * sizeof(double));
+    DALLOC(ctx, ctx->u, nd * sizeof(double));
+    DALLOC(ctx, ctx->p, ne * sizeof(double));
+    DALLOC(ctx, ctx->mask_d, nd);
+    DALLOC(ctx, ctx->bcval_d, nd * sizeof(double));
+    ctx->mask_h = (unsigned char *)calloc(nd, 1);
+    ctx->bcval_h = (double *)calloc(nd, sizeof(double));
+    int nq = dim == 3 ? 27 : 9, nv = 1 << dim;
+    DALLOC(ctx, ctx->WK, (size_t)nq * ne * sizeof(double));
+    DALLOC(ctx, ctx->WG, (size_t)nv * ne * sizeof(double));
+    DALLOC(ctx, ctx->WF, (size_t)nv * ne * sizeof(double));
+    DALLOC(ctx, ctx->approxS, ne * sizeof(double));
+    DALLOC(ctx, ctx->f, nd * sizeof(double));
+    DALLOC(ctx, ctx->h, ne * sizeof(double));
+    DALLOC(ctx, ctx->tmpv, nd * sizeof(double));
+    DALLOC(ctx, ctx->tmpp, ne * sizeof(double));
+    ctx->has_mesh = 1;
+    ctx->prev_solution = 0;
+    ctx->nullspace = 0;
+    return UST_OK;
+}
+
+int ust_set_materials(ust_context *ctx, int nmat, const ust_material *mats) {
+    if (nmat < 1 || nmat > UST_MAX_MATERIALS) UST_FAIL(ctx, UST_EINVAL, "nmat must be in [1,%d]", UST_MAX_MATERIALS);
+    for (int m = 0; m < nmat; m++) {
+        if (mats[m].nrh < 1 || mats[m].nrh > UST_MAX_RHEOLOGIES)
+            UST_FAIL(ctx, UST_EINVAL, "material %d: no rheology (\"No material available\", ConstitutiveMatrix.c:301)", m);
+        for (int r = 0; r < mats[m].nrh; r++)
+            if (mats[m].rh[r].type < UST_RH_MATERIALVISCOSITY || mats[m].rh[r].type > UST_RH_DRUCKERPRAGER)
+                UST_FAIL(ctx, UST_EINVAL, "material %d rheology %d: unknown type %d", m, r, mats[m].rh[r].type);
+    }
+    memcpy(ctx->mats_h, mats, sizeof(ust_material) * nmat);
+    ctx->nmat = nmat;
+    CKD(ctx, ust_memcpy_h2d(ctx->mats_d, mats, sizeof(ust_material) * nmat, ctx->stream));
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
+static int streq(const char *a, const char *b) { return strcmp(a, b) == 0; }
+
+int ust_rheology_from_xml(const char *type_name, int nkeys, const char *const *keys, const double *values,
+                          ust_rheology *out) {
+    memset(out, 0, sizeof(*out));
+    /* defaults as the reference's Stg_ComponentFactory_GetDouble calls */
+    if (streq(type_name, "MaterialViscosity")) {
+        out->type = UST_RH_MATERIALVISCOSITY;
+        out->p[0] = 1.0; /* eta0 */
+    } else if (streq(type_name, "Arrhenius")) {
+        out->type = UST_RH_ARRHENIUS;
+        out->p[0] = 1.0; /* eta0 */
+        out->p[1] = 0.0; /* activationEnergy */
+        out->p[2] = 0.0; /* activationVolume */
+        out->p[3] = 1.0; /* referenceTemperature */
+    } else if (streq(type_name, "ViscosityLimiter")) {
+        out->type = UST_RH_VISCOSITYLIMITER;
+        out->p[0] = 1e4;  /* maxViscosity */
+        out->p[1] = 1e-2; /* minViscosity */
+    } else if (streq(type_name, "VonMises")) {
+        out->type = UST_RH_VONMISES;
+    } else if (streq(type_name, "DruckerPrager")) {
+        out->type = UST_RH_DRUCKERPRAGER;
+    } else {
+        return UST_EINVAL;
+    }
+    for (int k = 0; k < nkeys; k++) {
+        const char *key = keys[k];
+        double v = values[k];
+        int t = out->type, slot = -1;
+        if (t == UST_RH_MATERIALVISCOSITY && streq(key, "eta0")) slot = 0;
+        if (t == UST_RH_ARRHENIUS) {
+            if (streq(key, "eta0")) slot = 0;
+            if (streq(key, "activationEnergy")) slot = 1;
+            if (streq(key, "activationVolume")) slot = 2;
+            if (streq(key, "referenceTemperature")) slot = 3;
+        }
+        if (t == UST_RH_VISCOSITYLIMITER) {
+            if (streq(key, "maxViscosity")) slot = 0;
+            if (streq(key, "minViscosity")) slot = 1;
+        }
+        if (t == UST_RH_VONMISES) {
+            if (streq(key, "cohesion")) slot = 0;
+            if (streq(key, "minimumViscosity")) slot = 1;
+            if (streq(key, "strainRateSoftening")) slot = 2;
+        }
+        if (t == UST_RH_DRUCKERPRAGER) {
+            if (streq(key, "cohesion")) slot = 0;
+            if (streq(key, "frictionCoefficient")) slot = 1;
+            if (streq(key, "minimumYieldStress")) slot = 2;
+            if (streq(key, "minimumViscosity")) slot = 3;
+            if (streq(key, "strainRateSoftening")) slot = 4;
+        }
+        if (slot < 0) return UST_EINVAL;
+        out->p[slot] = v;
+    }
+    return UST_OK;
+}
+
+int ust_swarm_upload_dev(ust_context *ctx, int np, const int *cell_off_dev, const double *xi0_dev,
+                         const double *xi1_dev, const double *xi2_dev, const double *w_dev,
+                         const unsigned char *material_dev, const float *coh_scale_dev, int max_ppc) {
+    if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "ust_set_mesh must come before the swarm");
+    if (np < 0 || !cell_off_dev || !xi0_dev || !xi1_dev || !w_dev || (ctx->grid.dim == 3 && !xi2_dev))
+        UST_FAIL(ctx, UST_EINVAL, "swarm arrays missing");
+    free_swarm(ctx);
+    size_t n = (size_t)np;
+    ctx->np = np;
+    ctx->max_ppc = max_ppc;
+    DALLOC(ctx, ctx->cell_off, (size_t)(ctx->nel + 1) * sizeof(int));
+    DALLOC(ctx, ctx->xi0, n * sizeof(double));
+    DALLOC(ctx, ctx->xi1, n * sizeof(double));
+    if (ctx->grid.dim == 3) DALLOC(ctx, ctx->xi2, n * sizeof(double));
+    DALLOC(ctx, ctx->w, n * sizeof(double));
+    DALLOC(ctx, ctx->eta, n * sizeof(double));
+    DALLOC(ctx, ctx->yielded, n);
+    CKD(ctx, ust_memcpy_d2d(ctx->cell_off, cell_off_dev, (size_t)(ctx->nel + 1) * sizeof(int), ctx->stream));
+    CKD(ctx, ust_memcpy_d2d(ctx->xi0, xi0_dev, n * sizeof(double), ctx->stream));
+    CKD(ctx, ust_memcpy_d2d(ctx->xi1, xi1_dev, n * sizeof(double), ctx->stream));
+    if (ctx->grid.dim == 3) CKD(ctx, ust_memcpy_d2d(ctx->xi2, xi2_dev, n * sizeof(double), ctx->stream));
+    CKD(ctx, ust_memcpy_d2d(ctx->w, w_dev, n * sizeof(double), ctx->stream));
+    if (material_dev) {
+        DALLOC(ctx, ctx->mat, n);
+        CKD(ctx, ust_memcpy_d2d(ctx->mat, material_dev, n, ctx->stream));
+    }
+    if (coh_scale_dev) {
+        DALLOC(ctx, ctx->coh, n * sizeof(float));
+        CKD(ctx, ust_memcpy_d2d(ctx->coh, coh_scale_dev, n * sizeof(float), ctx->stream));
+    }
+    CKD(ctx, ust_stream_sync(ctx->stream));
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
+int ust_swarm_upload(ust_context *ctx, int np, const int *cell_off, const double *xi, const double *w,
+                     const unsigned char *material, const float *coh_scale) {
+    if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "ust_set_mesh must come before the swarm");
+    if (np < 0 || !cell_off || !xi || !w) UST_FAIL(ctx, UST_EINVAL, "swarm arrays missing");
+    if (cell_off[0] != 0 || cell_off[ctx->nel] != np) UST_FAIL(ctx, UST_EINVAL, "cell_off must run 0..np");
+    int max_ppc = 0;
+    for (int e = 0; e < ctx->nel; e++) {
+        int c = cell_off[e + 1] - cell_off[e];
+        if (c < 0) UST_FAIL(ctx, UST_EINVAL, "cell_off not monotone at element %d", e);
+        if (c == 0)
+            UST_FAIL(ctx, UST_EINVAL, "element %d has no particles (ConstantWeights divides by the cell count)", e);
+        if (c > max_ppc) max_ppc = c;
+    }
+    if (material)
+        for (int i = 0; i < np; i++)
+            if (material[i] >= ctx->nmat)
+                UST_FAIL(ctx, UST_EINVAL, "particle %d: material %d not defined (\"No material available\")", i,
+                         material[i]);
+    size_t n = (size_t)np;
+    double *soa = (double *)malloc(n * sizeof(double) * 3 + 8);
+    if (!soa) UST_FAIL(ctx, UST_ENOMEM, "host allocation failed");
+    for (size_t i = 0; i < n; i++) {
+        soa[i] = xi[3 * i];
+        soa[n + i] = xi[3 * i + 1];
+        soa[2 * n + i] = xi[3 * i + 2];
+    }
+    void *d_off = NULL, *d_xi = NULL, *d_w = NULL, *d_m = NULL, *d_c = NULL;
+    int rc = UST_OK;
+    rc = ust_dev_malloc(&d_off, (size_t)(ctx->nel + 1) * sizeof(int));
+    if (!rc) rc = ust_dev_malloc(&d_xi, 3 * n * sizeof(double) + 8);
+    if (!rc) rc = ust_dev_malloc(&d_w, n * sizeof(double) + 8);
+    if (!rc && material) rc = ust_dev_malloc(&d_m, n + 8);
+    if (!rc && coh_scale) rc = ust_dev_malloc(&d_c, n * sizeof(float) + 8);
+    if (!rc) rc = ust_memcpy_h2d(d_off, cell_off, (size_t)(ctx->nel + 1) * sizeof(int), ctx->stream);
+    if (!rc) rc = ust_memcpy_h2d(d_xi, soa, 3 * n * sizeof(double), ctx->stream);
+    if (!rc) rc = ust_memcpy_h2d(d_w, w, n * sizeof(double), ctx->stream);
+    if (!rc && material) rc = ust_memcpy_h2d(d_m, material, n, ctx->stream);
+    if (!rc && coh_scale) rc = ust_memcpy_h2d(d_c, coh_scale, n * sizeof(float), ctx->stream);
+    free(soa);
+    if (!rc)
+        rc = ust_swarm_upload_dev(ctx, np, (const int *)d_off, (const double *)d_xi, (const double *)d_xi + n,
+                                  (const double *)d_xi + 2 * n, (const double *)d_w, (const unsigned char *)d_m,
+                                  (const float *)d_c, max_ppc);
+    else
+        snprintf(ctx->err, sizeof(ctx->err), "swarm upload failed: %s", ust_dev_error_string());
+    ust_dev_free(d_off);
+    ust_dev_free(d_xi);
+    ust_dev_free(d_w);
+    ust_dev_free(d_m);
+    ust_dev_free(d_c);
+    return rc;
+}
+
+int ust_set_temperature(ust_context *ctx, const double *T) {
+    if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "no mesh");
+    CKD(ctx, ust_memcpy_h2d(ctx->T, T, (size_t)ctx->nnodes * sizeof(double), ctx->stream));
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
+int ust_set_solution(ust_context *ctx, const double *u, const double *p) {
+    if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "no mesh");
+    if (u) CKD(ctx, ust_memcpy_h2d(ctx->u, u, (size_t)ctx->ndof * sizeof(double), ctx->stream));
+    if (p) CKD(ctx, ust_memcpy_h2d(ctx->p, p, (size_t)ctx->nel * sizeof(double), ctx->stream));
+    return UST_OK;
+}
+
+int ust_set_previous_solution_exists(ust_context *ctx, int flag) {
+    ctx->prev_solution = flag ? 1 : 0;
+    return UST_OK;
+}
+
+int ust_set_bcs(ust_context *ctx, const unsigned char *mask, const double *values) {
+    if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "no mesh");
+    memcpy(ctx->mask_h, mask, (size_t)ctx->ndof);
+    for (int i = 0; i < ctx->ndof; i++) ctx->bcval_h[i] = (mask[i] && values) ? values[i] : 0.0;
+    CKD(ctx, ust_memcpy_h2d(ctx->mask_d, ctx->mask_h, (size_t)ctx->ndof, ctx->stream));
+    CKD(ctx, ust_memcpy_h2d(ctx->bcval_d, ctx->bcval_h, (size_t)ctx->ndof * sizeof(double), ctx->stream));
+    ctx->has_bcs = 1;
+    ust_levels_free(ctx);
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
+/* K, G, approxS and RHS assembly.  The particle kernel runs the rheology chain of each
+ * particle and reduces the quadrature weights; everything else derives from the weights. */
+int ust_assemble(ust_context *ctx, double gravity) {
+    if (!ctx->has_mesh || !ctx->np || !ctx->nmat) UST_FAIL(ctx, UST_ESTATE, "mesh, materials and swarm are required");
+    if (!ctx->has_bcs) {
+        unsigned char *m0 = (unsigned char *)calloc((size_t)ctx->ndof, 1);
+        int rc = ust_set_bcs(ctx, m0, NULL);
+        free(m0);
+        if (rc) return rc;
+    }
+    size_t nd = (size_t)ctx->ndof, ne = (size_t)ctx->nel;
+    CKD(ctx, ust_k_particle_weights(&ctx->grid, ctx->cell_off, ctx->max_ppc, ctx->xi0, ctx->xi1, ctx->xi2, ctx->w,
+                                    ctx->mat, ctx->coh, ctx->mats_d, ctx->nmat, ctx->T, ctx->u, ctx->p,
+                                    ctx->prev_solution, gravity, ctx->eta, ctx->yielded, ctx->WK, ctx->WG, ctx->WF,
+                                    ctx->stream));
+    CKD(ctx, ust_k_uzawa_diag(&ctx->grid, ctx->WK, ctx->WG, ctx->approxS, ctx->stream));
+    /* f = buoyancy - K_{free,bc} u_bc ; h = -G_bc^T u_bc */
+    CKD(ctx, ust_k_fill_f64(ctx->f, 0.0, nd, ctx->stream));
+    CKD(ctx, ust_k_force(&ctx->grid, ctx->WF, gravity, ctx->mask_d, ctx->f, ctx->stream));
+    CKD(ctx, ust_k_lift_K(&ctx->grid, ctx->WK, ctx->mask_d, ctx->bcval_d, ctx->f, ctx->stream));
+    CKD(ctx, ust_k_fill_f64(ctx->h, 0.0, ne, ctx->stream));
+    CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, ctx->bcval_d, ctx->h, 1, ctx->stream));
+    /* is the constant pressure t in the null space of G?  ||G t|| / ||t|| < nstol ||D||_inf, nstol = 1e-7
+     * (KSPRemovePressureNullspace_BSSCR, ksp_pressure_nullspace.c:139-158; BSSCR.c:391).  With particle
+     * quadrature G t is usually not zero, and the constant is then part of the solution. */
+    {
+        double g1, gnorm;
+        CKD(ctx, ust_k_fill_f64(ctx->tmpp, 1.0, ne, ctx->stream));
+        CKD(ctx, ust_k_apply_G(&ctx->grid, ctx->WG, ctx->mask_d, ctx->tmpp, ctx->tmpv, ctx->stream));
+        CKD(ctx, ust_k_dot_f64(ctx->tmpv, ctx->tmpv, nd, ctx->partial, ctx->scal + SC_T0, ctx->stream));
+        CKD(ctx, ust_read_scalar(ctx, SC_T0, &g1));
+        CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, ctx->tmpv, ctx->tmpp, 2, ctx->stream));
+        CKD(ctx, ust_k_absmax_f64(ctx->tmpp, ne, ctx->partial, ctx->scal + SC_T0, ctx->stream));
+        CKD(ctx, ust_read_scalar(ctx, SC_T0, &gnorm));
+        ctx->ns_ratio = gnorm > 0 ? sqrt(g1) / sqrt((double)ne) / gnorm : 0.0;
+        ctx->nullspace = ctx->ns_ratio < 1e-7;
+    }
+    ctx->gravity = gravity;
+    ctx->assembled = 1;
+    ctx->levels_ready = 0;
+    ctx->cache_valid = 0;
+    return UST_OK;
+}
+
+static int need_assembled(ust_context *ctx) {
+    if (!ctx->assembled) UST_FAIL(ctx, UST_ESTATE, "ust_assemble has not run since the inputs changed");
+    return UST_OK;
+}
+
+int ust_get_element_matrices_dev(ust_context *ctx, int first, int count, double *Ke_dev) {
+    int rc = need_assembled(ctx);
+    if (rc) return rc;
+    if (first < 0 || count < 0 || first + count > ctx->nel) UST_FAIL(ctx, UST_EINVAL, "element range out of bounds");
+    CKD(ctx, ust_k_element_matrices(&ctx->grid, ctx->WK, first, count, Ke_dev, ctx->stream));
+    return UST_OK;
+}
+
+/* element batches are bounded so that the staging buffer stays modest */
+#define UST_BATCH 65536
+
+int ust_get_element_matrices(ust_context *ctx, int first, int count, double *Ke) {
+    int rc = need_assembled(ctx);
+    if (rc) return rc;
+    if (first < 0 || count < 0 || first + count > ctx->nel) UST_FAIL(ctx, UST_EINVAL, "element range out of bounds");
+    int nd = (1 << ctx->grid.dim) * ctx->grid.dim;
+    double *buf = NULL;
+    int bs = count < UST_BATCH ? count : UST_BATCH;
+    if (bs == 0) return UST_OK;
+    CKD(ctx, ust_dev_malloc((void **)&buf, (size_t)bs * nd * nd * sizeof(double)));
+    for (int s = 0; s < count; s += bs) {
+        int c = count - s < bs ? count - s : bs;
+        rc = ust_k_element_matrices(&ctx->grid, ctx->WK, first + s, c, buf, ctx->stream);
+        if (!rc) rc = ust_memcpy_d2h(Ke + (size_t)s * nd * nd, buf, (size_t)c * nd * nd * sizeof(double), ctx->stream);
+        if (rc) break;
+    }
+    ust_dev_free(buf);
+    if (rc) UST_FAIL(ctx, rc, "element matrices: %s", ust_dev_error_string());
+    return UST_OK;
+}
+
+static int get_elem_vec(ust_context *ctx, int first, int count, double *out, int which) {
+    int rc = need_assembled(ctx);
+    if (rc) return rc;
+    if (first < 0 || count < 0 || first + count > ctx->nel) UST_FAIL(ctx, UST_EINVAL, "element range out of bounds");
+    if (count == 0) return UST_OK;
+    int nd = (1 << ctx->grid.dim) * ctx->grid.dim;
+    double *buf = NULL;
+    CKD(ctx, ust_dev_malloc((void **)&buf, (size_t)count * nd * sizeof(double)));
+    if (which == 0) rc = ust_k_element_G(&ctx->grid, ctx->WG, first, count, buf, ctx->stream);
+    else rc = ust_k_element_f(&ctx->grid, ctx->WF, ctx->gravity, first, count, buf, ctx->stream);
+    if (!rc) rc = ust_memcpy_d2h(out, buf, (size_t)count * nd * sizeof(double), ctx->stream);
+    ust_dev_free(buf);
+    if (rc) UST_FAIL(ctx, rc, "element vectors: %s", ust_dev_error_string());
+    return UST_OK;
+}
+
+int ust_get_element_G(ust_context *ctx, int first, int count, double *Ge) { return get_elem_vec(ctx, first, count, Ge, 0); }
+int ust_get_element_f(ust_context *ctx, int first, int count, double *fe) { return get_elem_vec(ctx, first, count, fe, 1); }
+
+int ust_get_approxS(ust_context *ctx, double *approxS) {
+    int rc = need_assembled(ctx);
+    if (rc) return rc;
+    CKD(ctx, ust_memcpy_d2h(approxS, ctx->approxS, (size_t)ctx->nel * sizeof(double), ctx->stream));
+    return UST_OK;
+}
+
+int ust_get_particle_viscosity(ust_context *ctx, double *eta, unsigned char *yielded) {
+    int rc = need_assembled(ctx);
+    if (rc) return rc;
+    if (eta) CKD(ctx, ust_memcpy_d2h(eta, ctx->eta, (size_t)ctx->np * sizeof(double), ctx->stream));
+    if (yielded) CKD(ctx, ust_memcpy_d2h(yielded, ctx->yielded, (size_t)ctx->np, ctx->stream));
+    return UST_OK;
+}
+
+int ust_get_weights(ust_context *ctx, double *WK, double *WG, double *WF) {
+    int rc = need_assembled(ctx);
+    if (rc) return rc;
+    int nq = ctx->grid.dim == 3 ? 27 : 9, nv = 1 << ctx->grid.dim;
+    size_t ne = (size_t)ctx->nel;
+    if (WK) CKD(ctx, ust_memcpy_d2h(WK, ctx->WK, nq * ne * sizeof(double), ctx->stream));
+    if (WG) CKD(ctx, ust_memcpy_d2h(WG, ctx->WG, nv * ne * sizeof(double), ctx->stream));
+    if (WF) CKD(ctx, ust_memcpy_d2h(WF, ctx->WF, nv * ne * sizeof(double), ctx->stream));
+    return UST_OK;
+}
+
+int ust_get_rhs(ust_context *ctx, double *f, double *h) {
+    int rc = need_assembled(ctx);
+    if (rc) return rc;
+    if (f) CKD(ctx, ust_memcpy_d2h(f, ctx->f, (size_t)ctx->ndof * sizeof(double), ctx->stream));
+    if (h) CKD(ctx, ust_memcpy_d2h(h, ctx->h, (size_t)ctx->nel * sizeof(double), ctx->stream));
+    return UST_OK;
+}
+
+int ust_apply_K_dev(ust_context *ctx, const double *u, double *y) {
+    int rc = need_assembled(ctx);
+    if (rc) return rc;
+    CKD(ctx, ust_k_apply_K_f64(&ctx->grid, ctx->WK, ctx->mask_d, u, y, ctx->stream));
+    return UST_OK;
+}
+
+int ust_apply_K_f32_dev(ust_context *ctx, const float *u, float *y) {
+    int rc = need_assembled(ctx);
+    if (rc) return rc;
+    if (!ctx->levels_ready || !ctx->levels_prec32) {
+        ust_solve_opts o;
+        ust_solve_opts_default(&o);
+        rc = ust_levels_setup(ctx, &o, 1);
+        if (rc) return rc;
+    }
+    CKD(ctx, ust_k_apply_K_f32(&ctx->grid, ctx->lev[0].WK32, ctx->mask_d, u, y, ctx->stream));
+    return UST_OK;
+}
+
+static int host_op(ust_context *ctx, const double *in, size_t nin, double *out, size_t nout, int which) {
+    int rc = need_assembled(ctx);
+    if (rc) return rc;
+    double *a = NULL, *b = NULL;
+    CKD(ctx, ust_dev_malloc((void **)&a, nin * sizeof(double)));
+    rc = ust_dev_malloc((void **)&b, nout * sizeof(double));
+    if (!rc) rc = ust_memcpy_h2d(a, in, nin * sizeof(double), ctx->stream);
+    if (!rc) {
+        if (which == 0) rc = ust_k_apply_K_f64(&ctx->grid, ctx->WK, ctx->mask_d, a, b, ctx->stream);
+        else if (which == 1) rc = ust_k_apply_G(&ctx->grid, ctx->WG, ctx->mask_d, a, b, ctx->stream);
+        else rc = ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, a, b, 0, ctx->stream);
+    }
+    if (!rc) rc = ust_memcpy_d2h(out, b, nout * sizeof(double), ctx->stream);
+    ust_dev_free(a);
+    ust_dev_free(b);
+    if (rc) UST_FAIL(ctx, rc, "operator apply failed: %s", ust_dev_error_string());
+    return UST_OK;
+}
+
+int ust_apply_K(ust_context *ctx, const double *u, double *y) {
+    return host_op(ctx, u, (size_t)ctx->ndof, y, (size_t)ctx->ndof, 0);
+}
+int ust_apply_G(ust_context *ctx, const double *p, double *y) {
+    return host_op(ctx, p, (size_t)ctx->nel, y, (size_t)ctx->ndof, 1);
+}
+int ust_apply_Gt(ust_context *ctx, const double *u, double *q) {
+    return host_op(ctx, u, (size_t)ctx->ndof, q, (size_t)ctx->nel, 2);
+}
+
+int ust_get_solution(ust_context *ctx, double *u, double *p) {
+    if (u) CKD(ctx, ust_memcpy_d2h(u, ctx->u, (size_t)ctx->ndof * sizeof(double), ctx->stream));
+    if (p) CKD(ctx, ust_memcpy_d2h(p, ctx->p, (size_t)ctx->nel * sizeof(double), ctx->stream));
+    return UST_OK;
+}
+
+int ust_get_solution_dev(ust_context *ctx, const double **u, const double **p) {
+    if (u) *u = ctx->u;
+    if (p) *p = ctx->p;
+    return UST_OK;
+}
+
+/* ---- swarm generated on the device ---- */
+int ust_swarm_random(ust_context *ctx, const int *cell_off_host, int ppc, long seed) {
+    if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "ust_set_mesh must come before the swarm");
+    int ne = ctx->nel;
+    int *off = (int *)malloc(sizeof(int) * (size_t)(ne + 1));
+    if (!off) UST_FAIL(ctx, UST_ENOMEM, "host allocation failed");
+    if (cell_off_host) {
+        memcpy(off, cell_off_host, sizeof(int) * (size_t)(ne + 1));
+    } else {
+        if (ppc < 1) {
+            free(off);
+            UST_FAIL(ctx, UST_EINVAL, "particlesPerCell must be >= 1");
+        }
+        for (int e = 0; e <= ne; e++) off[e] = e * ppc;
+    }
+    int np = off[ne], max_ppc = 0;
+    for (int e = 0; e < ne; e++) {
+        int c = off[e + 1] - off[e];
+        if (c < 1) {
+            free(off);
+            UST_FAIL(ctx, UST_EINVAL, "element %d has no particles", e);
+        }
+        if (c > max_ppc) max_ppc = c;
+    }
+    free_swarm(ctx);
+    size_t n = (size_t)np;
+    ctx->np = np;
+    ctx->max_ppc = max_ppc;
+    int rc = ust_dev_alloc_zero(ctx, (void **)&ctx->cell_off, (size_t)(ne + 1) * sizeof(int));
+    if (!rc) rc = ust_memcpy_h2d(ctx->cell_off, off, (size_t)(ne + 1) * sizeof(int), ctx->stream);
+    free(off);
+    if (rc) UST_FAIL(ctx, rc, "swarm upload failed: %s", ust_dev_error_string());
+    DALLOC(ctx, ctx->xi0, n * sizeof(double));
+    DALLOC(ctx, ctx->xi1, n * sizeof(double));
+    if (ctx->grid.dim == 3) DALLOC(ctx, ctx->xi2, n * sizeof(double));
+    DALLOC(ctx, ctx->w, n * sizeof(double));
+    DALLOC(ctx, ctx->eta, n * sizeof(double));
+    DALLOC(ctx, ctx->yielded, n);
+    DALLOC(ctx, ctx->mat, n);
+    CKD(ctx, ust_k_random_layout(&ctx->grid, ctx->cell_off, np, seed, ctx->xi0, ctx->xi1, ctx->xi2, ctx->w,
+                                 ctx->stream));
+    CKD(ctx, ust_stream_sync(ctx->stream));
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
+int ust_swarm_assign_materials(ust_context *ctx, int nshape, const ust_shape *shapes,
+                               const unsigned char *cell_material) {
+    if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
+    if (nshape < 0 || nshape > 4096) UST_FAIL(ctx, UST_EINVAL, "bad shape count");
+    for (int s = 0; s < nshape; s++)
+        if (shapes[s].material < 0 || shapes[s].material >= ctx->nmat)
+            UST_FAIL(ctx, UST_EINVAL, "shape %d names material %d (only %d defined)", s, shapes[s].material, ctx->nmat);
+    if (!ctx->mat) DALLOC(ctx, ctx->mat, (size_t)ctx->np);
+    void *d_sh = NULL, *d_cm = NULL;
+    int rc = UST_OK;
+    if (nshape) {
+        rc = ust_dev_malloc(&d_sh, sizeof(ust_shape) * (size_t)nshape);
+        if (!rc) rc = ust_memcpy_h2d(d_sh, shapes, sizeof(ust_shape) * (size_t)nshape, ctx->stream);
+    }
+    if (!rc && cell_material) {
+        for (int e = 0; e < ctx->nel; e++)
+            if (cell_material[e] >= ctx->nmat) {
+                ust_dev_free(d_sh);
+                UST_FAIL(ctx, UST_EINVAL, "cell %d names material %d (only %d defined)", e, cell_material[e], ctx->nmat);
+            }
+        rc = ust_dev_malloc(&d_cm, (size_t)ctx->nel);
+        if (!rc) rc = ust_memcpy_h2d(d_cm, cell_material, (size_t)ctx->nel, ctx->stream);
+    }
+    if (!rc)
+        rc = ust_k_assign_materials(&ctx->grid, ctx->cell_off, ctx->np, ctx->xi0, ctx->xi1, ctx->xi2,
+                                    (const ust_shape *)d_sh, nshape, (const unsigned char *)d_cm, ctx->mat, ctx->stream);
+    if (!rc) rc = ust_stream_sync(ctx->stream);
+    ust_dev_free(d_sh);
+    ust_dev_free(d_cm);
+    if (rc) UST_FAIL(ctx, rc, "material assignment failed: %s", ust_dev_error_string());
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
+int ust_swarm_cohesion_noise(ust_context *ctx, float amplitude, unsigned long long seed) {
+    if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
+    if (!ctx->coh) DALLOC(ctx, ctx->coh, (size_t)ctx->np * sizeof(float));
+    CKD(ctx, ust_k_noise_scale(ctx->coh, ctx->np, amplitude, seed, ctx->stream));
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
+int ust_swarm_download(ust_context *ctx, int *cell_off, double *xi, double *w, unsigned char *material) {
+    if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
+    size_t n = (size_t)ctx->np;
+    if (cell_off) CKD(ctx, ust_memcpy_d2h(cell_off, ctx->cell_off, (size_t)(ctx->nel + 1) * sizeof(int), ctx->stream));
+    if (xi) {
+        double *tmp = (double *)malloc(n * sizeof(double));
+        if (!tmp) UST_FAIL(ctx, UST_ENOMEM, "host allocation failed");
+        const double *src[3] = {ctx->xi0, ctx->xi1, ctx->xi2};
+        for (int d = 0; d < 3; d++) {
+            if (d < ctx->grid.dim) {
+                int rc = ust_memcpy_d2h(tmp, src[d], n * sizeof(double), ctx->stream);
+                if (rc) {
+                    free(tmp);
+                    UST_FAIL(ctx, rc, "download failed: %s", ust_dev_error_string());
+                }
+            } else {
+                memset(tmp, 0, n * sizeof(double));
+            }
+            for (size_t i = 0; i < n; i++) xi[3 * i + d] = tmp[i];
+        }
+        free(tmp);
+    }
+    if (w) CKD(ctx, ust_memcpy_d2h(w, ctx->w, n * sizeof(double), ctx->stream));
+    if (material) {
+        if (ctx->mat) CKD(ctx, ust_memcpy_d2h(material, ctx->mat, n, ctx->stream));
+        else memset(material, 0, n);
+    }
+    return UST_OK;
+}
+
+int ust_swarm_count(ust_context *ctx) { return ctx->np; }

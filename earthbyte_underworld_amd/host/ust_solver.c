@@ -1,0 +1,561 @@
+/*
+ * ust_solver.c -- BSSCR Schur-complement solve on the GPU (C host driving the kernel ABI).
+ *
+ * Restates BSSCR_DRIVER_auglag (Solvers/KSPSolvers/src/BSSCR/auglag-driver-DGTGD.c:191-403) with
+ * penalty 0: S = G^T K^-1 G (the reference's -D K^-1 G; CG iterates are identical), h_hat =
+ * G^T K^-1 f - h, CG on S with the "uw" diagonal approxS (preconditioner.c:138-166), constant
+ * pressure null space removed (ksp_pressure_nullspace.c:121-176), back-solve u = K^-1 (f - G p).
+ * The A11 solves use CG preconditioned by a geometric multigrid V-cycle (PETScMGSolver.c:459-716
+ * with the SROpGenerator transfers, SROpGenerator.c:1000-1041) whose Galerkin coarse operators
+ * are formed exactly in the quadrature-weight representation; smoothing is Jacobi-Chebyshev
+ * (a parallel smoother in place of the reference's sequential SOR) and the coarsest level is a
+ * dense FP64 inverse.  Mixed precision runs the A11 solves in FP32 inside FP64 iterative
+ * refinement; outer Krylov, residuals and the coarse solve stay FP64.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "ust_internal.h"
+
+#define S(ctx) ((ctx)->stream)
+
+static double wall_ms(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+static void dfree(void **p) {
+    if (*p) ust_dev_free(*p);
+    *p = NULL;
+}
+#define DFREE(x) dfree((void **)&(x))
+#define DALLOC(ctx, ptr, bytes) CKD(ctx, ust_dev_alloc_zero(ctx, (void **)&(ptr), (bytes)))
+
+static int grid_ndof(const ust_grid *g) {
+    int n = 1;
+    for (int d = 0; d < g->dim; d++) n *= g->nel[d] + 1;
+    return n * g->dim;
+}
+static int grid_nel(const ust_grid *g) {
+    int n = 1;
+    for (int d = 0; d < g->dim; d++) n *= g->nel[d];
+    return n;
+}
+
+void ust_levels_free(ust_context *ctx) {
+    for (int l = 0; l < ctx->nlev; l++) {
+        ust_level *L = &ctx->lev[l];
+        if (l > 0) {
+            DFREE(L->WK64);
+            DFREE(L->mask);
+            free(L->mask_h);
+        }
+        DFREE(L->WK32);
+        DFREE(L->invd64);
+        DFREE(L->invd32);
+        DFREE(L->x64); DFREE(L->b64); DFREE(L->ax64); DFREE(L->d64);
+        DFREE(L->x32); DFREE(L->b32); DFREE(L->ax32); DFREE(L->d32);
+        memset(L, 0, sizeof(*L));
+    }
+    DFREE(ctx->Ainv);
+    ctx->nlev = 0;
+    ctx->levels_ready = 0;
+    ctx->levels_prec32 = 0;
+}
+
+/* ---------------- level operations (prec: 0 = FP64, 1 = FP32) ---------------- */
+static int lev_apply(ust_context *ctx, int l, int f32, const void *x, void *y) {
+    ust_level *L = &ctx->lev[l];
+    if (f32) CKD(ctx, ust_k_apply_K_f32(&L->g, L->WK32, L->mask, (const float *)x, (float *)y, S(ctx)));
+    else CKD(ctx, ust_k_apply_K_f64(&L->g, L->WK64, L->mask, (const double *)x, (double *)y, S(ctx)));
+    return UST_OK;
+}
+
+/* Chebyshev smoothing of A x = b; x_zero: x is zero on entry (its first product is skipped) */
+static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void *x, int x_zero) {
+    ust_level *L = &ctx->lev[l];
+    size_t n = (size_t)L->ndof;
+    for (int k = 0; k < L->cheb_deg; k++) {
+        if (k == 0 && x_zero) {
+            if (f32) CKD(ctx, ust_k_fill_f32(L->ax32, 0.0f, n, S(ctx)));
+            else CKD(ctx, ust_k_fill_f64(L->ax64, 0.0, n, S(ctx)));
+        } else {
+            int rc = lev_apply(ctx, l, f32, x, f32 ? (void *)L->ax32 : (void *)L->ax64);
+            if (rc) return rc;
+        }
+        if (f32)
+            CKD(ctx, ust_k_cheby_step_f32((const float *)b, L->ax32, L->invd32, L->d32, (float *)x, (float)L->c1[k],
+                                          (float)L->c2[k], n, S(ctx)));
+        else
+            CKD(ctx, ust_k_cheby_step_f64((const double *)b, L->ax64, L->invd64, L->d64, (double *)x, L->c1[k],
+                                          L->c2[k], n, S(ctx)));
+    }
+    return UST_OK;
+}
+
+/* one V-cycle: x = M^-1 b on level l (x is overwritten) */
+static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x) {
+    ust_level *L = &ctx->lev[l];
+    size_t n = (size_t)L->ndof;
+    int rc;
+    if (l == ctx->nlev - 1) {
+        if (f32) CKD(ctx, ust_k_dense_matvec_f32(ctx->Ainv, ctx->nc, (const float *)b, (float *)x, S(ctx)));
+        else CKD(ctx, ust_k_dense_matvec_f64(ctx->Ainv, ctx->nc, (const double *)b, (double *)x, S(ctx)));
+        return UST_OK;
+    }
+    ust_level *C = &ctx->lev[l + 1];
+    if (f32) CKD(ctx, ust_k_fill_f32((float *)x, 0.0f, n, S(ctx)));
+    else CKD(ctx, ust_k_fill_f64((double *)x, 0.0, n, S(ctx)));
+    if ((rc = lev_smooth(ctx, l, f32, b, x, 1))) return rc;
+    if ((rc = lev_apply(ctx, l, f32, x, f32 ? (void *)L->ax32 : (void *)L->ax64))) return rc;
+    if (f32) {
+        CKD(ctx, ust_k_residual_f32((const float *)b, L->ax32, L->ax32, n, S(ctx)));
+        CKD(ctx, ust_k_restrict_f32(&L->g, L->mask, C->mask, L->ax32, C->b32, S(ctx)));
+        if ((rc = vcycle(ctx, l + 1, 1, C->b32, C->x32))) return rc;
+        CKD(ctx, ust_k_prolong_add_f32(&L->g, L->mask, C->x32, (float *)x, S(ctx)));
+    } else {
+        CKD(ctx, ust_k_residual_f64((const double *)b, L->ax64, L->ax64, n, S(ctx)));
+        CKD(ctx, ust_k_restrict_f64(&L->g, L->mask, C->mask, L->ax64, C->b64, S(ctx)));
+        if ((rc = vcycle(ctx, l + 1, 0, C->b64, C->x64))) return rc;
+        CKD(ctx, ust_k_prolong_add_f64(&L->g, L->mask, C->x64, (double *)x, S(ctx)));
+    }
+    return lev_smooth(ctx, l, f32, b, x, 0);
+}
+
+/* lambda_max of D^-1 A by power iteration (FP64) */
+static int estimate_lmax(ust_context *ctx, int l, double *lmax) {
+    ust_level *L = &ctx->lev[l];
+    size_t n = (size_t)L->ndof;
+    double *x = L->x64, *y = L->ax64, nrm, lam = 0.0;
+    CKD(ctx, ust_k_fill_hash_f64(x, n, 12345 + l, S(ctx)));
+    CKD(ctx, ust_k_dot_f64(x, x, n, ctx->partial, ctx->scal + SC_T0, S(ctx)));
+    CKD(ctx, ust_read_scalar(ctx, SC_T0, &nrm));
+    CKD(ctx, ust_k_axpby_f64(0.0, x, 1.0 / sqrt(nrm), x, n, S(ctx)));
+    for (int it = 0; it < 15; it++) {
+        int rc = lev_apply(ctx, l, 0, x, y);
+        if (rc) return rc;
+        CKD(ctx, ust_k_scale_mul_f64(L->invd64, y, y, n, S(ctx)));
+        CKD(ctx, ust_k_dot_f64(y, y, n, ctx->partial, ctx->scal + SC_T0, S(ctx)));
+        CKD(ctx, ust_read_scalar(ctx, SC_T0, &nrm));
+        nrm = sqrt(nrm);
+        if (!(nrm > 0.0)) break;
+        lam = nrm;
+        CKD(ctx, ust_k_axpby_f64(1.0 / nrm, y, 0.0, x, n, S(ctx)));
+    }
+    *lmax = lam;
+    return UST_OK;
+}
+
+int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
+    if (ctx->levels_ready && (!need32 || ctx->levels_prec32) && ctx->lev[0].cheb_deg == o->smoother_degree)
+        return UST_OK;
+    ust_levels_free(ctx);
+    int dim = ctx->grid.dim, maxlev = o->mg_levels > 0 ? o->mg_levels : UST_MAX_LEVELS;
+    if (maxlev > UST_MAX_LEVELS) maxlev = UST_MAX_LEVELS;
+    ust_grid g = ctx->grid;
+    int nlev = 1;
+    while (nlev < maxlev && grid_ndof(&g) > 600) {
+        int ok = 1;
+        for (int d = 0; d < dim; d++)
+            if (g.nel[d] % 2 || g.nel[d] < 2) ok = 0;
+        if (!ok) break;
+        for (int d = 0; d < dim; d++) {
+            g.nel[d] /= 2;
+            g.h[d] *= 2.0;
+        }
+        nlev++;
+    }
+    if (grid_ndof(&g) > 4096)
+        UST_FAIL(ctx, UST_EINVAL,
+                 "multigrid: coarsest grid has %d dofs (> 4096); use element counts with more factors of two",
+                 grid_ndof(&g));
+    ctx->nlev = nlev;
+    int nq = dim == 3 ? 27 : 9;
+    g = ctx->grid;
+    for (int l = 0; l < nlev; l++) {
+        ust_level *L = &ctx->lev[l];
+        L->g = g;
+        L->ndof = grid_ndof(&g);
+        L->nel = grid_nel(&g);
+        L->nnodes = L->ndof / dim;
+        if (l == 0) {
+            L->WK64 = ctx->WK;
+            L->mask = ctx->mask_d;
+            L->mask_h = ctx->mask_h;
+        } else {
+            ust_level *F = &ctx->lev[l - 1];
+            DALLOC(ctx, L->WK64, (size_t)nq * L->nel * sizeof(double));
+            CKD(ctx, ust_k_coarsen_weights(&F->g, F->WK64, L->WK64, S(ctx)));
+            /* a coarse node coincides with fine node (2I, 2J, 2K): it inherits that node's BC flags */
+            L->mask_h = (unsigned char *)calloc((size_t)L->ndof, 1);
+            int fnx = F->g.nel[0] + 1, fny = F->g.nel[1] + 1, cnx = g.nel[0] + 1, cny = g.nel[1] + 1;
+            for (int n = 0; n < L->nnodes; n++) {
+                int I = n % cnx, J = (n / cnx) % cny, K = n / (cnx * cny);
+                int fn = 2 * I + fnx * (2 * J + fny * 2 * K);
+                for (int c = 0; c < dim; c++) L->mask_h[n * dim + c] = F->mask_h[fn * dim + c];
+            }
+            DALLOC(ctx, L->mask, (size_t)L->ndof);
+            CKD(ctx, ust_memcpy_h2d(L->mask, L->mask_h, (size_t)L->ndof, S(ctx)));
+        }
+        size_t n = (size_t)L->ndof;
+        DALLOC(ctx, L->invd64, n * sizeof(double));
+        DALLOC(ctx, L->x64, n * sizeof(double));
+        DALLOC(ctx, L->b64, n * sizeof(double));
+        DALLOC(ctx, L->ax64, n * sizeof(double));
+        DALLOC(ctx, L->d64, n * sizeof(double));
+        if (need32) {
+            DALLOC(ctx, L->invd32, n * sizeof(float));
+            DALLOC(ctx, L->WK32, (size_t)nq * L->nel * sizeof(float));
+            CKD(ctx, ust_k_copy_f64_to_f32(L->WK64, L->WK32, (size_t)nq * L->nel, S(ctx)));
+            DALLOC(ctx, L->x32, n * sizeof(float));
+            DALLOC(ctx, L->b32, n * sizeof(float));
+            DALLOC(ctx, L->ax32, n * sizeof(float));
+            DALLOC(ctx, L->d32, n * sizeof(float));
+        }
+        CKD(ctx, ust_k_inv_diag(&L->g, L->WK64, L->mask, L->invd64, L->invd32, S(ctx)));
+        if (l + 1 < nlev)
+            for (int d = 0; d < dim; d++) {
+                g.nel[d] /= 2;
+                g.h[d] *= 2.0;
+            }
+    }
+    /* coarsest: dense FP64 inverse */
+    ust_level *Lc = &ctx->lev[nlev - 1];
+    ctx->nc = Lc->ndof;
+    DALLOC(ctx, ctx->Ainv, (size_t)ctx->nc * ctx->nc * sizeof(double));
+    CKD(ctx, ust_k_dense_assemble(&Lc->g, Lc->WK64, Lc->mask, ctx->Ainv, S(ctx)));
+    CKD(ctx, ust_k_dense_spd_inverse(ctx->Ainv, ctx->nc, S(ctx)));
+    /* Chebyshev polynomials */
+    int deg = o->smoother_degree > 0 ? o->smoother_degree : 2;
+    if (deg > UST_MAX_CHEBY) deg = UST_MAX_CHEBY;
+    double ratio = o->cheby_ratio > 1.0 ? o->cheby_ratio : 12.0;
+    for (int l = 0; l + 1 < nlev; l++) {
+        ust_level *L = &ctx->lev[l];
+        double lm;
+        int rc = estimate_lmax(ctx, l, &lm);
+        if (rc) return rc;
+        L->lmax = 1.1 * lm;
+        double hi = L->lmax, lo = hi / ratio;
+        double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta, rho = 1.0 / sigma;
+        L->cheb_deg = deg;
+        L->c1[0] = 0.0;
+        L->c2[0] = 1.0 / theta;
+        for (int k = 1; k < deg; k++) {
+            double rn = 1.0 / (2.0 * sigma - rho);
+            L->c1[k] = rn * rho;
+            L->c2[k] = 2.0 * rn / delta;
+            rho = rn;
+        }
+    }
+    ctx->lev[0].cheb_deg = deg;
+    ctx->levels_ready = 1;
+    ctx->levels_prec32 = need32;
+    CKD(ctx, ust_stream_sync(S(ctx)));
+    return UST_OK;
+}
+
+static int ensure_work(ust_context *ctx, int need32) {
+    size_t n = (size_t)ctx->ndof, ne = (size_t)ctx->nel;
+    if (!ctx->vr) {
+        DALLOC(ctx, ctx->vr, n * sizeof(double));
+        DALLOC(ctx, ctx->vz, n * sizeof(double));
+        DALLOC(ctx, ctx->vp, n * sizeof(double));
+        DALLOC(ctx, ctx->vq, n * sizeof(double));
+        DALLOC(ctx, ctx->vt, n * sizeof(double));
+        DALLOC(ctx, ctx->va, n * sizeof(double));
+        DALLOC(ctx, ctx->vb, n * sizeof(double));
+        DALLOC(ctx, ctx->uprev, n * sizeof(double));
+        DALLOC(ctx, ctx->pr, ne * sizeof(double));
+        DALLOC(ctx, ctx->pz, ne * sizeof(double));
+        DALLOC(ctx, ctx->pd, ne * sizeof(double));
+        DALLOC(ctx, ctx->pq, ne * sizeof(double));
+        DALLOC(ctx, ctx->ph, ne * sizeof(double));
+        DALLOC(ctx, ctx->ppc, ne * sizeof(double));
+        DALLOC(ctx, ctx->pprev, ne * sizeof(double));
+    }
+    if (need32 && !ctx->wr) {
+        DALLOC(ctx, ctx->wr, n * sizeof(float));
+        DALLOC(ctx, ctx->wz, n * sizeof(float));
+        DALLOC(ctx, ctx->wp, n * sizeof(float));
+        DALLOC(ctx, ctx->wq, n * sizeof(float));
+        DALLOC(ctx, ctx->wx, n * sizeof(float));
+    }
+    return UST_OK;
+}
+
+static int dotv(ust_context *ctx, const double *a, const double *b, size_t n, int slot, double *out) {
+    CKD(ctx, ust_k_dot_f64(a, b, n, ctx->partial, ctx->scal + slot, S(ctx)));
+    if (out) CKD(ctx, ust_read_scalar(ctx, slot, out));
+    return UST_OK;
+}
+
+/* PCG with a V-cycle preconditioner; x = 0 on entry.  Stops at ||r|| <= rtol ||b||. */
+static int pcg(ust_context *ctx, int f32, const void *b, void *x, double rtol, int maxit, int *its) {
+    size_t n = (size_t)ctx->ndof;
+    double bb, rr;
+    int rc;
+    void *r = f32 ? (void *)ctx->wr : (void *)ctx->vr, *z = f32 ? (void *)ctx->wz : (void *)ctx->vz;
+    void *p = f32 ? (void *)ctx->wp : (void *)ctx->vp, *q = f32 ? (void *)ctx->wq : (void *)ctx->vq;
+    if (f32) {
+        CKD(ctx, ust_k_fill_f32((float *)x, 0.0f, n, S(ctx)));
+        CKD(ctx, ust_memcpy_d2d(r, b, n * sizeof(float), S(ctx)));
+        CKD(ctx, ust_k_dot_f32((const float *)b, (const float *)b, n, ctx->partial, ctx->scal + SC_BB, S(ctx)));
+    } else {
+        CKD(ctx, ust_k_fill_f64((double *)x, 0.0, n, S(ctx)));
+        CKD(ctx, ust_memcpy_d2d(r, b, n * sizeof(double), S(ctx)));
+        CKD(ctx, ust_k_dot_f64((const double *)b, (const double *)b, n, ctx->partial, ctx->scal + SC_BB, S(ctx)));
+    }
+    CKD(ctx, ust_read_scalar(ctx, SC_BB, &bb));
+    *its = 0;
+    if (!(bb > 0.0)) return UST_OK;
+    const double tol2 = rtol * rtol * bb;
+    for (int it = 0; it < maxit; it++) {
+        if ((rc = vcycle(ctx, 0, f32, r, z))) return rc;
+        if (f32) {
+            CKD(ctx, ust_k_dot_f32((const float *)r, (const float *)z, n, ctx->partial, ctx->scal + SC_RZ, S(ctx)));
+            CKD(ctx, ust_k_cg_update_p_f32((const float *)z, (float *)p, ctx->scal + SC_RZ, ctx->scal + SC_RZOLD,
+                                           it == 0, n, S(ctx)));
+        } else {
+            CKD(ctx, ust_k_dot_f64((const double *)r, (const double *)z, n, ctx->partial, ctx->scal + SC_RZ, S(ctx)));
+            CKD(ctx, ust_k_cg_update_p_f64((const double *)z, (double *)p, ctx->scal + SC_RZ, ctx->scal + SC_RZOLD,
+                                           it == 0, n, S(ctx)));
+        }
+        if ((rc = lev_apply(ctx, 0, f32, p, q))) return rc;
+        if (f32) {
+            CKD(ctx, ust_k_dot_f32((const float *)p, (const float *)q, n, ctx->partial, ctx->scal + SC_PQ, S(ctx)));
+            CKD(ctx, ust_k_cg_update_xr_f32((float *)x, (float *)r, (const float *)p, (const float *)q,
+                                            ctx->scal + SC_RZ, ctx->scal + SC_PQ, n, S(ctx)));
+            CKD(ctx, ust_k_dot_f32((const float *)r, (const float *)r, n, ctx->partial, ctx->scal + SC_RR, S(ctx)));
+        } else {
+            CKD(ctx, ust_k_dot_f64((const double *)p, (const double *)q, n, ctx->partial, ctx->scal + SC_PQ, S(ctx)));
+            CKD(ctx, ust_k_cg_update_xr_f64((double *)x, (double *)r, (const double *)p, (const double *)q,
+                                            ctx->scal + SC_RZ, ctx->scal + SC_PQ, n, S(ctx)));
+            CKD(ctx, ust_k_dot_f64((const double *)r, (const double *)r, n, ctx->partial, ctx->scal + SC_RR, S(ctx)));
+        }
+        CKD(ctx, ust_memcpy_d2d(ctx->scal + SC_RZOLD, ctx->scal + SC_RZ, sizeof(double), S(ctx)));
+        CKD(ctx, ust_read_scalar(ctx, SC_RR, &rr));
+        *its = it + 1;
+        if (!(rr == rr)) UST_FAIL(ctx, UST_ENOTCONV, "A11 CG produced NaN (StokesBlockKSPInterface.c:32 check)");
+        if (rr <= tol2) break;
+    }
+    return UST_OK;
+}
+
+/* velocity solve K x = b (FP64 result); mixed precision refines FP32 PCG solves in FP64 */
+static int a11_solve(ust_context *ctx, const ust_solve_opts *o, const double *b, double *x, ust_solve_stats *st) {
+    size_t n = (size_t)ctx->ndof;
+    int its = 0, rc;
+    st->inner_solves++;
+    if (o->precision == UST_PREC_FP64) {
+        rc = pcg(ctx, 0, b, x, o->inner_rtol, o->max_inner, &its);
+        st->inner_its += its;
+        return rc;
+    }
+    double bb, rr;
+    CKD(ctx, ust_k_fill_f64(x, 0.0, n, S(ctx)));
+    CKD(ctx, ust_memcpy_d2d(ctx->va, b, n * sizeof(double), S(ctx)));
+    CKD(ctx, dotv(ctx, b, b, n, SC_T1, &bb));
+    if (!(bb > 0.0)) return UST_OK;
+    rr = bb;
+    for (int ir = 0; ir < 40; ir++) {
+        if (rr <= o->inner_rtol * o->inner_rtol * bb) break;
+        CKD(ctx, ust_k_copy_f64_to_f32(ctx->va, ctx->lev[0].b32, n, S(ctx)));
+        rc = pcg(ctx, 1, ctx->lev[0].b32, ctx->wx, o->mixed_inner_rtol, o->max_inner, &its);
+        st->inner_its += its;
+        if (rc) return rc;
+        CKD(ctx, ust_k_add_f32_to_f64(ctx->wx, x, n, S(ctx)));
+        rc = lev_apply(ctx, 0, 0, x, ctx->vt);
+        if (rc) return rc;
+        CKD(ctx, ust_k_residual_f64(b, ctx->vt, ctx->va, n, S(ctx)));
+        CKD(ctx, dotv(ctx, ctx->va, ctx->va, n, SC_T1, &rr));
+        st->refinements++;
+        if (!(rr == rr)) UST_FAIL(ctx, UST_ENOTCONV, "iterative refinement produced NaN");
+    }
+    return UST_OK;
+}
+
+int ust_a11_solve_dev(ust_context *ctx, const ust_solve_opts *o, const double *b, double *x, ust_solve_stats *st) {
+    return a11_solve(ctx, o, b, x, st);
+}
+
+/* q = G^T K^-1 G d  (the Schur complement apply of MatSchurComplement) */
+static int schur_apply(ust_context *ctx, const ust_solve_opts *o, const double *d, double *q, ust_solve_stats *st) {
+    CKD(ctx, ust_k_apply_G(&ctx->grid, ctx->WG, ctx->mask_d, d, ctx->vb, S(ctx)));
+    int rc = a11_solve(ctx, o, ctx->vb, ctx->uprev, st);
+    if (rc) return rc;
+    CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, ctx->uprev, q, 0, S(ctx)));
+    return UST_OK;
+}
+
+static int remove_mean(ust_context *ctx, double *v) {
+    CKD(ctx, ust_k_sum_f64(v, (size_t)ctx->nel, ctx->partial, ctx->scal + SC_SUM, S(ctx)));
+    CKD(ctx, ust_k_remove_mean_f64(v, (size_t)ctx->nel, ctx->scal + SC_SUM, S(ctx)));
+    return UST_OK;
+}
+
+/* preconditioned CG on the Schur complement from the current p; returns its and the final ratio */
+static int schur_cg(ust_context *ctx, const ust_solve_opts *o, int ns, const double *pc, int stall_window,
+                    double hn, ust_solve_stats *st, int *stalled) {
+    size_t ne = (size_t)ctx->nel;
+    double rn2, hist[4096];
+    int rc;
+    *stalled = 0;
+    CKD(ctx, dotv(ctx, ctx->pr, ctx->pr, ne, SC_O_RR, &rn2));
+    for (int it = 0; it < o->max_outer; it++) {
+        double rn = sqrt(rn2);
+        if (it < 4096) hist[it] = rn;
+        if (o->verbose) fprintf(stderr, "[ust] scr it %d |r|/|h| = %.3e\n", st->outer_its, rn / hn);
+        if (rn <= o->outer_rtol * hn) break;
+        if (stall_window > 0 && it >= stall_window && it < 4096 && rn > 0.5 * hist[it - stall_window]) {
+            *stalled = 1;
+            break;
+        }
+        CKD(ctx, ust_k_div_f64(ctx->pr, pc, ctx->pz, ne, S(ctx)));
+        CKD(ctx, ust_k_dot_f64(ctx->pr, ctx->pz, ne, ctx->partial, ctx->scal + SC_O_RZ, S(ctx)));
+        CKD(ctx, ust_k_cg_update_p_f64(ctx->pz, ctx->pd, ctx->scal + SC_O_RZ, ctx->scal + SC_O_RZOLD, it == 0, ne,
+                                       S(ctx)));
+        CKD(ctx, ust_memcpy_d2d(ctx->scal + SC_O_RZOLD, ctx->scal + SC_O_RZ, sizeof(double), S(ctx)));
+        if ((rc = schur_apply(ctx, o, ctx->pd, ctx->pq, st))) return rc;
+        CKD(ctx, ust_k_dot_f64(ctx->pd, ctx->pq, ne, ctx->partial, ctx->scal + SC_O_PQ, S(ctx)));
+        CKD(ctx, ust_k_cg_update_xr_f64(ctx->p, ctx->pr, ctx->pd, ctx->pq, ctx->scal + SC_O_RZOLD, ctx->scal + SC_O_PQ,
+                                        ne, S(ctx)));
+        CKD(ctx, dotv(ctx, ctx->pr, ctx->pr, ne, SC_O_RR, &rn2));
+        st->outer_its++;
+        if (!(rn2 == rn2)) UST_FAIL(ctx, UST_ENOTCONV, "Schur CG produced NaN");
+    }
+    return UST_OK;
+}
+
+static int stokes_residual(ust_context *ctx, ust_solve_stats *st) {
+    size_t n = (size_t)ctx->ndof, ne = (size_t)ctx->nel;
+    double ff, hh, rm, rc2;
+    int rc;
+    /* momentum: f - K u - G p on free dofs (u here holds only the free part) */
+    if ((rc = lev_apply(ctx, 0, 0, ctx->uprev, ctx->vt))) return rc;
+    CKD(ctx, ust_k_apply_G(&ctx->grid, ctx->WG, ctx->mask_d, ctx->p, ctx->vb, S(ctx)));
+    CKD(ctx, ust_k_residual_f64(ctx->f, ctx->vt, ctx->va, n, S(ctx)));
+    CKD(ctx, ust_k_axpy_f64(-1.0, ctx->vb, ctx->va, n, S(ctx)));
+    CKD(ctx, dotv(ctx, ctx->va, ctx->va, n, SC_T0, &rm));
+    CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, ctx->uprev, ctx->pq, 0, S(ctx)));
+    CKD(ctx, ust_k_residual_f64(ctx->h, ctx->pq, ctx->pq, ne, S(ctx)));
+    CKD(ctx, dotv(ctx, ctx->pq, ctx->pq, ne, SC_T0, &rc2));
+    CKD(ctx, dotv(ctx, ctx->f, ctx->f, n, SC_T0, &ff));
+    CKD(ctx, dotv(ctx, ctx->h, ctx->h, ne, SC_T0, &hh));
+    double den = ff + hh > 0 ? ff + hh : 1.0;
+    st->momentum_residual = sqrt(rm / den);
+    st->continuity_residual = sqrt(rc2 / den);
+    st->stokes_residual = sqrt((rm + rc2) / den);
+    return UST_OK;
+}
+
+int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st) {
+    ust_solve_opts o;
+    if (opts) o = *opts;
+    else ust_solve_opts_default(&o);
+    memset(st, 0, sizeof(*st));
+    if (!ctx->assembled) UST_FAIL(ctx, UST_ESTATE, "ust_assemble must run before ust_solve");
+    int need32 = o.precision == UST_PREC_MIXED, rc;
+    size_t n = (size_t)ctx->ndof, ne = (size_t)ctx->nel;
+    double t0 = wall_ms();
+    if ((rc = ust_levels_setup(ctx, &o, need32))) return rc;
+    if ((rc = ensure_work(ctx, need32))) return rc;
+    CKD(ctx, ust_stream_sync(S(ctx)));
+    double t1 = wall_ms();
+    st->mg_levels = ctx->nlev;
+    int ns = o.remove_nullspace < 0 ? ctx->nullspace : o.remove_nullspace;
+    /* h_hat = G^T K^-1 f - h */
+    if ((rc = a11_solve(ctx, &o, ctx->f, ctx->uprev, st))) return rc;
+    CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, ctx->uprev, ctx->ph, 0, S(ctx)));
+    CKD(ctx, ust_k_axpy_f64(-1.0, ctx->h, ctx->ph, ne, S(ctx)));
+    if (ns && (rc = remove_mean(ctx, ctx->ph))) return rc;
+    double hh;
+    CKD(ctx, dotv(ctx, ctx->ph, ctx->ph, ne, SC_T0, &hh));
+    double hn = sqrt(hh) > 0 ? sqrt(hh) : 1.0;
+    CKD(ctx, ust_k_fill_f64(ctx->p, 0.0, ne, S(ctx)));
+    CKD(ctx, ust_memcpy_d2d(ctx->pr, ctx->ph, ne * sizeof(double), S(ctx)));
+    int stalled = 0;
+    const double *pc = ctx->approxS;
+    if (o.schur_pc == UST_SCHUR_LEARNED) {
+        if ((rc = ust_mlp_schur_pc(ctx, ctx->ppc))) return rc;
+        pc = ctx->ppc;
+        if ((rc = schur_cg(ctx, &o, ns, pc, o.stall_window, hn, st, &stalled))) return rc;
+        if (stalled) {
+            /* plain BSSCR takes over from the current iterate: r = h_hat - S p */
+            st->fallback_used = 1;
+            if ((rc = schur_apply(ctx, &o, ctx->p, ctx->pq, st))) return rc;
+            CKD(ctx, ust_k_sub_f64(ctx->ph, ctx->pq, ctx->pr, ne, S(ctx)));
+            if ((rc = schur_cg(ctx, &o, ns, ctx->approxS, 0, hn, st, &stalled))) return rc;
+        }
+    } else {
+        if ((rc = schur_cg(ctx, &o, ns, pc, 0, hn, st, &stalled))) return rc;
+    }
+    if (ns && (rc = remove_mean(ctx, ctx->p))) return rc;
+    /* back-solve u = K^-1 (f - G p) */
+    CKD(ctx, ust_k_apply_G(&ctx->grid, ctx->WG, ctx->mask_d, ctx->p, ctx->vb, S(ctx)));
+    CKD(ctx, ust_k_residual_f64(ctx->f, ctx->vb, ctx->vb, n, S(ctx)));
+    if ((rc = a11_solve(ctx, &o, ctx->vb, ctx->uprev, st))) return rc;
+    CKD(ctx, ust_stream_sync(S(ctx)));
+    double t2 = wall_ms();
+    if ((rc = stokes_residual(ctx, st))) return rc;
+    /* nodal solution with BC values (rheology input of the next assembly) */
+    CKD(ctx, ust_memcpy_d2d(ctx->u, ctx->uprev, n * sizeof(double), S(ctx)));
+    CKD(ctx, ust_k_axpy_f64(1.0, ctx->bcval_d, ctx->u, n, S(ctx)));
+    CKD(ctx, ust_stream_sync(S(ctx)));
+    st->t_setup_ms = t1 - t0;
+    st->t_solve_ms = t2 - t1;
+    return UST_OK;
+}
+
+int ust_velocity_solve(ust_context *ctx, const ust_solve_opts *opts, const double *b, double *u, int *its) {
+    ust_solve_opts o;
+    if (opts) o = *opts;
+    else ust_solve_opts_default(&o);
+    if (!ctx->assembled) UST_FAIL(ctx, UST_ESTATE, "ust_assemble must run first");
+    int need32 = o.precision == UST_PREC_MIXED, rc;
+    size_t n = (size_t)ctx->ndof;
+    if ((rc = ust_levels_setup(ctx, &o, need32))) return rc;
+    if ((rc = ensure_work(ctx, need32))) return rc;
+    ust_solve_stats st;
+    memset(&st, 0, sizeof(st));
+    CKD(ctx, ust_memcpy_h2d(ctx->vb, b, n * sizeof(double), S(ctx)));
+    /* the right-hand side lives on the free dofs */
+    CKD(ctx, ust_k_mask_zero_f64(ctx->vb, ctx->mask_d, n, S(ctx)));
+    if ((rc = a11_solve(ctx, &o, ctx->vb, ctx->uprev, &st))) return rc;
+    CKD(ctx, ust_memcpy_d2h(u, ctx->uprev, n * sizeof(double), S(ctx)));
+    if (its) *its = st.inner_its;
+    return UST_OK;
+}
+
+int ust_picard(ust_context *ctx, double gravity, double tol, int max_its, const ust_solve_opts *opts, int *its_done,
+               double *final_change, ust_solve_stats *last) {
+    size_t n = (size_t)ctx->ndof;
+    int rc;
+    double change = 1.0;
+    int it;
+    for (it = 0; it < max_its; it++) {
+        if (it > 0) CKD(ctx, ust_memcpy_d2d(ctx->va, ctx->u, n * sizeof(double), S(ctx)));
+        if ((rc = ust_assemble(ctx, gravity))) return rc;
+        if ((rc = ust_solve(ctx, opts, last))) return rc;
+        /* the first solve enables yielding (SystemLinearEquations.c:808-809) */
+        ctx->prev_solution = 1;
+        if (it > 0) {
+            double dd, uu;
+            CKD(ctx, ust_k_sub_f64(ctx->u, ctx->va, ctx->vt, n, S(ctx)));
+            CKD(ctx, dotv(ctx, ctx->vt, ctx->vt, n, SC_T0, &dd));
+            CKD(ctx, dotv(ctx, ctx->u, ctx->u, n, SC_T0, &uu));
+            change = uu > 0 ? sqrt(dd / uu) : 0.0; /* SystemLinearEquations.c:852-855 */
+            if (opts && opts->verbose) fprintf(stderr, "[ust] picard %d change %.3e\n", it, change);
+            if (change < tol) {
+                it++;
+                break;
+            }
+        }
+    }
+    if (its_done) *its_done = it;
+    if (final_change) *final_change = change;
+    return UST_OK;
+}

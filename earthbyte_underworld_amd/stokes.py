@@ -1,0 +1,403 @@
+"""Python host interface over the C ABI, named after the reference components it stands in for.
+
+    sys = StokesSystem(dim=3, elementRes=(32, 32, 32))          # CartesianGenerator
+    sys.set_materials([RheologyMaterial([MaterialViscosity(eta0=1.0)], density=0.0)])
+    sys.random_particle_layout(particlesPerCell=20, seed=13)     # RandomParticleLayout + ConstantWeights
+    sys.set_bcs(*free_slip_walls(sys))
+    sys.assemble(gravity=1.0)                                    # StiffnessMatrix_Assemble (K, G, approxS, f)
+    stats = sys.solve()                                          # BSSCR (KSPSolve_BSSCR)
+    u, p = sys.solution()
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class UstError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ---- Underworld/Rheology component types (XML Type names and keys) ----
+def _rh(type_name, **keys):
+    lib = L.load()
+    r = L.Rheology()
+    names = list(keys)
+    karr = (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+    varr = (C.c_double * max(1, len(names)))(*[float(keys[n]) for n in names])
+    rc = lib.ust_rheology_from_xml(type_name.encode(), len(names), C.cast(karr, C.c_void_p),
+                                   C.cast(varr, C.c_void_p), C.byref(r))
+    if rc != 0:
+        raise UstError(f"unknown rheology {type_name} or key in {names}")
+    return r
+
+
+def MaterialViscosity(**keys):
+    return _rh("MaterialViscosity", **keys)
+
+
+def Arrhenius(**keys):
+    return _rh("Arrhenius", **keys)
+
+
+def ViscosityLimiter(**keys):
+    return _rh("ViscosityLimiter", **keys)
+
+
+def VonMises(**keys):
+    return _rh("VonMises", **keys)
+
+
+def DruckerPrager(**keys):
+    return _rh("DruckerPrager", **keys)
+
+
+def RheologyMaterial(rheologies, density=0.0):
+    m = L.Material()
+    if not 1 <= len(rheologies) <= 4:
+        raise UstError("a RheologyMaterial holds 1..4 rheologies")
+    m.nrh = len(rheologies)
+    for i, r in enumerate(rheologies):
+        m.rh[i] = r
+    m.density = float(density)
+    return m
+
+
+def shape(kind, material, *params):
+    s = L.Shape()
+    s.type = kind
+    s.material = material
+    for i, v in enumerate(params):
+        s.p[i] = float(v)
+    return s
+
+
+def Everywhere(material):
+    return shape(L.SHAPE_EVERYWHERE, material)
+
+
+def BelowCosinePlane(material, offset, amplitude, wavelength, phase=0.0):
+    return shape(L.SHAPE_BELOWCOSINEPLANE, material, offset, amplitude, wavelength, phase)
+
+
+def Sphere(material, centre, radius):
+    c = list(centre) + [0.0] * (3 - len(centre))
+    return shape(L.SHAPE_SPHERE, material, c[0], c[1], c[2], radius)
+
+
+def Box(material, lo, hi):
+    lo = list(lo) + [0.0] * (3 - len(lo))
+    hi = list(hi) + [0.0] * (3 - len(hi))
+    return shape(L.SHAPE_BOX, material, *lo, *hi)
+
+
+def default_opts(**kw):
+    o = L.SolveOpts()
+    L.load().ust_solve_opts_default(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise UstError(f"unknown solver option {k}")
+        setattr(o, k, v)
+    return o
+
+
+class StokesSystem:
+    """One GPU context holding a Q1/P0 Cartesian mesh, its PIC swarm and the Stokes operators."""
+
+    def __init__(self, dim, elementRes, minCoord=None, maxCoord=None, device=0):
+        self.lib = L.load()
+        self.ctx = C.c_void_p()
+        rc = self.lib.ust_create(C.byref(self.ctx), device)
+        if rc != 0:
+            raise UstError(f"ust_create failed ({rc}): {self.lib.ust_dev_error_string().decode()}")
+        self.dim = dim
+        self.nel_axis = tuple(int(n) for n in elementRes[:dim])
+        minCoord = minCoord or (0.0,) * dim
+        maxCoord = maxCoord or (1.0,) * dim
+        self.min = tuple(float(x) for x in minCoord[:dim])
+        self.max = tuple(float(x) for x in maxCoord[:dim])
+        nel = (C.c_int * 3)(*(list(self.nel_axis) + [1] * (3 - dim)))
+        mn = (C.c_double * 3)(*(list(self.min) + [0.0] * (3 - dim)))
+        mx = (C.c_double * 3)(*(list(self.max) + [1.0] * (3 - dim)))
+        self._check(self.lib.ust_set_mesh(self.ctx, dim, nel, mn, mx))
+        self.nel = int(np.prod(self.nel_axis))
+        self.nnodes = int(np.prod([n + 1 for n in self.nel_axis]))
+        self.ndof = self.nnodes * dim
+        self.nd = (1 << dim) * dim
+
+    def _check(self, rc):
+        if rc != 0:
+            raise UstError(f"ust error {rc}: {self.lib.ust_error(self.ctx).decode()}")
+
+    def close(self):
+        if self.ctx:
+            self.lib.ust_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- geometry helpers ----
+    def node_coords(self):
+        axes = [np.linspace(self.min[d], self.max[d], self.nel_axis[d] + 1) for d in range(self.dim)]
+        axes = [self.min[d] + np.arange(self.nel_axis[d] + 1) * (self.max[d] - self.min[d]) / self.nel_axis[d]
+                for d in range(self.dim)]
+        grids = np.meshgrid(*axes, indexing="ij")
+        return np.stack([g.transpose().reshape(-1) for g in grids], axis=1)  # node id = i + nx*(j + ny*k)
+
+    def cell_centres(self):
+        axes = [self.min[d] + (np.arange(self.nel_axis[d]) + 0.5) * (self.max[d] - self.min[d]) / self.nel_axis[d]
+                for d in range(self.dim)]
+        grids = np.meshgrid(*axes, indexing="ij")
+        return np.stack([g.transpose().reshape(-1) for g in grids], axis=1)
+
+    # ---- inputs ----
+    def set_materials(self, materials):
+        arr = (L.Material * len(materials))(*materials)
+        self._check(self.lib.ust_set_materials(self.ctx, len(materials), arr))
+        self.materials = list(materials)
+
+    def upload_swarm(self, cell_off, xi, w, material=None, coh_scale=None):
+        cell_off = np.ascontiguousarray(cell_off, dtype=np.int32)
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        mat = None if material is None else np.ascontiguousarray(material, dtype=np.uint8)
+        coh = None if coh_scale is None else np.ascontiguousarray(coh_scale, dtype=np.float32)
+        self._check(self.lib.ust_swarm_upload(self.ctx, int(w.shape[0]), _ptr(cell_off), _ptr(xi), _ptr(w),
+                                              _ptr(mat), _ptr(coh)))
+
+    def random_particle_layout(self, particlesPerCell=20, seed=13, cell_counts=None):
+        if cell_counts is not None:
+            off = np.concatenate([[0], np.cumsum(np.asarray(cell_counts, dtype=np.int64))]).astype(np.int32)
+            self._check(self.lib.ust_swarm_random(self.ctx, _ptr(off), 0, seed))
+        else:
+            self._check(self.lib.ust_swarm_random(self.ctx, None, int(particlesPerCell), seed))
+
+    def assign_materials(self, shapes=(), cell_material=None):
+        arr = (L.Shape * max(1, len(shapes)))(*shapes)
+        cm = None if cell_material is None else np.ascontiguousarray(cell_material, dtype=np.uint8)
+        self._check(self.lib.ust_swarm_assign_materials(self.ctx, len(shapes), arr, _ptr(cm)))
+
+    def cohesion_noise(self, amplitude, seed):
+        self._check(self.lib.ust_swarm_cohesion_noise(self.ctx, float(amplitude), int(seed)))
+
+    def swarm(self):
+        n = self.lib.ust_swarm_count(self.ctx)
+        off = np.zeros(self.nel + 1, np.int32)
+        xi = np.zeros((n, 3))
+        w = np.zeros(n)
+        mat = np.zeros(n, np.uint8)
+        self._check(self.lib.ust_swarm_download(self.ctx, _ptr(off), _ptr(xi), _ptr(w), _ptr(mat)))
+        return off, xi, w, mat
+
+    def set_temperature(self, T):
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        assert T.shape[0] == self.nnodes
+        self._check(self.lib.ust_set_temperature(self.ctx, _ptr(T)))
+
+    def set_solution(self, u=None, p=None):
+        u = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+        p = None if p is None else np.ascontiguousarray(p, dtype=np.float64)
+        self._check(self.lib.ust_set_solution(self.ctx, _ptr(u), _ptr(p)))
+
+    def set_previous_solution_exists(self, flag=True):
+        self._check(self.lib.ust_set_previous_solution_exists(self.ctx, int(bool(flag))))
+
+    def set_bcs(self, mask, values=None):
+        mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        assert mask.shape[0] == self.ndof
+        vals = None if values is None else np.ascontiguousarray(values, dtype=np.float64)
+        self._check(self.lib.ust_set_bcs(self.ctx, _ptr(mask), _ptr(vals)))
+        self.bc_mask = mask
+        self.bc_values = np.zeros(self.ndof) if vals is None else vals
+
+    # ---- assembly and results ----
+    def assemble(self, gravity=1.0):
+        self._check(self.lib.ust_assemble(self.ctx, float(gravity)))
+
+    def element_matrices(self, first=0, count=None):
+        count = self.nel - first if count is None else count
+        Ke = np.zeros((count, self.nd, self.nd))
+        self._check(self.lib.ust_get_element_matrices(self.ctx, first, count, _ptr(Ke)))
+        return Ke
+
+    def element_G(self, first=0, count=None):
+        count = self.nel - first if count is None else count
+        Ge = np.zeros((count, self.nd))
+        self._check(self.lib.ust_get_element_G(self.ctx, first, count, _ptr(Ge)))
+        return Ge
+
+    def element_f(self, first=0, count=None):
+        count = self.nel - first if count is None else count
+        fe = np.zeros((count, self.nd))
+        self._check(self.lib.ust_get_element_f(self.ctx, first, count, _ptr(fe)))
+        return fe
+
+    def approxS(self):
+        a = np.zeros(self.nel)
+        self._check(self.lib.ust_get_approxS(self.ctx, _ptr(a)))
+        return a
+
+    def particle_viscosity(self):
+        n = self.lib.ust_swarm_count(self.ctx)
+        eta = np.zeros(n)
+        y = np.zeros(n, np.uint8)
+        self._check(self.lib.ust_get_particle_viscosity(self.ctx, _ptr(eta), _ptr(y)))
+        return eta, y
+
+    def weights(self):
+        nq, nv = 3 ** self.dim, 2 ** self.dim
+        WK = np.zeros((nq, self.nel))
+        WG = np.zeros((nv, self.nel))
+        WF = np.zeros((nv, self.nel))
+        self._check(self.lib.ust_get_weights(self.ctx, _ptr(WK), _ptr(WG), _ptr(WF)))
+        return WK, WG, WF
+
+    def rhs(self):
+        f = np.zeros(self.ndof)
+        h = np.zeros(self.nel)
+        self._check(self.lib.ust_get_rhs(self.ctx, _ptr(f), _ptr(h)))
+        return f, h
+
+    def apply_K(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        y = np.zeros(self.ndof)
+        self._check(self.lib.ust_apply_K(self.ctx, _ptr(u), _ptr(y)))
+        return y
+
+    def apply_G(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        y = np.zeros(self.ndof)
+        self._check(self.lib.ust_apply_G(self.ctx, _ptr(p), _ptr(y)))
+        return y
+
+    def apply_Gt(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        q = np.zeros(self.nel)
+        self._check(self.lib.ust_apply_Gt(self.ctx, _ptr(u), _ptr(q)))
+        return q
+
+    # ---- solves ----
+    def solve(self, opts=None, **kw):
+        o = opts if opts is not None else default_opts(**kw)
+        st = L.SolveStats()
+        self._check(self.lib.ust_solve(self.ctx, C.byref(o), C.byref(st)))
+        return st.as_dict()
+
+    def solution(self):
+        u = np.zeros(self.ndof)
+        p = np.zeros(self.nel)
+        self._check(self.lib.ust_get_solution(self.ctx, _ptr(u), _ptr(p)))
+        return u, p
+
+    def velocity_solve(self, b, opts=None, **kw):
+        o = opts if opts is not None else default_opts(**kw)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        u = np.zeros(self.ndof)
+        its = C.c_int()
+        self._check(self.lib.ust_velocity_solve(self.ctx, C.byref(o), _ptr(b), _ptr(u), C.byref(its)))
+        return u, its.value
+
+    def picard(self, gravity=1.0, tol=1e-4, max_its=30, opts=None, **kw):
+        o = opts if opts is not None else default_opts(**kw)
+        its = C.c_int()
+        change = C.c_double()
+        st = L.SolveStats()
+        self._check(self.lib.ust_picard(self.ctx, float(gravity), float(tol), int(max_its), C.byref(o),
+                                        C.byref(its), C.byref(change), C.byref(st)))
+        return its.value, change.value, st.as_dict()
+
+    # ---- learned Schur scaling ----
+    def mlp_param_counts(self, hidden=64):
+        nw, nb = C.c_int(), C.c_int()
+        self._check(self.lib.ust_mlp_param_counts(self.dim, hidden, C.byref(nw), C.byref(nb)))
+        return nw.value, nb.value
+
+    def mlp_init(self, seed=1, hidden=64):
+        self._check(self.lib.ust_mlp_init(self.ctx, hidden, int(seed)))
+
+    def mlp_set_params(self, w, b, hidden=64):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        self._check(self.lib.ust_mlp_set_params(self.ctx, hidden, _ptr(w), _ptr(b)))
+
+    def mlp_params(self):
+        nw, nb = self.mlp_param_counts()
+        w = np.zeros(nw, np.float32)
+        b = np.zeros(nb, np.float32)
+        self._check(self.lib.ust_mlp_get_params(self.ctx, _ptr(w), _ptr(b)))
+        return w, b
+
+    def mlp_features(self):
+        nin = 3 ** self.dim
+        f = np.zeros((self.nel, nin), np.float32)
+        self._check(self.lib.ust_mlp_features(self.ctx, _ptr(f)))
+        return f
+
+    def mlp_probe_target(self, opts=None, **kw):
+        o = opts if opts is not None else default_opts(**kw)
+        t = np.zeros(self.nel, np.float32)
+        self._check(self.lib.ust_mlp_probe_target(self.ctx, C.byref(o), 0, 0, _ptr(t)))
+        return t
+
+    def mlp_train(self, feat, target, epochs=20, batch=4096, lr=1e-3, seed=1):
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self.lib.ust_mlp_train(self.ctx, int(target.shape[0]), _ptr(feat), _ptr(target), int(epochs),
+                                           int(batch), float(lr), int(seed), C.byref(loss)))
+        return loss.value
+
+    def mlp_predict(self, feat):
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        out = np.zeros(feat.shape[0], np.float32)
+        self._check(self.lib.ust_mlp_predict(self.ctx, int(feat.shape[0]), _ptr(feat), _ptr(out)))
+        return out
+
+    def mlp_gradients(self, feat, target):
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        nw, nb = self.mlp_param_counts()
+        gw = np.zeros(nw, np.float32)
+        gb = np.zeros(nb, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self.lib.ust_mlp_gradients(self.ctx, int(target.shape[0]), _ptr(feat), _ptr(target), _ptr(gw),
+                                               _ptr(gb), _ptr(loss)))
+        return gw, gb, float(loss[0])
+
+
+# ---- boundary-condition helpers (VariableCondition XMLs) ----
+def wall_mask(sys, walls):
+    """walls: dict {(axis, side): components}, side 0 = min wall, 1 = max wall."""
+    dim = sys.dim
+    n = [a + 1 for a in sys.nel_axis]
+    mask = np.zeros((sys.nnodes, dim), np.uint8)
+    idx = np.arange(sys.nnodes)
+    ijk = [idx % n[0], (idx // n[0]) % n[1]] + ([idx // (n[0] * n[1])] if dim == 3 else [])
+    for (axis, side), comps in walls.items():
+        on = ijk[axis] == (0 if side == 0 else n[axis] - 1)
+        for c in comps:
+            mask[on, c] = 1
+    return mask.reshape(-1)
+
+
+def free_slip_walls(sys):
+    """velocityBCs.freeSlip: the normal component is fixed (zero) on every wall."""
+    walls = {(a, s): [a] for a in range(sys.dim) for s in (0, 1)}
+    return wall_mask(sys, walls), None
+
+
+def freeslip_sides_noslip_topbottom(sys):
+    """velocityBCs.freeslipSides.xml: no-slip bottom/top, free-slip on the other walls."""
+    walls = {(a, s): [a] for a in range(sys.dim) for s in (0, 1) if a != 1}
+    walls[(1, 0)] = list(range(sys.dim))
+    walls[(1, 1)] = list(range(sys.dim))
+    return wall_mask(sys, walls), None

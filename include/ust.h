@@ -1,0 +1,179 @@
+/*
+ * ust.h -- C host API of the MI355X-native PIC Stokes hot path (Underworld 1.x drop-in).
+ *
+ * The drop-in boundary.  Every entry point is extern "C", takes plain pointers and sizes and
+ * returns an int status (UST_OK = 0, negative on error; ust_error() gives the text).  One
+ * context drives one GPU through one HIP stream; a context is not re-entrant (like the
+ * reference's StiffnessMatrixTerm objects, StiffnessMatrixTerm.c:247-258, which keep scratch
+ * on self).  Host pointers unless the name ends in _dev (device pointers).
+ *
+ * Which reference interface each entry replaces is stated on the declaration.
+ */
+#ifndef UST_H
+#define UST_H
+#include "ust_kernels.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ust_context ust_context;
+
+#define UST_PREC_FP64 0   /* plain: FP64 operators everywhere */
+#define UST_PREC_MIXED 1  /* FP32 inner solves + FP64 iterative refinement, outer Krylov and coarse solve */
+
+#define UST_SCHUR_UW 0       /* approxS = g^T diag(K)^-1 g per element ("-Q22_pc_type uw") */
+#define UST_SCHUR_LEARNED 1  /* approxS scaled by the MLP; plain "uw" remains the fallback */
+
+typedef struct {
+    int precision;           /* UST_PREC_* */
+    int schur_pc;            /* UST_SCHUR_* */
+    int mg_levels;           /* 0: as many as the grid allows */
+    int smoother_degree;     /* Chebyshev degree of each pre/post smoothing (default 2) */
+    double cheby_ratio;      /* lambda_max / lambda_min of the Chebyshev interval (default 12) */
+    double outer_rtol;       /* "-scr_ksp_rtol" */
+    double inner_rtol;       /* "-A11_ksp_rtol" */
+    double mixed_inner_rtol; /* FP32 inner PCG tolerance per refinement step */
+    int max_outer;           /* "-scr_ksp_max_it" */
+    int max_inner;           /* "-A11_ksp_max_it" */
+    int remove_nullspace;    /* -1: when every wall fixes its normal velocity */
+    int stall_window;        /* learned PC: fall back to plain if the residual does not halve in this many its */
+    int verbose;
+} ust_solve_opts;
+
+typedef struct {
+    int outer_its;
+    int inner_its;           /* total inner (velocity) Krylov iterations */
+    int inner_solves;
+    int refinements;         /* FP64 iterative-refinement steps (mixed) */
+    int fallback_used;       /* learned PC stalled and plain BSSCR finished the solve */
+    int mg_levels;
+    double stokes_residual;  /* ||[f-Ku-Gp; h-G^T u]|| / ||[f;h]|| in FP64 */
+    double momentum_residual, continuity_residual;
+    double t_assemble_ms, t_setup_ms, t_solve_ms;
+} ust_solve_stats;
+
+void ust_solve_opts_default(ust_solve_opts *o);
+
+/* ---- lifecycle ---- */
+int ust_create(ust_context **ctx, int device);
+void ust_destroy(ust_context *ctx);
+const char *ust_error(const ust_context *ctx);
+void *ust_get_stream(ust_context *ctx);
+
+/* ---- mesh: CartesianGenerator "elementResI/J/K", "minX..maxZ" (CartesianGenerator.c:976-1050) ---- */
+int ust_set_mesh(ust_context *ctx, int dim, const int nel[3], const double xmin[3], const double xmax[3]);
+
+/* ---- materials: RheologyMaterial "Rheology" list, in registration order (RheologyMaterial.c:240-270) ---- */
+int ust_set_materials(ust_context *ctx, int nmat, const ust_material *mats);
+/* XML component Type + keys -> rheology (Arrhenius.c:110-133, MaterialViscosity.c:115,
+ * ViscosityLimiter.c:101-103, VonMises.c:157-159, YieldRheology.c:145, DruckerPrager.c:173-189) */
+int ust_rheology_from_xml(const char *type_name, int nkeys, const char *const *keys, const double *values,
+                          ust_rheology *out);
+
+/* ---- swarm: IntegrationPointsSwarm cell tables (cellParticleTbl/cellParticleCountTbl) ----
+ * cell_off (nel+1), xi (np*3, local coords), w (np), material (np, may be NULL = 0),
+ * coh_scale (np, may be NULL = 1): per-particle cohesion factor. */
+int ust_swarm_upload(ust_context *ctx, int np, const int *cell_off, const double *xi, const double *w,
+                     const unsigned char *material, const float *coh_scale);
+/* Same from device SoA arrays (copied). */
+int ust_swarm_upload_dev(ust_context *ctx, int np, const int *cell_off_dev, const double *xi0_dev,
+                         const double *xi1_dev, const double *xi2_dev, const double *w_dev,
+                         const unsigned char *material_dev, const float *coh_scale_dev, int max_ppc);
+
+/* Swarm generated on the device: RandomParticleLayout + coincident mapping + ConstantWeights.
+ * cell_off (host, nel+1) gives the particle count of every cell (NULL: ppc in every cell). */
+int ust_swarm_random(ust_context *ctx, const int *cell_off, int ppc, long seed);
+/* material index per particle from cell_material (host, nel; may be NULL) and shapes in order */
+int ust_swarm_assign_materials(ust_context *ctx, int nshape, const ust_shape *shapes,
+                               const unsigned char *cell_material);
+/* per-particle cohesion factor 1 + amplitude*u, u uniform in (-1,1) (seeded) */
+int ust_swarm_cohesion_noise(ust_context *ctx, float amplitude, unsigned long long seed);
+/* copy the swarm back (any pointer may be NULL): cell_off (nel+1), xi (np*3), w (np), material (np) */
+int ust_swarm_download(ust_context *ctx, int *cell_off, double *xi, double *w, unsigned char *material);
+int ust_swarm_count(ust_context *ctx);
+
+/* ---- fields read by the rheologies (PpcManager lookups, PpcManager.c:503-606) ---- */
+int ust_set_temperature(ust_context *ctx, const double *T_nodal);
+/* velocity (nodal, n_nodes*dim, BC values included) and pressure (per element) used by yield */
+int ust_set_solution(ust_context *ctx, const double *u_nodal, const double *p);
+int ust_set_previous_solution_exists(ust_context *ctx, int flag);
+
+/* ---- boundary conditions: mask (n_nodes*dim, 1 = Dirichlet) and values ---- */
+int ust_set_bcs(ust_context *ctx, const unsigned char *mask, const double *values);
+
+/* ---- assembly (replaces the K, G, preconditioner and force assembly of the Stokes SLE:
+ * __StiffnessMatrix_NewAssemble StiffnessMatrix.c:825-1017 with ConstitutiveMatrixCartesian,
+ * GradientStiffnessMatrixTerm, UzawaPreconditionerTerm, ForceVector_Assemble + BuoyancyForceTerm) ---- */
+int ust_assemble(ust_context *ctx, double gravity);
+
+/* batched element results (drop-in for the per-element elStiffMat callbacks) */
+int ust_get_element_matrices(ust_context *ctx, int first, int count, double *Ke); /* count*nd*nd */
+int ust_get_element_matrices_dev(ust_context *ctx, int first, int count, double *Ke_dev);
+int ust_get_element_G(ust_context *ctx, int first, int count, double *Ge);         /* count*nd */
+int ust_get_element_f(ust_context *ctx, int first, int count, double *fe);         /* count*nd */
+int ust_get_approxS(ust_context *ctx, double *approxS);                            /* nel */
+int ust_get_particle_viscosity(ust_context *ctx, double *eta, unsigned char *yielded);
+int ust_get_weights(ust_context *ctx, double *WK, double *WG, double *WF);         /* SoA (nq|nv)*nel */
+/* right-hand sides after BC lifting: f (nodal, zero on BC dofs), h (per element) */
+int ust_get_rhs(ust_context *ctx, double *f_nodal, double *h);
+
+/* ---- operators (MatMult inside MatSchurComplement / PCMG), nodal vectors, BC dofs = 0 ---- */
+int ust_apply_K(ust_context *ctx, const double *u, double *y);
+int ust_apply_K_dev(ust_context *ctx, const double *u_dev, double *y_dev);
+int ust_apply_K_f32_dev(ust_context *ctx, const float *u_dev, float *y_dev);
+int ust_apply_G(ust_context *ctx, const double *p, double *y);
+int ust_apply_Gt(ust_context *ctx, const double *u, double *q);
+
+/* ---- BSSCR solve (KSPSolve_BSSCR BSSCR.c:151-229 / BSSCR_DRIVER_auglag auglag-driver-DGTGD.c:53-441) ---- */
+int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *stats);
+int ust_get_solution(ust_context *ctx, double *u_nodal, double *p);
+int ust_get_solution_dev(ust_context *ctx, const double **u_nodal_dev, const double **p_dev);
+/* K u = b velocity solve alone (one A11 solve) on free dofs; b, u nodal */
+int ust_velocity_solve(ust_context *ctx, const ust_solve_opts *opts, const double *b, double *u, int *its);
+
+/* ---- nonlinear Picard loop (SystemLinearEquations_NonLinearExecute, SystemLinearEquations.c:750-934) ---- */
+int ust_picard(ust_context *ctx, double gravity, double tol, int max_its, const ust_solve_opts *opts,
+               int *its_done, double *final_change, ust_solve_stats *last_stats);
+
+/* ---- learned Schur scaling (MLP 27 -> H -> H -> 1, BF16 weights, FP32 accumulation) ---- */
+/* feature = log10 of element viscosity (mean over the element's particles, from the K weights)
+ * over the 3^dim neighbourhood of each pressure cell */
+int ust_mlp_init(ust_context *ctx, int hidden, unsigned long long seed);
+int ust_mlp_set_params(ust_context *ctx, int hidden, const float *w, const float *b);
+int ust_mlp_get_params(ust_context *ctx, float *w, float *b);
+int ust_mlp_param_counts(int dim, int hidden, int *nw, int *nb);
+/* features of the current assembly (nel * 3^dim) */
+int ust_mlp_features(ust_context *ctx, float *feat);
+/* target for training: log10 of the ratio between a probed Schur diagonal and approxS */
+int ust_mlp_probe_target(ust_context *ctx, const ust_solve_opts *opts, int nprobe, unsigned long long seed,
+                         float *target);
+/* n samples (host arrays) -> epochs of Adam on the GPU; returns the final loss */
+int ust_mlp_train(ust_context *ctx, int n, const float *feat, const float *target, int epochs, int batch,
+                  float lr, unsigned long long seed, float *final_loss);
+int ust_mlp_predict(ust_context *ctx, int n, const float *feat, float *out);
+/* gradients of the loss for one batch (for checking against a CPU reference) */
+int ust_mlp_gradients(ust_context *ctx, int n, const float *feat, const float *target, float *grad_w,
+                      float *grad_b, float *loss);
+
+/* ---- reference plugin API mirror (StiffnessMatrixTerm_AssembleElementFunction,
+ * StiffnessMatrixTerm.h:52-58): bind a term object to a context, then the callbacks fill
+ * elStiffMat (+=) from the batched assembly ---- */
+int ust_stg_bind_term(void *term, ust_context *ctx, int kind);
+#define UST_TERM_CONSTITUTIVE 1   /* ConstitutiveMatrixCartesian */
+#define UST_TERM_GRADIENT 2       /* GradientStiffnessMatrixTerm */
+#define UST_TERM_UZAWA_PC 3       /* UzawaPreconditionerTerm */
+#define UST_TERM_BUOYANCY 4       /* BuoyancyForceTerm */
+void ust_ConstitutiveMatrixCartesian_AssembleElement(void *constitutiveMatrix, void *stiffnessMatrix,
+                                                     int lElement_I, void *sle, void *context,
+                                                     double **elStiffMat);
+void ust_GradientStiffnessMatrixTerm_AssembleElement(void *matrixTerm, void *stiffnessMatrix, int lElement_I,
+                                                     void *sle, void *context, double **elStiffMat);
+void ust_UzawaPreconditionerTerm_AssembleElement(void *matrixTerm, void *stiffnessMatrix, int lElement_I,
+                                                 void *sle, void *context, double **elPreconditioner);
+void ust_BuoyancyForceTerm_AssembleElement(void *forceTerm, void *forceVector, int lElement_I,
+                                           double *elForceVec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,247 @@
+/*
+ * ust_kernels.h -- thin C ABI over the gfx950 HIP kernels of the PIC Stokes hot path.
+ *
+ * Plain device pointers and sizes, one HIP stream per call (passed as void*).  Every
+ * launcher returns 0 on success or a negative UST_E* code; it never allocates.
+ * The C host layer (ust.h) is built on these entry points only.
+ *
+ * Quadrature-weight representation (the MI355X-first form of the PIC element integrals):
+ * on a rectilinear Q1 element with isotropic D, every integrand that
+ * ConstitutiveMatrixCartesian_AssembleElement (ConstitutiveMatrixCartesian.c:277-410)
+ * sums over the particles is a polynomial of degree <= 2 per reference axis.  The particle
+ * sum is therefore reproduced exactly by the 3^dim Lagrange points xi in {-1,0,1}^dim with
+ * weights  WK[q] = sum_p w_p detJ eta_p L_q(xi_p),  and
+ *   K_e = sum_q WK[q] B(xi_q)^T D0 B(xi_q),    D0 = diag(2,..,2,1,..,1).
+ * G_e (GradientStiffnessMatrixTerm.c:220-249) and f_e (BuoyancyForceTerm.c:435-448) are
+ * multilinear: WG[b] = sum_p w_p detJ N_b(xi_p), WF[b] = sum_p w_p detJ rho_p N_b(xi_p).
+ * Weight arrays are SoA: W[q*nel + e].
+ */
+#ifndef UST_KERNELS_H
+#define UST_KERNELS_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UST_OK 0
+#define UST_EINVAL (-1)
+#define UST_EDEVICE (-2)
+#define UST_ENOMEM (-3)
+#define UST_ESTATE (-4)
+#define UST_ENOTCONV (-5)
+
+/* Rheology kinds (Underworld/Rheology component Type names); parameters as in ust.h. */
+#define UST_RH_NONE 0
+#define UST_RH_MATERIALVISCOSITY 1
+#define UST_RH_ARRHENIUS 2
+#define UST_RH_VISCOSITYLIMITER 3
+#define UST_RH_VONMISES 4
+#define UST_RH_DRUCKERPRAGER 5
+#define UST_MAX_RHEOLOGIES 4
+#define UST_MAX_MATERIALS 64
+
+typedef struct {
+    int type;
+    double p[6];
+} ust_rheology;
+
+typedef struct {
+    int nrh;
+    ust_rheology rh[UST_MAX_RHEOLOGIES];
+    double density;
+} ust_material;
+
+/* Geometry of one (multigrid) level of the Cartesian mesh. */
+typedef struct {
+    int dim;
+    int nel[3];
+    double h[3];      /* element size per axis */
+    double xmin[3];
+    double ymax;      /* top of the box (Arrhenius depth) */
+    double xmax[3];   /* box extent; vertices are xmin + i*(xmax-xmin)/nel as CartesianGenerator writes them */
+} ust_grid;
+
+/* Material shapes evaluated per particle (StgDomain/Shape role): later shapes win. */
+#define UST_SHAPE_EVERYWHERE 0
+#define UST_SHAPE_BELOWCOSINEPLANE 1 /* p: offset, amplitude, wavelength, phase (BelowCosinePlane.c:195-210) */
+#define UST_SHAPE_SPHERE 2           /* p: cx, cy, cz, radius */
+#define UST_SHAPE_BOX 3              /* p: minx, miny, minz, maxx, maxy, maxz */
+typedef struct {
+    int type;
+    int material;
+    double p[6];
+} ust_shape;
+
+/* --- swarm set-up --- */
+/* RandomParticleLayout (RandomParticleLayout.c:195-213, Random.c:56-73): particle p of the cell-sorted
+ * swarm takes MINSTD draws 3p+1..3p+3 of the stream seeded with seed^123459876 (jump-ahead, so every
+ * particle is independent), xi by the Q1 inverse map, w = 2^dim / n_cell (ConstantWeights.c:180-189). */
+int ust_k_random_layout(const ust_grid *g, const int *cell_off, int np, long seed, double *xi0, double *xi1,
+                        double *xi2, double *w, void *stream);
+/* material of each particle: cell_material[e] (or 0) overridden by each containing shape in order */
+int ust_k_assign_materials(const ust_grid *g, const int *cell_off, int np, const double *xi0, const double *xi1,
+                           const double *xi2, const ust_shape *shapes_dev, int nshape,
+                           const unsigned char *cell_material, unsigned char *mat, void *stream);
+/* coh[p] = 1 + amp * u, u uniform in (-1, 1) from a counter hash of (seed, p) */
+int ust_k_noise_scale(float *coh, int np, float amp, unsigned long long seed, void *stream);
+
+/* --- memory / stream plumbing (HIP runtime behind a C ABI) --- */
+int ust_dev_set(int device);
+int ust_dev_malloc(void **p, size_t bytes);
+int ust_dev_free(void *p);
+int ust_host_malloc_pinned(void **p, size_t bytes);
+int ust_host_free_pinned(void *p);
+int ust_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int ust_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int ust_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int ust_stream_create(void **stream);
+int ust_stream_destroy(void *stream);
+int ust_stream_sync(void *stream);
+int ust_event_create(void **ev);
+int ust_event_destroy(void *ev);
+int ust_event_record(void *ev, void *stream);
+int ust_event_elapsed_ms(void *ev_start, void *ev_stop, float *ms);
+const char *ust_dev_error_string(void);
+
+/* --- assembly (particle -> quadrature weights, fused per-particle rheology) --- */
+/* eta_out/yield_out (np) may be NULL.  temperature (nodal), velocity (nodal, interleaved dim),
+ * pressure (per element), coh_scale (per particle) may be NULL when no rheology needs them.
+ * WK (nq*nel), WG (nv*nel), WF (nv*nel): nq = 3^dim, nv = 2^dim. */
+int ust_k_particle_weights(const ust_grid *g, const int *cell_off, int max_ppc, const double *xi0,
+                           const double *xi1, const double *xi2, const double *w,
+                           const unsigned char *mat, const float *coh_scale,
+                           const ust_material *materials_dev, int nmat, const double *temperature,
+                           const double *velocity, const double *pressure, int prev_solution,
+                           double gravity, double *eta_out, unsigned char *yield_out, double *WK,
+                           double *WG, double *WF, void *stream);
+
+/* K_e for elements [first, first+count): Ke (count * nd * nd) row-major, FP64 MFMA. */
+int ust_k_element_matrices(const ust_grid *g, const double *WK, int first, int count, double *Ke,
+                           void *stream);
+/* G_e (count*nd) and f_e (count*nd) from the vertex weights */
+int ust_k_element_G(const ust_grid *g, const double *WG, int first, int count, double *Ge, void *stream);
+int ust_k_element_f(const ust_grid *g, const double *WF, double gravity, int first, int count,
+                    double *fe, void *stream);
+/* approxS_e = sum_v G_e[v]^2 / K_e[v][v] (UzawaPreconditionerTerm.c:200-212) */
+int ust_k_uzawa_diag(const ust_grid *g, const double *WK, const double *WG, double *approxS, void *stream);
+
+/* --- operators.  mask: per dof (n_nodes*dim), 1 = Dirichlet.  y is overwritten. --- */
+/* y = K u on the free dofs (input and output BC dofs treated as zero) */
+int ust_k_apply_K_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
+                      double *y, void *stream);
+int ust_k_apply_K_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
+                      float *y, void *stream);
+/* y = -K_{free,bc} u_bc  (BC lifting: input restricted to BC dofs, output to free dofs), accumulated */
+int ust_k_lift_K(const ust_grid *g, const double *WK, const unsigned char *mask, const double *ubc,
+                 double *y, void *stream);
+/* y = G p (velocity dofs, free only); ust_k_apply_Gt mode (bc_only): 0 q = G^T u on the free dofs,
+ * 1 q -= G^T u_bc over the BC dofs (continuity lifting), 2 q_e = sum over free dofs of |G_e| (row
+ * sums of |D| for the infinity norm) */
+int ust_k_apply_G(const ust_grid *g, const double *WG, const unsigned char *mask, const double *p,
+                  double *y, void *stream);
+int ust_k_apply_Gt(const ust_grid *g, const double *WG, const unsigned char *mask, const double *u,
+                   double *q, int bc_only, void *stream);
+/* nodal force vector f (free dofs) from WF; accumulated into f */
+int ust_k_force(const ust_grid *g, const double *WF, double gravity, const unsigned char *mask, double *f,
+                void *stream);
+/* inverse diagonal of K on free dofs (1 on BC dofs) */
+int ust_k_inv_diag(const ust_grid *g, const double *WK, const unsigned char *mask, double *invd64,
+                   float *invd32, void *stream);
+
+/* --- multigrid transfers (fine grid gf, coarse has nel/2) --- */
+int ust_k_coarsen_weights(const ust_grid *gf, const double *WKf, double *WKc, void *stream);
+int ust_k_restrict_f64(const ust_grid *gf, const unsigned char *maskf, const unsigned char *maskc,
+                       const double *rf, double *bc, void *stream);
+int ust_k_prolong_add_f64(const ust_grid *gf, const unsigned char *maskf, const double *xc, double *xf,
+                          void *stream);
+int ust_k_restrict_f32(const ust_grid *gf, const unsigned char *maskf, const unsigned char *maskc,
+                       const float *rf, float *bc, void *stream);
+int ust_k_prolong_add_f32(const ust_grid *gf, const unsigned char *maskf, const float *xc, float *xf,
+                          void *stream);
+
+/* --- dense coarse-grid solve (FP64) --- */
+/* A (n x n, n = n_nodes*dim) from the weights; BC dofs get identity rows/columns */
+int ust_k_dense_assemble(const ust_grid *g, const double *WK, const unsigned char *mask, double *A,
+                         void *stream);
+/* in-place inverse of an SPD matrix (one workgroup; n <= 4096) */
+int ust_k_dense_spd_inverse(double *A, int n, void *stream);
+/* y = Ainv x in FP64; the f32 variant converts at the boundary */
+int ust_k_dense_matvec_f64(const double *Ainv, int n, const double *x, double *y, void *stream);
+int ust_k_dense_matvec_f32(const double *Ainv, int n, const float *x, float *y, void *stream);
+
+/* --- vector kernels (scalars that the device computes live in device memory) --- */
+int ust_k_fill_f64(double *x, double v, size_t n, void *stream);
+int ust_k_fill_f32(float *x, float v, size_t n, void *stream);
+/* deterministic pseudo-random fill in (-1, 1) (power-iteration start vectors, probes) */
+int ust_k_fill_hash_f64(double *x, size_t n, unsigned long long seed, void *stream);
+/* Rademacher (+-1) fill */
+int ust_k_fill_sign_f64(double *x, size_t n, unsigned long long seed, void *stream);
+int ust_k_copy_f64_to_f32(const double *x, float *y, size_t n, void *stream);
+int ust_k_copy_f32_to_f64(const float *x, double *y, size_t n, void *stream);
+int ust_k_axpy_f64(double a, const double *x, double *y, size_t n, void *stream);     /* y += a x */
+int ust_k_axpby_f64(double a, const double *x, double b, double *y, size_t n, void *stream);
+int ust_k_add_f32_to_f64(const float *x, double *y, size_t n, void *stream);        /* y += x */
+int ust_k_sub_f64(const double *a, const double *b, double *y, size_t n, void *stream); /* y = a - b */
+int ust_k_scale_mul_f64(const double *a, const double *b, double *y, size_t n, void *stream); /* y = a .* b */
+int ust_k_div_f64(const double *a, const double *b, double *y, size_t n, void *stream); /* y = a ./ b */
+/* deterministic two-stage dot products: out[0] = x.y ; partial must hold 1024 doubles */
+int ust_k_dot_f64(const double *x, const double *y, size_t n, double *partial, double *out, void *stream);
+int ust_k_dot_f32(const float *x, const float *y, size_t n, double *partial, double *out, void *stream);
+/* max |x_i| -> out[0] */
+int ust_k_absmax_f64(const double *x, size_t n, double *partial, double *out, void *stream);
+/* sum(x) -> out[0] */
+int ust_k_sum_f64(const double *x, size_t n, double *partial, double *out, void *stream);
+/* x -= s[0]/n  (constant-pressure null space projection) */
+/* x[i] = 0 where mask[i] (restrict a nodal vector to the free dofs) */
+int ust_k_mask_zero_f64(double *x, const unsigned char *mask, size_t n, void *stream);
+int ust_k_remove_mean_f64(double *x, size_t n, const double *sum, void *stream);
+
+/* CG building blocks with device scalars: sc = {rz, pq, rz_old, ...} */
+/* p = z + (rz_new/rz_old) p  (first: p = z) */
+int ust_k_cg_update_p_f64(const double *z, double *p, const double *rz_new, const double *rz_old, int first,
+                          size_t n, void *stream);
+int ust_k_cg_update_p_f32(const float *z, float *p, const double *rz_new, const double *rz_old, int first,
+                          size_t n, void *stream);
+/* x += a p ; r -= a q with a = num/den */
+int ust_k_cg_update_xr_f64(double *x, double *r, const double *p, const double *q, const double *num,
+                           const double *den, size_t n, void *stream);
+int ust_k_cg_update_xr_f32(float *x, float *r, const float *p, const float *q, const double *num,
+                           const double *den, size_t n, void *stream);
+
+/* Chebyshev smoother step (Jacobi-preconditioned).  r = b - y (y = A x given), then
+ * d = c1*d + c2*invd.*r ; x += d  (c1 = 0 on the first step) */
+int ust_k_cheby_step_f64(const double *b, const double *Ax, const double *invd, double *d, double *x,
+                         double c1, double c2, size_t n, void *stream);
+int ust_k_cheby_step_f32(const float *b, const float *Ax, const float *invd, float *d, float *x, float c1,
+                         float c2, size_t n, void *stream);
+/* r = b - Ax */
+int ust_k_residual_f64(const double *b, const double *Ax, double *r, size_t n, void *stream);
+int ust_k_residual_f32(const float *b, const float *Ax, float *r, size_t n, void *stream);
+
+/* --- learned Schur-preconditioner scaling (MLP 27 -> H -> H -> 1, BF16 weights, FP32 accumulation) --- */
+/* feature: log10 of the element viscosity average over the 3^dim neighbourhood (clamped at the box) */
+int ust_k_mlp_features(const ust_grid *g, const double *WK, float *feat, void *stream);
+/* out[e] = MLP(feat[e]) ; weights packed (see ust_mlp_layout) */
+int ust_k_mlp_forward(const float *feat, int n, int nin, int hid, const unsigned short *wbf16, const float *bias,
+                      float *out, float *act1, float *act2, void *stream);
+/* gradients of 0.5*sum((out-target)^2)/n w.r.t. weights and biases (FP32, deterministic block
+ * partials; act1 is a workspace of >= 256*(nw+nb) floats, act2 of >= 256 floats) */
+int ust_k_mlp_backward(const float *feat, const float *target, int n, int nin, int hid,
+                       const unsigned short *wbf16, const float *bias, const float *act1, const float *act2,
+                       const float *out, float *grad_w, float *grad_b, float *loss, void *stream);
+int ust_k_mlp_adam(float *master_w, unsigned short *wbf16, float *bias, const float *grad_w, const float *grad_b,
+                   float *m_w, float *v_w, float *m_b, float *v_b, int nw, int nb, float lr, float beta1,
+                   float beta2, float eps, int step, void *stream);
+/* pc[e] = approxS[e] * 10^clip(out[e], -4, 4) */
+int ust_k_apply_scaling(const double *approxS, const float *out, double *pc, int n, void *stream);
+int ust_k_mlp_to_bf16(const float *x, unsigned short *y, int n, void *stream);
+/* probing of the Schur diagonal with a 3^dim colouring of the pressure cells */
+int ust_k_color_indicator(const ust_grid *g, int color, double *v, void *stream);
+int ust_k_color_gather(const ust_grid *g, int color, const double *q, double *d, void *stream);
+/* t[e] = log10(d[e] / approxS[e]) */
+int ust_k_log_ratio(const double *d, const double *approxS, float *t, int n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

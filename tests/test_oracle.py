@@ -1,0 +1,239 @@
+"""The FP64 CPU oracle pinned against the reference's own fixtures, known answers and closed forms."""
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+G2 = 1.0 / np.sqrt(3.0)
+
+
+def _gauss_swarm(O, mesh, nel):
+    """2x2 Gauss points (weights 1) per element: the GaussSwarm of testStiffnessMatrixAssembly.xml."""
+    gp = [(-G2, -G2), (G2, -G2), (-G2, G2), (G2, G2)]
+    xi = np.zeros((nel * 4, 3))
+    for e in range(nel):
+        for k, (a, b) in enumerate(gp):
+            xi[4 * e + k, :2] = (a, b)
+    return np.arange(0, nel * 4 + 1, 4).astype(np.int32), xi, np.ones(nel * 4)
+
+
+def _sinusoid_T(n):
+    """LinearWithSinusoidalPerturbation on an n x n element unit square (temperatureICs.sinusoid.xml)."""
+    i, j = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="xy")
+    x, y = i.reshape(-1) / n, j.reshape(-1) / n
+    return 0.0 + 1.0 * (1.0 - y) + 0.1 * np.cos(np.pi * x) * np.sin(np.pi * y)
+
+
+def _fixture_err(O, K, name):
+    E = O.read_petsc_matrix(os.path.join(GOLDEN, name)).toarray()
+    return np.linalg.norm(K.toarray() - E) / np.linalg.norm(E)
+
+
+def test_material_viscosity_fixture(oracle):
+    """ViscousSuite: testMaterialViscosity2D.matrix (eta0 = 1, 10x10, 242x242), reference tol 1e-4."""
+    O = oracle
+    m = O.make_mesh(2, (10, 10))
+    off, xi, w = _gauss_swarm(O, m, 100)
+    Ke, *_ = O.assemble_elements(m, off, xi, w, np.ones(400))
+    assert _fixture_err(O, O.assemble_K_full(m, Ke), "testMaterialViscosity2D.matrix") < 1e-13
+
+
+@pytest.mark.parametrize("limiter,name", [(False, "testArrhenius2D.matrix"), (True, "testArrheniusWithLimiter.matrix")])
+def test_arrhenius_fixtures(oracle, limiter, name):
+    O = oracle
+    m = O.make_mesh(2, (10, 10))
+    off, xi, w = _gauss_swarm(O, m, 100)
+    rh = [(O.RH_ARRHENIUS, (1e-6, 27.63102112, 0.0, 1.0))]
+    if limiter:
+        rh.append((O.RH_VISCOSITYLIMITER, (10e4, 10.0)))
+    eta, _ = O.eval_viscosity(m, [O.make_material(rh)], off, xi, np.zeros(400, np.int32), temperature=_sinusoid_T(10))
+    Ke, *_ = O.assemble_elements(m, off, xi, w, eta)
+    assert _fixture_err(O, O.assemble_K_full(m, Ke), name) < 1e-13
+
+
+def test_isoviscous_stiffness_fixture(oracle):
+    """IsoviscousStiffnessSuite: testIsoviscousSiffness.matrix (3x3 elements, 32x32)."""
+    O = oracle
+    m = O.make_mesh(2, (3, 3))
+    off, xi, w = _gauss_swarm(O, m, 9)
+    Ke, *_ = O.assemble_elements(m, off, xi, w, np.ones(36))
+    assert _fixture_err(O, O.assemble_K_full(m, Ke), "testIsoviscousSiffness.matrix") < 1e-13
+
+
+def test_minstd_known_answer(oracle):
+    """Park & Miller 1988: from state 1 the 10000th state is 1043618065 (seed 123459877 ^ mask = 1)."""
+    s = oracle.rng_sequence(123459877, 10000)
+    assert int(s[0]) == 16807 and int(s[-1]) == 1043618065
+
+
+def test_layout_golden(oracle):
+    """First particles of the seed-13 layout on the Rayleigh-Taylor box (golden vector)."""
+    gold = np.load(os.path.join(GOLDEN, "layout_seed13.npz"))
+    m = oracle.make_mesh(2, (4, 4), (0, 0), (0.9142, 1.0))
+    c = oracle.layout_random(m, 3, 13)
+    assert np.array_equal(c, gold["coords"])
+
+
+def test_global_to_local_closed_form(oracle):
+    m = oracle.make_mesh(3, (4, 3, 5), (0, -1, 2), (2, 1, 3))
+    rng = np.random.default_rng(0)
+    for e in [0, 17, 59]:
+        nodes_lo = np.array([(e % 4) * 0.5, -1 + ((e // 4) % 3) * (2 / 3), 2 + (e // 12) * 0.2])
+        h = np.array([0.5, 2 / 3, 0.2])
+        xi = rng.uniform(-1, 1, 3)
+        x = nodes_lo + (xi + 1) * h / 2
+        assert np.max(np.abs(oracle.global_to_local(m, e, x) - xi)) < 1e-14
+
+
+def test_shape_derivatives_closed_form(oracle):
+    """Rectangular elements: detJ = prod(h/2), GNx = (2/h) dN/dxi, sum_a GNx[d][a] = 0."""
+    m = oracle.make_mesh(3, (4, 3, 5), (0, 0, 0), (2, 1, 3))
+    h = np.array([0.5, 1 / 3, 0.6])
+    xi = np.array([0.3, -0.7, 0.1])
+    detJ, GNx = oracle.shape_global_derivs(m, 13, xi)
+    assert abs(detJ - np.prod(h / 2)) < 1e-15
+    for a in range(8):
+        s = np.array([2 * (a & 1) - 1, 2 * ((a >> 1) & 1) - 1, 2 * ((a >> 2) & 1) - 1])
+        dN = np.array([s[0] * (1 + s[1] * xi[1]) * (1 + s[2] * xi[2]), s[1] * (1 + s[0] * xi[0]) * (1 + s[2] * xi[2]),
+                       s[2] * (1 + s[0] * xi[0]) * (1 + s[1] * xi[1])]) / 8
+        assert np.allclose(GNx[:3, a], 2 / h * dN, rtol=1e-14, atol=1e-14)
+    assert np.allclose(GNx[:3].sum(axis=1), 0.0, atol=1e-14)
+
+
+def _random_swarm(O, dim, nel, ppc, seed):
+    m = O.make_mesh(dim, nel)
+    c = O.layout_random(m, ppc, seed)
+    ne = int(np.prod(nel))
+    off = np.arange(0, ne * ppc + 1, ppc).astype(np.int32)
+    xi, w = O.map_particles(m, c, off)
+    return m, off, xi, w, c
+
+
+def test_element_terms_closed_forms(oracle):
+    """G_e = -sum_p w detJ GNx (sum over a of G_e per direction = 0), f_e sums to -rho g vol,
+    approxS = sum_v G_v^2 / K_vv, and K_e is symmetric positive semi-definite with the rigid
+    translations in its kernel."""
+    O = oracle
+    m, off, xi, w, _ = _random_swarm(O, 3, (3, 2, 2), 25, 7)
+    eta = np.exp(np.random.default_rng(1).normal(size=len(w)))
+    rho = np.full(len(w), 2.5)
+    Ke, Ge, fe, aS = O.assemble_elements(m, off, xi, w, eta, rho=rho, gravity=1.5)
+    vol = (1 / 3) * (1 / 2) * (1 / 2)
+    for e in range(12):
+        assert np.allclose(Ge[e].reshape(8, 3).sum(axis=0), 0.0, atol=1e-15)
+        assert abs(fe[e].reshape(8, 3)[:, 1].sum() + 2.5 * 1.5 * vol) < 1e-14
+        assert np.allclose(fe[e].reshape(8, 3)[:, [0, 2]], 0.0)
+        assert abs(aS[e] - np.sum(Ge[e] ** 2 / np.diag(Ke[e]))) < 1e-14 * aS[e]
+        assert np.allclose(Ke[e], Ke[e].T, rtol=0, atol=1e-15 * np.abs(Ke[e]).max())
+        for c in range(3):
+            t = np.zeros(24)
+            t[c::3] = 1.0
+            assert np.abs(Ke[e] @ t).max() < 1e-13 * np.abs(Ke[e]).max()
+        assert np.linalg.eigvalsh(Ke[e]).min() > -1e-12 * np.abs(Ke[e]).max()
+
+
+def test_element_K_brute_force(oracle):
+    """K_e against an independent numpy evaluation of sum_p w detJ eta B^T D B (engineering shear)."""
+    O = oracle
+    m, off, xi, w, _ = _random_swarm(O, 2, (2, 2), 6, 3)
+    eta = np.linspace(0.5, 3.0, len(w))
+    Ke, *_ = O.assemble_elements(m, off, xi, w, eta)
+    h = np.array([0.5, 0.5])
+    for e in range(4):
+        K = np.zeros((8, 8))
+        for p in range(off[e], off[e + 1]):
+            x, y = xi[p, :2]
+            B = np.zeros((3, 8))
+            for a in range(4):
+                sx, sy = 2 * (a & 1) - 1, 2 * (a >> 1) - 1
+                dx = sx * (1 + sy * y) / 4 * 2 / h[0]
+                dy = sy * (1 + sx * x) / 4 * 2 / h[1]
+                B[0, 2 * a], B[1, 2 * a + 1], B[2, 2 * a], B[2, 2 * a + 1] = dx, dy, dy, dx
+            K += w[p] * np.prod(h / 2) * eta[p] * B.T @ np.diag([2, 2, 1]) @ B
+        assert np.max(np.abs(K - Ke[e])) < 1e-14 * np.abs(K).max()
+
+
+def test_arrhenius_and_yield_scalar_forms(oracle):
+    """Single particles against the closed forms of Arrhenius.c:193-197, VonMises.c:251-256 and
+    DruckerPrager.c:292-315 (plane strain and 3D)."""
+    O = oracle
+    m = O.make_mesh(2, (1, 1))
+    T = np.array([0.2, 0.4, 0.6, 0.8])
+    xi = np.array([[0.1, -0.3, 0.0]])
+    off = np.array([0, 1], np.int32)
+    eta, _ = O.eval_viscosity(m, [O.make_material([(O.RH_ARRHENIUS, (2.0, 3.0, 0.0, 0.5))])], off, xi,
+                              np.zeros(1, np.int32), temperature=T)
+    N = np.array([(1 - 0.1) * (1 + 0.3), (1 + 0.1) * (1 + 0.3), (1 - 0.1) * (1 - 0.3), (1 + 0.1) * (1 - 0.3)]) / 4
+    assert abs(eta[0] - 2.0 * np.exp(3.0 / (N @ T + 0.5))) < 1e-14 * eta[0]
+    # pure shear velocity u = (y, 0): strain rate xy = 1/2, tau_II = eta
+    vel = np.array([0, 0, 0, 0, 0, 0, 0, 0], float).reshape(4, 2)
+    vel[:, 0] = [0, 0, 1, 1]
+    for kind, params, crit in [(O.RH_VONMISES, (0.4, 0.01, 0), 0.4),
+                               (O.RH_DRUCKERPRAGER, (0.3, 0.6, 0.0, 0.01, 0),
+                                np.sin(np.arctan(0.6)) * 2.0 + 0.3 * np.cos(np.arctan(0.6)))]:
+        mat = O.make_material([(O.RH_MATERIALVISCOSITY, (3.0,)), (kind, params)])
+        eta, y = O.eval_viscosity(m, [mat], off, xi, np.zeros(1, np.int32), velocity=vel.reshape(-1),
+                                  pressure=np.array([2.0]), previous_solution_exists=True)
+        assert y[0] == 1
+        assert abs(eta[0] - 3.0 * crit / 3.0) < 1e-14  # eta * tau_y / tau_II with tau_II = eta = 3
+    # not yielded before the first solve
+    eta, y = O.eval_viscosity(m, [O.make_material([(O.RH_MATERIALVISCOSITY, (3.0,)), (O.RH_VONMISES, (0.4, 0.01, 0))])],
+                              off, xi, np.zeros(1, np.int32), velocity=vel.reshape(-1), previous_solution_exists=False)
+    assert eta[0] == 3.0 and y[0] == 0
+    # 3D Drucker-Prager constants
+    m3 = O.make_mesh(3, (1, 1, 1))
+    v3 = np.zeros((8, 3))
+    v3[:, 0] = [0, 0, 1, 1, 0, 0, 1, 1]
+    mat = O.make_material([(O.RH_MATERIALVISCOSITY, (3.0,)), (O.RH_DRUCKERPRAGER, (0.3, 0.6, 0.0, 0.01, 0))])
+    eta, y = O.eval_viscosity(m3, [mat], off, np.array([[0.0, 0.0, 0.0]]), np.zeros(1, np.int32),
+                              velocity=v3.reshape(-1), pressure=np.array([2.0]), previous_solution_exists=True)
+    phi = np.arctan(0.6)
+    k = 6 * np.sin(phi) / (np.sqrt(3) * (3 - np.sin(phi)))
+    c = 6 * 0.3 * np.cos(phi) / (np.sqrt(3) * (3 - np.sin(phi)))
+    assert abs(eta[0] - (k * 2.0 + c)) < 1e-14
+
+
+def test_bsscr_matches_direct_solve(oracle):
+    """The oracle's BSSCR (GMG-preconditioned inner CG, approxS-preconditioned outer CG) against a
+    sparse direct solve of the assembled saddle-point system (PIC quadrature: no pressure null space)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    O = oracle
+    m, off, xi, w, c = _random_swarm(O, 2, (16, 16), 12, 13)
+    light = c[:, 1] <= 0.2 + 0.02 * np.cos(2 * np.pi * c[:, 0] / 1.8284)
+    eta = np.where(light, 0.1, 1.0)
+    Ke, Ge, fe, aS = O.assemble_elements(m, off, xi, w, eta, rho=np.where(light, 0.0, 1.0), gravity=1.0)
+    nn = 17 * 17
+    mask = np.zeros((nn, 2), np.uint8)
+    i, j = np.arange(nn) % 17, np.arange(nn) // 17
+    mask[(i == 0) | (i == 16), 0] = 1
+    mask[(j == 0) | (j == 16), :] = 1
+    gs = O.GlobalSystem(m, mask.reshape(-1), np.zeros(2 * nn), Ke, Ge, fe)
+    u, p, st = gs.solve(aS, inner_rtol=1e-12, outer_rtol=1e-12)
+    K, G = gs.scipy()
+    A = sp.bmat([[K, G], [G.T, None]]).tocsc()
+    x = spla.spsolve(A, np.concatenate([gs.f, gs.h]))
+    n = K.shape[0]
+    assert st.stokes_rel_residual < 1e-9
+    assert np.linalg.norm(u - x[:n]) / np.linalg.norm(x[:n]) < 1e-8
+    assert np.linalg.norm(p - x[n:]) / np.linalg.norm(x[n:]) < 1e-8
+
+
+def test_bsscr_nullspace_gauss_quadrature(oracle):
+    """With exact (Gauss) quadrature G 1 = 0 on a closed box: the constant is detected, projected out of
+    h_hat and p, and the solve still reaches its tolerance."""
+    O = oracle
+    m = O.make_mesh(2, (8, 8))
+    off, xi, w = _gauss_swarm(O, m, 64)
+    rho = np.where(np.repeat(np.arange(64) % 8 < 4, 4), 1.0, 0.0)
+    Ke, Ge, fe, aS = O.assemble_elements(m, off, xi, w, np.ones(256), rho=rho, gravity=1.0)
+    nn = 81
+    mask = np.zeros((nn, 2), np.uint8)
+    i, j = np.arange(nn) % 9, np.arange(nn) // 9
+    mask[(i == 0) | (i == 8), 0] = 1
+    mask[(j == 0) | (j == 8), 1] = 1
+    gs = O.GlobalSystem(m, mask.reshape(-1), np.zeros(2 * nn), Ke, Ge, fe)
+    u, p, st = gs.solve(aS, inner_rtol=1e-12, outer_rtol=1e-10)
+    assert abs(p.sum()) < 1e-12
+    assert st.stokes_rel_residual < 1e-8
