@@ -88,9 +88,16 @@ _SIGS = {
     "ust_get_particle_viscosity": (I, [vp, vp, vp]),
     "ust_get_weights": (I, [vp, vp, vp, vp]),
     "ust_get_rhs": (I, [vp, vp, vp]),
+    "ust_get_device_views": (I, [vp, C.POINTER(Grid), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),
+    "ust_k_apply_K_ref_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp]),
+    "ust_k_apply_K_ref_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp]),
+    "ust_k_copy_f64_to_f32": (I, [vp, vp, C.c_size_t, vp]),
+    "ust_k_weights_len": (C.c_size_t, [C.POINTER(Grid)]),
     "ust_apply_K": (I, [vp, vp, vp]),
     "ust_apply_K_dev": (I, [vp, vp, vp]),
     "ust_apply_K_f32_dev": (I, [vp, vp, vp]),
+    "ust_apply_K_f32": (I, [vp, vp, vp]),
+    "ust_get_cohesion_scale": (I, [vp, vp]),
     "ust_apply_G": (I, [vp, vp, vp]),
     "ust_apply_Gt": (I, [vp, vp, vp]),
     "ust_solve": (I, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
@@ -122,6 +129,11 @@ _SIGS = {
     "ust_event_record": (I, [vp, vp]),
     "ust_event_elapsed_ms": (I, [vp, vp, C.POINTER(C.c_float)]),
     "ust_stream_sync": (I, [vp]),
+    "ust_dev_malloc": (I, [C.POINTER(vp), C.c_size_t]),
+    "ust_dev_free": (I, [vp]),
+    "ust_memcpy_h2d": (I, [vp, vp, C.c_size_t, vp]),
+    "ust_memcpy_d2h": (I, [vp, vp, C.c_size_t, vp]),
+    "ust_k_fill_hash_f64": (I, [vp, C.c_size_t, C.c_ulonglong, vp]),
     "ust_dev_error_string": (C.c_char_p, []),
 }
 

@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void k_particle_weights(GridDev g, const int *
     }
     if (valid && k == 0) {
 #pragma unroll
-        for (int q = 0; q < NQ; q++) WK[(size_t)q * g.nel + e] = aK[q];
+        for (int q = 0; q < NQ; q++) WK[widx(g, e, q)] = aK[q];
 #pragma unroll
         for (int b = 0; b < NV; b++) {
             WG[(size_t)b * g.nel + e] = aG[b];
@@ -249,7 +249,8 @@ __device__ __forceinline__ double bdb_entry(int q, int row, int col, const doubl
 typedef double ust_double4 __attribute__((ext_vector_type(4)));
 
 // K_e = W (count x NQ) * M (NQ x nd^2) on v_mfma_f64_16x16x4f64.  One wavefront per 16 elements;
-// A fragment: lane l holds A[l%16][l/16], B fragment: B[l/16][l%16], D: acc[r] = D[4(l/16)+r][l%16].
+// A fragment: lane l holds A[l%16][l/16], B fragment: B[l/16][l%16], D: acc[r] = D[4r + l/16][l%16]
+// (output layout measured on gfx950 against the oracle's element matrices).
 template <int DIM>
 __global__ __launch_bounds__(256) void k_element_matrices(GridDev g, const double *__restrict__ WK, int first,
                                                           int count, double *__restrict__ Ke) {
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(256) void k_element_matrices(GridDev g, const doubl
 #pragma unroll
     for (int ks = 0; ks < KSTEPS; ks++) {
         const int q = ks * 4 + lk, el = e0 + li;
-        afrag[ks] = (q < NQ && el < count) ? WK[(size_t)q * g.nel + first + el] : 0.0;
+        afrag[ks] = (q < NQ && el < count) ? WK[widx(g, first + el, q)] : 0.0;
     }
     for (int ct = 0; ct < NCOL / 16; ct++) {
         ust_double4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(256) void k_element_matrices(GridDev g, const doubl
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int el = e0 + 4 * lk + r;
+            const int el = e0 + 4 * r + lk;
             if (el < count) Ke[(size_t)el * NCOL + col] = acc[r];
         }
     }
@@ -335,7 +336,7 @@ __device__ __forceinline__ void element_K_diag(const GridDev &g, const double *W
 #pragma unroll
     for (int v = 0; v < NV * DIM; v++) dg[v] = 0.0;
     for (int q = 0; q < NQ; q++) {
-        const double wq = WK[(size_t)q * g.nel + e];
+        const double wq = WK[widx(g, e, q)];
 #pragma unroll
         for (int a = 0; a < NV; a++) {
             double gk[3], s2 = 0.0;

@@ -37,15 +37,40 @@ __device__ __forceinline__ void lagrange3(double x, double *L) {
 __host__ __device__ constexpr int cfac(int a, int q) { return a ? q : 2 - q; }
 __host__ __device__ constexpr int sgn(int a) { return 2 * a - 1; }
 
+// Element tiles of the operator kernels: one 256-thread workgroup per tile, one element per thread.
+// The K weights are stored tile-blocked, W[(tile*NQ + q)*256 + local], so that every load of a
+// quadrature weight by a wavefront is one contiguous 64-element run.
+#define UST_TX3 8
+#define UST_TY3 8
+#define UST_TZ3 4
+#define UST_TX2 16
+#define UST_TY2 16
+#define UST_TILE 256
+
 struct GridDev {
     int dim;
     int nx, ny, nz;       // elements per axis (nz = 1 in 2D)
     int nnx, nny, nnz;    // nodes per axis
     int nel;
+    int tnx, tny, tnz;    // tiles per axis
+    int ntiles;
     double hx, hy, hz;
     double xmin, ymin, zmin;
     double ymax;
 };
+
+// index of quadrature weight q of element e in the tile-blocked layout
+__host__ __device__ __forceinline__ size_t widx(const GridDev &g, int e, int q) {
+    const int ex = e % g.nx, ey = (e / g.nx) % g.ny, ez = e / (g.nx * g.ny);
+    if (g.dim == 3) {
+        const int t = ex / UST_TX3 + g.tnx * (ey / UST_TY3 + g.tny * (ez / UST_TZ3));
+        const int l = ex % UST_TX3 + UST_TX3 * (ey % UST_TY3 + UST_TY3 * (ez % UST_TZ3));
+        return ((size_t)t * 27 + q) * UST_TILE + l;
+    }
+    const int t = ex / UST_TX2 + g.tnx * (ey / UST_TY2);
+    const int l = ex % UST_TX2 + UST_TX2 * (ey % UST_TY2);
+    return ((size_t)t * 9 + q) * UST_TILE + l;
+}
 
 static inline GridDev grid_dev(const ust_grid *g) {
     GridDev d;
@@ -57,6 +82,16 @@ static inline GridDev grid_dev(const ust_grid *g) {
     d.nny = d.ny + 1;
     d.nnz = g->dim == 3 ? d.nz + 1 : 1;
     d.nel = d.nx * d.ny * d.nz;
+    if (g->dim == 3) {
+        d.tnx = (d.nx + UST_TX3 - 1) / UST_TX3;
+        d.tny = (d.ny + UST_TY3 - 1) / UST_TY3;
+        d.tnz = (d.nz + UST_TZ3 - 1) / UST_TZ3;
+    } else {
+        d.tnx = (d.nx + UST_TX2 - 1) / UST_TX2;
+        d.tny = (d.ny + UST_TY2 - 1) / UST_TY2;
+        d.tnz = 1;
+    }
+    d.ntiles = d.tnx * d.tny * d.tnz;
     d.hx = g->h[0];
     d.hy = g->h[1];
     d.hz = g->dim == 3 ? g->h[2] : 1.0;

@@ -23,7 +23,7 @@ __device__ __forceinline__ unsigned short f32_to_bf16(float f) {
 __device__ __forceinline__ double elem_logeta(const GridDev &g, const double *WK, int e) {
     const int nq = g.dim == 3 ? 27 : 9;
     double s = 0.0;
-    for (int q = 0; q < nq; q++) s += WK[(size_t)q * g.nel + e];
+    for (int q = 0; q < nq; q++) s += WK[widx(g, e, q)];
     const double vol = g.dim == 3 ? g.hx * g.hy * g.hz : g.hx * g.hy;
     return log10(fmax(s / vol, 1e-300));
 }

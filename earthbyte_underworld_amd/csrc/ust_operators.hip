@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_apply_K(GridDev g, const T *__restrict_
             ye[a][i] = (T)0;
         }
 #pragma unroll
-    for (int q = 0; q < NQ; q++) wq[q] = W[(size_t)q * g.nel + e];
+    for (int q = 0; q < NQ; q++) wq[q] = W[widx(g, e, q)];
     const T s[3] = {(T)(2.0 / g.hx), (T)(2.0 / g.hy), (T)(2.0 / g.hz)};
     element_apply<DIM, T>(wq, s, ue, ye);
 #pragma unroll
@@ -210,7 +210,7 @@ __global__ void k_diag_acc(GridDev g, const double *__restrict__ WK, const unsig
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         const int qd[3] = {q % 3, (q / 3) % 3, q / 9};
-        const double wq = WK[(size_t)q * g.nel + e] / (double)(1 << (2 * DIM));
+        const double wq = WK[widx(g, e, q)] / (double)(1 << (2 * DIM));
 #pragma unroll
         for (int a = 0; a < NV; a++) {
             double gk[3], s2 = 0.0;
@@ -273,7 +273,7 @@ __global__ void k_coarsen_weights(GridDev gf, GridDev gc, const double *__restri
         const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
         const int ef = (2 * I + cx) + gf.nx * ((2 * J + cy) + gf.ny * (DIM == 3 ? (2 * K + cz) : 0));
         for (int q = 0; q < NQ; q++) {
-            const double wf = Wf[(size_t)q * gf.nel + ef];
+            const double wf = Wf[widx(gf, ef, q)];
             const int qx = q % 3, qy = (q / 3) % 3, qz = q / 9;
 #pragma unroll
             for (int qp = 0; qp < NQ; qp++) {
@@ -285,7 +285,7 @@ __global__ void k_coarsen_weights(GridDev gf, GridDev gc, const double *__restri
         }
     }
 #pragma unroll
-    for (int q = 0; q < NQ; q++) Wc[(size_t)q * gc.nel + E] = acc[q];
+    for (int q = 0; q < NQ; q++) Wc[widx(gc, E, q)] = acc[q];
 }
 
 // restriction b_c = P^T r_f : coarse node gathers fine nodes 2I+{-1,0,1} (weights 1/2, 1, 1/2)
@@ -381,7 +381,7 @@ __global__ void k_dense_assemble(GridDev g, const double *__restrict__ WK, const
         if (i == j)
             for (int k = 0; k < DIM; k++) m += ga[k] * gb[k];
         m += ga[j] * gb[i];
-        v += WK[(size_t)q * g.nel + e] * m;
+        v += WK[widx(g, e, q)] * m;
     }
     unsafeAtomicAdd(&A[(size_t)R * n + C], v);
 }
@@ -428,8 +428,10 @@ __global__ void k_dense_matvec(const double *__restrict__ Ainv, int n, const T *
 // ------------------------------------------------------------------------------------------
 extern "C" {
 
-int ust_k_apply_K_f64(const ust_grid *gr, const double *WK, const unsigned char *mask, const double *u, double *y,
-                      void *stream) {
+// Element-per-thread variant with per-dof atomics: the plain restatement the tiled kernels of
+// ust_apply.hip are checked against.
+int ust_k_apply_K_ref_f64(const ust_grid *gr, const double *WK, const unsigned char *mask, const double *u, double *y,
+                          void *stream) {
     GridDev g = grid_dev(gr);
     if (!WK || !mask || !u || !y) return UST_EINVAL;
     const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * g.dim;
@@ -445,8 +447,8 @@ int ust_k_apply_K_f64(const ust_grid *gr, const double *WK, const unsigned char 
     return UST_OK;
 }
 
-int ust_k_apply_K_f32(const ust_grid *gr, const float *WK, const unsigned char *mask, const float *u, float *y,
-                      void *stream) {
+int ust_k_apply_K_ref_f32(const ust_grid *gr, const float *WK, const unsigned char *mask, const float *u, float *y,
+                          void *stream) {
     GridDev g = grid_dev(gr);
     if (!WK || !mask || !u || !y) return UST_EINVAL;
     const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * g.dim;

@@ -160,7 +160,8 @@ int ust_set_mesh(ust_context *ctx, int dim, const int nel[3], const double xmin[
     ctx->mask_h = (unsigned char *)calloc(nd, 1);
     ctx->bcval_h = (double *)calloc(nd, sizeof(double));
     int nq = dim == 3 ? 27 : 9, nv = 1 << dim;
-    DALLOC(ctx, ctx->WK, (size_t)nq * ne * sizeof(double));
+    (void)nq;
+    DALLOC(ctx, ctx->WK, ust_k_weights_len(&ctx->grid) * sizeof(double));
     DALLOC(ctx, ctx->WG, (size_t)nv * ne * sizeof(double));
     DALLOC(ctx, ctx->WF, (size_t)nv * ne * sizeof(double));
     DALLOC(ctx, ctx->approxS, ne * sizeof(double));
@@ -487,9 +488,33 @@ int ust_get_weights(ust_context *ctx, double *WK, double *WG, double *WF) {
     if (rc) return rc;
     int nq = ctx->grid.dim == 3 ? 27 : 9, nv = 1 << ctx->grid.dim;
     size_t ne = (size_t)ctx->nel;
-    if (WK) CKD(ctx, ust_memcpy_d2h(WK, ctx->WK, nq * ne * sizeof(double), ctx->stream));
+    if (WK) {
+        /* device layout is tile-blocked; hand back [q][e] */
+        size_t len = ust_k_weights_len(&ctx->grid);
+        double *tmp = (double *)malloc(len * sizeof(double));
+        if (!tmp) UST_FAIL(ctx, UST_ENOMEM, "host allocation failed");
+        rc = ust_memcpy_d2h(tmp, ctx->WK, len * sizeof(double), ctx->stream);
+        if (rc) {
+            free(tmp);
+            UST_FAIL(ctx, rc, "weights download failed: %s", ust_dev_error_string());
+        }
+        for (int q = 0; q < nq; q++)
+            for (int e = 0; e < ctx->nel; e++) WK[(size_t)q * ne + e] = tmp[ust_k_weights_index(&ctx->grid, e, q)];
+        free(tmp);
+    }
     if (WG) CKD(ctx, ust_memcpy_d2h(WG, ctx->WG, nv * ne * sizeof(double), ctx->stream));
     if (WF) CKD(ctx, ust_memcpy_d2h(WF, ctx->WF, nv * ne * sizeof(double), ctx->stream));
+    return UST_OK;
+}
+
+int ust_get_device_views(ust_context *ctx, ust_grid *grid, const double **WK, const double **WG, const double **WF,
+                         const unsigned char **mask) {
+    if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "no mesh");
+    if (grid) *grid = ctx->grid;
+    if (WK) *WK = ctx->WK;
+    if (WG) *WG = ctx->WG;
+    if (WF) *WF = ctx->WF;
+    if (mask) *mask = ctx->mask_d;
     return UST_OK;
 }
 
@@ -518,6 +543,29 @@ int ust_apply_K_f32_dev(ust_context *ctx, const float *u, float *y) {
         if (rc) return rc;
     }
     CKD(ctx, ust_k_apply_K_f32(&ctx->grid, ctx->lev[0].WK32, ctx->mask_d, u, y, ctx->stream));
+    return UST_OK;
+}
+
+int ust_apply_K_f32(ust_context *ctx, const float *u, float *y) {
+    size_t n = (size_t)ctx->ndof;
+    float *a = NULL, *b = NULL;
+    CKD(ctx, ust_dev_malloc((void **)&a, n * sizeof(float)));
+    int rc = ust_dev_malloc((void **)&b, n * sizeof(float));
+    if (!rc) rc = ust_memcpy_h2d(a, u, n * sizeof(float), ctx->stream);
+    if (!rc) rc = ust_apply_K_f32_dev(ctx, a, b);
+    if (!rc) rc = ust_memcpy_d2h(y, b, n * sizeof(float), ctx->stream);
+    ust_dev_free(a);
+    ust_dev_free(b);
+    return rc;
+}
+
+int ust_get_cohesion_scale(ust_context *ctx, float *coh) {
+    if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
+    if (ctx->coh) {
+        CKD(ctx, ust_memcpy_d2h(coh, ctx->coh, (size_t)ctx->np * sizeof(float), ctx->stream));
+    } else {
+        for (int i = 0; i < ctx->np; i++) coh[i] = 1.0f;
+    }
     return UST_OK;
 }
 

@@ -174,7 +174,6 @@ int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
                  "multigrid: coarsest grid has %d dofs (> 4096); use element counts with more factors of two",
                  grid_ndof(&g));
     ctx->nlev = nlev;
-    int nq = dim == 3 ? 27 : 9;
     g = ctx->grid;
     for (int l = 0; l < nlev; l++) {
         ust_level *L = &ctx->lev[l];
@@ -188,7 +187,7 @@ int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
             L->mask_h = ctx->mask_h;
         } else {
             ust_level *F = &ctx->lev[l - 1];
-            DALLOC(ctx, L->WK64, (size_t)nq * L->nel * sizeof(double));
+            DALLOC(ctx, L->WK64, ust_k_weights_len(&L->g) * sizeof(double));
             CKD(ctx, ust_k_coarsen_weights(&F->g, F->WK64, L->WK64, S(ctx)));
             /* a coarse node coincides with fine node (2I, 2J, 2K): it inherits that node's BC flags */
             L->mask_h = (unsigned char *)calloc((size_t)L->ndof, 1);
@@ -209,8 +208,8 @@ int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
         DALLOC(ctx, L->d64, n * sizeof(double));
         if (need32) {
             DALLOC(ctx, L->invd32, n * sizeof(float));
-            DALLOC(ctx, L->WK32, (size_t)nq * L->nel * sizeof(float));
-            CKD(ctx, ust_k_copy_f64_to_f32(L->WK64, L->WK32, (size_t)nq * L->nel, S(ctx)));
+            DALLOC(ctx, L->WK32, ust_k_weights_len(&L->g) * sizeof(float));
+            CKD(ctx, ust_k_copy_f64_to_f32(L->WK64, L->WK32, ust_k_weights_len(&L->g), S(ctx)));
             DALLOC(ctx, L->x32, n * sizeof(float));
             DALLOC(ctx, L->b32, n * sizeof(float));
             DALLOC(ctx, L->ax32, n * sizeof(float));

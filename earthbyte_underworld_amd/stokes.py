@@ -273,6 +273,18 @@ class StokesSystem:
         self._check(self.lib.ust_apply_K(self.ctx, _ptr(u), _ptr(y)))
         return y
 
+    def apply_K_f32(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float32)
+        y = np.zeros(self.ndof, np.float32)
+        self._check(self.lib.ust_apply_K_f32(self.ctx, _ptr(u), _ptr(y)))
+        return y
+
+    def cohesion_scale(self):
+        n = self.lib.ust_swarm_count(self.ctx)
+        c = np.zeros(n, np.float32)
+        self._check(self.lib.ust_get_cohesion_scale(self.ctx, _ptr(c)))
+        return c
+
     def apply_G(self, p):
         p = np.ascontiguousarray(p, dtype=np.float64)
         y = np.zeros(self.ndof)

@@ -91,6 +91,8 @@ int ust_swarm_cohesion_noise(ust_context *ctx, float amplitude, unsigned long lo
 /* copy the swarm back (any pointer may be NULL): cell_off (nel+1), xi (np*3), w (np), material (np) */
 int ust_swarm_download(ust_context *ctx, int *cell_off, double *xi, double *w, unsigned char *material);
 int ust_swarm_count(ust_context *ctx);
+/* per-particle cohesion factors (np floats; 1 where none were set) */
+int ust_get_cohesion_scale(ust_context *ctx, float *coh);
 
 /* ---- fields read by the rheologies (PpcManager lookups, PpcManager.c:503-606) ---- */
 int ust_set_temperature(ust_context *ctx, const double *T_nodal);
@@ -114,6 +116,9 @@ int ust_get_element_f(ust_context *ctx, int first, int count, double *fe);      
 int ust_get_approxS(ust_context *ctx, double *approxS);                            /* nel */
 int ust_get_particle_viscosity(ust_context *ctx, double *eta, unsigned char *yielded);
 int ust_get_weights(ust_context *ctx, double *WK, double *WG, double *WF);         /* SoA (nq|nv)*nel */
+/* device views for callers that drive the kernel ABI directly (valid until the next ust_set_mesh) */
+int ust_get_device_views(ust_context *ctx, ust_grid *grid, const double **WK, const double **WG,
+                         const double **WF, const unsigned char **mask);
 /* right-hand sides after BC lifting: f (nodal, zero on BC dofs), h (per element) */
 int ust_get_rhs(ust_context *ctx, double *f_nodal, double *h);
 
@@ -121,6 +126,7 @@ int ust_get_rhs(ust_context *ctx, double *f_nodal, double *h);
 int ust_apply_K(ust_context *ctx, const double *u, double *y);
 int ust_apply_K_dev(ust_context *ctx, const double *u_dev, double *y_dev);
 int ust_apply_K_f32_dev(ust_context *ctx, const float *u_dev, float *y_dev);
+int ust_apply_K_f32(ust_context *ctx, const float *u, float *y);
 int ust_apply_G(ust_context *ctx, const double *p, double *y);
 int ust_apply_Gt(ust_context *ctx, const double *u, double *q);
 

