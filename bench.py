@@ -178,19 +178,21 @@ def main():
     WK, WG, WF, mask = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
     lib.ust_get_device_views(ctx, C.byref(g), C.byref(WK), C.byref(WG), C.byref(WF), C.byref(mask))
     wl = lib.ust_k_weights_len(C.byref(g))
-    W32, u32, y32 = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    W32, u32, y32, wk32 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
     lib.ust_dev_malloc(C.byref(W32), wl * 4)
+    lib.ust_dev_malloc(C.byref(wk32), lib.ust_k_apply_work_len(C.byref(g)) * 4)
     lib.ust_dev_malloc(C.byref(u32), s.ndof * 4)
     lib.ust_dev_malloc(C.byref(y32), s.ndof * 4)
     lib.ust_k_copy_f64_to_f32(WK, W32, wl, stream)
     u0 = np.random.default_rng(0).normal(size=s.ndof).astype(np.float32)
     lib.ust_memcpy_h2d(u32, u0.ctypes.data_as(C.c_void_p), s.ndof * 4, stream)
     t_apply32 = time_on_stream(lib, stream,
-                               lambda: lib.ust_k_apply_K_f32(C.byref(g), W32, mask, u32, y32, stream), 50)
+                               lambda: lib.ust_k_apply_K_f32(C.byref(g), W32, mask, u32, y32, wk32, stream), 50)
     apply_bytes = s.nel * 27 * 4 + 2 * s.ndof * 4
     lib.ust_dev_free(W32)
     lib.ust_dev_free(u32)
     lib.ust_dev_free(y32)
+    lib.ust_dev_free(wk32)
     achieved = apply_bytes / (t_apply32 * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

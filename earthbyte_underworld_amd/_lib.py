@@ -120,8 +120,9 @@ _SIGS = {
     "ust_UzawaPreconditionerTerm_AssembleElement": (None, [vp, vp, I, vp, vp, vp]),
     "ust_BuoyancyForceTerm_AssembleElement": (None, [vp, vp, I, vp]),
     # thin kernel ABI used directly by tests and the benchmark timers
-    "ust_k_apply_K_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp]),
-    "ust_k_apply_K_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp]),
+    "ust_k_apply_work_len": (C.c_size_t, [C.POINTER(Grid)]),
+    "ust_k_apply_K_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp]),
+    "ust_k_apply_K_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp]),
     "ust_k_particle_weights": (I, [C.POINTER(Grid), vp, I, vp, vp, vp, vp, vp, vp, vp, I, vp, vp, vp, I, D, vp,
                                    vp, vp, vp, vp, vp]),
     "ust_event_create": (I, [C.POINTER(vp)]),

@@ -6,8 +6,9 @@
 // thread.  The tile's nodal velocities are staged in LDS, each thread applies its element with sum
 // factorisation (gradients at the Lagrange points from edge differences, stress, and the transposed
 // interpolation back to the vertices, ~0.8 kflop per 3D element), element results are reduced in
-// LDS, and the tile writes its nodes once: nodes interior to the tile (or on the box boundary) with
-// plain stores, nodes on faces shared with a neighbouring tile with one atomic add per dof.
+// LDS, and the tile writes its whole node block to its own slot of a workspace.  A second, node-
+// ordered pass sums the slots covering each dof (one for interior nodes, two to eight on tile faces)
+// and writes y, so y needs no clearing and no global atomics are used.
 #include "ust_common.h"
 
 // axes orthogonal to k, in increasing order
@@ -184,10 +185,24 @@ __device__ __forceinline__ void element_apply2(const T u[4][2], const T W[9], co
 }
 
 // ------------------------------------------------------------------------------------------
+// Phase 1: one workgroup per tile.  The tile's nodal velocities are staged in LDS, each thread
+// applies its element, and the element results are reduced into the tile's node block in LDS.
+// Nodes that belong to this tile only are written to y directly; nodes on a face shared with a
+// neighbouring tile go to the tile's slot of a workspace.  Phase 2: one thread per shared dof sums
+// the 2, 4 or 8 slots that hold it.  No atomics touch global memory and the summation order is
+// fixed, so K u is bitwise reproducible.
+// ------------------------------------------------------------------------------------------
+
+// does node coordinate gc lie on a face between two tiles along an axis of tile size T (tn tiles)?
+__device__ __forceinline__ bool on_tile_face(int gc, int T, int tn) {
+    return gc > 0 && gc % T == 0 && gc / T < tn;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_applyK3_tiled(GridDev g, const T *__restrict__ W,
                                                        const unsigned char *__restrict__ mask,
-                                                       const T *__restrict__ u, T *__restrict__ y) {
+                                                       const T *__restrict__ u, T *__restrict__ y,
+                                                       T *__restrict__ work) {
     constexpr int TX = UST_TX3, TY = UST_TY3, TZ = UST_TZ3;
     constexpr int NX = TX + 1, NY = TY + 1, NZ = TZ + 1, NN = NX * NY * NZ;
     __shared__ T su[NN * 3];
@@ -207,8 +222,10 @@ __global__ __launch_bounds__(256) void k_applyK3_tiled(GridDev g, const T *__res
     }
     __syncthreads();
     const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
-    if (tx0 + lx < g.nx && ty0 + ly < g.ny && tz0 + lz < g.nz) {
-        T ue[8][3], ye[8][3], wq[27];
+    const bool active = tx0 + lx < g.nx && ty0 + ly < g.ny && tz0 + lz < g.nz;
+    T ye[8][3];
+    if (active) {
+        T ue[8][3], wq[27];
 #pragma unroll
         for (int a = 0; a < 8; a++) {
             const int ln = (lx + (a & 1)) + NX * ((ly + ((a >> 1) & 1)) + NY * (lz + (a >> 2)));
@@ -222,32 +239,40 @@ __global__ __launch_bounds__(256) void k_applyK3_tiled(GridDev g, const T *__res
         for (int q = 0; q < 27; q++) wq[q] = W[((size_t)tile * 27 + q) * UST_TILE + threadIdx.x];
         const T s[3] = {(T)(2.0 / g.hx), (T)(2.0 / g.hy), (T)(2.0 / g.hz)};
         element_apply3<T>(ue, wq, s, ye);
-#pragma unroll
-        for (int a = 0; a < 8; a++) {
-            const int ln = (lx + (a & 1)) + NX * ((ly + ((a >> 1) & 1)) + NY * (lz + (a >> 2)));
-#pragma unroll
-            for (int i = 0; i < 3; i++) atomicAdd(&sy[ln * 3 + i], ye[a][i]);
-        }
     }
-    __syncthreads();
+    // Wave w holds the element layer lz = w, whose nodes are layers w and w+1: even waves never share a
+    // node, nor do odd waves, and within one LDS add the 64 lanes hit 64 distinct nodes.  Adding in two
+    // wave-parity phases therefore fixes the order in which every node receives its contributions.
+    for (int phase = 0; phase < 2; phase++) {
+        if (active && (lz & 1) == phase) {
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                const int ln = (lx + (a & 1)) + NX * ((ly + ((a >> 1) & 1)) + NY * (lz + (a >> 2)));
+#pragma unroll
+                for (int i = 0; i < 3; i++) atomicAdd(&sy[ln * 3 + i], ye[a][i]);
+            }
+        }
+        __syncthreads();
+    }
+    T *slot = work + (size_t)tile * NN * 3;
     for (int t = threadIdx.x; t < NN * 3; t += blockDim.x) {
         const int c = t % 3, n = t / 3;
-        const int nlx = n % NX, nly = (n / NX) % NY, nlz = n / (NX * NY);
-        const int gx = tx0 + nlx, gy = ty0 + nly, gz = tz0 + nlz;
+        const int gx = tx0 + n % NX, gy = ty0 + (n / NX) % NY, gz = tz0 + n / (NX * NY);
         if (gx >= g.nnx || gy >= g.nny || gz >= g.nnz) continue;
-        const size_t dof = ((size_t)gx + (size_t)g.nnx * ((size_t)gy + (size_t)g.nny * gz)) * 3 + c;
-        if (mask[dof]) continue;
-        const bool shared = (nlx == 0 && tx0 > 0) || (nlx == TX && tx0 + TX < g.nx) || (nly == 0 && ty0 > 0) ||
-                            (nly == TY && ty0 + TY < g.ny) || (nlz == 0 && tz0 > 0) || (nlz == TZ && tz0 + TZ < g.nz);
-        if (shared) unsafeAtomicAdd(&y[dof], sy[t]);
-        else y[dof] = sy[t];
+        if (on_tile_face(gx, TX, g.tnx) || on_tile_face(gy, TY, g.tny) || on_tile_face(gz, TZ, g.tnz)) {
+            slot[t] = sy[t];
+        } else {
+            const size_t dof = ((size_t)gx + (size_t)g.nnx * ((size_t)gy + (size_t)g.nny * gz)) * 3 + c;
+            y[dof] = mask[dof] ? (T)0 : sy[t];
+        }
     }
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const T *__restrict__ W,
                                                        const unsigned char *__restrict__ mask,
-                                                       const T *__restrict__ u, T *__restrict__ y) {
+                                                       const T *__restrict__ u, T *__restrict__ y,
+                                                       T *__restrict__ work) {
     constexpr int TX = UST_TX2, TY = UST_TY2;
     constexpr int NX = TX + 1, NY = TY + 1, NN = NX * NY;
     __shared__ T su[NN * 2];
@@ -267,8 +292,10 @@ __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const T *__res
     }
     __syncthreads();
     const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
-    if (tx0 + lx < g.nx && ty0 + ly < g.ny) {
-        T ue[4][2], ye[4][2], wq[9];
+    const bool active = tx0 + lx < g.nx && ty0 + ly < g.ny;
+    T ye[4][2];
+    if (active) {
+        T ue[4][2], wq[9];
 #pragma unroll
         for (int a = 0; a < 4; a++) {
             const int ln = (lx + (a & 1)) + NX * (ly + (a >> 1));
@@ -280,59 +307,112 @@ __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const T *__res
         for (int q = 0; q < 9; q++) wq[q] = W[((size_t)tile * 9 + q) * UST_TILE + threadIdx.x];
         const T s[3] = {(T)(2.0 / g.hx), (T)(2.0 / g.hy), (T)1};
         element_apply2<T>(ue, wq, s, ye);
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-            const int ln = (lx + (a & 1)) + NX * (ly + (a >> 1));
-            atomicAdd(&sy[ln * 2], ye[a][0]);
-            atomicAdd(&sy[ln * 2 + 1], ye[a][1]);
-        }
     }
-    __syncthreads();
+    // wave w holds element rows 4w..4w+3 (node rows 4w..4w+4): same two-phase ordering as in 3D
+    const int wave = threadIdx.x / 64;
+    for (int phase = 0; phase < 2; phase++) {
+        if (active && (wave & 1) == phase) {
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int ln = (lx + (a & 1)) + NX * (ly + (a >> 1));
+                atomicAdd(&sy[ln * 2], ye[a][0]);
+                atomicAdd(&sy[ln * 2 + 1], ye[a][1]);
+            }
+        }
+        __syncthreads();
+    }
+    T *slot = work + (size_t)tile * NN * 2;
     for (int t = threadIdx.x; t < NN * 2; t += blockDim.x) {
         const int c = t % 2, n = t / 2;
-        const int nlx = n % NX, nly = n / NX;
-        const int gx = tx0 + nlx, gy = ty0 + nly;
+        const int gx = tx0 + n % NX, gy = ty0 + n / NX;
         if (gx >= g.nnx || gy >= g.nny) continue;
-        const size_t dof = ((size_t)gx + (size_t)g.nnx * gy) * 2 + c;
-        if (mask[dof]) continue;
-        const bool shared =
-            (nlx == 0 && tx0 > 0) || (nlx == TX && tx0 + TX < g.nx) || (nly == 0 && ty0 > 0) || (nly == TY && ty0 + TY < g.ny);
-        if (shared) unsafeAtomicAdd(&y[dof], sy[t]);
-        else y[dof] = sy[t];
+        if (on_tile_face(gx, TX, g.tnx) || on_tile_face(gy, TY, g.tny)) {
+            slot[t] = sy[t];
+        } else {
+            const size_t dof = ((size_t)gx + (size_t)g.nnx * gy) * 2 + c;
+            y[dof] = mask[dof] ? (T)0 : sy[t];
+        }
     }
 }
 
-template <typename T>
-__global__ void k_zero(T *x, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        x[i] = (T)0;
+// tiles holding node coordinate gc along one axis as (tile, local index) pairs: 1 or 2 of them
+__device__ __forceinline__ int node_tiles(int gc, int T, int tn, int *tt, int *ll) {
+    int t = gc / T;
+    if (t >= tn) t = tn - 1;
+    tt[0] = t;
+    ll[0] = gc - t * T;
+    if (ll[0] == 0 && t > 0) {
+        tt[1] = t - 1;
+        ll[1] = T;
+        return 2;
+    }
+    return 1;
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned char *__restrict__ mask,
+                                                      const T *__restrict__ work, T *__restrict__ y) {
+    const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * DIM;
+    const size_t dof = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (dof >= ndof) return;
+    const int c = (int)(dof % DIM);
+    const size_t n = dof / DIM;
+    const int gx = (int)(n % g.nnx), gy = (int)((n / g.nnx) % g.nny), gz = (int)(n / ((size_t)g.nnx * g.nny));
+    constexpr int TX = DIM == 3 ? UST_TX3 : UST_TX2, TY = DIM == 3 ? UST_TY3 : UST_TY2, TZ = DIM == 3 ? UST_TZ3 : 1;
+    constexpr int NX = TX + 1, NY = TY + 1, NN = NX * NY * (DIM == 3 ? TZ + 1 : 1);
+    int tx[2], lxs[2], ty[2], lys[2], tz[2] = {0, 0}, lzs[2] = {0, 0};
+    const int nxs = node_tiles(gx, TX, g.tnx, tx, lxs);
+    const int nys = node_tiles(gy, TY, g.tny, ty, lys);
+    const int nzs = DIM == 3 ? node_tiles(gz, TZ, g.tnz, tz, lzs) : 1;
+    if (nxs * nys * nzs == 1) return;  // written by its only tile
+    if (mask[dof]) {
+        y[dof] = (T)0;
+        return;
+    }
+    T acc = (T)0;
+    for (int a = 0; a < nzs; a++)
+        for (int b = 0; b < nys; b++)
+            for (int d = 0; d < nxs; d++) {
+                const size_t tile = (size_t)tx[d] + (size_t)g.tnx * ((size_t)ty[b] + (size_t)g.tny * tz[a]);
+                const int ln = lxs[d] + NX * (lys[b] + NY * lzs[a]);
+                acc += work[(tile * NN + ln) * DIM + c];
+            }
+    y[dof] = acc;
+}
+
+static inline size_t work_len(const GridDev &g) {
+    const int nn = g.dim == 3 ? (UST_TX3 + 1) * (UST_TY3 + 1) * (UST_TZ3 + 1) : (UST_TX2 + 1) * (UST_TY2 + 1);
+    return (size_t)g.ntiles * nn * g.dim;
 }
 
 template <typename T>
-static int apply_tiled(const ust_grid *gr, const T *W, const unsigned char *mask, const T *u, T *y, void *stream) {
+static int apply_tiled(const ust_grid *gr, const T *W, const unsigned char *mask, const T *u, T *y, T *work,
+                       void *stream) {
     GridDev g = grid_dev(gr);
-    if (!W || !mask || !u || !y) return UST_EINVAL;
+    if (!W || !mask || !u || !y || !work) return UST_EINVAL;
     const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * g.dim;
     hipStream_t st = ust_stream(stream);
-    size_t zb = (ndof + UST_BLOCK - 1) / UST_BLOCK;
-    if (zb > 8192) zb = 8192;
-    hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)zb), dim3(UST_BLOCK), 0, st, y, ndof);
-    if (g.dim == 3)
-        hipLaunchKernelGGL(k_applyK3_tiled<T>, dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, y);
-    else
-        hipLaunchKernelGGL(k_applyK2_tiled<T>, dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, y);
+    if (g.dim == 3) {
+        hipLaunchKernelGGL(k_applyK3_tiled<T>, dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, y, work);
+        hipLaunchKernelGGL((k_gather_tiles<T, 3>), dim3(ust_grid_size(ndof, 256)), dim3(256), 0, st, g, mask, work, y);
+    } else {
+        hipLaunchKernelGGL(k_applyK2_tiled<T>, dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, y, work);
+        hipLaunchKernelGGL((k_gather_tiles<T, 2>), dim3(ust_grid_size(ndof, 256)), dim3(256), 0, st, g, mask, work, y);
+    }
     UST_LAUNCH_CHECK();
     return UST_OK;
 }
 
+extern "C" size_t ust_k_apply_work_len(const ust_grid *gr) { return work_len(grid_dev(gr)); }
+
 extern "C" int ust_k_apply_K_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
-                                 double *y, void *stream) {
-    return apply_tiled<double>(g, WK, mask, u, y, stream);
+                                 double *y, double *work, void *stream) {
+    return apply_tiled<double>(g, WK, mask, u, y, work, stream);
 }
 
 extern "C" int ust_k_apply_K_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
-                                 float *y, void *stream) {
-    return apply_tiled<float>(g, WK, mask, u, y, stream);
+                                 float *y, float *work, void *stream) {
+    return apply_tiled<float>(g, WK, mask, u, y, work, stream);
 }
 
 extern "C" size_t ust_k_weights_len(const ust_grid *gr) {

@@ -15,7 +15,7 @@
 // ------------------------------------------------------------------------------------------
 // per-particle rheology (isotropic D carried as eta)
 // ------------------------------------------------------------------------------------------
-template <int DIM>
+template <int DIM, bool NEED_T, bool NEED_VEL>
 __device__ double particle_viscosity(const GridDev &g, const ust_material &M, const int *nodes, const double *xi,
                                      const double *T, const double *vel, const double *pres, int e, float coh,
                                      int prev, int *yflag) {
@@ -25,7 +25,7 @@ __device__ double particle_viscosity(const GridDev &g, const ust_material &M, co
         const ust_rheology &rh = M.rh[r];
         if (rh.type == UST_RH_MATERIALVISCOSITY) {
             eta = rh.p[0];
-        } else if (rh.type == UST_RH_ARRHENIUS) {
+        } else if (NEED_T && rh.type == UST_RH_ARRHENIUS) {
             double Tp = 0.0, y = 0.0;
 #pragma unroll
             for (int a = 0; a < NV; a++) {
@@ -47,7 +47,7 @@ __device__ double particle_viscosity(const GridDev &g, const ust_material &M, co
         } else if (rh.type == UST_RH_VISCOSITYLIMITER) {
             if (eta > rh.p[0]) eta = rh.p[0];
             if (eta < rh.p[1]) eta = rh.p[1];
-        } else if (rh.type == UST_RH_VONMISES || rh.type == UST_RH_DRUCKERPRAGER) {
+        } else if (NEED_VEL && (rh.type == UST_RH_VONMISES || rh.type == UST_RH_DRUCKERPRAGER)) {
             const bool dp = rh.type == UST_RH_DRUCKERPRAGER;
             const double minVisc = dp ? rh.p[3] : rh.p[1];
             const int srSoft = (int)(dp ? rh.p[4] : rh.p[2]);
@@ -120,40 +120,34 @@ __device__ double particle_viscosity(const GridDev &g, const ust_material &M, co
 
 // ------------------------------------------------------------------------------------------
 // particles -> weights.  A wavefront covers 64/S consecutive elements; lane k of a segment takes
-// particles k, k+S, ... of its element (coalesced: particles are cell-sorted), then the segment
-// reduces its partial sums with cross-lane shuffles and one lane stores.
+// particles k, k+S, ... of its element (coalesced: particles are cell-sorted) and accumulates its
+// 43 (3D) partial sums in registers.  The wave then transposes the partials through LDS and lane t
+// sums column t of one segment, so the reduction costs one LDS write and S reads per value instead of
+// log2(S) cross-lane exchanges of every accumulator.  NEED_T / NEED_VEL compile the temperature and
+// strain-rate lookups (Arrhenius, yield) in only when a material uses them.
 // ------------------------------------------------------------------------------------------
-template <int DIM, int S>
-__global__ __launch_bounds__(256) void k_particle_weights(GridDev g, const int *__restrict__ cell_off,
-                                                          const double *__restrict__ xi0,
-                                                          const double *__restrict__ xi1,
-                                                          const double *__restrict__ xi2,
-                                                          const double *__restrict__ w,
-                                                          const unsigned char *__restrict__ mat,
-                                                          const float *__restrict__ coh,
-                                                          const ust_material *__restrict__ mats, int nmat,
-                                                          const double *__restrict__ T,
-                                                          const double *__restrict__ vel,
-                                                          const double *__restrict__ pres, int prev,
-                                                          double *__restrict__ eta_out,
-                                                          unsigned char *__restrict__ yield_out,
-                                                          double *__restrict__ WK, double *__restrict__ WG,
-                                                          double *__restrict__ WF) {
+#define PW_WAVES 2
+template <int DIM, int S, bool NEED_T, bool NEED_VEL>
+__global__ __launch_bounds__(64 * PW_WAVES) void k_particle_weights(
+    GridDev g, const int *__restrict__ cell_off, const double *__restrict__ xi0, const double *__restrict__ xi1,
+    const double *__restrict__ xi2, const double *__restrict__ w, const unsigned char *__restrict__ mat,
+    const float *__restrict__ coh, const ust_material *__restrict__ mats, int nmat, const double *__restrict__ T,
+    const double *__restrict__ vel, const double *__restrict__ pres, int prev, double *__restrict__ eta_out,
+    unsigned char *__restrict__ yield_out, double *__restrict__ WK, double *__restrict__ WG,
+    double *__restrict__ WF) {
     constexpr int NQ = DIM == 3 ? 27 : 9;
     constexpr int NV = 1 << DIM;
-    const int lane = threadIdx.x & 63;
+    constexpr int NA = NQ + 2 * NV;  // accumulators per lane
+    constexpr int LD = NA + 1;       // padded LDS row
+    __shared__ double red[PW_WAVES][64 * LD];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int seg = lane / S, k = lane % S;
-    const int e = (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / S) + seg);
-    const bool valid = e < g.nel;
-    double aK[NQ], aG[NV], aF[NV];
+    const int ebase = (blockIdx.x * PW_WAVES + wv) * (64 / S);
+    const int e = ebase + seg;
+    double acc[NA];
 #pragma unroll
-    for (int q = 0; q < NQ; q++) aK[q] = 0.0;
-#pragma unroll
-    for (int b = 0; b < NV; b++) {
-        aG[b] = 0.0;
-        aF[b] = 0.0;
-    }
-    if (valid) {
+    for (int t = 0; t < NA; t++) acc[t] = 0.0;
+    if (e < g.nel) {
         const int ie = e % g.nx, je = (e / g.nx) % g.ny, ke = e / (g.nx * g.ny);
         const int n0 = ie + g.nnx * (je + g.nny * ke);
         int nodes[NV];
@@ -170,52 +164,49 @@ __global__ __launch_bounds__(256) void k_particle_weights(GridDev g, const int *
             const int m = mat ? (int)mat[pi] : 0;
             const ust_material &M = mats[m < nmat ? m : 0];
             int yf = -1;
-            const double eta =
-                particle_viscosity<DIM>(g, M, nodes, xi, T, vel, pres, e, coh ? coh[pi] : 1.0f, prev, &yf);
+            const double eta = particle_viscosity<DIM, NEED_T, NEED_VEL>(g, M, nodes, xi, T, vel, pres, e,
+                                                                         coh ? coh[pi] : 1.0f, prev, &yf);
             if (eta_out) eta_out[pi] = eta;
-            if (yield_out && yf >= 0) yield_out[pi] = (unsigned char)yf;
+            if (NEED_VEL && yield_out && yf >= 0) yield_out[pi] = (unsigned char)yf;
             const double wd = w[pi] * detJ;
             const double et = wd * eta;
-            const double rho = M.density;
+            const double wr = wd * M.density;
             double Lx[3], Ly[3], Lz[3];
             lagrange3(xi[0], Lx);
             lagrange3(xi[1], Ly);
             if (DIM == 3) lagrange3(xi[2], Lz);
 #pragma unroll
-            for (int q = 0; q < NQ; q++) {
-                const int qx = q % 3, qy = (q / 3) % 3, qz = q / 9;
-                double v = et * Lx[qx] * Ly[qy];
-                if (DIM == 3) v *= Lz[qz];
-                aK[q] += v;
-            }
+            for (int qz = 0; qz < (DIM == 3 ? 3 : 1); qz++)
+#pragma unroll
+                for (int qy = 0; qy < 3; qy++) {
+                    const double eyz = et * Ly[qy] * (DIM == 3 ? Lz[qz] : 1.0);
+#pragma unroll
+                    for (int qx = 0; qx < 3; qx++) acc[qx + 3 * qy + 9 * qz] += eyz * Lx[qx];
+                }
+            double Nx[2] = {0.5 * (1.0 - xi[0]), 0.5 * (1.0 + xi[0])};
+            double Ny[2] = {0.5 * (1.0 - xi[1]), 0.5 * (1.0 + xi[1])};
+            double Nz[2] = {0.5 * (1.0 - xi[2]), 0.5 * (1.0 + xi[2])};
 #pragma unroll
             for (int b = 0; b < NV; b++) {
-                double N = 1.0;
-#pragma unroll
-                for (int d = 0; d < DIM; d++) N *= 0.5 * (1.0 + sgn((b >> d) & 1) * xi[d]);
-                aG[b] += wd * N;
-                aF[b] += wd * rho * N;
+                double N = Nx[b & 1] * Ny[(b >> 1) & 1];
+                if (DIM == 3) N *= Nz[(b >> 2) & 1];
+                acc[NQ + b] += wd * N;
+                acc[NQ + NV + b] += wr * N;
             }
         }
     }
+    double *r = red[wv];
 #pragma unroll
-    for (int off = S / 2; off > 0; off >>= 1) {
-#pragma unroll
-        for (int q = 0; q < NQ; q++) aK[q] += __shfl_xor(aK[q], off, S);
-#pragma unroll
-        for (int b = 0; b < NV; b++) {
-            aG[b] += __shfl_xor(aG[b], off, S);
-            aF[b] += __shfl_xor(aF[b], off, S);
-        }
-    }
-    if (valid && k == 0) {
-#pragma unroll
-        for (int q = 0; q < NQ; q++) WK[widx(g, e, q)] = aK[q];
-#pragma unroll
-        for (int b = 0; b < NV; b++) {
-            WG[(size_t)b * g.nel + e] = aG[b];
-            WF[(size_t)b * g.nel + e] = aF[b];
-        }
+    for (int t = 0; t < NA; t++) r[lane * LD + t] = acc[t];
+    __syncthreads();
+    for (int t = lane; t < (64 / S) * NA; t += 64) {
+        const int sg = t / NA, col = t % NA, el = ebase + sg;
+        if (el >= g.nel) continue;
+        double sum = 0.0;
+        for (int row = sg * S; row < sg * S + S; row++) sum += r[row * LD + col];
+        if (col < NQ) WK[widx(g, el, col)] = sum;
+        else if (col < NQ + NV) WG[(size_t)(col - NQ) * g.nel + el] = sum;
+        else WF[(size_t)(col - NQ - NV) * g.nel + el] = sum;
     }
 }
 
@@ -380,24 +371,34 @@ extern "C" int ust_k_particle_weights(const ust_grid *gr, const int *cell_off, i
     if (nmat < 1 || nmat > UST_MAX_MATERIALS) return UST_EINVAL;
     GridDev g = grid_dev(gr);
     if (g.nel <= 0) return UST_OK;
+    // temperature / velocity lookups only when a rheology reads them (the caller passes NULL otherwise)
+    const bool needT = temperature != nullptr, needV = velocity != nullptr && pressure != nullptr;
     // segment width: the smallest of 16/32/64 that covers the typical cell in one pass
     const int S = max_ppc <= 16 ? 16 : (max_ppc <= 32 ? 32 : 64);
-    const int elems_per_block = (UST_BLOCK / 64) * (64 / S);
+    const int elems_per_block = PW_WAVES * (64 / S);
     const unsigned grid = ust_grid_size(g.nel, elems_per_block);
     hipStream_t st = ust_stream(stream);
-#define LAUNCH_PW(D, SS)                                                                                        \
-    hipLaunchKernelGGL((k_particle_weights<D, SS>), dim3(grid), dim3(UST_BLOCK), 0, st, g, cell_off, xi0, xi1, \
-                       xi2, w, mat, coh_scale, materials_dev, nmat, temperature, velocity, pressure,            \
+#define LAUNCH_PW(D, SS, NT, NVL)                                                                                \
+    hipLaunchKernelGGL((k_particle_weights<D, SS, NT, NVL>), dim3(grid), dim3(64 * PW_WAVES), 0, st, g, cell_off, \
+                       xi0, xi1, xi2, w, mat, coh_scale, materials_dev, nmat, temperature, velocity, pressure,     \
                        prev_solution, eta_out, yield_out, WK, WG, WF)
-    if (g.dim == 3) {
-        if (S == 16) LAUNCH_PW(3, 16);
-        else if (S == 32) LAUNCH_PW(3, 32);
-        else LAUNCH_PW(3, 64);
-    } else {
-        if (S == 16) LAUNCH_PW(2, 16);
-        else if (S == 32) LAUNCH_PW(2, 32);
-        else LAUNCH_PW(2, 64);
-    }
+#define LAUNCH_S(D, NT, NVL)                   \
+    do {                                       \
+        if (S == 16) LAUNCH_PW(D, 16, NT, NVL); \
+        else if (S == 32) LAUNCH_PW(D, 32, NT, NVL); \
+        else LAUNCH_PW(D, 64, NT, NVL);        \
+    } while (0)
+#define LAUNCH_F(D)                                      \
+    do {                                                 \
+        if (needT && needV) LAUNCH_S(D, true, true);     \
+        else if (needT) LAUNCH_S(D, true, false);        \
+        else if (needV) LAUNCH_S(D, false, true);        \
+        else LAUNCH_S(D, false, false);                  \
+    } while (0)
+    if (g.dim == 3) LAUNCH_F(3);
+    else LAUNCH_F(2);
+#undef LAUNCH_F
+#undef LAUNCH_S
 #undef LAUNCH_PW
     UST_LAUNCH_CHECK();
     return UST_OK;

@@ -126,26 +126,38 @@ __device__ __forceinline__ double g_entry(const GridDev &g, const double *wv, in
     return -(double)sgn((a >> i) & 1) / h * (wv[a & ~(1 << i)] + wv[a | (1 << i)]);
 }
 
+// one thread per node: y[n, i] = sum over the <= 2^DIM elements around n of G_e[a, i] p_e, a being the
+// node's vertex index in e.  Every dof is written once (0 on Dirichlet dofs) in a fixed order.
 template <int DIM>
 __global__ __launch_bounds__(256) void k_apply_G(GridDev g, const double *__restrict__ WG,
                                                  const unsigned char *__restrict__ mask, const double *__restrict__ p,
                                                  double *__restrict__ y) {
     constexpr int NV = 1 << DIM;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= g.nel) return;
-    int nodes[NV];
-    elem_nodes<DIM>(g, e, nodes);
-    double wv[NV];
+    const size_t nnodes = (size_t)g.nnx * g.nny * g.nnz;
+    const size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (n >= nnodes) return;
+    const int gx = (int)(n % g.nnx), gy = (int)((n / g.nnx) % g.nny), gz = (int)(n / ((size_t)g.nnx * g.nny));
+    double acc[DIM];
 #pragma unroll
-    for (int b = 0; b < NV; b++) wv[b] = WG[(size_t)b * g.nel + e];
-    const double pe = p[e];
+    for (int i = 0; i < DIM; i++) acc[i] = 0.0;
 #pragma unroll
-    for (int a = 0; a < NV; a++)
+    for (int a = 0; a < NV; a++) {
+        const int ex = gx - (a & 1), ey = gy - ((a >> 1) & 1), ez = DIM == 3 ? gz - (a >> 2) : 0;
+        if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+        const size_t e = (size_t)ex + (size_t)g.nx * ((size_t)ey + (size_t)g.ny * ez);
+        const double pe = p[e], wa = WG[(size_t)a * g.nel + e];
 #pragma unroll
         for (int i = 0; i < DIM; i++) {
-            const size_t dof = (size_t)nodes[a] * DIM + i;
-            if (!mask[dof]) unsafeAtomicAdd(&y[dof], g_entry<DIM>(g, wv, a, i) * pe);
+            const double h = i == 0 ? g.hx : (i == 1 ? g.hy : g.hz);
+            const double gai = -(double)sgn((a >> i) & 1) / h * (wa + WG[(size_t)(a ^ (1 << i)) * g.nel + e]);
+            acc[i] += gai * pe;
         }
+    }
+#pragma unroll
+    for (int i = 0; i < DIM; i++) {
+        const size_t dof = n * DIM + i;
+        y[dof] = mask[dof] ? 0.0 : acc[i];
+    }
 }
 
 // MODE 0: q_e = sum_free G_e u ;  MODE 1: q_e -= sum_bc G_e u  (continuity-RHS lifting)
@@ -178,19 +190,25 @@ __global__ __launch_bounds__(256) void k_apply_Gt(GridDev g, const double *__res
     else q[e] = s;
 }
 
+// one thread per node: f[n, y] += -g sum over the elements around n of WF[a][e]  (free dofs only)
 template <int DIM>
 __global__ void k_force(GridDev g, const double *__restrict__ WF, double gravity,
                         const unsigned char *__restrict__ mask, double *__restrict__ f) {
     constexpr int NV = 1 << DIM;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= g.nel) return;
-    int nodes[NV];
-    elem_nodes<DIM>(g, e, nodes);
+    const size_t nnodes = (size_t)g.nnx * g.nny * g.nnz;
+    const size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (n >= nnodes) return;
+    const size_t dof = n * DIM + 1;
+    if (mask[dof]) return;
+    const int gx = (int)(n % g.nnx), gy = (int)((n / g.nnx) % g.nny), gz = (int)(n / ((size_t)g.nnx * g.nny));
+    double acc = 0.0;
 #pragma unroll
     for (int a = 0; a < NV; a++) {
-        const size_t dof = (size_t)nodes[a] * DIM + 1;
-        if (!mask[dof]) unsafeAtomicAdd(&f[dof], -gravity * WF[(size_t)a * g.nel + e]);
+        const int ex = gx - (a & 1), ey = gy - ((a >> 1) & 1), ez = DIM == 3 ? gz - (a >> 2) : 0;
+        if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+        acc += WF[(size_t)a * g.nel + (size_t)ex + (size_t)g.nx * ((size_t)ey + (size_t)g.ny * ez)];
     }
+    f[dof] += -gravity * acc;
 }
 
 template <int DIM>
@@ -484,12 +502,12 @@ int ust_k_apply_G(const ust_grid *gr, const double *WG, const unsigned char *mas
     if (!WG || !mask || !p || !y) return UST_EINVAL;
     const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * g.dim;
     hipStream_t st = ust_stream(stream);
-    hipLaunchKernelGGL(k_fill<double>, dim3(ust_grid_size(ndof, UST_BLOCK)), dim3(UST_BLOCK), 0, st, y, 0.0, ndof);
+    const size_t nnodes = ndof / g.dim;
     if (g.dim == 3)
-        hipLaunchKernelGGL(k_apply_G<3>, dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WG, mask,
+        hipLaunchKernelGGL(k_apply_G<3>, dim3(ust_grid_size(nnodes, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WG, mask,
                            p, y);
     else
-        hipLaunchKernelGGL(k_apply_G<2>, dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WG, mask,
+        hipLaunchKernelGGL(k_apply_G<2>, dim3(ust_grid_size(nnodes, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WG, mask,
                            p, y);
     UST_LAUNCH_CHECK();
     return UST_OK;
@@ -518,7 +536,7 @@ int ust_k_force(const ust_grid *gr, const double *WF, double gravity, const unsi
                 void *stream) {
     GridDev g = grid_dev(gr);
     if (!WF || !mask || !f) return UST_EINVAL;
-    const unsigned grid = ust_grid_size(g.nel, UST_BLOCK);
+    const unsigned grid = ust_grid_size((size_t)g.nnx * g.nny * g.nnz, UST_BLOCK);
     if (g.dim == 3)
         hipLaunchKernelGGL(k_force<3>, dim3(grid), dim3(UST_BLOCK), 0, ust_stream(stream), g, WF, gravity, mask, f);
     else

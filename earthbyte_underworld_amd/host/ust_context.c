@@ -86,6 +86,9 @@ static void free_mesh_data(ust_context *ctx) {
     DFREE(ctx->h);
     DFREE(ctx->tmpv);
     DFREE(ctx->tmpp);
+    DFREE(ctx->kwork);
+    DFREE(ctx->WK32_0);
+    ctx->wk32_0_valid = 0;
     DFREE(ctx->vr); DFREE(ctx->vz); DFREE(ctx->vp); DFREE(ctx->vq); DFREE(ctx->vt);
     DFREE(ctx->va); DFREE(ctx->vb);
     DFREE(ctx->wr); DFREE(ctx->wz); DFREE(ctx->wp); DFREE(ctx->wq); DFREE(ctx->wx);
@@ -169,6 +172,7 @@ int ust_set_mesh(ust_context *ctx, int dim, const int nel[3], const double xmin[
     DALLOC(ctx, ctx->h, ne * sizeof(double));
     DALLOC(ctx, ctx->tmpv, nd * sizeof(double));
     DALLOC(ctx, ctx->tmpp, ne * sizeof(double));
+    DALLOC(ctx, ctx->kwork, ust_k_apply_work_len(&ctx->grid) * sizeof(double));
     ctx->has_mesh = 1;
     ctx->prev_solution = 0;
     ctx->nullspace = 0;
@@ -380,10 +384,19 @@ int ust_assemble(ust_context *ctx, double gravity) {
         if (rc) return rc;
     }
     size_t nd = (size_t)ctx->ndof, ne = (size_t)ctx->nel;
+    /* the temperature and strain-rate/pressure lookups are compiled into the particle kernel only when a
+     * material's rheology chain reads them */
+    int needT = 0, needV = 0;
+    for (int m = 0; m < ctx->nmat; m++)
+        for (int r = 0; r < ctx->mats_h[m].nrh; r++) {
+            int t = ctx->mats_h[m].rh[r].type;
+            if (t == UST_RH_ARRHENIUS) needT = 1;
+            if (t == UST_RH_VONMISES || t == UST_RH_DRUCKERPRAGER) needV = 1;
+        }
     CKD(ctx, ust_k_particle_weights(&ctx->grid, ctx->cell_off, ctx->max_ppc, ctx->xi0, ctx->xi1, ctx->xi2, ctx->w,
-                                    ctx->mat, ctx->coh, ctx->mats_d, ctx->nmat, ctx->T, ctx->u, ctx->p,
-                                    ctx->prev_solution, gravity, ctx->eta, ctx->yielded, ctx->WK, ctx->WG, ctx->WF,
-                                    ctx->stream));
+                                    ctx->mat, ctx->coh, ctx->mats_d, ctx->nmat, needT ? ctx->T : NULL,
+                                    needV ? ctx->u : NULL, needV ? ctx->p : NULL, ctx->prev_solution, gravity,
+                                    ctx->eta, ctx->yielded, ctx->WK, ctx->WG, ctx->WF, ctx->stream));
     CKD(ctx, ust_k_uzawa_diag(&ctx->grid, ctx->WK, ctx->WG, ctx->approxS, ctx->stream));
     /* f = buoyancy - K_{free,bc} u_bc ; h = -G_bc^T u_bc */
     CKD(ctx, ust_k_fill_f64(ctx->f, 0.0, nd, ctx->stream));
@@ -409,6 +422,7 @@ int ust_assemble(ust_context *ctx, double gravity) {
     ctx->gravity = gravity;
     ctx->assembled = 1;
     ctx->levels_ready = 0;
+    ctx->wk32_0_valid = 0;
     ctx->cache_valid = 0;
     return UST_OK;
 }
@@ -529,20 +543,20 @@ int ust_get_rhs(ust_context *ctx, double *f, double *h) {
 int ust_apply_K_dev(ust_context *ctx, const double *u, double *y) {
     int rc = need_assembled(ctx);
     if (rc) return rc;
-    CKD(ctx, ust_k_apply_K_f64(&ctx->grid, ctx->WK, ctx->mask_d, u, y, ctx->stream));
+    CKD(ctx, ust_k_apply_K_f64(&ctx->grid, ctx->WK, ctx->mask_d, u, y, (double *)ctx->kwork, ctx->stream));
     return UST_OK;
 }
 
 int ust_apply_K_f32_dev(ust_context *ctx, const float *u, float *y) {
     int rc = need_assembled(ctx);
     if (rc) return rc;
-    if (!ctx->levels_ready || !ctx->levels_prec32) {
-        ust_solve_opts o;
-        ust_solve_opts_default(&o);
-        rc = ust_levels_setup(ctx, &o, 1);
-        if (rc) return rc;
+    size_t len = ust_k_weights_len(&ctx->grid);
+    if (!ctx->WK32_0) DALLOC(ctx, ctx->WK32_0, len * sizeof(float));
+    if (!ctx->wk32_0_valid) {
+        CKD(ctx, ust_k_copy_f64_to_f32(ctx->WK, ctx->WK32_0, len, ctx->stream));
+        ctx->wk32_0_valid = 1;
     }
-    CKD(ctx, ust_k_apply_K_f32(&ctx->grid, ctx->lev[0].WK32, ctx->mask_d, u, y, ctx->stream));
+    CKD(ctx, ust_k_apply_K_f32(&ctx->grid, ctx->WK32_0, ctx->mask_d, u, y, (float *)ctx->kwork, ctx->stream));
     return UST_OK;
 }
 
@@ -577,7 +591,7 @@ static int host_op(ust_context *ctx, const double *in, size_t nin, double *out, 
     rc = ust_dev_malloc((void **)&b, nout * sizeof(double));
     if (!rc) rc = ust_memcpy_h2d(a, in, nin * sizeof(double), ctx->stream);
     if (!rc) {
-        if (which == 0) rc = ust_k_apply_K_f64(&ctx->grid, ctx->WK, ctx->mask_d, a, b, ctx->stream);
+        if (which == 0) rc = ust_k_apply_K_f64(&ctx->grid, ctx->WK, ctx->mask_d, a, b, (double *)ctx->kwork, ctx->stream);
         else if (which == 1) rc = ust_k_apply_G(&ctx->grid, ctx->WG, ctx->mask_d, a, b, ctx->stream);
         else rc = ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, a, b, 0, ctx->stream);
     }

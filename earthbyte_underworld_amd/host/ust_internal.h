@@ -6,7 +6,7 @@
 #include "../../include/ust.h"
 
 #define UST_MAX_LEVELS 12
-#define UST_MAX_CHEBY 8
+#define UST_MAX_CHEBY 24
 
 /* device scalar slots */
 enum { SC_RZ = 0, SC_RZOLD, SC_PQ, SC_RR, SC_BB, SC_SUM, SC_T0, SC_T1, SC_T2, SC_T3,
@@ -72,6 +72,7 @@ struct ust_context {
     int nullspace; /* constant pressure found in the null space of G (ksp_pressure_nullspace.c:139-158 test) */
     double ns_ratio; /* ||G 1|| / ||1|| / ||D||_inf of the last assembly */
     double *tmpv, *tmpp; /* velocity- and pressure-sized scratch of the assembly */
+    void *kwork;         /* K u workspace (per-tile node partials), sized for the finest grid in FP64 */
     /* assembly */
     int assembled;
     double gravity;
@@ -81,6 +82,10 @@ struct ust_context {
     double *h; /* continuity RHS after lifting */
     /* multigrid hierarchy */
     int levels_ready, levels_prec32, nlev;
+    int levels_struct, levels_deg, levels_req, coarse_dense;
+    double levels_ratio;
+    float *WK32_0; /* FP32 copy of the fine weights for ust_apply_K_f32 without a hierarchy */
+    int wk32_0_valid;
     ust_level lev[UST_MAX_LEVELS];
     double *Ainv;
     int nc;

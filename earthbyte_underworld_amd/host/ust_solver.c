@@ -65,13 +65,14 @@ void ust_levels_free(ust_context *ctx) {
     ctx->nlev = 0;
     ctx->levels_ready = 0;
     ctx->levels_prec32 = 0;
+    ctx->levels_struct = 0;
 }
 
 /* ---------------- level operations (prec: 0 = FP64, 1 = FP32) ---------------- */
 static int lev_apply(ust_context *ctx, int l, int f32, const void *x, void *y) {
     ust_level *L = &ctx->lev[l];
-    if (f32) CKD(ctx, ust_k_apply_K_f32(&L->g, L->WK32, L->mask, (const float *)x, (float *)y, S(ctx)));
-    else CKD(ctx, ust_k_apply_K_f64(&L->g, L->WK64, L->mask, (const double *)x, (double *)y, S(ctx)));
+    if (f32) CKD(ctx, ust_k_apply_K_f32(&L->g, L->WK32, L->mask, (const float *)x, (float *)y, (float *)ctx->kwork, S(ctx)));
+    else CKD(ctx, ust_k_apply_K_f64(&L->g, L->WK64, L->mask, (const double *)x, (double *)y, (double *)ctx->kwork, S(ctx)));
     return UST_OK;
 }
 
@@ -103,9 +104,14 @@ static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x) {
     size_t n = (size_t)L->ndof;
     int rc;
     if (l == ctx->nlev - 1) {
-        if (f32) CKD(ctx, ust_k_dense_matvec_f32(ctx->Ainv, ctx->nc, (const float *)b, (float *)x, S(ctx)));
-        else CKD(ctx, ust_k_dense_matvec_f64(ctx->Ainv, ctx->nc, (const double *)b, (double *)x, S(ctx)));
-        return UST_OK;
+        if (ctx->coarse_dense) {
+            if (f32) CKD(ctx, ust_k_dense_matvec_f32(ctx->Ainv, ctx->nc, (const float *)b, (float *)x, S(ctx)));
+            else CKD(ctx, ust_k_dense_matvec_f64(ctx->Ainv, ctx->nc, (const double *)b, (double *)x, S(ctx)));
+            return UST_OK;
+        }
+        if (f32) CKD(ctx, ust_k_fill_f32((float *)x, 0.0f, n, S(ctx)));
+        else CKD(ctx, ust_k_fill_f64((double *)x, 0.0, n, S(ctx)));
+        return lev_smooth(ctx, l, f32, b, x, 1);
     }
     ust_level *C = &ctx->lev[l + 1];
     if (f32) CKD(ctx, ust_k_fill_f32((float *)x, 0.0f, n, S(ctx)));
@@ -150,9 +156,25 @@ static int estimate_lmax(ust_context *ctx, int l, double *lmax) {
     return UST_OK;
 }
 
-int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
-    if (ctx->levels_ready && (!need32 || ctx->levels_prec32) && ctx->lev[0].cheb_deg == o->smoother_degree)
-        return UST_OK;
+static void cheby_coeffs(ust_level *L, double lmax, double ratio, int deg) {
+    double hi = lmax, lo = hi / ratio;
+    double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta, rho = 1.0 / sigma;
+    L->lmax = lmax;
+    L->cheb_deg = deg;
+    L->c1[0] = 0.0;
+    L->c2[0] = 1.0 / theta;
+    for (int k = 1; k < deg; k++) {
+        double rn = 1.0 / (2.0 * sigma - rho);
+        L->c1[k] = rn * rho;
+        L->c2[k] = 2.0 * rn / delta;
+        rho = rn;
+    }
+}
+
+/* Structure (allocations and BC masks of every level) is built once per mesh, BC set and precision;
+ * the numeric part (Galerkin weights, FP32 copies, diagonals, coarse operator, Chebyshev bounds)
+ * after every assembly. */
+static int levels_structure(ust_context *ctx, const ust_solve_opts *o, int need32) {
     ust_levels_free(ctx);
     int dim = ctx->grid.dim, maxlev = o->mg_levels > 0 ? o->mg_levels : UST_MAX_LEVELS;
     if (maxlev > UST_MAX_LEVELS) maxlev = UST_MAX_LEVELS;
@@ -169,11 +191,8 @@ int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
         }
         nlev++;
     }
-    if (grid_ndof(&g) > 4096)
-        UST_FAIL(ctx, UST_EINVAL,
-                 "multigrid: coarsest grid has %d dofs (> 4096); use element counts with more factors of two",
-                 grid_ndof(&g));
     ctx->nlev = nlev;
+    ctx->coarse_dense = grid_ndof(&g) <= 4096;
     g = ctx->grid;
     for (int l = 0; l < nlev; l++) {
         ust_level *L = &ctx->lev[l];
@@ -188,9 +207,9 @@ int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
         } else {
             ust_level *F = &ctx->lev[l - 1];
             DALLOC(ctx, L->WK64, ust_k_weights_len(&L->g) * sizeof(double));
-            CKD(ctx, ust_k_coarsen_weights(&F->g, F->WK64, L->WK64, S(ctx)));
             /* a coarse node coincides with fine node (2I, 2J, 2K): it inherits that node's BC flags */
             L->mask_h = (unsigned char *)calloc((size_t)L->ndof, 1);
+            if (!L->mask_h) UST_FAIL(ctx, UST_ENOMEM, "host allocation failed");
             int fnx = F->g.nel[0] + 1, fny = F->g.nel[1] + 1, cnx = g.nel[0] + 1, cny = g.nel[1] + 1;
             for (int n = 0; n < L->nnodes; n++) {
                 int I = n % cnx, J = (n / cnx) % cny, K = n / (cnx * cny);
@@ -209,50 +228,61 @@ int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
         if (need32) {
             DALLOC(ctx, L->invd32, n * sizeof(float));
             DALLOC(ctx, L->WK32, ust_k_weights_len(&L->g) * sizeof(float));
-            CKD(ctx, ust_k_copy_f64_to_f32(L->WK64, L->WK32, ust_k_weights_len(&L->g), S(ctx)));
             DALLOC(ctx, L->x32, n * sizeof(float));
             DALLOC(ctx, L->b32, n * sizeof(float));
             DALLOC(ctx, L->ax32, n * sizeof(float));
             DALLOC(ctx, L->d32, n * sizeof(float));
         }
-        CKD(ctx, ust_k_inv_diag(&L->g, L->WK64, L->mask, L->invd64, L->invd32, S(ctx)));
         if (l + 1 < nlev)
             for (int d = 0; d < dim; d++) {
                 g.nel[d] /= 2;
                 g.h[d] *= 2.0;
             }
     }
-    /* coarsest: dense FP64 inverse */
+    if (ctx->coarse_dense) {
+        ctx->nc = ctx->lev[nlev - 1].ndof;
+        DALLOC(ctx, ctx->Ainv, (size_t)ctx->nc * ctx->nc * sizeof(double));
+    }
+    ctx->levels_prec32 = need32;
+    ctx->levels_struct = 1;
+    ctx->levels_deg = o->smoother_degree;
+    ctx->levels_req = o->mg_levels;
+    return UST_OK;
+}
+
+int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
+    int rc;
+    if (!ctx->levels_struct || (need32 && !ctx->levels_prec32) || ctx->levels_deg != o->smoother_degree ||
+        ctx->levels_req != o->mg_levels) {
+        if ((rc = levels_structure(ctx, o, need32 || ctx->levels_prec32))) return rc;
+    }
+    if (ctx->levels_ready && ctx->levels_ratio == o->cheby_ratio) return UST_OK;
+    int nlev = ctx->nlev;
+    for (int l = 0; l < nlev; l++) {
+        ust_level *L = &ctx->lev[l];
+        if (l > 0) CKD(ctx, ust_k_coarsen_weights(&ctx->lev[l - 1].g, ctx->lev[l - 1].WK64, L->WK64, S(ctx)));
+        if (L->WK32) CKD(ctx, ust_k_copy_f64_to_f32(L->WK64, L->WK32, ust_k_weights_len(&L->g), S(ctx)));
+        CKD(ctx, ust_k_inv_diag(&L->g, L->WK64, L->mask, L->invd64, L->invd32, S(ctx)));
+    }
     ust_level *Lc = &ctx->lev[nlev - 1];
-    ctx->nc = Lc->ndof;
-    DALLOC(ctx, ctx->Ainv, (size_t)ctx->nc * ctx->nc * sizeof(double));
-    CKD(ctx, ust_k_dense_assemble(&Lc->g, Lc->WK64, Lc->mask, ctx->Ainv, S(ctx)));
-    CKD(ctx, ust_k_dense_spd_inverse(ctx->Ainv, ctx->nc, S(ctx)));
-    /* Chebyshev polynomials */
+    if (ctx->coarse_dense) { /* coarsest: dense FP64 inverse */
+        CKD(ctx, ust_k_dense_assemble(&Lc->g, Lc->WK64, Lc->mask, ctx->Ainv, S(ctx)));
+        CKD(ctx, ust_k_dense_spd_inverse(ctx->Ainv, ctx->nc, S(ctx)));
+    }
     int deg = o->smoother_degree > 0 ? o->smoother_degree : 2;
     if (deg > UST_MAX_CHEBY) deg = UST_MAX_CHEBY;
     double ratio = o->cheby_ratio > 1.0 ? o->cheby_ratio : 12.0;
-    for (int l = 0; l + 1 < nlev; l++) {
-        ust_level *L = &ctx->lev[l];
+    for (int l = 0; l < nlev; l++) {
+        if (l == nlev - 1 && ctx->coarse_dense) break;
         double lm;
-        int rc = estimate_lmax(ctx, l, &lm);
-        if (rc) return rc;
-        L->lmax = 1.1 * lm;
-        double hi = L->lmax, lo = hi / ratio;
-        double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta, rho = 1.0 / sigma;
-        L->cheb_deg = deg;
-        L->c1[0] = 0.0;
-        L->c2[0] = 1.0 / theta;
-        for (int k = 1; k < deg; k++) {
-            double rn = 1.0 / (2.0 * sigma - rho);
-            L->c1[k] = rn * rho;
-            L->c2[k] = 2.0 * rn / delta;
-            rho = rn;
-        }
+        if ((rc = estimate_lmax(ctx, l, &lm))) return rc;
+        if (l == nlev - 1)  /* grid too coarse-resistant for a dense solve: a long Chebyshev polynomial */
+            cheby_coeffs(&ctx->lev[l], 1.1 * lm, 100.0, UST_MAX_CHEBY);
+        else
+            cheby_coeffs(&ctx->lev[l], 1.1 * lm, ratio, deg);
     }
-    ctx->lev[0].cheb_deg = deg;
     ctx->levels_ready = 1;
-    ctx->levels_prec32 = need32;
+    ctx->levels_ratio = o->cheby_ratio;
     CKD(ctx, ust_stream_sync(S(ctx)));
     return UST_OK;
 }
@@ -293,49 +323,40 @@ static int dotv(ust_context *ctx, const double *a, const double *b, size_t n, in
 }
 
 /* PCG with a V-cycle preconditioner; x = 0 on entry.  Stops at ||r|| <= rtol ||b||. */
-static int pcg(ust_context *ctx, int f32, const void *b, void *x, double rtol, int maxit, int *its) {
+/* FP64 PCG on K x = b with one V-cycle as the preconditioner; x = 0 on entry.  pc32: the V-cycle runs in
+ * FP32 (FP32 weights, smoothers and transfers, FP64 coarse solve) on the FP64 residual rounded to FP32;
+ * the Krylov vectors, K applies, dots and the stopping test stay FP64, so each iteration is a refinement
+ * step that restores FP64 accuracy.  Stops at ||r|| <= rtol ||b||. */
+static int pcg(ust_context *ctx, int pc32, const double *b, double *x, double rtol, int maxit, int *its) {
     size_t n = (size_t)ctx->ndof;
     double bb, rr;
     int rc;
-    void *r = f32 ? (void *)ctx->wr : (void *)ctx->vr, *z = f32 ? (void *)ctx->wz : (void *)ctx->vz;
-    void *p = f32 ? (void *)ctx->wp : (void *)ctx->vp, *q = f32 ? (void *)ctx->wq : (void *)ctx->vq;
-    if (f32) {
-        CKD(ctx, ust_k_fill_f32((float *)x, 0.0f, n, S(ctx)));
-        CKD(ctx, ust_memcpy_d2d(r, b, n * sizeof(float), S(ctx)));
-        CKD(ctx, ust_k_dot_f32((const float *)b, (const float *)b, n, ctx->partial, ctx->scal + SC_BB, S(ctx)));
-    } else {
-        CKD(ctx, ust_k_fill_f64((double *)x, 0.0, n, S(ctx)));
-        CKD(ctx, ust_memcpy_d2d(r, b, n * sizeof(double), S(ctx)));
-        CKD(ctx, ust_k_dot_f64((const double *)b, (const double *)b, n, ctx->partial, ctx->scal + SC_BB, S(ctx)));
-    }
+    double *r = ctx->vr, *z = ctx->vz, *p = ctx->vp, *q = ctx->vq;
+    CKD(ctx, ust_k_fill_f64(x, 0.0, n, S(ctx)));
+    CKD(ctx, ust_memcpy_d2d(r, b, n * sizeof(double), S(ctx)));
+    CKD(ctx, ust_k_dot_f64(b, b, n, ctx->partial, ctx->scal + SC_BB, S(ctx)));
     CKD(ctx, ust_read_scalar(ctx, SC_BB, &bb));
     *its = 0;
     if (!(bb > 0.0)) return UST_OK;
     const double tol2 = rtol * rtol * bb;
     for (int it = 0; it < maxit; it++) {
-        if ((rc = vcycle(ctx, 0, f32, r, z))) return rc;
-        if (f32) {
-            CKD(ctx, ust_k_dot_f32((const float *)r, (const float *)z, n, ctx->partial, ctx->scal + SC_RZ, S(ctx)));
-            CKD(ctx, ust_k_cg_update_p_f32((const float *)z, (float *)p, ctx->scal + SC_RZ, ctx->scal + SC_RZOLD,
-                                           it == 0, n, S(ctx)));
+        /* the two rz slots alternate: this iteration's rz, the previous one's */
+        double *rz_new = ctx->scal + ((it & 1) ? SC_RZOLD : SC_RZ);
+        double *rz_old = ctx->scal + ((it & 1) ? SC_RZ : SC_RZOLD);
+        if (pc32) {
+            ust_level *L0 = &ctx->lev[0];
+            CKD(ctx, ust_k_copy_f64_to_f32(r, L0->b32, n, S(ctx)));
+            if ((rc = vcycle(ctx, 0, 1, L0->b32, ctx->wz))) return rc;
+            CKD(ctx, ust_k_copy_f32_to_f64(ctx->wz, z, n, S(ctx)));
         } else {
-            CKD(ctx, ust_k_dot_f64((const double *)r, (const double *)z, n, ctx->partial, ctx->scal + SC_RZ, S(ctx)));
-            CKD(ctx, ust_k_cg_update_p_f64((const double *)z, (double *)p, ctx->scal + SC_RZ, ctx->scal + SC_RZOLD,
-                                           it == 0, n, S(ctx)));
+            if ((rc = vcycle(ctx, 0, 0, r, z))) return rc;
         }
-        if ((rc = lev_apply(ctx, 0, f32, p, q))) return rc;
-        if (f32) {
-            CKD(ctx, ust_k_dot_f32((const float *)p, (const float *)q, n, ctx->partial, ctx->scal + SC_PQ, S(ctx)));
-            CKD(ctx, ust_k_cg_update_xr_f32((float *)x, (float *)r, (const float *)p, (const float *)q,
-                                            ctx->scal + SC_RZ, ctx->scal + SC_PQ, n, S(ctx)));
-            CKD(ctx, ust_k_dot_f32((const float *)r, (const float *)r, n, ctx->partial, ctx->scal + SC_RR, S(ctx)));
-        } else {
-            CKD(ctx, ust_k_dot_f64((const double *)p, (const double *)q, n, ctx->partial, ctx->scal + SC_PQ, S(ctx)));
-            CKD(ctx, ust_k_cg_update_xr_f64((double *)x, (double *)r, (const double *)p, (const double *)q,
-                                            ctx->scal + SC_RZ, ctx->scal + SC_PQ, n, S(ctx)));
-            CKD(ctx, ust_k_dot_f64((const double *)r, (const double *)r, n, ctx->partial, ctx->scal + SC_RR, S(ctx)));
-        }
-        CKD(ctx, ust_memcpy_d2d(ctx->scal + SC_RZOLD, ctx->scal + SC_RZ, sizeof(double), S(ctx)));
+        CKD(ctx, ust_k_dot_f64(r, z, n, ctx->partial, rz_new, S(ctx)));
+        CKD(ctx, ust_k_cg_update_p_f64(z, p, rz_new, rz_old, it == 0, n, S(ctx)));
+        if ((rc = lev_apply(ctx, 0, 0, p, q))) return rc;
+        CKD(ctx, ust_k_dot_f64(p, q, n, ctx->partial, ctx->scal + SC_PQ, S(ctx)));
+        CKD(ctx, ust_k_cg_update_xr_f64(x, r, p, q, rz_new, ctx->scal + SC_PQ, n, S(ctx)));
+        CKD(ctx, ust_k_dot_f64(r, r, n, ctx->partial, ctx->scal + SC_RR, S(ctx)));
         CKD(ctx, ust_read_scalar(ctx, SC_RR, &rr));
         *its = it + 1;
         if (!(rr == rr)) UST_FAIL(ctx, UST_ENOTCONV, "A11 CG produced NaN (StokesBlockKSPInterface.c:32 check)");
@@ -344,37 +365,14 @@ static int pcg(ust_context *ctx, int f32, const void *b, void *x, double rtol, i
     return UST_OK;
 }
 
-/* velocity solve K x = b (FP64 result); mixed precision refines FP32 PCG solves in FP64 */
+/* velocity solve K x = b (FP64 result) */
 static int a11_solve(ust_context *ctx, const ust_solve_opts *o, const double *b, double *x, ust_solve_stats *st) {
-    size_t n = (size_t)ctx->ndof;
     int its = 0, rc;
     st->inner_solves++;
-    if (o->precision == UST_PREC_FP64) {
-        rc = pcg(ctx, 0, b, x, o->inner_rtol, o->max_inner, &its);
-        st->inner_its += its;
-        return rc;
-    }
-    double bb, rr;
-    CKD(ctx, ust_k_fill_f64(x, 0.0, n, S(ctx)));
-    CKD(ctx, ust_memcpy_d2d(ctx->va, b, n * sizeof(double), S(ctx)));
-    CKD(ctx, dotv(ctx, b, b, n, SC_T1, &bb));
-    if (!(bb > 0.0)) return UST_OK;
-    rr = bb;
-    for (int ir = 0; ir < 40; ir++) {
-        if (rr <= o->inner_rtol * o->inner_rtol * bb) break;
-        CKD(ctx, ust_k_copy_f64_to_f32(ctx->va, ctx->lev[0].b32, n, S(ctx)));
-        rc = pcg(ctx, 1, ctx->lev[0].b32, ctx->wx, o->mixed_inner_rtol, o->max_inner, &its);
-        st->inner_its += its;
-        if (rc) return rc;
-        CKD(ctx, ust_k_add_f32_to_f64(ctx->wx, x, n, S(ctx)));
-        rc = lev_apply(ctx, 0, 0, x, ctx->vt);
-        if (rc) return rc;
-        CKD(ctx, ust_k_residual_f64(b, ctx->vt, ctx->va, n, S(ctx)));
-        CKD(ctx, dotv(ctx, ctx->va, ctx->va, n, SC_T1, &rr));
-        st->refinements++;
-        if (!(rr == rr)) UST_FAIL(ctx, UST_ENOTCONV, "iterative refinement produced NaN");
-    }
-    return UST_OK;
+    rc = pcg(ctx, o->precision == UST_PREC_MIXED, b, x, o->inner_rtol, o->max_inner, &its);
+    st->inner_its += its;
+    if (o->precision == UST_PREC_MIXED) st->refinements += its;
+    return rc;
 }
 
 int ust_a11_solve_dev(ust_context *ctx, const ust_solve_opts *o, const double *b, double *x, ust_solve_stats *st) {

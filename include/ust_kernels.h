@@ -133,11 +133,14 @@ int ust_k_element_f(const ust_grid *g, const double *WF, double gravity, int fir
 int ust_k_uzawa_diag(const ust_grid *g, const double *WK, const double *WG, double *approxS, void *stream);
 
 /* --- operators.  mask: per dof (n_nodes*dim), 1 = Dirichlet.  y is overwritten. --- */
-/* y = K u on the free dofs (input and output BC dofs treated as zero) */
+/* y = K u on the free dofs (input and output BC dofs treated as zero).  work: device workspace of
+ * ust_k_apply_work_len(g) elements of the operator's type (per-tile node partial sums); one
+ * workspace may be shared by all calls on one stream. */
+size_t ust_k_apply_work_len(const ust_grid *g);
 int ust_k_apply_K_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
-                      double *y, void *stream);
+                      double *y, double *work, void *stream);
 int ust_k_apply_K_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
-                      float *y, void *stream);
+                      float *y, float *work, void *stream);
 /* same operator, one element per thread with per-dof atomic adds (reference variant for checks) */
 int ust_k_apply_K_ref_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
                           double *y, void *stream);

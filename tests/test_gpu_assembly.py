@@ -118,7 +118,7 @@ def test_yield_rheologies(oracle, dim):
     s.assign_materials([ust.Box(1, (0.0,) * dim, (0.5,) * dim), ust.Sphere(2, (0.7,) * dim, 0.2)])
     s.cohesion_noise(0.05, 99)
     rng = np.random.default_rng(1)
-    u = rng.normal(size=s.ndof)
+    u = 0.01 * rng.normal(size=s.ndof)
     p = rng.normal(size=s.nel)
     T = np.linspace(0.0, 1.0, s.nnodes)
     s.set_temperature(T)
@@ -131,7 +131,7 @@ def test_yield_rheologies(oracle, dim):
     eta, yf = s.particle_viscosity()
     assert rel_err(eta, o.eta) < 1e-13
     assert np.array_equal(yf, o.yielded)
-    assert 0.05 < yf.mean() < 0.95  # both branches exercised
+    assert 0.02 < yf.mean() < 0.98  # both branches exercised
     assert max_rel_err(s.element_matrices(), o.Ke) < TOL
 
 
