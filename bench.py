@@ -54,7 +54,7 @@ def time_on_stream(lib, stream, fn, reps):
 def solve_opts(ust, precision, learned):
     from earthbyte_underworld_amd import _lib as L
     return ust.default_opts(precision=precision, schur_pc=L.SCHUR_LEARNED if learned else L.SCHUR_UW,
-                            outer_rtol=1e-7, inner_rtol=1e-8, mixed_inner_rtol=1e-4, max_outer=2000)
+                            outer_rtol=1e-7, inner_rtol=1e-8, max_outer=2000)
 
 
 def cpu_baseline(size, ppc, nsph, seed, cpu_size, threads):

@@ -38,7 +38,7 @@ class Grid(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("schur_pc", C.c_int), ("mg_levels", C.c_int),
                 ("smoother_degree", C.c_int), ("cheby_ratio", C.c_double), ("outer_rtol", C.c_double),
-                ("inner_rtol", C.c_double), ("mixed_inner_rtol", C.c_double), ("max_outer", C.c_int),
+                ("inner_rtol", C.c_double), ("max_outer", C.c_int),
                 ("max_inner", C.c_int), ("remove_nullspace", C.c_int), ("stall_window", C.c_int),
                 ("verbose", C.c_int)]
 
@@ -134,6 +134,7 @@ _SIGS = {
     "ust_dev_free": (I, [vp]),
     "ust_memcpy_h2d": (I, [vp, vp, C.c_size_t, vp]),
     "ust_memcpy_d2h": (I, [vp, vp, C.c_size_t, vp]),
+    "ust_memcpy_d2d": (I, [vp, vp, C.c_size_t, vp]),
     "ust_k_fill_hash_f64": (I, [vp, C.c_size_t, C.c_ulonglong, vp]),
     "ust_dev_error_string": (C.c_char_p, []),
 }

@@ -28,7 +28,10 @@ __device__ __forceinline__ double elem_logeta(const GridDev &g, const double *WK
     return log10(fmax(s / vol, 1e-300));
 }
 
-// feat[e*nin + k] = log10 eta(neighbour k) - log10 eta(e), neighbours clamped at the box
+// feat[e*nin + k] = log10 eta(neighbour k) - log10 eta(e).  A neighbour outside the box reads as
+// MLP_WALL: the walls act on a cell like a much stiffer neighbour, and the constant tells the network
+// which faces are walls.
+#define MLP_WALL 3.0f
 __global__ void k_features(GridDev g, const double *__restrict__ WK, float *__restrict__ feat) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= g.nel) return;
@@ -37,10 +40,11 @@ __global__ void k_features(GridDev g, const double *__restrict__ WK, float *__re
     const double c = elem_logeta(g, WK, e);
     for (int t = 0; t < nin; t++) {
         const int di = t % 3 - 1, dj = (t / 3) % 3 - 1, dk = g.dim == 3 ? t / 9 - 1 : 0;
-        const int ii = min(max(i + di, 0), g.nx - 1), jj = min(max(j + dj, 0), g.ny - 1),
-                  kk = min(max(k + dk, 0), g.nz - 1);
-        const int en = ii + g.nx * (jj + g.ny * kk);
-        feat[(size_t)e * nin + t] = (float)(elem_logeta(g, WK, en) - c);
+        const int ii = i + di, jj = j + dj, kk = k + dk;
+        float v = MLP_WALL;
+        if (ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz)
+            v = (float)(elem_logeta(g, WK, ii + g.nx * (jj + g.ny * kk)) - c);
+        feat[(size_t)e * nin + t] = v;
     }
 }
 

@@ -1,4 +1,4 @@
-// HIP runtime plumbing behind the thin C ABI (memory, streams, events).
+// HIP runtime plumbing behind the thin C ABI (memory, streams, events, graphs).
 #include <stdio.h>
 #include <string.h>
 
@@ -112,6 +112,46 @@ int ust_event_record(void *ev, void *stream) {
 int ust_event_elapsed_ms(void *a, void *b, float *ms) {
     HIPCALL(hipEventSynchronize((hipEvent_t)b));
     HIPCALL(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return UST_OK;
+}
+
+// Stream capture of a fixed launch sequence into an instantiated HIP graph.  Only kernel launches are
+// captured by the callers (no memset nodes), so replays see exactly the captured arguments.
+int ust_graph_begin(void *stream) {
+    HIPCALL(hipStreamBeginCapture(ust_stream(stream), hipStreamCaptureModeThreadLocal));
+    return UST_OK;
+}
+
+int ust_graph_end(void *stream, void **exec) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    *exec = nullptr;
+    HIPCALL(hipStreamEndCapture(ust_stream(stream), &g));
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        ust_set_hip_error(e, __FILE__, __LINE__);
+        return UST_EDEVICE;
+    }
+    *exec = (void *)ge;
+    return UST_OK;
+}
+
+int ust_graph_abort(void *stream) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ust_stream(stream), &g);
+    if (g) hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return e == hipSuccess ? UST_OK : UST_EDEVICE;
+}
+
+int ust_graph_launch(void *exec, void *stream) {
+    HIPCALL(hipGraphLaunch((hipGraphExec_t)exec, ust_stream(stream)));
+    return UST_OK;
+}
+
+int ust_graph_destroy(void *exec) {
+    if (exec) HIPCALL(hipGraphExecDestroy((hipGraphExec_t)exec));
     return UST_OK;
 }
 

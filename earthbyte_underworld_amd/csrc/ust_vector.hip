@@ -133,6 +133,46 @@ __global__ void kv_cg_xr(T *x, T *r, const T *p, const T *q, const double *num, 
     }
 }
 
+// Device-side CG control block c: c[0] = stopping threshold on r.r, c[1] = state (0 running,
+// 1 converged, 2 NaN), c[2] = iterations taken, c[3] = beta.  Once the state leaves 0 every update
+// kernel below returns at once, so iterations launched past convergence change nothing.
+__global__ void kv_cg_start(const double *bb, double rtol2, double *c, double *rz_old) {
+    c[0] = rtol2 * bb[0];
+    c[1] = bb[0] > 0.0 ? 0.0 : 1.0;
+    c[2] = 0.0;
+    c[3] = 0.0;
+    rz_old[0] = 1.0;
+}
+
+__global__ void kv_cg_beta(const double *rz, double *rz_old, double *c) {
+    if (c[1] != 0.0) return;
+    c[3] = rz[0] / rz_old[0];
+    rz_old[0] = rz[0];
+}
+
+__global__ void kv_cg_p_guarded(const double *z, double *p, const double *c, size_t n) {
+    if (c[1] != 0.0) return;
+    const double beta = c[3];
+    GRID_STRIDE(i, n) p[i] = z[i] + beta * p[i];
+}
+
+__global__ void kv_cg_xr_guarded(double *x, double *r, const double *p, const double *q, const double *rz,
+                                 const double *pq, const double *c, size_t n) {
+    if (c[1] != 0.0) return;
+    const double a = rz[0] / pq[0];
+    GRID_STRIDE(i, n) {
+        x[i] += a * p[i];
+        r[i] -= a * q[i];
+    }
+}
+
+__global__ void kv_cg_check(const double *rr, double *c) {
+    if (c[1] != 0.0) return;
+    c[2] += 1.0;
+    if (!(rr[0] == rr[0])) c[1] = 2.0;
+    else if (rr[0] <= c[0]) c[1] = 1.0;
+}
+
 template <typename T>
 __global__ void kv_cheby(const T *b, const T *Ax, const T *invd, T *d, T *x, T c1, T c2, int first, size_t n) {
     GRID_STRIDE(i, n) {
@@ -233,6 +273,29 @@ int ust_k_cg_update_xr_f64(double *x, double *r, const double *p, const double *
 int ust_k_cg_update_xr_f32(float *x, float *r, const float *p, const float *q, const double *num, const double *den,
                            size_t n, void *stream) {
     LV(kv_cg_xr<float>, n, x, r, p, q, num, den, n);
+}
+
+int ust_k_cg_start(const double *bb, double rtol2, double *ctl, double *rz_old, void *stream) {
+    hipLaunchKernelGGL(kv_cg_start, dim3(1), dim3(1), 0, ust_stream(stream), bb, rtol2, ctl, rz_old);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+int ust_k_cg_beta(const double *rz, double *rz_old, double *ctl, void *stream) {
+    hipLaunchKernelGGL(kv_cg_beta, dim3(1), dim3(1), 0, ust_stream(stream), rz, rz_old, ctl);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+int ust_k_cg_p_guarded_f64(const double *z, double *p, const double *ctl, size_t n, void *stream) {
+    LV(kv_cg_p_guarded, n, z, p, ctl, n);
+}
+int ust_k_cg_xr_guarded_f64(double *x, double *r, const double *p, const double *q, const double *rz,
+                            const double *pq, const double *ctl, size_t n, void *stream) {
+    LV(kv_cg_xr_guarded, n, x, r, p, q, rz, pq, ctl, n);
+}
+int ust_k_cg_check(const double *rr, double *ctl, void *stream) {
+    hipLaunchKernelGGL(kv_cg_check, dim3(1), dim3(1), 0, ust_stream(stream), rr, ctl);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
 }
 
 int ust_k_cheby_step_f64(const double *b, const double *Ax, const double *invd, double *d, double *x, double c1,

@@ -34,7 +34,6 @@ void ust_solve_opts_default(ust_solve_opts *o) {
     o->cheby_ratio = 12.0;
     o->outer_rtol = 1e-7;
     o->inner_rtol = 1e-9;
-    o->mixed_inner_rtol = 1e-4;
     o->max_outer = 1000;
     o->max_inner = 500;
     o->remove_nullspace = -1;
@@ -89,7 +88,8 @@ static void free_mesh_data(ust_context *ctx) {
     DFREE(ctx->kwork);
     DFREE(ctx->WK32_0);
     ctx->wk32_0_valid = 0;
-    DFREE(ctx->vr); DFREE(ctx->vz); DFREE(ctx->vp); DFREE(ctx->vq); DFREE(ctx->vt);
+    ust_graphs_reset(ctx);
+    DFREE(ctx->vr); DFREE(ctx->vz); DFREE(ctx->vp); DFREE(ctx->vq); DFREE(ctx->vt); DFREE(ctx->vx);
     DFREE(ctx->va); DFREE(ctx->vb);
     DFREE(ctx->wr); DFREE(ctx->wz); DFREE(ctx->wp); DFREE(ctx->wq); DFREE(ctx->wx);
     DFREE(ctx->pr); DFREE(ctx->pz); DFREE(ctx->pd); DFREE(ctx->pq); DFREE(ctx->ph); DFREE(ctx->ppc);

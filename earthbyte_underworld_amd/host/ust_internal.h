@@ -10,7 +10,9 @@
 
 /* device scalar slots */
 enum { SC_RZ = 0, SC_RZOLD, SC_PQ, SC_RR, SC_BB, SC_SUM, SC_T0, SC_T1, SC_T2, SC_T3,
-       SC_O_RZ, SC_O_RZOLD, SC_O_PQ, SC_O_RR, SC_COUNT = 32 };
+       SC_O_RZ, SC_O_RZOLD, SC_O_PQ, SC_O_RR,
+       SC_CG = 16, /* 4 slots: device-controlled inner CG (threshold, state, iterations, beta) */
+       SC_COUNT = 32 };
 
 typedef struct {
     ust_grid g;
@@ -92,6 +94,9 @@ struct ust_context {
     /* solver work */
     double *partial, *scal, *scal_h;
     double *vr, *vz, *vp, *vq, *vt;   /* FP64 velocity work (inner PCG) */
+    double *vx;                       /* inner PCG iterate */
+    void *pcg_exec[2];                /* captured inner PCG iteration (FP64 / FP32 V-cycle), NULL if stale */
+    int pcg_hint;                     /* iterations of the last inner solve: first convergence check */
     float *wr, *wz, *wp, *wq, *wx;    /* FP32 velocity work (mixed inner PCG) */
     double *va, *vb;                  /* FP64 velocity work (Schur / IR) */
     double *pr, *pz, *pd, *pq, *ph, *ppc, *pprev; /* pressure work */
@@ -123,10 +128,13 @@ struct ust_context {
 
 int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32);
 void ust_levels_free(ust_context *ctx);
+void ust_graphs_reset(ust_context *ctx);
 int ust_read_scalar(ust_context *ctx, int slot, double *v);
 int ust_mlp_schur_pc(ust_context *ctx, double *pc_dev);
 void ust_mlp_free(ust_context *ctx);
 int ust_dev_alloc_zero(ust_context *ctx, void **p, size_t bytes);
+/* multigrid levels and work vectors for solves with options o (after an assembly) */
+int ust_solver_prepare(ust_context *ctx, const ust_solve_opts *o);
 /* K x = b on device vectors (levels and work vectors must be set up) */
 int ust_a11_solve_dev(ust_context *ctx, const ust_solve_opts *o, const double *b, double *x, ust_solve_stats *st);
 

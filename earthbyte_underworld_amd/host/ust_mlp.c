@@ -258,8 +258,7 @@ int ust_mlp_probe_target(ust_context *ctx, const ust_solve_opts *opts, int nprob
     ust_solve_opts o;
     if (opts) o = *opts;
     else ust_solve_opts_default(&o);
-    ust_solve_stats st;
-    int rc = ust_solve(ctx, &o, &st); /* sets up levels and work vectors; a plain solve of this field */
+    int rc = ust_solver_prepare(ctx, &o);
     if (rc) return rc;
     int ncol = ctx->grid.dim == 3 ? 27 : 9;
     size_t ne = (size_t)ctx->nel;

@@ -46,7 +46,15 @@ static int grid_nel(const ust_grid *g) {
     return n;
 }
 
+void ust_graphs_reset(ust_context *ctx) {
+    for (int k = 0; k < 2; k++) {
+        if (ctx->pcg_exec[k]) ust_graph_destroy(ctx->pcg_exec[k]);
+        ctx->pcg_exec[k] = NULL;
+    }
+}
+
 void ust_levels_free(ust_context *ctx) {
+    ust_graphs_reset(ctx);
     for (int l = 0; l < ctx->nlev; l++) {
         ust_level *L = &ctx->lev[l];
         if (l > 0) {
@@ -257,6 +265,7 @@ int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
         if ((rc = levels_structure(ctx, o, need32 || ctx->levels_prec32))) return rc;
     }
     if (ctx->levels_ready && ctx->levels_ratio == o->cheby_ratio) return UST_OK;
+    ust_graphs_reset(ctx); /* captured smoother coefficients change with the operator */
     int nlev = ctx->nlev;
     for (int l = 0; l < nlev; l++) {
         ust_level *L = &ctx->lev[l];
@@ -295,6 +304,7 @@ static int ensure_work(ust_context *ctx, int need32) {
         DALLOC(ctx, ctx->vp, n * sizeof(double));
         DALLOC(ctx, ctx->vq, n * sizeof(double));
         DALLOC(ctx, ctx->vt, n * sizeof(double));
+        DALLOC(ctx, ctx->vx, n * sizeof(double));
         DALLOC(ctx, ctx->va, n * sizeof(double));
         DALLOC(ctx, ctx->vb, n * sizeof(double));
         DALLOC(ctx, ctx->uprev, n * sizeof(double));
@@ -322,47 +332,84 @@ static int dotv(ust_context *ctx, const double *a, const double *b, size_t n, in
     return UST_OK;
 }
 
-/* PCG with a V-cycle preconditioner; x = 0 on entry.  Stops at ||r|| <= rtol ||b||. */
-/* FP64 PCG on K x = b with one V-cycle as the preconditioner; x = 0 on entry.  pc32: the V-cycle runs in
- * FP32 (FP32 weights, smoothers and transfers, FP64 coarse solve) on the FP64 residual rounded to FP32;
- * the Krylov vectors, K applies, dots and the stopping test stay FP64, so each iteration is a refinement
- * step that restores FP64 accuracy.  Stops at ||r|| <= rtol ||b||. */
+/* One iteration of FP64 PCG on K x = b with one V-cycle as the preconditioner, on the context's work
+ * vectors (x in vx).  pc32: the V-cycle runs in FP32 (FP32 weights, smoothers and transfers, FP64 coarse
+ * solve) on the FP64 residual rounded to FP32; the Krylov vectors, K applies, dots and the stopping test
+ * stay FP64, so each iteration is a refinement step that restores FP64 accuracy.  The scalars and the
+ * stopping test live on the device (SC_CG block), so the sequence has no host round trip and is captured
+ * once into a HIP graph. */
+static int pcg_iteration(ust_context *ctx, int pc32) {
+    size_t n = (size_t)ctx->ndof;
+    double *r = ctx->vr, *z = ctx->vz, *p = ctx->vp, *q = ctx->vq, *sc = ctx->scal;
+    int rc;
+    if (pc32) {
+        ust_level *L0 = &ctx->lev[0];
+        CKD(ctx, ust_k_copy_f64_to_f32(r, L0->b32, n, S(ctx)));
+        if ((rc = vcycle(ctx, 0, 1, L0->b32, ctx->wz))) return rc;
+        CKD(ctx, ust_k_copy_f32_to_f64(ctx->wz, z, n, S(ctx)));
+    } else {
+        if ((rc = vcycle(ctx, 0, 0, r, z))) return rc;
+    }
+    CKD(ctx, ust_k_dot_f64(r, z, n, ctx->partial, sc + SC_RZ, S(ctx)));
+    CKD(ctx, ust_k_cg_beta(sc + SC_RZ, sc + SC_RZOLD, sc + SC_CG, S(ctx)));
+    CKD(ctx, ust_k_cg_p_guarded_f64(z, p, sc + SC_CG, n, S(ctx)));
+    if ((rc = lev_apply(ctx, 0, 0, p, q))) return rc;
+    CKD(ctx, ust_k_dot_f64(p, q, n, ctx->partial, sc + SC_PQ, S(ctx)));
+    CKD(ctx, ust_k_cg_xr_guarded_f64(ctx->vx, r, p, q, sc + SC_RZ, sc + SC_PQ, sc + SC_CG, n, S(ctx)));
+    CKD(ctx, ust_k_dot_f64(r, r, n, ctx->partial, sc + SC_RR, S(ctx)));
+    CKD(ctx, ust_k_cg_check(sc + SC_RR, sc + SC_CG, S(ctx)));
+    return UST_OK;
+}
+
+static int pcg_graph(ust_context *ctx, int pc32) {
+    if (ctx->pcg_exec[pc32]) return UST_OK;
+    CKD(ctx, ust_graph_begin(S(ctx)));
+    int rc = pcg_iteration(ctx, pc32);
+    if (rc) {
+        ust_graph_abort(S(ctx));
+        return rc;
+    }
+    CKD(ctx, ust_graph_end(S(ctx), &ctx->pcg_exec[pc32]));
+    return UST_OK;
+}
+
+/* PCG: x = K^-1 b to ||r|| <= rtol ||b|| from x = 0 (KSPSolve with KSPCG, StokesBlockKSPInterface.c:242).
+ * The iterations run as graph launches; the host reads the device state after the number of iterations
+ * the previous inner solve needed, then after every further one. */
 static int pcg(ust_context *ctx, int pc32, const double *b, double *x, double rtol, int maxit, int *its) {
     size_t n = (size_t)ctx->ndof;
-    double bb, rr;
+    double *sc = ctx->scal;
     int rc;
-    double *r = ctx->vr, *z = ctx->vz, *p = ctx->vp, *q = ctx->vq;
-    CKD(ctx, ust_k_fill_f64(x, 0.0, n, S(ctx)));
-    CKD(ctx, ust_memcpy_d2d(r, b, n * sizeof(double), S(ctx)));
-    CKD(ctx, ust_k_dot_f64(b, b, n, ctx->partial, ctx->scal + SC_BB, S(ctx)));
-    CKD(ctx, ust_read_scalar(ctx, SC_BB, &bb));
-    *its = 0;
-    if (!(bb > 0.0)) return UST_OK;
-    const double tol2 = rtol * rtol * bb;
-    for (int it = 0; it < maxit; it++) {
-        /* the two rz slots alternate: this iteration's rz, the previous one's */
-        double *rz_new = ctx->scal + ((it & 1) ? SC_RZOLD : SC_RZ);
-        double *rz_old = ctx->scal + ((it & 1) ? SC_RZ : SC_RZOLD);
-        if (pc32) {
-            ust_level *L0 = &ctx->lev[0];
-            CKD(ctx, ust_k_copy_f64_to_f32(r, L0->b32, n, S(ctx)));
-            if ((rc = vcycle(ctx, 0, 1, L0->b32, ctx->wz))) return rc;
-            CKD(ctx, ust_k_copy_f32_to_f64(ctx->wz, z, n, S(ctx)));
-        } else {
-            if ((rc = vcycle(ctx, 0, 0, r, z))) return rc;
-        }
-        CKD(ctx, ust_k_dot_f64(r, z, n, ctx->partial, rz_new, S(ctx)));
-        CKD(ctx, ust_k_cg_update_p_f64(z, p, rz_new, rz_old, it == 0, n, S(ctx)));
-        if ((rc = lev_apply(ctx, 0, 0, p, q))) return rc;
-        CKD(ctx, ust_k_dot_f64(p, q, n, ctx->partial, ctx->scal + SC_PQ, S(ctx)));
-        CKD(ctx, ust_k_cg_update_xr_f64(x, r, p, q, rz_new, ctx->scal + SC_PQ, n, S(ctx)));
-        CKD(ctx, ust_k_dot_f64(r, r, n, ctx->partial, ctx->scal + SC_RR, S(ctx)));
-        CKD(ctx, ust_read_scalar(ctx, SC_RR, &rr));
-        *its = it + 1;
-        if (!(rr == rr)) UST_FAIL(ctx, UST_ENOTCONV, "A11 CG produced NaN (StokesBlockKSPInterface.c:32 check)");
-        if (rr <= tol2) break;
+    CKD(ctx, ust_k_fill_f64(ctx->vx, 0.0, n, S(ctx)));
+    CKD(ctx, ust_k_fill_f64(ctx->vp, 0.0, n, S(ctx)));
+    CKD(ctx, ust_memcpy_d2d(ctx->vr, b, n * sizeof(double), S(ctx)));
+    CKD(ctx, ust_k_dot_f64(b, b, n, ctx->partial, sc + SC_BB, S(ctx)));
+    CKD(ctx, ust_k_cg_start(sc + SC_BB, rtol * rtol, sc + SC_CG, sc + SC_RZOLD, S(ctx)));
+    if ((rc = pcg_graph(ctx, pc32))) return rc;
+    int done = 0, first = ctx->pcg_hint > 2 ? ctx->pcg_hint - 1 : 1;
+    double state = 0.0, count = 0.0;
+    while (done < maxit) {
+        int k = done == 0 ? first : 1;
+        if (k > maxit - done) k = maxit - done;
+        for (int j = 0; j < k; j++) CKD(ctx, ust_graph_launch(ctx->pcg_exec[pc32], S(ctx)));
+        done += k;
+        CKD(ctx, ust_memcpy_d2h(ctx->scal_h + SC_CG + 1, sc + SC_CG + 1, 2 * sizeof(double), S(ctx)));
+        state = ctx->scal_h[SC_CG + 1];
+        count = ctx->scal_h[SC_CG + 2];
+        if (state != 0.0) break;
     }
+    *its = (int)count;
+    if (state == 2.0) UST_FAIL(ctx, UST_ENOTCONV, "A11 CG produced NaN (StokesBlockKSPInterface.c:32 check)");
+    if (*its > 0) ctx->pcg_hint = *its;
+    CKD(ctx, ust_memcpy_d2d(x, ctx->vx, n * sizeof(double), S(ctx)));
     return UST_OK;
+}
+
+int ust_solver_prepare(ust_context *ctx, const ust_solve_opts *o) {
+    int need32 = o->precision == UST_PREC_MIXED, rc;
+    if (!ctx->assembled) UST_FAIL(ctx, UST_ESTATE, "assemble first");
+    if ((rc = ust_levels_setup(ctx, o, need32))) return rc;
+    return ensure_work(ctx, need32);
 }
 
 /* velocity solve K x = b (FP64 result) */

@@ -1,0 +1,124 @@
+"""Train the learned Schur-preconditioner scaling on the GPU from solves of the plain method.
+
+For every training field the plain BSSCR operators are assembled and the diagonal of the Schur complement
+S = G^T K^-1 G is probed with the 3^dim colouring (one A11 multigrid-CG solve per colour).  The target of a
+pressure cell is log10(diag(S)_e / approxS_e), approxS being the "uw" diagonal g^T diag(K_e)^-1 g of
+UzawaPreconditionerTerm.c:195-212.  The network (27 -> 64 -> 64 -> 1, BF16 weights) is then trained with
+Adam by the library's own forward/backward kernels, and evaluated on held-out seeds by the outer BSSCR
+iteration count against the plain approxS preconditioner.
+
+    python -m earthbyte_underworld_amd.train_schur_mlp --size 32 --train 24 --heldout 4
+
+writes earthbyte_underworld_amd/data/schur_mlp_3d.npz (w, b, and a JSON training record).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if os.path.dirname(HERE) not in sys.path:
+    sys.path.insert(0, os.path.dirname(HERE))
+
+from earthbyte_underworld_amd import _lib as L  # noqa: E402
+from earthbyte_underworld_amd import configs  # noqa: E402
+from earthbyte_underworld_amd.stokes import default_opts  # noqa: E402
+
+DEFAULT_OUT = os.path.join(HERE, "data", "schur_mlp_3d.npz")
+
+
+def field(kind, n, seed, ppc):
+    """kind 'spheres': 40 seeded spheres (log10 eta U[-2,4]); 'grf': Gaussian random field mapped to [-3,3]."""
+    if kind == "spheres":
+        return configs.spheres_3d(nel=(n, n, n), ppc=ppc, nsph=40, seed=seed)
+    return configs.random_field_3d(nel=(n, n, n // 2), ppc=ppc, seed=seed)
+
+
+def probe(s, inner_rtol):
+    s.assemble()
+    o = default_opts(precision=L.PREC_FP64, inner_rtol=inner_rtol)
+    return s.mlp_features(), s.mlp_probe_target(o)
+
+
+def outer_its(s, learned, rtol=1e-7):
+    o = default_opts(precision=L.PREC_FP64, schur_pc=L.SCHUR_LEARNED if learned else L.SCHUR_UW,
+                     outer_rtol=rtol, inner_rtol=1e-8, max_outer=3000, stall_window=0)
+    st = s.solve(o)
+    return st
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--size", type=int, default=32)
+    ap.add_argument("--ppc", type=int, default=12)
+    ap.add_argument("--train", type=int, default=24, help="training fields (half spheres, half random fields)")
+    ap.add_argument("--heldout", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--inner-rtol", type=float, default=1e-7)
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    a = ap.parse_args(argv)
+
+    t0 = time.time()
+    X, Y = [], []
+    kinds = ["spheres", "grf"]
+    for k in range(a.train):
+        kind = kinds[k % 2]
+        s = field(kind, a.size, 1000 + k, a.ppc)
+        f, t = probe(s, a.inner_rtol)
+        s.close()
+        X.append(f)
+        Y.append(t)
+        print(f"[train] field {k} ({kind}): target mean {t.mean():+.3f} std {t.std():.3f} "
+              f"({time.time() - t0:.1f}s)", flush=True)
+    X = np.concatenate(X)
+    Y = np.concatenate(Y)
+    t_probe = time.time() - t0
+
+    s = field("spheres", a.size, 1, a.ppc)
+    s.mlp_init(seed=a.seed)
+    t1 = time.time()
+    losses = []
+    lr = a.lr
+    for stage in range(3):
+        losses.append(s.mlp_train(X, Y, epochs=a.epochs // 3, batch=a.batch, lr=lr, seed=a.seed + stage))
+        print(f"[train] stage {stage}: lr {lr:.1e} loss {losses[-1]:.5f} (target variance {Y.var():.5f})",
+              flush=True)
+        lr *= 0.3
+    t_train = time.time() - t1
+    w, b = s.mlp_params()
+    s.close()
+
+    # held-out evaluation: the outer iteration count of plain vs learned BSSCR
+    evals = []
+    for k in range(a.heldout):
+        kind = kinds[k % 2]
+        s = field(kind, a.size, 5000 + k, a.ppc)
+        s.assemble()
+        s.mlp_set_params(w, b)
+        plain = outer_its(s, False)
+        learned = outer_its(s, True)
+        evals.append({"kind": kind, "seed": 5000 + k, "plain_outer": plain["outer_its"],
+                      "learned_outer": learned["outer_its"], "plain_inner": plain["inner_its"],
+                      "learned_inner": learned["inner_its"], "plain_residual": plain["stokes_residual"],
+                      "learned_residual": learned["stokes_residual"]})
+        print(f"[eval] {kind} seed {5000 + k}: outer its plain {plain['outer_its']} learned "
+              f"{learned['outer_its']}", flush=True)
+        s.close()
+    record = {"size": a.size, "ppc": a.ppc, "train_fields": a.train, "samples": int(len(Y)),
+              "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "final_loss": float(losses[-1]),
+              "target_variance": float(Y.var()), "probe_seconds": round(t_probe, 1),
+              "train_seconds": round(t_train, 1), "heldout": evals}
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    np.savez(a.out, w=w.astype(np.float32), b=b.astype(np.float32), record=np.array(json.dumps(record)))
+    print(json.dumps(record), flush=True)
+    return record
+
+
+if __name__ == "__main__":
+    main()

@@ -19,7 +19,7 @@ extern "C" {
 typedef struct ust_context ust_context;
 
 #define UST_PREC_FP64 0   /* plain: FP64 operators everywhere */
-#define UST_PREC_MIXED 1  /* FP32 inner solves + FP64 iterative refinement, outer Krylov and coarse solve */
+#define UST_PREC_MIXED 1  /* FP32 multigrid V-cycle inside FP64 inner PCG; outer Krylov, residuals and coarse solve FP64 */
 
 #define UST_SCHUR_UW 0       /* approxS = g^T diag(K)^-1 g per element ("-Q22_pc_type uw") */
 #define UST_SCHUR_LEARNED 1  /* approxS scaled by the MLP; plain "uw" remains the fallback */
@@ -32,10 +32,9 @@ typedef struct {
     double cheby_ratio;      /* lambda_max / lambda_min of the Chebyshev interval (default 12) */
     double outer_rtol;       /* "-scr_ksp_rtol" */
     double inner_rtol;       /* "-A11_ksp_rtol" */
-    double mixed_inner_rtol; /* FP32 inner PCG tolerance per refinement step */
     int max_outer;           /* "-scr_ksp_max_it" */
     int max_inner;           /* "-A11_ksp_max_it" */
-    int remove_nullspace;    /* -1: when every wall fixes its normal velocity */
+    int remove_nullspace;    /* -1: when ||G 1|| / ||1|| < 1e-7 ||D||_inf (ksp_pressure_nullspace.c:139-158); 0/1 force */
     int stall_window;        /* learned PC: fall back to plain if the residual does not halve in this many its */
     int verbose;
 } ust_solve_opts;
@@ -142,8 +141,9 @@ int ust_picard(ust_context *ctx, double gravity, double tol, int max_its, const 
                int *its_done, double *final_change, ust_solve_stats *last_stats);
 
 /* ---- learned Schur scaling (MLP 27 -> H -> H -> 1, BF16 weights, FP32 accumulation) ---- */
-/* feature = log10 of element viscosity (mean over the element's particles, from the K weights)
- * over the 3^dim neighbourhood of each pressure cell */
+/* feature k of pressure cell e = log10 eta(neighbour k) - log10 eta(e) over the 3^dim neighbourhood
+ * (k = dx+1 + 3(dy+1) + 9(dz+1)), eta the element mean viscosity recovered from the K weights; a
+ * neighbour outside the box reads 3.0 */
 int ust_mlp_init(ust_context *ctx, int hidden, unsigned long long seed);
 int ust_mlp_set_params(ust_context *ctx, int hidden, const float *w, const float *b);
 int ust_mlp_get_params(ust_context *ctx, float *w, float *b);

@@ -14,7 +14,7 @@
  *   K_e = sum_q WK[q] B(xi_q)^T D0 B(xi_q),    D0 = diag(2,..,2,1,..,1).
  * G_e (GradientStiffnessMatrixTerm.c:220-249) and f_e (BuoyancyForceTerm.c:435-448) are
  * multilinear: WG[b] = sum_p w_p detJ N_b(xi_p), WF[b] = sum_p w_p detJ rho_p N_b(xi_p).
- * Weight arrays are SoA: W[q*nel + e].
+ * WG and WF are SoA (W[b*nel + e]); WK is tile-blocked (ust_k_weights_len below).
  */
 #ifndef UST_KERNELS_H
 #define UST_KERNELS_H
@@ -108,6 +108,13 @@ int ust_event_create(void **ev);
 int ust_event_destroy(void *ev);
 int ust_event_record(void *ev, void *stream);
 int ust_event_elapsed_ms(void *ev_start, void *ev_stop, float *ms);
+/* capture the launches issued on stream between begin and end into an instantiated graph (exec);
+ * abort ends a capture that failed part-way and drops it */
+int ust_graph_begin(void *stream);
+int ust_graph_end(void *stream, void **exec);
+int ust_graph_abort(void *stream);
+int ust_graph_launch(void *exec, void *stream);
+int ust_graph_destroy(void *exec);
 const char *ust_dev_error_string(void);
 
 /* --- assembly (particle -> quadrature weights, fused per-particle rheology) --- */
@@ -222,6 +229,18 @@ int ust_k_cg_update_xr_f64(double *x, double *r, const double *p, const double *
                            const double *den, size_t n, void *stream);
 int ust_k_cg_update_xr_f32(float *x, float *r, const float *p, const float *q, const double *num,
                            const double *den, size_t n, void *stream);
+/* Device-controlled CG (no host round trip per iteration).  ctl[4] on the device: ctl[0] = stopping
+ * threshold rtol2 * b.b, ctl[1] = state (0 running, 1 converged, 2 NaN), ctl[2] = iterations, ctl[3] =
+ * beta.  start: ctl from bb = b.b and rz_old = 1 (with p = 0 the first update gives p = z).  beta:
+ * ctl[3] = rz/rz_old, rz_old = rz.  The guarded updates and check do nothing once ctl[1] != 0, so
+ * iterations launched past convergence leave x and r unchanged (KSPSolve's loop with the rnorm test
+ * of KSPConvergedDefault, evaluated on the device). */
+int ust_k_cg_start(const double *bb, double rtol2, double *ctl, double *rz_old, void *stream);
+int ust_k_cg_beta(const double *rz, double *rz_old, double *ctl, void *stream);
+int ust_k_cg_p_guarded_f64(const double *z, double *p, const double *ctl, size_t n, void *stream);
+int ust_k_cg_xr_guarded_f64(double *x, double *r, const double *p, const double *q, const double *rz,
+                            const double *pq, const double *ctl, size_t n, void *stream);
+int ust_k_cg_check(const double *rr, double *ctl, void *stream);
 
 /* Chebyshev smoother step (Jacobi-preconditioned).  r = b - y (y = A x given), then
  * d = c1*d + c2*invd.*r ; x += d  (c1 = 0 on the first step) */

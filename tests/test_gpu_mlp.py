@@ -1,0 +1,172 @@
+"""Learned Schur scaling on the GPU against the MLP oracle (oracle/mlp_oracle.py) and the FP64 operator oracle.
+
+Bars: features within 1e-5 absolute (FP32 log differences); forward within 2e-4 of the float64 oracle with
+the same BF16 weights; gradients within 1e-3 relative L2 per layer (FP32 accumulation in a different order);
+one Adam step within 1e-5 relative; probed Schur-diagonal targets within 1e-5 absolute of the dense FP64
+S = G^T K^-1 G probed the same way.  The gain test runs the shipped network on a held-out seed.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+from tests.helpers import OracleProblem, rel_err
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "earthbyte_underworld_amd",
+                       "data", "schur_mlp_3d.npz")
+
+
+def _spheres(nel, ppc=6, seed=5, nsph=6):
+    import earthbyte_underworld_amd as ust
+    s = ust.StokesSystem(3, nel)
+    s.random_particle_layout(particlesPerCell=ppc, seed=seed)
+    rng = np.random.default_rng(seed)
+    mats = [ust.RheologyMaterial([ust.MaterialViscosity(eta0=1.0)], density=0.0)]
+    shapes = []
+    for k in range(nsph):
+        mats.append(ust.RheologyMaterial([ust.MaterialViscosity(eta0=10 ** rng.uniform(-2, 4))], density=1.0))
+        shapes.append(ust.Sphere(k + 1, rng.uniform(0.2, 0.8, 3), rng.uniform(0.1, 0.3)))
+    s.set_materials(mats)
+    s.assign_materials(shapes)
+    s.set_bcs(*ust.free_slip_walls(s))
+    s.assemble()
+    return s
+
+
+def _element_log_eta(s):
+    WK, _, _ = s.weights()
+    vol = np.prod([(b - a) / n for a, b, n in zip(s.min, s.max, s.nel_axis)])
+    return np.log10(WK.sum(0) / vol)
+
+
+def test_features_match_oracle():
+    s = _spheres((7, 5, 4))
+    F = s.mlp_features()
+    Fo = M.features(_element_log_eta(s), s.nel_axis)
+    assert np.max(np.abs(F - Fo)) < 1e-5
+    assert np.sum(F == M.WALL) == np.sum(Fo == M.WALL) > 0
+    s.close()
+
+
+def test_forward_and_gradients_match_oracle():
+    s = _spheres((4, 4, 4))
+    s.mlp_init(seed=3)
+    w, b = s.mlp_params()
+    b = np.random.default_rng(0).normal(size=b.shape).astype(np.float32) * 0.1
+    s.mlp_set_params(w, b)
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(1000, 27)).astype(np.float32)
+    t = rng.normal(size=1000).astype(np.float32)
+    out = s.mlp_predict(x)
+    ref, _ = M.forward(x, w, b)
+    assert np.max(np.abs(out - ref)) < 2e-4 * max(1.0, np.max(np.abs(ref)))
+    gw, gb, loss = s.mlp_gradients(x, t)
+    lo, gwo, gbo = M.loss_and_grads(x, t, w, b)
+    assert abs(loss - lo) < 1e-4 * lo
+    nw1, nw2 = 64 * 27, 64 * 64
+    for sl in (slice(0, nw1), slice(nw1, nw1 + nw2), slice(nw1 + nw2, None)):
+        assert rel_err(gw[sl], gwo[sl]) < 1e-3
+    assert rel_err(gb, gbo) < 1e-3
+    s.close()
+
+
+def test_adam_step_matches_oracle():
+    s = _spheres((4, 4, 4))
+    s.mlp_init(seed=5)
+    w0, b0 = s.mlp_params()
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(512, 27)).astype(np.float32)
+    t = rng.normal(size=512).astype(np.float32)
+    s.mlp_train(x, t, epochs=1, batch=512, lr=1e-3, seed=1)  # one full-batch step
+    w1, b1 = s.mlp_params()
+    _, gwo, gbo = M.loss_and_grads(x, t, w0, b0)
+    wo, bo, _ = M.adam_step(w0.astype(np.float64), b0.astype(np.float64), gwo, gbo,
+                            M.adam_state(len(w0), len(b0)), lr=1e-3)
+    # the first Adam step moves each parameter by ~lr sign(g): compare the steps themselves
+    big = np.abs(gwo) > 1e-4 * np.max(np.abs(gwo))
+    assert np.max(np.abs((w1 - w0)[big] - (wo - w0)[big])) < 1e-5
+    bigb = np.abs(gbo) > 1e-4 * np.max(np.abs(gbo))
+    assert np.max(np.abs((b1 - b0)[bigb] - (bo - b0)[bigb])) < 1e-5
+    s.close()
+
+
+def test_training_fits_a_teacher_network():
+    s = _spheres((4, 4, 4))
+    rng = np.random.default_rng(7)
+    x = rng.normal(size=(8192, 27)).astype(np.float32) * 0.5
+    s.mlp_init(seed=11)
+    wt, bt = s.mlp_params()
+    t = M.forward(x, wt * 2.0, bt)[0].astype(np.float32)
+    s.mlp_init(seed=12)
+    l0 = s.mlp_gradients(x, t)[2]
+    l1 = s.mlp_train(x, t, epochs=30, batch=1024, lr=2e-3, seed=3)
+    assert l1 < 0.1 * l0, (l0, l1)
+    s.close()
+
+
+def test_probe_target_matches_dense_schur(oracle):
+    import earthbyte_underworld_amd as ust
+    from earthbyte_underworld_amd import _lib as L
+    s = _spheres((5, 4, 4), ppc=8)
+    o = OracleProblem(s)
+    gs = o.system(s.bc_mask)
+    K, G = gs.scipy()
+    import scipy.sparse.linalg as spla
+    S = np.asarray(G.T @ spla.splu(K.tocsc()).solve(G.toarray()))
+    nx, ny, nz = s.nel_axis
+    e = np.arange(s.nel)
+    col = e % nx % 3 + 3 * ((e // nx) % ny % 3) + 9 * ((e // (nx * ny)) % 3)
+    d = np.zeros(s.nel)
+    for c in range(27):
+        d[col == c] = (S @ (col == c).astype(float))[col == c]
+    ref = np.log10(d / o.approxS)
+    t = s.mlp_probe_target(ust.default_opts(precision=L.PREC_FP64, inner_rtol=1e-12, max_inner=2000))
+    assert np.max(np.abs(t - ref)) < 1e-5
+    s.close()
+
+
+def test_learned_scaling_gain_on_heldout_seed():
+    """The shipped network against plain BSSCR on a seed it never saw: fewer outer iterations, same tolerance,
+    same solution."""
+    from earthbyte_underworld_amd import _lib as L
+    from earthbyte_underworld_amd.configs import spheres_3d
+    assert os.path.exists(WEIGHTS), "trained network missing (earthbyte_underworld_amd/train_schur_mlp.py)"
+    d = np.load(WEIGHTS)
+    import earthbyte_underworld_amd as ust
+    s = spheres_3d(nel=(24, 24, 24), ppc=10, nsph=40, seed=7777)
+    s.assemble()
+    s.mlp_set_params(d["w"], d["b"])
+    kw = dict(outer_rtol=1e-7, inner_rtol=1e-9, max_outer=3000, stall_window=0)
+    plain = s.solve(ust.default_opts(precision=L.PREC_FP64, schur_pc=L.SCHUR_UW, **kw))
+    u0, p0 = s.solution()
+    learned = s.solve(ust.default_opts(precision=L.PREC_FP64, schur_pc=L.SCHUR_LEARNED, **kw))
+    u1, p1 = s.solution()
+    print(f"outer its: plain {plain['outer_its']}, learned {learned['outer_its']}")
+    assert learned["fallback_used"] == 0
+    assert learned["outer_its"] <= 0.75 * plain["outer_its"]
+    assert plain["stokes_residual"] < 1e-6 and learned["stokes_residual"] < 1e-6
+    assert rel_err(u1, u0) < 1e-5 and rel_err(p1, p0) < 1e-5
+    s.close()
+
+
+def test_fallback_to_plain_bsscr_on_stall():
+    """A network that scrambles the scaling stalls the learned run; plain BSSCR takes over from the current
+    iterate and reaches the same tolerance and solution."""
+    import earthbyte_underworld_amd as ust
+    from earthbyte_underworld_amd import _lib as L
+    s = _spheres((8, 8, 8), ppc=6, nsph=8)
+    nw, nb = s.mlp_param_counts()
+    rng = np.random.default_rng(9)
+    s.mlp_set_params(rng.normal(size=nw).astype(np.float32) * 3.0, rng.normal(size=nb).astype(np.float32))
+    kw = dict(outer_rtol=1e-7, inner_rtol=1e-9, max_outer=3000)
+    plain = s.solve(ust.default_opts(precision=L.PREC_FP64, schur_pc=L.SCHUR_UW, **kw))
+    u0, p0 = s.solution()
+    st = s.solve(ust.default_opts(precision=L.PREC_FP64, schur_pc=L.SCHUR_LEARNED, stall_window=2, **kw))
+    u1, p1 = s.solution()
+    assert st["fallback_used"] == 1
+    assert st["stokes_residual"] < 1e-6
+    assert rel_err(u1, u0) < 1e-5 and rel_err(p1, p0) < 1e-5
+    s.close()

@@ -93,3 +93,44 @@ def test_velocity_solve_matches_scipy(oracle):
     ref = spla.spsolve(K.tocsc(), b[free])
     assert rel_err(u[free], ref) < 1e-9
     assert its < 100
+
+
+def test_picard_viscoplastic_2d(oracle):
+    """Arrhenius + Drucker-Prager with cohesion noise and vx = +-1 sides (the visco-plastic configuration at
+    32x16): ust_picard against the same Picard loop driven through the oracle (re-evaluate the rheology with the
+    last u, p; re-assemble; BSSCR; stop at ||u - u_prev|| / ||u|| < tol, SystemLinearEquations.c:852-871)."""
+    from earthbyte_underworld_amd.configs import viscoplastic_2d
+    s = viscoplastic_2d(nel=(32, 16), mean_ppc=10, seed=3)
+    tol = 1e-4
+    kw = dict(outer_rtol=1e-10, inner_rtol=1e-12, max_outer=4000)
+    its, change, st = s.picard(gravity=1.0, tol=tol, max_its=40, **kw)
+    u, p = s.solution()
+    assert change < tol and st["stokes_residual"] < 1e-6
+
+    T = 1.0 - s.node_coords()[:, 1]  # the configuration's temperature
+    coh = s.cohesion_scale()
+    mesh = oracle.make_mesh(2, s.nel_axis, s.min, s.max)
+    off, xi, w, mat = s.swarm()
+    from tests.helpers import to_oracle_materials
+    mats = to_oracle_materials(s.materials)
+    rho = np.array([mats[m].density for m in mat])
+    uo = np.zeros(s.ndof)
+    po = np.zeros(s.nel)
+    ito, cho = 0, 1.0
+    for k in range(40):
+        eta, _ = oracle.eval_viscosity(mesh, mats, off, xi, mat.astype(np.int32), temperature=T, velocity=uo,
+                                       pressure=po, coh_scale=coh, previous_solution_exists=k > 0)
+        Ke, Ge, fe, aS = oracle.assemble_elements(mesh, off, xi, w, eta, rho=rho, gravity=1.0)
+        gs = oracle.GlobalSystem(mesh, s.bc_mask, s.bc_values, Ke, Ge, fe)
+        uf, pn, sto = gs.solve(aS, inner_rtol=1e-12, outer_rtol=1e-10)
+        un = gs.expand_u(uf)
+        ito = k + 1
+        if k > 0:
+            cho = np.linalg.norm(un - uo) / np.linalg.norm(un)
+        uo, po = un, pn
+        if k > 0 and cho < tol:
+            break
+    print(f"picard its gpu {its} oracle {ito}; change {change:.2e} / {cho:.2e}")
+    assert its == ito
+    assert rel_err(u, uo) < 1e-5
+    assert rel_err(p, po) < 1e-5
