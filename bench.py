@@ -51,10 +51,10 @@ def time_on_stream(lib, stream, fn, reps):
     return ms.value / reps
 
 
-def solve_opts(ust, precision, learned):
+def solve_opts(ust, precision, learned, relax=0):
     from earthbyte_underworld_amd import _lib as L
     return ust.default_opts(precision=precision, schur_pc=L.SCHUR_LEARNED if learned else L.SCHUR_UW,
-                            outer_rtol=1e-7, inner_rtol=1e-8, max_outer=2000)
+                            outer_rtol=1e-7, inner_rtol=1e-8, max_outer=2000, inner_relax=relax)
 
 
 def cpu_baseline(size, ppc, nsph, seed, cpu_size, threads):
@@ -102,6 +102,72 @@ def cpu_baseline(size, ppc, nsph, seed, cpu_size, threads):
                       f"(iteration growth with resolution ignored: an upper bound on the CPU rate)"}
 
 
+def dominant_kernel(lib, ctx, s, stream, reps=50):
+    """The fused FP32 Jacobi-Chebyshev smoothing pass of the inner solves' V-cycle on the finest grid
+    (k_applyK3_tiled<float, EPI_CHEBY> + k_gather_tiles): its average time over reps launches on the library
+    stream (HIP events) and its algorithmic HBM bytes."""
+    from earthbyte_underworld_amd import _lib as L
+    g = L.Grid()
+    WK, WG, WF, mask = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    lib.ust_get_device_views(ctx, C.byref(g), C.byref(WK), C.byref(WG), C.byref(WF), C.byref(mask))
+    wl = lib.ust_k_weights_len(C.byref(g))
+    n = s.ndof
+    bufs = {}
+    for name, nbytes in (("W", wl * 4), ("x", n * 4), ("b", n * 4), ("invd", n * 4), ("d", n * 4),
+                         ("work", lib.ust_k_apply_work_len(C.byref(g)) * 4)):
+        bufs[name] = C.c_void_p()
+        assert lib.ust_dev_malloc(C.byref(bufs[name]), nbytes) == 0
+    lib.ust_k_copy_f64_to_f32(WK, bufs["W"], wl, stream)
+    rng = np.random.default_rng(0)
+    for name in ("x", "b", "d"):
+        a = rng.normal(size=n).astype(np.float32)
+        lib.ust_memcpy_h2d(bufs[name], a.ctypes.data_as(C.c_void_p), n * 4, stream)
+    a = np.full(n, 1e-3, np.float32)  # a contracting step keeps x bounded over the repetitions
+    lib.ust_memcpy_h2d(bufs["invd"], a.ctypes.data_as(C.c_void_p), n * 4, stream)
+    ms = time_on_stream(lib, stream, lambda: lib.ust_k_apply_K_cheby_f32(
+        C.byref(g), bufs["W"], mask, bufs["x"], bufs["b"], bufs["invd"], bufs["d"], C.c_float(0.1),
+        C.c_float(0.5), 0, bufs["work"], stream), reps)
+    for p in bufs.values():
+        lib.ust_dev_free(p)
+    # weights 27 x 4 B per element; per dof: x read + write, b, invd, d read + write (4 B each), mask 1 B
+    nbytes = s.nel * 27 * 4 + n * (6 * 4 + 1)
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_applyK3_tiled<float,EPI_CHEBY> + k_gather_tiles (fused FP32 K u + Chebyshev step)",
+            "avg_launch_us": round(ms * 1e3, 2), "algorithmic_bytes_per_launch": int(nbytes)}
+
+
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel pair from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), or None when it has not been measured."""
+    path = os.path.join(ROOT, "profiles", "k_applyK3_tiled_f32_cheby_96cube_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def kernel_only(args):
+    """Runs only the dominant kernel (50 launches) and a 256 MiB device copy (20 launches) for counter
+    collection under rocprofv3 --pmc."""
+    from earthbyte_underworld_amd.configs import spheres_3d
+    n = args.size
+    s = spheres_3d(nel=(n, n, n), ppc=args.ppc, nsph=args.nsph, seed=args.seed)
+    s.assemble(gravity=1.0)
+    lib, ctx = s.lib, s.ctx
+    stream = lib.ust_get_stream(ctx)
+    r = dominant_kernel(lib, ctx, s, stream)
+    nb = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    lib.ust_dev_malloc(C.byref(a), nb)
+    lib.ust_dev_malloc(C.byref(b), nb)
+    for _ in range(20):
+        lib.ust_memcpy_d2d(b, a, nb, stream)
+    lib.ust_stream_sync(stream)
+    print(json.dumps({"kernel_only": r, "copy_bytes": nb}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,12 +181,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=16)
     ap.add_argument("--mlp", default=MLP_WEIGHTS)
+    ap.add_argument("--relax", type=int, default=1, help="relaxed inner tolerance on the mixed+learned path")
+    ap.add_argument("--kernel-only", action="store_true", help="dominant kernel + copy only (counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is design-only.",
               file=sys.stderr)
         sys.exit(2)
 
+    if args.kernel_only:
+        kernel_only(args)
+        return
     import earthbyte_underworld_amd as ust
     from earthbyte_underworld_amd import _lib as L
     from earthbyte_underworld_amd.configs import spheres_3d
@@ -137,7 +208,7 @@ def main():
         log("learned Schur scaling loaded from", os.path.relpath(args.mlp, ROOT))
     else:
         log("no trained MLP found: the mixed run uses the plain approxS scaling")
-    opts_mixed = solve_opts(ust, L.PREC_MIXED, learned)
+    opts_mixed = solve_opts(ust, L.PREC_MIXED, learned, args.relax)
     opts_plain = solve_opts(ust, L.PREC_FP64, False)
 
     def step(opts):
@@ -173,31 +244,8 @@ def main():
     t_asm = time_on_stream(lib, stream, lambda: lib.ust_assemble(ctx, C.c_double(1.0)), 3)
     np_ = lib.ust_swarm_count(ctx)
     asm_bytes = np_ * (3 * 8 + 8 + 1 + 8) + s.nel * (27 + 8 + 8) * 8
-    # ---- dominant kernel: the FP32 matrix-free K u of the inner solves ----
-    g = L.Grid()
-    WK, WG, WF, mask = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
-    lib.ust_get_device_views(ctx, C.byref(g), C.byref(WK), C.byref(WG), C.byref(WF), C.byref(mask))
-    wl = lib.ust_k_weights_len(C.byref(g))
-    W32, u32, y32, wk32 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
-    lib.ust_dev_malloc(C.byref(W32), wl * 4)
-    lib.ust_dev_malloc(C.byref(wk32), lib.ust_k_apply_work_len(C.byref(g)) * 4)
-    lib.ust_dev_malloc(C.byref(u32), s.ndof * 4)
-    lib.ust_dev_malloc(C.byref(y32), s.ndof * 4)
-    lib.ust_k_copy_f64_to_f32(WK, W32, wl, stream)
-    u0 = np.random.default_rng(0).normal(size=s.ndof).astype(np.float32)
-    lib.ust_memcpy_h2d(u32, u0.ctypes.data_as(C.c_void_p), s.ndof * 4, stream)
-    t_apply32 = time_on_stream(lib, stream,
-                               lambda: lib.ust_k_apply_K_f32(C.byref(g), W32, mask, u32, y32, wk32, stream), 50)
-    apply_bytes = s.nel * 27 * 4 + 2 * s.ndof * 4
-    lib.ust_dev_free(W32)
-    lib.ust_dev_free(u32)
-    lib.ust_dev_free(y32)
-    lib.ust_dev_free(wk32)
-    achieved = apply_bytes / (t_apply32 * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_applyK3_tiled<float> (matrix-free K u, FP32)",
-                "avg_launch_us": round(t_apply32 * 1e3, 2), "algorithmic_bytes_per_launch": apply_bytes}
+    roofline = dominant_kernel(lib, ctx, s, stream)
+    roofline["traffic"] = measured_traffic()
 
     cpu = None
     if not args.no_cpu:
@@ -221,12 +269,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner solves + f64 outer/residuals",
+        "dtype": "f32 V-cycle inside f64 inner PCG; f64 outer CG, residuals, coarse solve",
         "data": "synthetic (seeded swarm and sphere set)",
         "config": {"workload": f"3D spheres: {n}^3 Q1/P0, {args.ppc} particles/cell, {args.nsph} spheres "
                                f"log10 eta U[-2,4], free-slip, BSSCR to 1e-6 relative Stokes residual",
                    "elements": s.nel, "particles": int(np_), "velocity_dofs": s.ndof,
-                   "learned_schur_pc": learned},
+                   "learned_schur_pc": learned, "inner_tolerance_relaxed": bool(args.relax)},
         "stokes_residual": worst_res,
         "outer_its": stats[-1]["outer_its"],
         "inner_its": stats[-1]["inner_its"],

@@ -39,7 +39,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("schur_pc", C.c_int), ("mg_levels", C.c_int),
                 ("smoother_degree", C.c_int), ("cheby_ratio", C.c_double), ("outer_rtol", C.c_double),
                 ("inner_rtol", C.c_double), ("max_outer", C.c_int),
-                ("max_inner", C.c_int), ("remove_nullspace", C.c_int), ("stall_window", C.c_int),
+                ("max_inner", C.c_int), ("remove_nullspace", C.c_int), ("stall_window", C.c_int), ("inner_relax", C.c_int),
                 ("verbose", C.c_int)]
 
 
@@ -123,6 +123,8 @@ _SIGS = {
     "ust_k_apply_work_len": (C.c_size_t, [C.POINTER(Grid)]),
     "ust_k_apply_K_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp]),
     "ust_k_apply_K_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp]),
+    "ust_k_apply_K_residual_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp]),
+    "ust_k_apply_K_cheby_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, D, D, I, vp, vp]),
     "ust_k_particle_weights": (I, [C.POINTER(Grid), vp, I, vp, vp, vp, vp, vp, vp, vp, I, vp, vp, vp, I, D, vp,
                                    vp, vp, vp, vp, vp]),
     "ust_event_create": (I, [C.POINTER(vp)]),

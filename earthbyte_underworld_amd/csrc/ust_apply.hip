@@ -198,34 +198,107 @@ __device__ __forceinline__ bool on_tile_face(int gc, int T, int tn) {
     return gc > 0 && gc % T == 0 && gc / T < tn;
 }
 
+// What an apply writes for dof i whose product (K u)_i is v (v = 0 on Dirichlet dofs):
+//   EPI_Y       y_i = v
+//   EPI_RES     y_i = b_i - v                                  (multigrid residual)
+//   EPI_CHEBY   r = b_i - v; d_i = c1 d_i + c2 invd_i r; x_i += d_i with x the input u: one Jacobi-
+//               Chebyshev smoothing step in place.  In place is safe: a node interior to a tile is
+//               staged only by that tile, before its update, and tile-face nodes are updated in
+//               phase 2, after every tile has staged them.
+//   EPI_CHEBY0  the same with c1 = 0 (the previous d is not read)
+enum { EPI_Y = 0, EPI_RES = 1, EPI_CHEBY = 2, EPI_CHEBY0 = 3 };
 template <typename T>
+struct Epi {
+    T *y;  // EPI_Y / EPI_RES: output; EPI_CHEBY*: x, the input u
+    const T *b;
+    const T *invd;
+    T *d;
+    T c1, c2;
+};
+
+// operands the epilogue reads, loaded ahead of the stores so that a thread's loads overlap
+template <typename T>
+struct EpiOps {
+    unsigned char m;
+    T b, invd, d, x;
+};
+
+template <int MODE, typename T>
+__device__ __forceinline__ EpiOps<T> epilogue_load(const Epi<T> &ep, const unsigned char *mask, size_t i) {
+    EpiOps<T> o;
+    o.m = mask[i];
+    o.b = o.invd = o.d = o.x = (T)0;
+    if (MODE != EPI_Y) o.b = ep.b[i];
+    if (MODE == EPI_CHEBY || MODE == EPI_CHEBY0) {
+        o.invd = ep.invd[i];
+        o.x = ep.y[i];
+    }
+    if (MODE == EPI_CHEBY) o.d = ep.d[i];
+    return o;
+}
+
+// v: the product (K u)_i before masking
+template <int MODE, typename T>
+__device__ __forceinline__ void epilogue_store(const Epi<T> &ep, const EpiOps<T> &o, size_t i, T v) {
+    v = o.m ? (T)0 : v;
+    if (MODE == EPI_Y) {
+        ep.y[i] = v;
+    } else if (MODE == EPI_RES) {
+        ep.y[i] = o.b - v;
+    } else {
+        const T dn = (MODE == EPI_CHEBY ? ep.c1 * o.d : (T)0) + ep.c2 * o.invd * (o.b - v);
+        ep.d[i] = dn;
+        ep.y[i] = o.x + dn;
+    }
+}
+
+template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_applyK3_tiled(GridDev g, const T *__restrict__ W,
-                                                       const unsigned char *__restrict__ mask,
-                                                       const T *__restrict__ u, T *__restrict__ y,
-                                                       T *__restrict__ work) {
+                                                       const unsigned char *__restrict__ mask, const T *u,
+                                                       Epi<T> ep, T *__restrict__ work) {
     constexpr int TX = UST_TX3, TY = UST_TY3, TZ = UST_TZ3;
     constexpr int NX = TX + 1, NY = TY + 1, NZ = TZ + 1, NN = NX * NY * NZ;
     __shared__ T su[NN * 3];
-    __shared__ T sy[NN * 3];
+    __shared__ double sy[NN * 3];  // FP64 accumulation in LDS for both precisions (ds_add_f64)
     const int tile = blockIdx.x;
     const int tx0 = (tile % g.tnx) * TX, ty0 = ((tile / g.tnx) % g.tny) * TY, tz0 = (tile / (g.tnx * g.tny)) * TZ;
-    for (int t = threadIdx.x; t < NN * 3; t += blockDim.x) {
+    // Loads are issued without branches, staged velocities first, then the element weights (which do not
+    // depend on them), so the LDS stores below wait only for the staging loads and the weights arrive
+    // while the tile is being staged.  Out-of-tile slots read dof 0 and are discarded; partial tiles are
+    // padded in W, so every load is in bounds.
+    constexpr int NST = (NN * 3 + UST_TILE - 1) / UST_TILE;
+    T stv[NST];
+    bool stok[NST];
+    unsigned char stm[NST];
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = threadIdx.x + UST_TILE * r;
         const int c = t % 3, n = t / 3;
         const int gx = tx0 + n % NX, gy = ty0 + (n / NX) % NY, gz = tz0 + n / (NX * NY);
-        T v = (T)0;
-        if (gx < g.nnx && gy < g.nny && gz < g.nnz) {
-            const size_t dof = ((size_t)gx + (size_t)g.nnx * ((size_t)gy + (size_t)g.nny * gz)) * 3 + c;
-            if (!mask[dof]) v = u[dof];
+        stok[r] = t < NN * 3 && gx < g.nnx && gy < g.nny && gz < g.nnz;
+        const size_t dof = stok[r] ? ((size_t)gx + (size_t)g.nnx * ((size_t)gy + (size_t)g.nny * gz)) * 3 + c : 0;
+        stm[r] = mask[dof];
+        stv[r] = u[dof];
+    }
+    T wq[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) wq[q] = W[((size_t)tile * 27 + q) * UST_TILE + threadIdx.x];
+#pragma unroll
+    for (int r = 0; r < NST; r++) stv[r] = (stok[r] && !stm[r]) ? stv[r] : (T)0;
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = threadIdx.x + UST_TILE * r;
+        if (t < NN * 3) {
+            su[t] = stv[r];
+            sy[t] = 0.0;
         }
-        su[t] = v;
-        sy[t] = (T)0;
     }
     __syncthreads();
     const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
     const bool active = tx0 + lx < g.nx && ty0 + ly < g.ny && tz0 + lz < g.nz;
     T ye[8][3];
     if (active) {
-        T ue[8][3], wq[27];
+        T ue[8][3];
 #pragma unroll
         for (int a = 0; a < 8; a++) {
             const int ln = (lx + (a & 1)) + NX * ((ly + ((a >> 1) & 1)) + NY * (lz + (a >> 2)));
@@ -235,8 +308,6 @@ __global__ __launch_bounds__(256) void k_applyK3_tiled(GridDev g, const T *__res
                 ye[a][i] = (T)0;
             }
         }
-#pragma unroll
-        for (int q = 0; q < 27; q++) wq[q] = W[((size_t)tile * 27 + q) * UST_TILE + threadIdx.x];
         const T s[3] = {(T)(2.0 / g.hx), (T)(2.0 / g.hy), (T)(2.0 / g.hz)};
         element_apply3<T>(ue, wq, s, ye);
     }
@@ -249,53 +320,78 @@ __global__ __launch_bounds__(256) void k_applyK3_tiled(GridDev g, const T *__res
             for (int a = 0; a < 8; a++) {
                 const int ln = (lx + (a & 1)) + NX * ((ly + ((a >> 1) & 1)) + NY * (lz + (a >> 2)));
 #pragma unroll
-                for (int i = 0; i < 3; i++) atomicAdd(&sy[ln * 3 + i], ye[a][i]);
+                for (int i = 0; i < 3; i++) atomicAdd(&sy[ln * 3 + i], (double)ye[a][i]);
             }
         }
         __syncthreads();
     }
+    // writeout: every load of the thread (mask, epilogue operands) first, then the stores
     T *slot = work + (size_t)tile * NN * 3;
-    for (int t = threadIdx.x; t < NN * 3; t += blockDim.x) {
+    int kind[NST];  // 0 none, 1 tile-face node (workspace slot), 2 node of this tile only (written here)
+    size_t wdof[NST];
+    EpiOps<T> ops[NST];
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = threadIdx.x + UST_TILE * r;
         const int c = t % 3, n = t / 3;
         const int gx = tx0 + n % NX, gy = ty0 + (n / NX) % NY, gz = tz0 + n / (NX * NY);
-        if (gx >= g.nnx || gy >= g.nny || gz >= g.nnz) continue;
-        if (on_tile_face(gx, TX, g.tnx) || on_tile_face(gy, TY, g.tny) || on_tile_face(gz, TZ, g.tnz)) {
-            slot[t] = sy[t];
-        } else {
-            const size_t dof = ((size_t)gx + (size_t)g.nnx * ((size_t)gy + (size_t)g.nny * gz)) * 3 + c;
-            y[dof] = mask[dof] ? (T)0 : sy[t];
-        }
+        const bool in = t < NN * 3 && gx < g.nnx && gy < g.nny && gz < g.nnz;
+        const bool face = on_tile_face(gx, TX, g.tnx) || on_tile_face(gy, TY, g.tny) || on_tile_face(gz, TZ, g.tnz);
+        kind[r] = in ? (face ? 1 : 2) : 0;
+        wdof[r] = kind[r] == 2 ? ((size_t)gx + (size_t)g.nnx * ((size_t)gy + (size_t)g.nny * gz)) * 3 + c : 0;
+        ops[r] = epilogue_load<MODE, T>(ep, mask, wdof[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = threadIdx.x + UST_TILE * r;
+        if (kind[r] == 1) slot[t] = (T)sy[t];
+        else if (kind[r] == 2) epilogue_store<MODE, T>(ep, ops[r], wdof[r], (T)sy[t]);
     }
 }
 
-template <typename T>
+template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const T *__restrict__ W,
-                                                       const unsigned char *__restrict__ mask,
-                                                       const T *__restrict__ u, T *__restrict__ y,
-                                                       T *__restrict__ work) {
+                                                       const unsigned char *__restrict__ mask, const T *u,
+                                                       Epi<T> ep, T *__restrict__ work) {
     constexpr int TX = UST_TX2, TY = UST_TY2;
     constexpr int NX = TX + 1, NY = TY + 1, NN = NX * NY;
     __shared__ T su[NN * 2];
-    __shared__ T sy[NN * 2];
+    __shared__ double sy[NN * 2];
     const int tile = blockIdx.x;
     const int tx0 = (tile % g.tnx) * TX, ty0 = (tile / g.tnx) * TY;
-    for (int t = threadIdx.x; t < NN * 2; t += blockDim.x) {
+    constexpr int NST = (NN * 2 + UST_TILE - 1) / UST_TILE;
+    T stv[NST];
+    bool stok[NST];
+    unsigned char stm[NST];
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = threadIdx.x + UST_TILE * r;
         const int c = t % 2, n = t / 2;
         const int gx = tx0 + n % NX, gy = ty0 + n / NX;
-        T v = (T)0;
-        if (gx < g.nnx && gy < g.nny) {
-            const size_t dof = ((size_t)gx + (size_t)g.nnx * gy) * 2 + c;
-            if (!mask[dof]) v = u[dof];
+        stok[r] = t < NN * 2 && gx < g.nnx && gy < g.nny;
+        const size_t dof = stok[r] ? ((size_t)gx + (size_t)g.nnx * gy) * 2 + c : 0;
+        stm[r] = mask[dof];
+        stv[r] = u[dof];
+    }
+    T wq[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) wq[q] = W[((size_t)tile * 9 + q) * UST_TILE + threadIdx.x];
+#pragma unroll
+    for (int r = 0; r < NST; r++) stv[r] = (stok[r] && !stm[r]) ? stv[r] : (T)0;
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = threadIdx.x + UST_TILE * r;
+        if (t < NN * 2) {
+            su[t] = stv[r];
+            sy[t] = 0.0;
         }
-        su[t] = v;
-        sy[t] = (T)0;
     }
     __syncthreads();
     const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
     const bool active = tx0 + lx < g.nx && ty0 + ly < g.ny;
     T ye[4][2];
     if (active) {
-        T ue[4][2], wq[9];
+        T ue[4][2];
 #pragma unroll
         for (int a = 0; a < 4; a++) {
             const int ln = (lx + (a & 1)) + NX * (ly + (a >> 1));
@@ -303,8 +399,6 @@ __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const T *__res
             ue[a][1] = su[ln * 2 + 1];
             ye[a][0] = ye[a][1] = (T)0;
         }
-#pragma unroll
-        for (int q = 0; q < 9; q++) wq[q] = W[((size_t)tile * 9 + q) * UST_TILE + threadIdx.x];
         const T s[3] = {(T)(2.0 / g.hx), (T)(2.0 / g.hy), (T)1};
         element_apply2<T>(ue, wq, s, ye);
     }
@@ -315,23 +409,32 @@ __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const T *__res
 #pragma unroll
             for (int a = 0; a < 4; a++) {
                 const int ln = (lx + (a & 1)) + NX * (ly + (a >> 1));
-                atomicAdd(&sy[ln * 2], ye[a][0]);
-                atomicAdd(&sy[ln * 2 + 1], ye[a][1]);
+                atomicAdd(&sy[ln * 2], (double)ye[a][0]);
+                atomicAdd(&sy[ln * 2 + 1], (double)ye[a][1]);
             }
         }
         __syncthreads();
     }
     T *slot = work + (size_t)tile * NN * 2;
-    for (int t = threadIdx.x; t < NN * 2; t += blockDim.x) {
+    int kind[NST];
+    size_t wdof[NST];
+    EpiOps<T> ops[NST];
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = threadIdx.x + UST_TILE * r;
         const int c = t % 2, n = t / 2;
         const int gx = tx0 + n % NX, gy = ty0 + n / NX;
-        if (gx >= g.nnx || gy >= g.nny) continue;
-        if (on_tile_face(gx, TX, g.tnx) || on_tile_face(gy, TY, g.tny)) {
-            slot[t] = sy[t];
-        } else {
-            const size_t dof = ((size_t)gx + (size_t)g.nnx * gy) * 2 + c;
-            y[dof] = mask[dof] ? (T)0 : sy[t];
-        }
+        const bool in = t < NN * 2 && gx < g.nnx && gy < g.nny;
+        const bool face = on_tile_face(gx, TX, g.tnx) || on_tile_face(gy, TY, g.tny);
+        kind[r] = in ? (face ? 1 : 2) : 0;
+        wdof[r] = kind[r] == 2 ? ((size_t)gx + (size_t)g.nnx * gy) * 2 + c : 0;
+        ops[r] = epilogue_load<MODE, T>(ep, mask, wdof[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = threadIdx.x + UST_TILE * r;
+        if (kind[r] == 1) slot[t] = (T)sy[t];
+        else if (kind[r] == 2) epilogue_store<MODE, T>(ep, ops[r], wdof[r], (T)sy[t]);
     }
 }
 
@@ -349,9 +452,9 @@ __device__ __forceinline__ int node_tiles(int gc, int T, int tn, int *tt, int *l
     return 1;
 }
 
-template <typename T, int DIM>
+template <typename T, int DIM, int MODE>
 __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned char *__restrict__ mask,
-                                                      const T *__restrict__ work, T *__restrict__ y) {
+                                                      const T *__restrict__ work, Epi<T> ep) {
     const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * DIM;
     const size_t dof = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (dof >= ndof) return;
@@ -365,10 +468,7 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned 
     const int nys = node_tiles(gy, TY, g.tny, ty, lys);
     const int nzs = DIM == 3 ? node_tiles(gz, TZ, g.tnz, tz, lzs) : 1;
     if (nxs * nys * nzs == 1) return;  // written by its only tile
-    if (mask[dof]) {
-        y[dof] = (T)0;
-        return;
-    }
+    const EpiOps<T> ops = epilogue_load<MODE, T>(ep, mask, dof);
     T acc = (T)0;
     for (int a = 0; a < nzs; a++)
         for (int b = 0; b < nys; b++)
@@ -377,7 +477,7 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned 
                 const int ln = lxs[d] + NX * (lys[b] + NY * lzs[a]);
                 acc += work[(tile * NN + ln) * DIM + c];
             }
-    y[dof] = acc;
+    epilogue_store<MODE, T>(ep, ops, dof, acc);
 }
 
 static inline size_t work_len(const GridDev &g) {
@@ -385,34 +485,82 @@ static inline size_t work_len(const GridDev &g) {
     return (size_t)g.ntiles * nn * g.dim;
 }
 
-template <typename T>
-static int apply_tiled(const ust_grid *gr, const T *W, const unsigned char *mask, const T *u, T *y, T *work,
+template <typename T, int MODE>
+static int apply_tiled(const ust_grid *gr, const T *W, const unsigned char *mask, const T *u, Epi<T> ep, T *work,
                        void *stream) {
     GridDev g = grid_dev(gr);
-    if (!W || !mask || !u || !y || !work) return UST_EINVAL;
+    if (!W || !mask || !u || !ep.y || !work) return UST_EINVAL;
+    if (MODE != EPI_Y && !ep.b) return UST_EINVAL;
+    if ((MODE == EPI_CHEBY || MODE == EPI_CHEBY0) && (!ep.invd || !ep.d || ep.y != u)) return UST_EINVAL;
     const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * g.dim;
     hipStream_t st = ust_stream(stream);
     if (g.dim == 3) {
-        hipLaunchKernelGGL(k_applyK3_tiled<T>, dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, y, work);
-        hipLaunchKernelGGL((k_gather_tiles<T, 3>), dim3(ust_grid_size(ndof, 256)), dim3(256), 0, st, g, mask, work, y);
+        hipLaunchKernelGGL((k_applyK3_tiled<T, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep, work);
+        hipLaunchKernelGGL((k_gather_tiles<T, 3, MODE>), dim3(ust_grid_size(ndof, 256)), dim3(256), 0, st, g, mask,
+                           work, ep);
     } else {
-        hipLaunchKernelGGL(k_applyK2_tiled<T>, dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, y, work);
-        hipLaunchKernelGGL((k_gather_tiles<T, 2>), dim3(ust_grid_size(ndof, 256)), dim3(256), 0, st, g, mask, work, y);
+        hipLaunchKernelGGL((k_applyK2_tiled<T, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep, work);
+        hipLaunchKernelGGL((k_gather_tiles<T, 2, MODE>), dim3(ust_grid_size(ndof, 256)), dim3(256), 0, st, g, mask,
+                           work, ep);
     }
     UST_LAUNCH_CHECK();
     return UST_OK;
+}
+
+template <typename T>
+static int apply_plain(const ust_grid *g, const T *W, const unsigned char *mask, const T *u, T *y, T *work,
+                       void *stream) {
+    Epi<T> ep = {y, nullptr, nullptr, nullptr, (T)0, (T)0};
+    return apply_tiled<T, EPI_Y>(g, W, mask, u, ep, work, stream);
+}
+
+template <typename T>
+static int apply_residual(const ust_grid *g, const T *W, const unsigned char *mask, const T *u, const T *b, T *r,
+                          T *work, void *stream) {
+    Epi<T> ep = {r, b, nullptr, nullptr, (T)0, (T)0};
+    return apply_tiled<T, EPI_RES>(g, W, mask, u, ep, work, stream);
+}
+
+template <typename T>
+static int apply_cheby(const ust_grid *g, const T *W, const unsigned char *mask, T *x, const T *b, const T *invd,
+                       T *d, T c1, T c2, int first, T *work, void *stream) {
+    Epi<T> ep = {x, b, invd, d, c1, c2};
+    return first ? apply_tiled<T, EPI_CHEBY0>(g, W, mask, x, ep, work, stream)
+                 : apply_tiled<T, EPI_CHEBY>(g, W, mask, x, ep, work, stream);
 }
 
 extern "C" size_t ust_k_apply_work_len(const ust_grid *gr) { return work_len(grid_dev(gr)); }
 
 extern "C" int ust_k_apply_K_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
                                  double *y, double *work, void *stream) {
-    return apply_tiled<double>(g, WK, mask, u, y, work, stream);
+    return apply_plain<double>(g, WK, mask, u, y, work, stream);
 }
 
 extern "C" int ust_k_apply_K_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
                                  float *y, float *work, void *stream) {
-    return apply_tiled<float>(g, WK, mask, u, y, work, stream);
+    return apply_plain<float>(g, WK, mask, u, y, work, stream);
+}
+
+extern "C" int ust_k_apply_K_residual_f64(const ust_grid *g, const double *WK, const unsigned char *mask,
+                                          const double *u, const double *b, double *r, double *work, void *stream) {
+    return apply_residual<double>(g, WK, mask, u, b, r, work, stream);
+}
+
+extern "C" int ust_k_apply_K_residual_f32(const ust_grid *g, const float *WK, const unsigned char *mask,
+                                          const float *u, const float *b, float *r, float *work, void *stream) {
+    return apply_residual<float>(g, WK, mask, u, b, r, work, stream);
+}
+
+extern "C" int ust_k_apply_K_cheby_f64(const ust_grid *g, const double *WK, const unsigned char *mask, double *x,
+                                       const double *b, const double *invd, double *d, double c1, double c2,
+                                       int first, double *work, void *stream) {
+    return apply_cheby<double>(g, WK, mask, x, b, invd, d, c1, c2, first, work, stream);
+}
+
+extern "C" int ust_k_apply_K_cheby_f32(const ust_grid *g, const float *WK, const unsigned char *mask, float *x,
+                                       const float *b, const float *invd, float *d, float c1, float c2, int first,
+                                       float *work, void *stream) {
+    return apply_cheby<float>(g, WK, mask, x, b, invd, d, c1, c2, first, work, stream);
 }
 
 extern "C" size_t ust_k_weights_len(const ust_grid *gr) {

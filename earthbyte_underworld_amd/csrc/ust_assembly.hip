@@ -119,17 +119,19 @@ __device__ double particle_viscosity(const GridDev &g, const ust_material &M, co
 }
 
 // ------------------------------------------------------------------------------------------
-// particles -> weights.  A wavefront covers 64/S consecutive elements; lane k of a segment takes
-// particles k, k+S, ... of its element (coalesced: particles are cell-sorted) and accumulates its
-// 43 (3D) partial sums in registers.  The wave then transposes the partials through LDS and lane t
-// sums column t of one segment, so the reduction costs one LDS write and S reads per value instead of
-// log2(S) cross-lane exchanges of every accumulator.  NEED_T / NEED_VEL compile the temperature and
-// strain-rate lookups (Arrhenius, yield) in only when a material uses them.
+// particles -> weights.  A wavefront covers 64/S consecutive elements (S chosen near the mean particle
+// count, so few lanes idle); lane k of a segment takes particles k, k+S, ... of its element (coalesced:
+// particles are cell-sorted) and accumulates its 43 (3D) partial sums in registers.  The partials are
+// then reduced through LDS in chunks of PW_CH columns: every lane writes its chunk, and lane t sums
+// column t of one segment.  Chunking keeps the LDS footprint at 64 * (PW_CH + 1) doubles per wave so
+// LDS does not limit occupancy.  NEED_T / NEED_VEL compile the temperature and strain-rate lookups
+// (Arrhenius, yield) in only when a material uses them.
 // ------------------------------------------------------------------------------------------
 #define PW_WAVES 2
-template <int DIM, int S, bool NEED_T, bool NEED_VEL>
+#define PW_CH 11
+template <int DIM, bool NEED_T, bool NEED_VEL>
 __global__ __launch_bounds__(64 * PW_WAVES) void k_particle_weights(
-    GridDev g, const int *__restrict__ cell_off, const double *__restrict__ xi0, const double *__restrict__ xi1,
+    GridDev g, int S, const int *__restrict__ cell_off, const double *__restrict__ xi0, const double *__restrict__ xi1,
     const double *__restrict__ xi2, const double *__restrict__ w, const unsigned char *__restrict__ mat,
     const float *__restrict__ coh, const ust_material *__restrict__ mats, int nmat, const double *__restrict__ T,
     const double *__restrict__ vel, const double *__restrict__ pres, int prev, double *__restrict__ eta_out,
@@ -138,16 +140,17 @@ __global__ __launch_bounds__(64 * PW_WAVES) void k_particle_weights(
     constexpr int NQ = DIM == 3 ? 27 : 9;
     constexpr int NV = 1 << DIM;
     constexpr int NA = NQ + 2 * NV;  // accumulators per lane
-    constexpr int LD = NA + 1;       // padded LDS row
+    constexpr int LD = PW_CH + 1;    // padded LDS row
     __shared__ double red[PW_WAVES][64 * LD];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nseg = 64 / S;
     const int seg = lane / S, k = lane % S;
-    const int ebase = (blockIdx.x * PW_WAVES + wv) * (64 / S);
+    const int ebase = (blockIdx.x * PW_WAVES + wv) * nseg;
     const int e = ebase + seg;
     double acc[NA];
 #pragma unroll
     for (int t = 0; t < NA; t++) acc[t] = 0.0;
-    if (e < g.nel) {
+    if (seg < nseg && e < g.nel) {
         const int ie = e % g.nx, je = (e / g.nx) % g.ny, ke = e / (g.nx * g.ny);
         const int n0 = ie + g.nnx * (je + g.nny * ke);
         int nodes[NV];
@@ -197,16 +200,23 @@ __global__ __launch_bounds__(64 * PW_WAVES) void k_particle_weights(
     }
     double *r = red[wv];
 #pragma unroll
-    for (int t = 0; t < NA; t++) r[lane * LD + t] = acc[t];
-    __syncthreads();
-    for (int t = lane; t < (64 / S) * NA; t += 64) {
-        const int sg = t / NA, col = t % NA, el = ebase + sg;
-        if (el >= g.nel) continue;
-        double sum = 0.0;
-        for (int row = sg * S; row < sg * S + S; row++) sum += r[row * LD + col];
-        if (col < NQ) WK[widx(g, el, col)] = sum;
-        else if (col < NQ + NV) WG[(size_t)(col - NQ) * g.nel + el] = sum;
-        else WF[(size_t)(col - NQ - NV) * g.nel + el] = sum;
+    for (int c0 = 0; c0 < NA; c0 += PW_CH) {
+#pragma unroll
+        for (int t = 0; t < PW_CH; t++)
+            if (c0 + t < NA) r[lane * LD + t] = acc[c0 + t];
+        __syncthreads();
+        const int nc = NA - c0 < PW_CH ? NA - c0 : PW_CH;
+        for (int t = lane; t < nseg * nc; t += 64) {
+            const int sg = t / nc, col = c0 + t % nc, el = ebase + sg;
+            if (el < g.nel) {
+                double sum = 0.0;
+                for (int row = sg * S; row < sg * S + S; row++) sum += r[row * LD + (col - c0)];
+                if (col < NQ) WK[widx(g, el, col)] = sum;
+                else if (col < NQ + NV) WG[(size_t)(col - NQ) * g.nel + el] = sum;
+                else WF[(size_t)(col - NQ - NV) * g.nel + el] = sum;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -358,7 +368,7 @@ __global__ void k_uzawa_diag(GridDev g, const double *__restrict__ WK, const dou
 }
 
 // ------------------------------------------------------------------------------------------
-extern "C" int ust_k_particle_weights(const ust_grid *gr, const int *cell_off, int max_ppc, const double *xi0,
+extern "C" int ust_k_particle_weights(const ust_grid *gr, const int *cell_off, int ppc_hint, const double *xi0,
                                       const double *xi1, const double *xi2, const double *w,
                                       const unsigned char *mat, const float *coh_scale,
                                       const ust_material *materials_dev, int nmat, const double *temperature,
@@ -373,21 +383,22 @@ extern "C" int ust_k_particle_weights(const ust_grid *gr, const int *cell_off, i
     if (g.nel <= 0) return UST_OK;
     // temperature / velocity lookups only when a rheology reads them (the caller passes NULL otherwise)
     const bool needT = temperature != nullptr, needV = velocity != nullptr && pressure != nullptr;
-    // segment width: the smallest of 16/32/64 that covers the typical cell in one pass
-    const int S = max_ppc <= 16 ? 16 : (max_ppc <= 32 ? 32 : 64);
+    // segment width: the smallest candidate >= the caller's particles-per-cell hint (the mean count; a cell
+    // with more particles than S loops), so few lanes idle; segments never straddle a wavefront
+    int S = 64;
+    const int cands[6] = {8, 12, 16, 21, 32, 64};
+    for (int c = 0; c < 6; c++)
+        if (cands[c] >= ppc_hint) {
+            S = cands[c];
+            break;
+        }
     const int elems_per_block = PW_WAVES * (64 / S);
     const unsigned grid = ust_grid_size(g.nel, elems_per_block);
     hipStream_t st = ust_stream(stream);
-#define LAUNCH_PW(D, SS, NT, NVL)                                                                                \
-    hipLaunchKernelGGL((k_particle_weights<D, SS, NT, NVL>), dim3(grid), dim3(64 * PW_WAVES), 0, st, g, cell_off, \
-                       xi0, xi1, xi2, w, mat, coh_scale, materials_dev, nmat, temperature, velocity, pressure,     \
+#define LAUNCH_S(D, NT, NVL)                                                                                      \
+    hipLaunchKernelGGL((k_particle_weights<D, NT, NVL>), dim3(grid), dim3(64 * PW_WAVES), 0, st, g, S, cell_off, \
+                       xi0, xi1, xi2, w, mat, coh_scale, materials_dev, nmat, temperature, velocity, pressure,    \
                        prev_solution, eta_out, yield_out, WK, WG, WF)
-#define LAUNCH_S(D, NT, NVL)                   \
-    do {                                       \
-        if (S == 16) LAUNCH_PW(D, 16, NT, NVL); \
-        else if (S == 32) LAUNCH_PW(D, 32, NT, NVL); \
-        else LAUNCH_PW(D, 64, NT, NVL);        \
-    } while (0)
 #define LAUNCH_F(D)                                      \
     do {                                                 \
         if (needT && needV) LAUNCH_S(D, true, true);     \
@@ -399,7 +410,6 @@ extern "C" int ust_k_particle_weights(const ust_grid *gr, const int *cell_off, i
     else LAUNCH_F(2);
 #undef LAUNCH_F
 #undef LAUNCH_S
-#undef LAUNCH_PW
     UST_LAUNCH_CHECK();
     return UST_OK;
 }

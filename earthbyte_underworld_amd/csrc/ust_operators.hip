@@ -306,36 +306,48 @@ __global__ void k_coarsen_weights(GridDev gf, GridDev gc, const double *__restri
     for (int q = 0; q < NQ; q++) Wc[widx(gc, E, q)] = acc[q];
 }
 
-// restriction b_c = P^T r_f : coarse node gathers fine nodes 2I+{-1,0,1} (weights 1/2, 1, 1/2)
+// restriction b_c = P^T r_f : coarse dof (node (I,J,K), component c) gathers fine nodes 2I+{-1,0,1}
+// (weights 1/2, 1, 1/2 per axis) over the free fine dofs.  One thread per coarse dof; the 3^DIM loads are
+// unrolled and issued together (out-of-range neighbours read a clamped in-range address with weight 0).
 template <int DIM, typename T>
-__global__ void k_restrict(GridDev gf, GridDev gc, const unsigned char *__restrict__ maskf,
-                           const unsigned char *__restrict__ maskc, const T *__restrict__ rf, T *__restrict__ bc) {
-    const int nnc = gc.nnx * gc.nny * gc.nnz;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nnc) return;
-    const int I = n % gc.nnx, J = (n / gc.nnx) % gc.nny, K = n / (gc.nnx * gc.nny);
-    T acc[DIM];
+__global__ __launch_bounds__(256) void k_restrict(GridDev gf, GridDev gc, const unsigned char *__restrict__ maskf,
+                                                  const unsigned char *__restrict__ maskc, const T *__restrict__ rf,
+                                                  T *__restrict__ bc) {
+    const size_t ndc = (size_t)gc.nnx * gc.nny * gc.nnz * DIM;
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= ndc) return;
+    if (maskc[t]) {
+        bc[t] = (T)0;
+        return;
+    }
+    const int c = (int)(t % DIM);
+    const size_t n = t / DIM;
+    const int I = (int)(n % gc.nnx), J = (int)((n / gc.nnx) % gc.nny), K = (int)(n / ((size_t)gc.nnx * gc.nny));
+    T acc = (T)0;
 #pragma unroll
-    for (int i = 0; i < DIM; i++) acc[i] = (T)0;
     for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); dz++) {
         const int k = 2 * K + dz;
-        if (k < 0 || k >= gf.nnz) continue;
+        const bool okz = k >= 0 && k < gf.nnz;
+        const int kc = okz ? k : 2 * K;
+#pragma unroll
         for (int dy = -1; dy <= 1; dy++) {
             const int j = 2 * J + dy;
-            if (j < 0 || j >= gf.nny) continue;
+            const bool oky = j >= 0 && j < gf.nny;
+            const int jc = oky ? j : 2 * J;
+#pragma unroll
             for (int dx = -1; dx <= 1; dx++) {
                 const int i = 2 * I + dx;
-                if (i < 0 || i >= gf.nnx) continue;
+                const bool okx = i >= 0 && i < gf.nnx;
+                const int ic = okx ? i : 2 * I;
+                const size_t fd = ((size_t)ic + (size_t)gf.nnx * ((size_t)jc + (size_t)gf.nny * kc)) * DIM + c;
+                const T v = rf[fd];
+                const bool use = okx && oky && okz && !maskf[fd];
                 const T wt = (T)((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0));
-                const size_t fn = (size_t)i + gf.nnx * ((size_t)j + gf.nny * k);
-#pragma unroll
-                for (int c = 0; c < DIM; c++)
-                    if (!maskf[fn * DIM + c]) acc[c] += wt * rf[fn * DIM + c];
+                acc += use ? wt * v : (T)0;
             }
         }
     }
-#pragma unroll
-    for (int c = 0; c < DIM; c++) bc[(size_t)n * DIM + c] = maskc[(size_t)n * DIM + c] ? (T)0 : acc[c];
+    bc[t] = acc;
 }
 
 // prolongation x_f += P x_c : even index injects, odd index averages its two coarse neighbours
@@ -597,10 +609,10 @@ int ust_k_coarsen_weights(const ust_grid *gf, const double *WKf, double *WKc, vo
         GridDev f = grid_dev(gf), c = grid_dev(&gcr);                                                          \
         const int nnc = c.nnx * c.nny * c.nnz;                                                                 \
         if (f.dim == 3)                                                                                        \
-            hipLaunchKernelGGL((k_restrict<3, T>), dim3(ust_grid_size(nnc, UST_BLOCK)), dim3(UST_BLOCK), 0,    \
+            hipLaunchKernelGGL((k_restrict<3, T>), dim3(ust_grid_size(nnc * 3, UST_BLOCK)), dim3(UST_BLOCK), 0,\
                                ust_stream(stream), f, c, maskf, maskc, rf, bc);                               \
         else                                                                                                   \
-            hipLaunchKernelGGL((k_restrict<2, T>), dim3(ust_grid_size(nnc, UST_BLOCK)), dim3(UST_BLOCK), 0,    \
+            hipLaunchKernelGGL((k_restrict<2, T>), dim3(ust_grid_size(nnc * 2, UST_BLOCK)), dim3(UST_BLOCK), 0,    \
                                ust_stream(stream), f, c, maskf, maskc, rf, bc);                               \
         UST_LAUNCH_CHECK();                                                                                    \
         return UST_OK;                                                                                         \

@@ -183,6 +183,16 @@ __global__ void kv_cheby(const T *b, const T *Ax, const T *invd, T *d, T *x, T c
     }
 }
 
+// first Chebyshev step from x = 0: d = c2 invd b, x = d
+template <typename T>
+__global__ void kv_cheby_first(const T *b, const T *invd, T *d, T *x, T c2, size_t n) {
+    GRID_STRIDE(i, n) {
+        const T dn = c2 * invd[i] * b[i];
+        d[i] = dn;
+        x[i] = dn;
+    }
+}
+
 template <typename T>
 __global__ void kv_residual(const T *b, const T *Ax, T *r, size_t n) {
     GRID_STRIDE(i, n) r[i] = b[i] - Ax[i];
@@ -305,6 +315,13 @@ int ust_k_cheby_step_f64(const double *b, const double *Ax, const double *invd, 
 int ust_k_cheby_step_f32(const float *b, const float *Ax, const float *invd, float *d, float *x, float c1, float c2,
                          size_t n, void *stream) {
     LV(kv_cheby<float>, n, b, Ax, invd, d, x, c1, c2, c1 == 0.0f ? 1 : 0, n);
+}
+int ust_k_cheby_first_f64(const double *b, const double *invd, double *d, double *x, double c2, size_t n,
+                          void *stream) {
+    LV(kv_cheby_first<double>, n, b, invd, d, x, c2, n);
+}
+int ust_k_cheby_first_f32(const float *b, const float *invd, float *d, float *x, float c2, size_t n, void *stream) {
+    LV(kv_cheby_first<float>, n, b, invd, d, x, c2, n);
 }
 int ust_k_residual_f64(const double *b, const double *Ax, double *r, size_t n, void *stream) {
     LV(kv_residual<double>, n, b, Ax, r, n);

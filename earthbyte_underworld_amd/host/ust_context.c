@@ -46,6 +46,10 @@ int ust_create(ust_context **out, int device) {
     ust_context *ctx = (ust_context *)calloc(1, sizeof(ust_context));
     if (!ctx) return UST_ENOMEM;
     ctx->device = device;
+    /* UST_NO_GRAPH=1 launches the inner Krylov iteration kernel by kernel instead of as a captured graph
+     * (same kernels and arguments; for tools that do not follow graph launches) */
+    const char *ng = getenv("UST_NO_GRAPH");
+    ctx->use_graphs = !(ng && ng[0] == '1');
     int rc = ust_dev_set(device);
     if (rc == UST_OK) rc = ust_stream_create(&ctx->stream);
     if (rc != UST_OK) {
@@ -93,7 +97,7 @@ static void free_mesh_data(ust_context *ctx) {
     DFREE(ctx->va); DFREE(ctx->vb);
     DFREE(ctx->wr); DFREE(ctx->wz); DFREE(ctx->wp); DFREE(ctx->wq); DFREE(ctx->wx);
     DFREE(ctx->pr); DFREE(ctx->pz); DFREE(ctx->pd); DFREE(ctx->pq); DFREE(ctx->ph); DFREE(ctx->ppc);
-    DFREE(ctx->pprev); DFREE(ctx->uprev);
+    DFREE(ctx->pprev); DFREE(ctx->uprev); DFREE(ctx->upic);
     DFREE(ctx->feat); DFREE(ctx->mlp_out); DFREE(ctx->act1); DFREE(ctx->act2);
     ctx->feat_cap = 0;
     free(ctx->ke_cache); free(ctx->ge_cache); free(ctx->fe_cache); free(ctx->as_cache);
@@ -393,7 +397,8 @@ int ust_assemble(ust_context *ctx, double gravity) {
             if (t == UST_RH_ARRHENIUS) needT = 1;
             if (t == UST_RH_VONMISES || t == UST_RH_DRUCKERPRAGER) needV = 1;
         }
-    CKD(ctx, ust_k_particle_weights(&ctx->grid, ctx->cell_off, ctx->max_ppc, ctx->xi0, ctx->xi1, ctx->xi2, ctx->w,
+    const int ppc_mean = (ctx->np + ctx->nel - 1) / ctx->nel;
+    CKD(ctx, ust_k_particle_weights(&ctx->grid, ctx->cell_off, ppc_mean, ctx->xi0, ctx->xi1, ctx->xi2, ctx->w,
                                     ctx->mat, ctx->coh, ctx->mats_d, ctx->nmat, needT ? ctx->T : NULL,
                                     needV ? ctx->u : NULL, needV ? ctx->p : NULL, ctx->prev_solution, gravity,
                                     ctx->eta, ctx->yielded, ctx->WK, ctx->WG, ctx->WF, ctx->stream));

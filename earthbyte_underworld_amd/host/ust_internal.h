@@ -96,11 +96,13 @@ struct ust_context {
     double *vr, *vz, *vp, *vq, *vt;   /* FP64 velocity work (inner PCG) */
     double *vx;                       /* inner PCG iterate */
     void *pcg_exec[2];                /* captured inner PCG iteration (FP64 / FP32 V-cycle), NULL if stale */
+    int use_graphs;                   /* 0: launch the inner iteration kernel by kernel (UST_NO_GRAPH=1) */
     int pcg_hint;                     /* iterations of the last inner solve: first convergence check */
     float *wr, *wz, *wp, *wq, *wx;    /* FP32 velocity work (mixed inner PCG) */
     double *va, *vb;                  /* FP64 velocity work (Schur / IR) */
     double *pr, *pz, *pd, *pq, *ph, *ppc, *pprev; /* pressure work */
     double *uprev;
+    double *upic;                     /* Picard: the previous iterate's nodal velocity */
     /* learned scaling */
     ust_mlp mlp;
     float *feat, *mlp_out, *act1, *act2;

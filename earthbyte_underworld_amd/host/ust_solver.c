@@ -84,32 +84,46 @@ static int lev_apply(ust_context *ctx, int l, int f32, const void *x, void *y) {
     return UST_OK;
 }
 
-/* Chebyshev smoothing of A x = b; x_zero: x is zero on entry (its first product is skipped) */
+/* r = b - A x on level l (fused product and residual) */
+static int lev_residual(ust_context *ctx, int l, int f32, const void *x, const void *b, void *r) {
+    ust_level *L = &ctx->lev[l];
+    if (f32)
+        CKD(ctx, ust_k_apply_K_residual_f32(&L->g, L->WK32, L->mask, (const float *)x, (const float *)b, (float *)r,
+                                            (float *)ctx->kwork, S(ctx)));
+    else
+        CKD(ctx, ust_k_apply_K_residual_f64(&L->g, L->WK64, L->mask, (const double *)x, (const double *)b,
+                                            (double *)r, (double *)ctx->kwork, S(ctx)));
+    return UST_OK;
+}
+
+/* Jacobi-Chebyshev smoothing of A x = b (PETScMGSolver.c:674-709 smoother role).  x_zero: x is not read
+ * (taken as zero); the first step is then d = c2 D^-1 b, x = d, without a product.  Every other step is
+ * one fused product + update pass. */
 static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void *x, int x_zero) {
     ust_level *L = &ctx->lev[l];
     size_t n = (size_t)L->ndof;
     for (int k = 0; k < L->cheb_deg; k++) {
         if (k == 0 && x_zero) {
-            if (f32) CKD(ctx, ust_k_fill_f32(L->ax32, 0.0f, n, S(ctx)));
-            else CKD(ctx, ust_k_fill_f64(L->ax64, 0.0, n, S(ctx)));
+            if (f32)
+                CKD(ctx, ust_k_cheby_first_f32((const float *)b, L->invd32, L->d32, (float *)x, (float)L->c2[0], n,
+                                               S(ctx)));
+            else
+                CKD(ctx, ust_k_cheby_first_f64((const double *)b, L->invd64, L->d64, (double *)x, L->c2[0], n,
+                                               S(ctx)));
+        } else if (f32) {
+            CKD(ctx, ust_k_apply_K_cheby_f32(&L->g, L->WK32, L->mask, (float *)x, (const float *)b, L->invd32, L->d32,
+                                             (float)L->c1[k], (float)L->c2[k], k == 0, (float *)ctx->kwork, S(ctx)));
         } else {
-            int rc = lev_apply(ctx, l, f32, x, f32 ? (void *)L->ax32 : (void *)L->ax64);
-            if (rc) return rc;
+            CKD(ctx, ust_k_apply_K_cheby_f64(&L->g, L->WK64, L->mask, (double *)x, (const double *)b, L->invd64,
+                                             L->d64, L->c1[k], L->c2[k], k == 0, (double *)ctx->kwork, S(ctx)));
         }
-        if (f32)
-            CKD(ctx, ust_k_cheby_step_f32((const float *)b, L->ax32, L->invd32, L->d32, (float *)x, (float)L->c1[k],
-                                          (float)L->c2[k], n, S(ctx)));
-        else
-            CKD(ctx, ust_k_cheby_step_f64((const double *)b, L->ax64, L->invd64, L->d64, (double *)x, L->c1[k],
-                                          L->c2[k], n, S(ctx)));
     }
     return UST_OK;
 }
 
-/* one V-cycle: x = M^-1 b on level l (x is overwritten) */
+/* one V-cycle: x = M^-1 b on level l (x is overwritten, not read) */
 static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x) {
     ust_level *L = &ctx->lev[l];
-    size_t n = (size_t)L->ndof;
     int rc;
     if (l == ctx->nlev - 1) {
         if (ctx->coarse_dense) {
@@ -117,22 +131,17 @@ static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x) {
             else CKD(ctx, ust_k_dense_matvec_f64(ctx->Ainv, ctx->nc, (const double *)b, (double *)x, S(ctx)));
             return UST_OK;
         }
-        if (f32) CKD(ctx, ust_k_fill_f32((float *)x, 0.0f, n, S(ctx)));
-        else CKD(ctx, ust_k_fill_f64((double *)x, 0.0, n, S(ctx)));
         return lev_smooth(ctx, l, f32, b, x, 1);
     }
     ust_level *C = &ctx->lev[l + 1];
-    if (f32) CKD(ctx, ust_k_fill_f32((float *)x, 0.0f, n, S(ctx)));
-    else CKD(ctx, ust_k_fill_f64((double *)x, 0.0, n, S(ctx)));
     if ((rc = lev_smooth(ctx, l, f32, b, x, 1))) return rc;
-    if ((rc = lev_apply(ctx, l, f32, x, f32 ? (void *)L->ax32 : (void *)L->ax64))) return rc;
     if (f32) {
-        CKD(ctx, ust_k_residual_f32((const float *)b, L->ax32, L->ax32, n, S(ctx)));
+        if ((rc = lev_residual(ctx, l, 1, x, b, L->ax32))) return rc;
         CKD(ctx, ust_k_restrict_f32(&L->g, L->mask, C->mask, L->ax32, C->b32, S(ctx)));
         if ((rc = vcycle(ctx, l + 1, 1, C->b32, C->x32))) return rc;
         CKD(ctx, ust_k_prolong_add_f32(&L->g, L->mask, C->x32, (float *)x, S(ctx)));
     } else {
-        CKD(ctx, ust_k_residual_f64((const double *)b, L->ax64, L->ax64, n, S(ctx)));
+        if ((rc = lev_residual(ctx, l, 0, x, b, L->ax64))) return rc;
         CKD(ctx, ust_k_restrict_f64(&L->g, L->mask, C->mask, L->ax64, C->b64, S(ctx)));
         if ((rc = vcycle(ctx, l + 1, 0, C->b64, C->x64))) return rc;
         CKD(ctx, ust_k_prolong_add_f64(&L->g, L->mask, C->x64, (double *)x, S(ctx)));
@@ -385,13 +394,16 @@ static int pcg(ust_context *ctx, int pc32, const double *b, double *x, double rt
     CKD(ctx, ust_memcpy_d2d(ctx->vr, b, n * sizeof(double), S(ctx)));
     CKD(ctx, ust_k_dot_f64(b, b, n, ctx->partial, sc + SC_BB, S(ctx)));
     CKD(ctx, ust_k_cg_start(sc + SC_BB, rtol * rtol, sc + SC_CG, sc + SC_RZOLD, S(ctx)));
-    if ((rc = pcg_graph(ctx, pc32))) return rc;
+    if (ctx->use_graphs && (rc = pcg_graph(ctx, pc32))) return rc;
     int done = 0, first = ctx->pcg_hint > 2 ? ctx->pcg_hint - 1 : 1;
     double state = 0.0, count = 0.0;
     while (done < maxit) {
         int k = done == 0 ? first : 1;
         if (k > maxit - done) k = maxit - done;
-        for (int j = 0; j < k; j++) CKD(ctx, ust_graph_launch(ctx->pcg_exec[pc32], S(ctx)));
+        for (int j = 0; j < k; j++) {
+            if (ctx->use_graphs) CKD(ctx, ust_graph_launch(ctx->pcg_exec[pc32], S(ctx)));
+            else if ((rc = pcg_iteration(ctx, pc32))) return rc;
+        }
         done += k;
         CKD(ctx, ust_memcpy_d2h(ctx->scal_h + SC_CG + 1, sc + SC_CG + 1, 2 * sizeof(double), S(ctx)));
         state = ctx->scal_h[SC_CG + 1];
@@ -426,10 +438,13 @@ int ust_a11_solve_dev(ust_context *ctx, const ust_solve_opts *o, const double *b
     return a11_solve(ctx, o, b, x, st);
 }
 
-/* q = G^T K^-1 G d  (the Schur complement apply of MatSchurComplement) */
-static int schur_apply(ust_context *ctx, const ust_solve_opts *o, const double *d, double *q, ust_solve_stats *st) {
+/* q = G^T K^-1 G d  (the Schur complement apply of MatSchurComplement), inner solve to rtol */
+static int schur_apply(ust_context *ctx, const ust_solve_opts *o, const double *d, double *q, double rtol,
+                       ust_solve_stats *st) {
     CKD(ctx, ust_k_apply_G(&ctx->grid, ctx->WG, ctx->mask_d, d, ctx->vb, S(ctx)));
-    int rc = a11_solve(ctx, o, ctx->vb, ctx->uprev, st);
+    ust_solve_opts oi = *o;
+    oi.inner_rtol = rtol;
+    int rc = a11_solve(ctx, &oi, ctx->vb, ctx->uprev, st);
     if (rc) return rc;
     CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, ctx->uprev, q, 0, S(ctx)));
     return UST_OK;
@@ -463,7 +478,13 @@ static int schur_cg(ust_context *ctx, const ust_solve_opts *o, int ns, const dou
         CKD(ctx, ust_k_cg_update_p_f64(ctx->pz, ctx->pd, ctx->scal + SC_O_RZ, ctx->scal + SC_O_RZOLD, it == 0, ne,
                                        S(ctx)));
         CKD(ctx, ust_memcpy_d2d(ctx->scal + SC_O_RZOLD, ctx->scal + SC_O_RZ, sizeof(double), S(ctx)));
-        if ((rc = schur_apply(ctx, o, ctx->pd, ctx->pq, st))) return rc;
+        double rtol = o->inner_rtol;
+        if (o->inner_relax && rn > 0.0) {
+            rtol = o->inner_rtol * hn / rn;
+            if (rtol > 1e-3) rtol = 1e-3;
+            if (rtol < o->inner_rtol) rtol = o->inner_rtol;
+        }
+        if ((rc = schur_apply(ctx, o, ctx->pd, ctx->pq, rtol, st))) return rc;
         CKD(ctx, ust_k_dot_f64(ctx->pd, ctx->pq, ne, ctx->partial, ctx->scal + SC_O_PQ, S(ctx)));
         CKD(ctx, ust_k_cg_update_xr_f64(ctx->p, ctx->pr, ctx->pd, ctx->pq, ctx->scal + SC_O_RZOLD, ctx->scal + SC_O_PQ,
                                         ne, S(ctx)));
@@ -530,7 +551,7 @@ int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st)
         if (stalled) {
             /* plain BSSCR takes over from the current iterate: r = h_hat - S p */
             st->fallback_used = 1;
-            if ((rc = schur_apply(ctx, &o, ctx->p, ctx->pq, st))) return rc;
+            if ((rc = schur_apply(ctx, &o, ctx->p, ctx->pq, o.inner_rtol, st))) return rc;
             CKD(ctx, ust_k_sub_f64(ctx->ph, ctx->pq, ctx->pr, ne, S(ctx)));
             if ((rc = schur_cg(ctx, &o, ns, ctx->approxS, 0, hn, st, &stalled))) return rc;
         }
@@ -580,15 +601,16 @@ int ust_picard(ust_context *ctx, double gravity, double tol, int max_its, const 
     int rc;
     double change = 1.0;
     int it;
+    if (!ctx->upic) DALLOC(ctx, ctx->upic, n * sizeof(double));
     for (it = 0; it < max_its; it++) {
-        if (it > 0) CKD(ctx, ust_memcpy_d2d(ctx->va, ctx->u, n * sizeof(double), S(ctx)));
+        if (it > 0) CKD(ctx, ust_memcpy_d2d(ctx->upic, ctx->u, n * sizeof(double), S(ctx)));
         if ((rc = ust_assemble(ctx, gravity))) return rc;
         if ((rc = ust_solve(ctx, opts, last))) return rc;
         /* the first solve enables yielding (SystemLinearEquations.c:808-809) */
         ctx->prev_solution = 1;
         if (it > 0) {
             double dd, uu;
-            CKD(ctx, ust_k_sub_f64(ctx->u, ctx->va, ctx->vt, n, S(ctx)));
+            CKD(ctx, ust_k_sub_f64(ctx->u, ctx->upic, ctx->vt, n, S(ctx)));
             CKD(ctx, dotv(ctx, ctx->vt, ctx->vt, n, SC_T0, &dd));
             CKD(ctx, dotv(ctx, ctx->u, ctx->u, n, SC_T0, &uu));
             change = uu > 0 ? sqrt(dd / uu) : 0.0; /* SystemLinearEquations.c:852-855 */

@@ -36,6 +36,8 @@ typedef struct {
     int max_inner;           /* "-A11_ksp_max_it" */
     int remove_nullspace;    /* -1: when ||G 1|| / ||1|| < 1e-7 ||D||_inf (ksp_pressure_nullspace.c:139-158); 0/1 force */
     int stall_window;        /* learned PC: fall back to plain if the residual does not halve in this many its */
+    int inner_relax;         /* 1: the inner tolerance of each Schur iteration is inner_rtol * ||h_hat|| / ||r_k||
+                                (capped at 1e-3), relaxed as the outer residual falls; 0: fixed, as the reference */
     int verbose;
 } ust_solve_opts;
 

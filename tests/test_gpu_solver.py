@@ -134,3 +134,23 @@ def test_picard_viscoplastic_2d(oracle):
     assert its == ito
     assert rel_err(u, uo) < 1e-5
     assert rel_err(p, po) < 1e-5
+
+
+def test_graph_replays_match_eager_launches(monkeypatch):
+    """The inner Krylov iteration replayed from its captured HIP graph (hundreds of replays, with the host's
+    own synchronisations between them) gives bitwise the same solution as launching it kernel by kernel."""
+    import earthbyte_underworld_amd as ust
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("UST_NO_GRAPH", flag)
+        s = _spheres_3d(nel=(12, 12, 12), ppc=8)
+        s.set_bcs(*ust.free_slip_walls(s))
+        s.assemble(gravity=1.0)
+        st = s.solve(outer_rtol=1e-8, inner_rtol=1e-10)
+        st32 = s.solve(precision=ust._lib.PREC_MIXED, outer_rtol=1e-8, inner_rtol=1e-10)
+        out.append((s.solution(), st, st32))
+        s.close()
+    (ua, pa), sta, sta32 = out[0]
+    (ub, pb), stb, stb32 = out[1]
+    assert sta["inner_its"] == stb["inner_its"] and sta32["inner_its"] == stb32["inner_its"]
+    assert np.array_equal(ua, ub) and np.array_equal(pa, pb)
