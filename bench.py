@@ -222,9 +222,10 @@ def main():
     t0 = time.perf_counter()
     stats = []
     for i in range(args.steps):
+        ts = time.perf_counter()
         stats.append(step(opts_mixed))
-        log(f"step {i}: {stats[-1]['outer_its']} outer / {stats[-1]['inner_its']} inner its, "
-            f"residual {stats[-1]['stokes_residual']:.2e}, fallback {stats[-1]['fallback_used']}")
+        log(f"step {i}: {time.perf_counter() - ts:.2f}s, {stats[-1]['outer_its']} outer / {stats[-1]['inner_its']} "
+            f"inner its, residual {stats[-1]['stokes_residual']:.2e}, fallback {stats[-1]['fallback_used']}")
     lib.ust_stream_sync(stream)
     t_mixed = (time.perf_counter() - t0) / args.steps
     worst_res = max(st["stokes_residual"] for st in stats)

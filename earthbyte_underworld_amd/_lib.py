@@ -125,6 +125,8 @@ _SIGS = {
     "ust_k_apply_K_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp]),
     "ust_k_apply_K_residual_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp]),
     "ust_k_apply_K_cheby_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, D, D, I, vp, vp]),
+    "ust_k_apply_K_residual_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp]),
+    "ust_k_apply_K_cheby_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, I, vp, vp]),
     "ust_k_particle_weights": (I, [C.POINTER(Grid), vp, I, vp, vp, vp, vp, vp, vp, vp, I, vp, vp, vp, I, D, vp,
                                    vp, vp, vp, vp, vp]),
     "ust_event_create": (I, [C.POINTER(vp)]),
@@ -157,8 +159,21 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
-    return lib
+    _lib = _Declared(lib)
+    return _lib
+
+
+class _Declared:
+    """The loaded library restricted to the entry points with declared signatures: ctypes would otherwise
+    pass 64-bit pointers and sizes of an undeclared call as 32-bit ints."""
+
+    def __init__(self, cdll):
+        self._cdll = cdll
+
+    def __getattr__(self, name):
+        if name not in _SIGS:
+            raise AttributeError(f"{name}: no signature declared in earthbyte_underworld_amd/_lib.py")
+        return getattr(self._cdll, name)
 
 
 def exported_symbols():
