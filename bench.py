@@ -178,6 +178,7 @@ def main():
     ap.add_argument("--nsph", type=int, default=40)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="plain FP64 solve only (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=16)
     ap.add_argument("--mlp", default=MLP_WEIGHTS)
@@ -215,6 +216,8 @@ def main():
         s.assemble(gravity=1.0)
         return s.solve(opts)
 
+    if args.no_mixed:
+        args.warmup, args.steps = 0, 0
     for i in range(args.warmup):
         st = step(opts_mixed)
         log(f"warmup {i}: {st['outer_its']} outer its, residual {st['stokes_residual']:.2e}")
@@ -227,8 +230,8 @@ def main():
         log(f"step {i}: {time.perf_counter() - ts:.2f}s, {stats[-1]['outer_its']} outer / {stats[-1]['inner_its']} "
             f"inner its, residual {stats[-1]['stokes_residual']:.2e}, fallback {stats[-1]['fallback_used']}")
     lib.ust_stream_sync(stream)
-    t_mixed = (time.perf_counter() - t0) / args.steps
-    worst_res = max(st["stokes_residual"] for st in stats)
+    t_mixed = (time.perf_counter() - t0) / args.steps if args.steps else None
+    worst_res = max(st["stokes_residual"] for st in stats) if stats else None
 
     plain = None
     if not args.no_plain:
@@ -258,6 +261,9 @@ def main():
         except Exception as ex:  # the CPU leg never blocks the GPU result
             log("cpu baseline failed:", ex)
 
+    if t_mixed is None:  # --no-mixed: the plain solve alone (profiling)
+        log("plain FP64 only:", json.dumps(plain))
+        return
     value = 3600.0 / t_mixed
     out = {
         "metric": "3D Stokes solves/hour to 1e-6 residual, mixed+learned vs plain; assembly GB/s",

@@ -253,7 +253,7 @@ __device__ __forceinline__ void epilogue_store(const Epi<T> &ep, const EpiOps<T>
 }
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_applyK3_tiled(GridDev g, const T *__restrict__ W,
+__global__ __launch_bounds__(256, 2) void k_applyK3_tiled(GridDev g, const T *__restrict__ W,
                                                        const unsigned char *__restrict__ mask, const T *u,
                                                        Epi<T> ep, T *__restrict__ work) {
     constexpr int TX = UST_TX3, TY = UST_TY3, TZ = UST_TZ3;
@@ -280,9 +280,14 @@ __global__ __launch_bounds__(256) void k_applyK3_tiled(GridDev g, const T *__res
         stm[r] = mask[dof];
         stv[r] = u[dof];
     }
+    // FP32: the weights are loaded here, ahead of the barrier; FP64 loads them after it, which keeps the
+    // register count within two waves per SIMD
+    constexpr bool early = sizeof(T) == 4;
     T wq[27];
+    if (early) {
 #pragma unroll
-    for (int q = 0; q < 27; q++) wq[q] = W[((size_t)tile * 27 + q) * UST_TILE + threadIdx.x];
+        for (int q = 0; q < 27; q++) wq[q] = W[((size_t)tile * 27 + q) * UST_TILE + threadIdx.x];
+    }
 #pragma unroll
     for (int r = 0; r < NST; r++) stv[r] = (stok[r] && !stm[r]) ? stv[r] : (T)0;
 #pragma unroll
@@ -298,6 +303,10 @@ __global__ __launch_bounds__(256) void k_applyK3_tiled(GridDev g, const T *__res
     const bool active = tx0 + lx < g.nx && ty0 + ly < g.ny && tz0 + lz < g.nz;
     T ye[8][3];
     if (active) {
+        if (!early) {
+#pragma unroll
+            for (int q = 0; q < 27; q++) wq[q] = W[((size_t)tile * 27 + q) * UST_TILE + threadIdx.x];
+        }
         T ue[8][3];
 #pragma unroll
         for (int a = 0; a < 8; a++) {
@@ -452,15 +461,16 @@ __device__ __forceinline__ int node_tiles(int gc, int T, int tn, int *tt, int *l
     return 1;
 }
 
+// one thread per node (32-bit index arithmetic: the grids here stay below 2^31 dofs)
 template <typename T, int DIM, int MODE>
 __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned char *__restrict__ mask,
                                                       const T *__restrict__ work, Epi<T> ep) {
-    const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * DIM;
-    const size_t dof = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (dof >= ndof) return;
-    const int c = (int)(dof % DIM);
-    const size_t n = dof / DIM;
-    const int gx = (int)(n % g.nnx), gy = (int)((n / g.nnx) % g.nny), gz = (int)(n / ((size_t)g.nnx * g.nny));
+    const unsigned nnodes = (unsigned)g.nnx * g.nny * g.nnz;
+    const unsigned n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnodes) return;
+    const unsigned nx = g.nnx, nxy = (unsigned)g.nnx * g.nny;
+    const int gz = (int)(n / nxy), rem = (int)(n - (unsigned)gz * nxy);
+    const int gy = rem / (int)nx, gx = rem - gy * (int)nx;
     constexpr int TX = DIM == 3 ? UST_TX3 : UST_TX2, TY = DIM == 3 ? UST_TY3 : UST_TY2, TZ = DIM == 3 ? UST_TZ3 : 1;
     constexpr int NX = TX + 1, NY = TY + 1, NN = NX * NY * (DIM == 3 ? TZ + 1 : 1);
     int tx[2], lxs[2], ty[2], lys[2], tz[2] = {0, 0}, lzs[2] = {0, 0};
@@ -468,16 +478,24 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned 
     const int nys = node_tiles(gy, TY, g.tny, ty, lys);
     const int nzs = DIM == 3 ? node_tiles(gz, TZ, g.tnz, tz, lzs) : 1;
     if (nxs * nys * nzs == 1) return;  // written by its only tile
-    const EpiOps<T> ops = epilogue_load<MODE, T>(ep, mask, dof);
-    T acc = (T)0;
+    EpiOps<T> ops[DIM];
+    T acc[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; c++) {
+        ops[c] = epilogue_load<MODE, T>(ep, mask, (size_t)n * DIM + c);
+        acc[c] = (T)0;
+    }
     for (int a = 0; a < nzs; a++)
         for (int b = 0; b < nys; b++)
             for (int d = 0; d < nxs; d++) {
-                const size_t tile = (size_t)tx[d] + (size_t)g.tnx * ((size_t)ty[b] + (size_t)g.tny * tz[a]);
-                const int ln = lxs[d] + NX * (lys[b] + NY * lzs[a]);
-                acc += work[(tile * NN + ln) * DIM + c];
+                const unsigned tile = (unsigned)tx[d] + (unsigned)g.tnx * ((unsigned)ty[b] + (unsigned)g.tny * tz[a]);
+                const unsigned ln = lxs[d] + NX * (lys[b] + NY * lzs[a]);
+                const T *src = work + ((size_t)tile * NN + ln) * DIM;
+#pragma unroll
+                for (int c = 0; c < DIM; c++) acc[c] += src[c];
             }
-    epilogue_store<MODE, T>(ep, ops, dof, acc);
+#pragma unroll
+    for (int c = 0; c < DIM; c++) epilogue_store<MODE, T>(ep, ops[c], (size_t)n * DIM + c, acc[c]);
 }
 
 static inline size_t work_len(const GridDev &g) {
@@ -496,11 +514,11 @@ static int apply_tiled(const ust_grid *gr, const T *W, const unsigned char *mask
     hipStream_t st = ust_stream(stream);
     if (g.dim == 3) {
         hipLaunchKernelGGL((k_applyK3_tiled<T, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep, work);
-        hipLaunchKernelGGL((k_gather_tiles<T, 3, MODE>), dim3(ust_grid_size(ndof, 256)), dim3(256), 0, st, g, mask,
+        hipLaunchKernelGGL((k_gather_tiles<T, 3, MODE>), dim3(ust_grid_size(ndof / 3, 256)), dim3(256), 0, st, g, mask,
                            work, ep);
     } else {
         hipLaunchKernelGGL((k_applyK2_tiled<T, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep, work);
-        hipLaunchKernelGGL((k_gather_tiles<T, 2, MODE>), dim3(ust_grid_size(ndof, 256)), dim3(256), 0, st, g, mask,
+        hipLaunchKernelGGL((k_gather_tiles<T, 2, MODE>), dim3(ust_grid_size(ndof / 2, 256)), dim3(256), 0, st, g, mask,
                            work, ep);
     }
     UST_LAUNCH_CHECK();

@@ -313,16 +313,17 @@ template <int DIM, typename T>
 __global__ __launch_bounds__(256) void k_restrict(GridDev gf, GridDev gc, const unsigned char *__restrict__ maskf,
                                                   const unsigned char *__restrict__ maskc, const T *__restrict__ rf,
                                                   T *__restrict__ bc) {
-    const size_t ndc = (size_t)gc.nnx * gc.nny * gc.nnz * DIM;
-    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const unsigned ndc = (unsigned)gc.nnx * gc.nny * gc.nnz * DIM;  // 32-bit index arithmetic (< 2^31 dofs)
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ndc) return;
     if (maskc[t]) {
         bc[t] = (T)0;
         return;
     }
     const int c = (int)(t % DIM);
-    const size_t n = t / DIM;
-    const int I = (int)(n % gc.nnx), J = (int)((n / gc.nnx) % gc.nny), K = (int)(n / ((size_t)gc.nnx * gc.nny));
+    const unsigned n = t / DIM, nxy = (unsigned)gc.nnx * gc.nny;
+    const int K = (int)(n / nxy), rem = (int)(n - (unsigned)K * nxy);
+    const int J = rem / gc.nnx, I = rem - J * gc.nnx;
     T acc = (T)0;
 #pragma unroll
     for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); dz++) {
