@@ -461,23 +461,60 @@ __device__ __forceinline__ int node_tiles(int gc, int T, int tn, int *tt, int *l
     return 1;
 }
 
-// one thread per node (32-bit index arithmetic: the grids here stay below 2^31 dofs)
+// Phase 2 visits only the nodes on tile faces, enumerated as three disjoint regular sets so that every
+// thread has work: Z = the z-face planes (all x, y), Y = the y-face planes off the z-faces, X = the
+// x-face lines off both.  An axis with tn tiles has tn - 1 face coordinates T, 2T, ..., (tn-1)T.
+__device__ __forceinline__ int face_coord(int m, int T) { return T * (m + 1); }
+
+// k-th coordinate that is not a tile face: 0, 1..T-1, T+1..2T-1, ..., (tn-1)T+1 .. end
+__device__ __forceinline__ int nonface_coord(int k, int T, int tn) {
+    if (k == 0) return 0;
+    const int kk = k - 1, blk = kk / (T - 1);
+    if (blk < tn - 1) return 1 + blk * T + kk % (T - 1);
+    return 1 + (tn - 1) * T + (kk - (tn - 1) * (T - 1));
+}
+
+template <int DIM>
+static inline unsigned face_node_count(const GridDev &g) {
+    const unsigned fx = g.tnx - 1, fy = g.tny - 1, fz = DIM == 3 ? g.tnz - 1 : 0;
+    return fz * g.nnx * g.nny + fy * g.nnx * (g.nnz - fz) + fx * (g.nny - fy) * (g.nnz - fz);
+}
+
+// one thread per tile-face node, all components (32-bit index arithmetic: grids stay below 2^31 dofs)
 template <typename T, int DIM, int MODE>
 __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned char *__restrict__ mask,
                                                       const T *__restrict__ work, Epi<T> ep) {
-    const unsigned nnodes = (unsigned)g.nnx * g.nny * g.nnz;
-    const unsigned n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nnodes) return;
-    const unsigned nx = g.nnx, nxy = (unsigned)g.nnx * g.nny;
-    const int gz = (int)(n / nxy), rem = (int)(n - (unsigned)gz * nxy);
-    const int gy = rem / (int)nx, gx = rem - gy * (int)nx;
     constexpr int TX = DIM == 3 ? UST_TX3 : UST_TX2, TY = DIM == 3 ? UST_TY3 : UST_TY2, TZ = DIM == 3 ? UST_TZ3 : 1;
     constexpr int NX = TX + 1, NY = TY + 1, NN = NX * NY * (DIM == 3 ? TZ + 1 : 1);
+    const int fx = g.tnx - 1, fy = g.tny - 1, fz = DIM == 3 ? g.tnz - 1 : 0;
+    const unsigned nZ = (unsigned)fz * g.nnx * g.nny;
+    const unsigned nY = (unsigned)fy * g.nnx * (g.nnz - fz);
+    const unsigned nX = (unsigned)fx * (g.nny - fy) * (g.nnz - fz);
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    int gx, gy, gz;
+    if (t < nZ) {
+        const unsigned r = t / g.nnx;
+        gx = (int)(t - r * g.nnx);
+        gy = (int)(r % g.nny);
+        gz = face_coord((int)(r / g.nny), TZ);
+    } else if ((t -= nZ) < nY) {
+        const unsigned r = t / g.nnx;
+        gx = (int)(t - r * g.nnx);
+        gy = face_coord((int)(r % fy), TY);
+        gz = DIM == 3 ? nonface_coord((int)(r / fy), TZ, g.tnz) : 0;
+    } else if ((t -= nY) < nX) {
+        const unsigned r = t / fx, ny = g.nny - fy;
+        gx = face_coord((int)(t - r * fx), TX);
+        gy = nonface_coord((int)(r % ny), TY, g.tny);
+        gz = DIM == 3 ? nonface_coord((int)(r / ny), TZ, g.tnz) : 0;
+    } else {
+        return;
+    }
+    const unsigned n = (unsigned)gx + (unsigned)g.nnx * ((unsigned)gy + (unsigned)g.nny * gz);
     int tx[2], lxs[2], ty[2], lys[2], tz[2] = {0, 0}, lzs[2] = {0, 0};
     const int nxs = node_tiles(gx, TX, g.tnx, tx, lxs);
     const int nys = node_tiles(gy, TY, g.tny, ty, lys);
     const int nzs = DIM == 3 ? node_tiles(gz, TZ, g.tnz, tz, lzs) : 1;
-    if (nxs * nys * nzs == 1) return;  // written by its only tile
     EpiOps<T> ops[DIM];
     T acc[DIM];
 #pragma unroll
@@ -510,16 +547,17 @@ static int apply_tiled(const ust_grid *gr, const T *W, const unsigned char *mask
     if (!W || !mask || !u || !ep.y || !work) return UST_EINVAL;
     if (MODE != EPI_Y && !ep.b) return UST_EINVAL;
     if ((MODE == EPI_CHEBY || MODE == EPI_CHEBY0) && (!ep.invd || !ep.d || ep.y != u)) return UST_EINVAL;
-    const size_t ndof = (size_t)g.nnx * g.nny * g.nnz * g.dim;
     hipStream_t st = ust_stream(stream);
     if (g.dim == 3) {
         hipLaunchKernelGGL((k_applyK3_tiled<T, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep, work);
-        hipLaunchKernelGGL((k_gather_tiles<T, 3, MODE>), dim3(ust_grid_size(ndof / 3, 256)), dim3(256), 0, st, g, mask,
-                           work, ep);
+        const unsigned nf = face_node_count<3>(g);
+        if (nf) hipLaunchKernelGGL((k_gather_tiles<T, 3, MODE>), dim3(ust_grid_size(nf, 256)), dim3(256), 0, st, g,
+                                   mask, work, ep);
     } else {
         hipLaunchKernelGGL((k_applyK2_tiled<T, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep, work);
-        hipLaunchKernelGGL((k_gather_tiles<T, 2, MODE>), dim3(ust_grid_size(ndof / 2, 256)), dim3(256), 0, st, g, mask,
-                           work, ep);
+        const unsigned nf = face_node_count<2>(g);
+        if (nf) hipLaunchKernelGGL((k_gather_tiles<T, 2, MODE>), dim3(ust_grid_size(nf, 256)), dim3(256), 0, st, g,
+                                   mask, work, ep);
     }
     UST_LAUNCH_CHECK();
     return UST_OK;
