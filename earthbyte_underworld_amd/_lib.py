@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(_PKG, "libust.so")
 
 UST_OK = 0
 PREC_FP64, PREC_MIXED = 0, 1
-SCHUR_UW, SCHUR_LEARNED = 0, 1
+SCHUR_UW, SCHUR_LEARNED, SCHUR_SCALED = 0, 1, 2
 RH_MATERIALVISCOSITY, RH_ARRHENIUS, RH_VISCOSITYLIMITER, RH_VONMISES, RH_DRUCKERPRAGER = 1, 2, 3, 4, 5
 SHAPE_EVERYWHERE, SHAPE_BELOWCOSINEPLANE, SHAPE_SPHERE, SHAPE_BOX = 0, 1, 2, 3
 TERM_CONSTITUTIVE, TERM_GRADIENT, TERM_UZAWA_PC, TERM_BUOYANCY = 1, 2, 3, 4
@@ -40,6 +40,7 @@ class SolveOpts(C.Structure):
                 ("smoother_degree", C.c_int), ("cheby_ratio", C.c_double), ("outer_rtol", C.c_double),
                 ("inner_rtol", C.c_double), ("max_outer", C.c_int),
                 ("max_inner", C.c_int), ("remove_nullspace", C.c_int), ("stall_window", C.c_int), ("inner_relax", C.c_int),
+                ("inner_relax_cap", C.c_double),
                 ("verbose", C.c_int)]
 
 
@@ -111,6 +112,7 @@ _SIGS = {
     "ust_mlp_param_counts": (I, [I, I, C.POINTER(I), C.POINTER(I)]),
     "ust_mlp_features": (I, [vp, vp]),
     "ust_mlp_probe_target": (I, [vp, C.POINTER(SolveOpts), I, C.c_ulonglong, vp]),
+    "ust_set_schur_scaling": (I, [vp, vp]),
     "ust_mlp_train": (I, [vp, I, vp, vp, I, I, C.c_float, C.c_ulonglong, C.POINTER(C.c_float)]),
     "ust_mlp_predict": (I, [vp, I, vp, vp]),
     "ust_mlp_gradients": (I, [vp, I, vp, vp, vp, vp, vp]),

@@ -38,6 +38,7 @@ void ust_solve_opts_default(ust_solve_opts *o) {
     o->max_inner = 500;
     o->remove_nullspace = -1;
     o->stall_window = 30;
+    o->inner_relax_cap = 1e-2;
     o->verbose = 0;
 }
 
@@ -98,7 +99,7 @@ static void free_mesh_data(ust_context *ctx) {
     DFREE(ctx->wr); DFREE(ctx->wz); DFREE(ctx->wp); DFREE(ctx->wq); DFREE(ctx->wx);
     DFREE(ctx->pr); DFREE(ctx->pz); DFREE(ctx->pd); DFREE(ctx->pq); DFREE(ctx->ph); DFREE(ctx->ppc);
     DFREE(ctx->pprev); DFREE(ctx->uprev); DFREE(ctx->upic);
-    DFREE(ctx->feat); DFREE(ctx->mlp_out); DFREE(ctx->act1); DFREE(ctx->act2);
+    DFREE(ctx->feat); DFREE(ctx->mlp_out); DFREE(ctx->act1); DFREE(ctx->act2); DFREE(ctx->user_scale);
     ctx->feat_cap = 0;
     free(ctx->ke_cache); free(ctx->ge_cache); free(ctx->fe_cache); free(ctx->as_cache);
     ctx->ke_cache = ctx->ge_cache = ctx->fe_cache = ctx->as_cache = NULL;

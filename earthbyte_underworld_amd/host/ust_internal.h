@@ -106,6 +106,7 @@ struct ust_context {
     /* learned scaling */
     ust_mlp mlp;
     float *feat, *mlp_out, *act1, *act2;
+    float *user_scale; /* UST_SCHUR_SCALED: log10 factors on approxS (nel) */
     int feat_cap;
     /* element-matrix cache for the per-element plugin callbacks */
     double *ke_cache, *ge_cache, *fe_cache, *as_cache;

@@ -417,6 +417,14 @@ static int pcg(ust_context *ctx, int pc32, const double *b, double *x, double rt
     return UST_OK;
 }
 
+int ust_set_schur_scaling(ust_context *ctx, const float *log10_scale) {
+    if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "set the mesh first");
+    if (!log10_scale) UST_FAIL(ctx, UST_EINVAL, "no scaling given");
+    if (!ctx->user_scale) DALLOC(ctx, ctx->user_scale, (size_t)ctx->nel * sizeof(float));
+    CKD(ctx, ust_memcpy_h2d(ctx->user_scale, log10_scale, (size_t)ctx->nel * sizeof(float), S(ctx)));
+    return UST_OK;
+}
+
 int ust_solver_prepare(ust_context *ctx, const ust_solve_opts *o) {
     int need32 = o->precision == UST_PREC_MIXED, rc;
     if (!ctx->assembled) UST_FAIL(ctx, UST_ESTATE, "assemble first");
@@ -481,7 +489,7 @@ static int schur_cg(ust_context *ctx, const ust_solve_opts *o, int ns, const dou
         double rtol = o->inner_rtol;
         if (o->inner_relax && rn > 0.0) {
             rtol = o->inner_rtol * hn / rn;
-            if (rtol > 1e-3) rtol = 1e-3;
+            if (rtol > o->inner_relax_cap) rtol = o->inner_relax_cap;
             if (rtol < o->inner_rtol) rtol = o->inner_rtol;
         }
         if ((rc = schur_apply(ctx, o, ctx->pd, ctx->pq, rtol, st))) return rc;
@@ -544,8 +552,13 @@ int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st)
     CKD(ctx, ust_memcpy_d2d(ctx->pr, ctx->ph, ne * sizeof(double), S(ctx)));
     int stalled = 0;
     const double *pc = ctx->approxS;
-    if (o.schur_pc == UST_SCHUR_LEARNED) {
-        if ((rc = ust_mlp_schur_pc(ctx, ctx->ppc))) return rc;
+    if (o.schur_pc == UST_SCHUR_LEARNED || o.schur_pc == UST_SCHUR_SCALED) {
+        if (o.schur_pc == UST_SCHUR_LEARNED) {
+            if ((rc = ust_mlp_schur_pc(ctx, ctx->ppc))) return rc;
+        } else {
+            if (!ctx->user_scale) UST_FAIL(ctx, UST_ESTATE, "UST_SCHUR_SCALED needs ust_set_schur_scaling");
+            CKD(ctx, ust_k_apply_scaling(ctx->approxS, ctx->user_scale, ctx->ppc, ctx->nel, S(ctx)));
+        }
         pc = ctx->ppc;
         if ((rc = schur_cg(ctx, &o, ns, pc, o.stall_window, hn, st, &stalled))) return rc;
         if (stalled) {

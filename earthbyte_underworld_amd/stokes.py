@@ -360,6 +360,12 @@ class StokesSystem:
         self._check(self.lib.ust_mlp_probe_target(self.ctx, C.byref(o), 0, 0, _ptr(t)))
         return t
 
+    def set_schur_scaling(self, log10_scale):
+        """log10 factors on approxS for schur_pc=SCHUR_SCALED (one per pressure cell)."""
+        a = np.ascontiguousarray(log10_scale, dtype=np.float32)
+        assert a.shape[0] == self.nel
+        self._check(self.lib.ust_set_schur_scaling(self.ctx, _ptr(a)))
+
     def mlp_train(self, feat, target, epochs=20, batch=4096, lr=1e-3, seed=1):
         feat = np.ascontiguousarray(feat, dtype=np.float32)
         target = np.ascontiguousarray(target, dtype=np.float32)

@@ -30,11 +30,12 @@ from earthbyte_underworld_amd.stokes import default_opts  # noqa: E402
 DEFAULT_OUT = os.path.join(HERE, "data", "schur_mlp_3d.npz")
 
 
-def field(kind, n, seed, ppc):
-    """kind 'spheres': 40 seeded spheres (log10 eta U[-2,4]); 'grf': Gaussian random field mapped to [-3,3]."""
+def field(kind, n, seed, ppc, grf_box=(2.0, 2.0, 1.0)):
+    """kind 'spheres': 40 seeded spheres (log10 eta U[-2,4]) in the unit cube at n^3; 'grf': Gaussian random
+    field (correlation length 0.1) mapped to [-3,3] on n x n x n/2 elements over grf_box."""
     if kind == "spheres":
         return configs.spheres_3d(nel=(n, n, n), ppc=ppc, nsph=40, seed=seed)
-    return configs.random_field_3d(nel=(n, n, n // 2), ppc=ppc, seed=seed)
+    return configs.random_field_3d(nel=(n, n, n // 2), ppc=ppc, seed=seed, box=tuple(grf_box))
 
 
 def probe(s, inner_rtol):
@@ -53,6 +54,8 @@ def outer_its(s, learned, rtol=1e-7):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--size", type=int, default=32)
+    ap.add_argument("--grf-size", type=int, default=0, help="random-field grid n (n x n x n/2); 0: --size")
+    ap.add_argument("--grf-box", type=float, nargs=3, default=(2.0, 2.0, 1.0))
     ap.add_argument("--ppc", type=int, default=12)
     ap.add_argument("--train", type=int, default=24, help="training fields (half spheres, half random fields)")
     ap.add_argument("--heldout", type=int, default=4)
@@ -69,7 +72,7 @@ def main(argv=None):
     kinds = ["spheres", "grf"]
     for k in range(a.train):
         kind = kinds[k % 2]
-        s = field(kind, a.size, 1000 + k, a.ppc)
+        s = field(kind, (a.grf_size or a.size) if kind == "grf" else a.size, 1000 + k, a.ppc, a.grf_box)
         f, t = probe(s, a.inner_rtol)
         s.close()
         X.append(f)
@@ -98,7 +101,7 @@ def main(argv=None):
     evals = []
     for k in range(a.heldout):
         kind = kinds[k % 2]
-        s = field(kind, a.size, 5000 + k, a.ppc)
+        s = field(kind, (a.grf_size or a.size) if kind == "grf" else a.size, 5000 + k, a.ppc, a.grf_box)
         s.assemble()
         s.mlp_set_params(w, b)
         plain = outer_its(s, False)
@@ -110,7 +113,7 @@ def main(argv=None):
         print(f"[eval] {kind} seed {5000 + k}: outer its plain {plain['outer_its']} learned "
               f"{learned['outer_its']}", flush=True)
         s.close()
-    record = {"size": a.size, "ppc": a.ppc, "train_fields": a.train, "samples": int(len(Y)),
+    record = {"size": a.size, "grf_size": a.grf_size or a.size, "grf_box": list(a.grf_box), "ppc": a.ppc, "train_fields": a.train, "samples": int(len(Y)),
               "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "final_loss": float(losses[-1]),
               "target_variance": float(Y.var()), "probe_seconds": round(t_probe, 1),
               "train_seconds": round(t_train, 1), "heldout": evals}

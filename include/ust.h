@@ -23,6 +23,7 @@ typedef struct ust_context ust_context;
 
 #define UST_SCHUR_UW 0       /* approxS = g^T diag(K)^-1 g per element ("-Q22_pc_type uw") */
 #define UST_SCHUR_LEARNED 1  /* approxS scaled by the MLP; plain "uw" remains the fallback */
+#define UST_SCHUR_SCALED 2   /* approxS scaled by 10^s, s set by ust_set_schur_scaling; plain "uw" the fallback */
 
 typedef struct {
     int precision;           /* UST_PREC_* */
@@ -37,7 +38,9 @@ typedef struct {
     int remove_nullspace;    /* -1: when ||G 1|| / ||1|| < 1e-7 ||D||_inf (ksp_pressure_nullspace.c:139-158); 0/1 force */
     int stall_window;        /* learned PC: fall back to plain if the residual does not halve in this many its */
     int inner_relax;         /* 1: the inner tolerance of each Schur iteration is inner_rtol * ||h_hat|| / ||r_k||
-                                (capped at 1e-3), relaxed as the outer residual falls; 0: fixed, as the reference */
+                                (capped at inner_relax_cap), relaxed as the outer residual falls; 0: fixed, as
+                                the reference */
+    double inner_relax_cap;  /* largest relaxed inner tolerance (default 1e-2) */
     int verbose;
 } ust_solve_opts;
 
@@ -153,6 +156,8 @@ int ust_mlp_param_counts(int dim, int hidden, int *nw, int *nb);
 /* features of the current assembly (nel * 3^dim) */
 int ust_mlp_features(ust_context *ctx, float *feat);
 /* target for training: log10 of the ratio between a probed Schur diagonal and approxS */
+/* per-cell log10 factor on approxS for UST_SCHUR_SCALED (e.g. the probed target itself), nel values */
+int ust_set_schur_scaling(ust_context *ctx, const float *log10_scale);
 int ust_mlp_probe_target(ust_context *ctx, const ust_solve_opts *opts, int nprobe, unsigned long long seed,
                          float *target);
 /* n samples (host arrays) -> epochs of Adam on the GPU; returns the final loss */
