@@ -129,9 +129,9 @@ __device__ double particle_viscosity(const GridDev &g, const ust_material &M, co
 // ------------------------------------------------------------------------------------------
 #define PW_WAVES 2
 #define PW_CH 11
-template <int DIM, bool NEED_T, bool NEED_VEL>
+template <int DIM, int S, bool NEED_T, bool NEED_VEL>
 __global__ __launch_bounds__(64 * PW_WAVES) void k_particle_weights(
-    GridDev g, int S, const int *__restrict__ cell_off, const double *__restrict__ xi0, const double *__restrict__ xi1,
+    GridDev g, const int *__restrict__ cell_off, const double *__restrict__ xi0, const double *__restrict__ xi1,
     const double *__restrict__ xi2, const double *__restrict__ w, const unsigned char *__restrict__ mat,
     const float *__restrict__ coh, const ust_material *__restrict__ mats, int nmat, const double *__restrict__ T,
     const double *__restrict__ vel, const double *__restrict__ pres, int prev, double *__restrict__ eta_out,
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(64 * PW_WAVES) void k_particle_weights(
     constexpr int LD = PW_CH + 1;    // padded LDS row
     __shared__ double red[PW_WAVES][64 * LD];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int nseg = 64 / S;
+    constexpr int nseg = 64 / S;
     const int seg = lane / S, k = lane % S;
     const int ebase = (blockIdx.x * PW_WAVES + wv) * nseg;
     const int e = ebase + seg;
@@ -209,8 +209,11 @@ __global__ __launch_bounds__(64 * PW_WAVES) void k_particle_weights(
         for (int t = lane; t < nseg * nc; t += 64) {
             const int sg = t / nc, col = c0 + t % nc, el = ebase + sg;
             if (el < g.nel) {
-                double sum = 0.0;
-                for (int row = sg * S; row < sg * S + S; row++) sum += r[row * LD + (col - c0)];
+                // S independent LDS reads into four partial sums (S is a compile-time constant)
+                double ps[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int row = 0; row < S; row++) ps[row & 3] += r[(sg * S + row) * LD + (col - c0)];
+                const double sum = (ps[0] + ps[1]) + (ps[2] + ps[3]);
                 if (col < NQ) WK[widx(g, el, col)] = sum;
                 else if (col < NQ + NV) WG[(size_t)(col - NQ) * g.nel + el] = sum;
                 else WF[(size_t)(col - NQ - NV) * g.nel + el] = sum;
@@ -395,10 +398,21 @@ extern "C" int ust_k_particle_weights(const ust_grid *gr, const int *cell_off, i
     const int elems_per_block = PW_WAVES * (64 / S);
     const unsigned grid = ust_grid_size(g.nel, elems_per_block);
     hipStream_t st = ust_stream(stream);
-#define LAUNCH_S(D, NT, NVL)                                                                                      \
-    hipLaunchKernelGGL((k_particle_weights<D, NT, NVL>), dim3(grid), dim3(64 * PW_WAVES), 0, st, g, S, cell_off, \
-                       xi0, xi1, xi2, w, mat, coh_scale, materials_dev, nmat, temperature, velocity, pressure,    \
+#define LAUNCH_PW(D, SS, NT, NVL)                                                                                 \
+    hipLaunchKernelGGL((k_particle_weights<D, SS, NT, NVL>), dim3(grid), dim3(64 * PW_WAVES), 0, st, g, cell_off, \
+                       xi0, xi1, xi2, w, mat, coh_scale, materials_dev, nmat, temperature, velocity, pressure,     \
                        prev_solution, eta_out, yield_out, WK, WG, WF)
+#define LAUNCH_S(D, NT, NVL)                                  \
+    do {                                                      \
+        switch (S) {                                          \
+            case 8: LAUNCH_PW(D, 8, NT, NVL); break;          \
+            case 12: LAUNCH_PW(D, 12, NT, NVL); break;        \
+            case 16: LAUNCH_PW(D, 16, NT, NVL); break;        \
+            case 21: LAUNCH_PW(D, 21, NT, NVL); break;        \
+            case 32: LAUNCH_PW(D, 32, NT, NVL); break;        \
+            default: LAUNCH_PW(D, 64, NT, NVL); break;        \
+        }                                                     \
+    } while (0)
 #define LAUNCH_F(D)                                      \
     do {                                                 \
         if (needT && needV) LAUNCH_S(D, true, true);     \
@@ -410,6 +424,7 @@ extern "C" int ust_k_particle_weights(const ust_grid *gr, const int *cell_off, i
     else LAUNCH_F(2);
 #undef LAUNCH_F
 #undef LAUNCH_S
+#undef LAUNCH_PW
     UST_LAUNCH_CHECK();
     return UST_OK;
 }

@@ -369,7 +369,11 @@ int ust_set_previous_solution_exists(ust_context *ctx, int flag) {
 int ust_set_bcs(ust_context *ctx, const unsigned char *mask, const double *values) {
     if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "no mesh");
     memcpy(ctx->mask_h, mask, (size_t)ctx->ndof);
-    for (int i = 0; i < ctx->ndof; i++) ctx->bcval_h[i] = (mask[i] && values) ? values[i] : 0.0;
+    ctx->bc_nonzero = 0;
+    for (int i = 0; i < ctx->ndof; i++) {
+        ctx->bcval_h[i] = (mask[i] && values) ? values[i] : 0.0;
+        if (ctx->bcval_h[i] != 0.0) ctx->bc_nonzero = 1;
+    }
     CKD(ctx, ust_memcpy_h2d(ctx->mask_d, ctx->mask_h, (size_t)ctx->ndof, ctx->stream));
     CKD(ctx, ust_memcpy_h2d(ctx->bcval_d, ctx->bcval_h, (size_t)ctx->ndof * sizeof(double), ctx->stream));
     ctx->has_bcs = 1;
@@ -407,9 +411,11 @@ int ust_assemble(ust_context *ctx, double gravity) {
     /* f = buoyancy - K_{free,bc} u_bc ; h = -G_bc^T u_bc */
     CKD(ctx, ust_k_fill_f64(ctx->f, 0.0, nd, ctx->stream));
     CKD(ctx, ust_k_force(&ctx->grid, ctx->WF, gravity, ctx->mask_d, ctx->f, ctx->stream));
-    CKD(ctx, ust_k_lift_K(&ctx->grid, ctx->WK, ctx->mask_d, ctx->bcval_d, ctx->f, ctx->stream));
     CKD(ctx, ust_k_fill_f64(ctx->h, 0.0, ne, ctx->stream));
-    CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, ctx->bcval_d, ctx->h, 1, ctx->stream));
+    if (ctx->bc_nonzero) { /* homogeneous Dirichlet values lift nothing */
+        CKD(ctx, ust_k_lift_K(&ctx->grid, ctx->WK, ctx->mask_d, ctx->bcval_d, ctx->f, ctx->stream));
+        CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, ctx->bcval_d, ctx->h, 1, ctx->stream));
+    }
     /* is the constant pressure t in the null space of G?  ||G t|| / ||t|| < nstol ||D||_inf, nstol = 1e-7
      * (KSPRemovePressureNullspace_BSSCR, ksp_pressure_nullspace.c:139-158; BSSCR.c:391).  With particle
      * quadrature G t is usually not zero, and the constant is then part of the solution. */

@@ -69,6 +69,7 @@ struct ust_context {
     int prev_solution;
     /* BCs */
     int has_bcs;
+    int bc_nonzero; /* some Dirichlet value is not zero (the RHS needs lifting) */
     unsigned char *mask_d, *mask_h;
     double *bcval_d, *bcval_h;
     int nullspace; /* constant pressure found in the null space of G (ksp_pressure_nullspace.c:139-158 test) */
