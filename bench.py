@@ -53,9 +53,10 @@ def time_on_stream(lib, stream, fn, reps):
 
 def solve_opts(ust, precision, learned, relax=0, degree=2):
     from earthbyte_underworld_amd import _lib as L
+    # stokes_rtol: the metric's 1e-6 bar on the FP64 Stokes residual is enforced, not assumed
     return ust.default_opts(precision=precision, schur_pc=L.SCHUR_LEARNED if learned else L.SCHUR_UW,
                             outer_rtol=1e-7, inner_rtol=1e-8, max_outer=2000, inner_relax=relax,
-                            smoother_degree=degree)
+                            smoother_degree=degree, stokes_rtol=1e-6)
 
 
 def cpu_baseline(size, ppc, nsph, seed, cpu_size, threads):

@@ -38,10 +38,9 @@ class Grid(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("schur_pc", C.c_int), ("mg_levels", C.c_int),
                 ("smoother_degree", C.c_int), ("cheby_ratio", C.c_double), ("outer_rtol", C.c_double),
-                ("inner_rtol", C.c_double), ("max_outer", C.c_int),
-                ("max_inner", C.c_int), ("remove_nullspace", C.c_int), ("stall_window", C.c_int), ("inner_relax", C.c_int),
-                ("inner_relax_cap", C.c_double),
-                ("verbose", C.c_int)]
+                ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("max_inner", C.c_int),
+                ("remove_nullspace", C.c_int), ("stall_window", C.c_int), ("inner_relax", C.c_int),
+                ("inner_relax_cap", C.c_double), ("stokes_rtol", C.c_double), ("verbose", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -49,7 +48,8 @@ class SolveStats(C.Structure):
                 ("refinements", C.c_int), ("fallback_used", C.c_int), ("mg_levels", C.c_int),
                 ("stokes_residual", C.c_double), ("momentum_residual", C.c_double),
                 ("continuity_residual", C.c_double), ("t_assemble_ms", C.c_double), ("t_setup_ms", C.c_double),
-                ("t_solve_ms", C.c_double)]
+                ("t_solve_ms", C.c_double), ("restarts", C.c_int), ("nullspace_removed", C.c_int),
+                ("nullspace_ratio", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

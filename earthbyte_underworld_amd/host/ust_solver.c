@@ -477,7 +477,9 @@ static int schur_cg(ust_context *ctx, const ust_solve_opts *o, int ns, const dou
         if (it < 4096) hist[it] = rn;
         if (o->verbose) fprintf(stderr, "[ust] scr it %d |r|/|h| = %.3e\n", st->outer_its, rn / hn);
         if (rn <= o->outer_rtol * hn) break;
-        if (stall_window > 0 && it >= stall_window && it < 4096 && rn > 0.5 * hist[it - stall_window]) {
+        /* stalled: less than 10 % residual reduction over the last stall_window iterations (a slowly converging
+         * learned run keeps going; a scaling that does not precondition is abandoned) */
+        if (stall_window > 0 && it >= stall_window && it < 4096 && rn > 0.9 * hist[it - stall_window]) {
             *stalled = 1;
             break;
         }
@@ -525,6 +527,31 @@ static int stokes_residual(ust_context *ctx, ust_solve_stats *st) {
     return UST_OK;
 }
 
+/* pressure null space, back-solve u = K^-1 (f - G p) and the FP64 Stokes residual.  With a Stokes-residual
+ * bar the back-solve tolerance is also tied to it: the momentum residual it leaves is its relative residual
+ * times ||f - G p||, which can exceed ||[f; h]|| by orders of magnitude when buoyancy and pressure gradient
+ * nearly cancel (bs_scale: the fraction of the bar given to the back-solve). */
+static int back_solve(ust_context *ctx, const ust_solve_opts *o, int ns, double bs_scale, ust_solve_stats *st) {
+    size_t n = (size_t)ctx->ndof, ne = (size_t)ctx->nel;
+    int rc;
+    if (ns && (rc = remove_mean(ctx, ctx->p))) return rc;
+    CKD(ctx, ust_k_apply_G(&ctx->grid, ctx->WG, ctx->mask_d, ctx->p, ctx->vb, S(ctx)));
+    CKD(ctx, ust_k_residual_f64(ctx->f, ctx->vb, ctx->vb, n, S(ctx)));
+    ust_solve_opts ob = *o;
+    if (o->stokes_rtol > 0.0) {
+        double bb, ff, hh;
+        CKD(ctx, dotv(ctx, ctx->vb, ctx->vb, n, SC_T1, &bb));
+        CKD(ctx, dotv(ctx, ctx->f, ctx->f, n, SC_T1, &ff));
+        CKD(ctx, dotv(ctx, ctx->h, ctx->h, ne, SC_T1, &hh));
+        if (bb > 0.0) {
+            double t = bs_scale * o->stokes_rtol * sqrt((ff + hh) / bb);
+            if (t < ob.inner_rtol) ob.inner_rtol = t;
+        }
+    }
+    if ((rc = a11_solve(ctx, &ob, ctx->vb, ctx->uprev, st))) return rc;
+    return stokes_residual(ctx, st);
+}
+
 int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st) {
     ust_solve_opts o;
     if (opts) o = *opts;
@@ -540,6 +567,8 @@ int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st)
     double t1 = wall_ms();
     st->mg_levels = ctx->nlev;
     int ns = o.remove_nullspace < 0 ? ctx->nullspace : o.remove_nullspace;
+    st->nullspace_removed = ns;
+    st->nullspace_ratio = ctx->ns_ratio;
     /* h_hat = G^T K^-1 f - h */
     if ((rc = a11_solve(ctx, &o, ctx->f, ctx->uprev, st))) return rc;
     CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, ctx->uprev, ctx->ph, 0, S(ctx)));
@@ -552,6 +581,7 @@ int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st)
     CKD(ctx, ust_memcpy_d2d(ctx->pr, ctx->ph, ne * sizeof(double), S(ctx)));
     int stalled = 0;
     const double *pc = ctx->approxS;
+    const double *pc_last = ctx->approxS; /* the preconditioner the Schur CG finished with */
     if (o.schur_pc == UST_SCHUR_LEARNED || o.schur_pc == UST_SCHUR_SCALED) {
         if (o.schur_pc == UST_SCHUR_LEARNED) {
             if ((rc = ust_mlp_schur_pc(ctx, ctx->ppc))) return rc;
@@ -560,8 +590,10 @@ int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st)
             CKD(ctx, ust_k_apply_scaling(ctx->approxS, ctx->user_scale, ctx->ppc, ctx->nel, S(ctx)));
         }
         pc = ctx->ppc;
+        pc_last = pc;
         if ((rc = schur_cg(ctx, &o, ns, pc, o.stall_window, hn, st, &stalled))) return rc;
         if (stalled) {
+            pc_last = ctx->approxS;
             /* plain BSSCR takes over from the current iterate: r = h_hat - S p */
             st->fallback_used = 1;
             if ((rc = schur_apply(ctx, &o, ctx->p, ctx->pq, o.inner_rtol, st))) return rc;
@@ -571,14 +603,22 @@ int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st)
     } else {
         if ((rc = schur_cg(ctx, &o, ns, pc, 0, hn, st, &stalled))) return rc;
     }
-    if (ns && (rc = remove_mean(ctx, ctx->p))) return rc;
-    /* back-solve u = K^-1 (f - G p) */
-    CKD(ctx, ust_k_apply_G(&ctx->grid, ctx->WG, ctx->mask_d, ctx->p, ctx->vb, S(ctx)));
-    CKD(ctx, ust_k_residual_f64(ctx->f, ctx->vb, ctx->vb, n, S(ctx)));
-    if ((rc = a11_solve(ctx, &o, ctx->vb, ctx->uprev, st))) return rc;
-    CKD(ctx, ust_stream_sync(S(ctx)));
+    double bs_scale = 0.3;
+    if ((rc = back_solve(ctx, &o, ns, bs_scale, st))) return rc;
+    for (int pass = 1; o.stokes_rtol > 0.0 && st->stokes_residual > o.stokes_rtol && pass <= 3; pass++) {
+        st->restarts++;
+        if (st->continuity_residual > 0.5 * o.stokes_rtol) {
+            /* inexact inner solves let the recursively updated Schur residual drift from the true one:
+             * restart from the current pressure on r = h_hat - S p with a tighter outer tolerance */
+            o.outer_rtol *= 0.1;
+            if ((rc = schur_apply(ctx, &o, ctx->p, ctx->pq, o.inner_rtol, st))) return rc;
+            CKD(ctx, ust_k_sub_f64(ctx->ph, ctx->pq, ctx->pr, ne, S(ctx)));
+            if ((rc = schur_cg(ctx, &o, ns, pc_last, 0, hn, st, &stalled))) return rc;
+        }
+        bs_scale *= 0.1;
+        if ((rc = back_solve(ctx, &o, ns, bs_scale, st))) return rc;
+    }
     double t2 = wall_ms();
-    if ((rc = stokes_residual(ctx, st))) return rc;
     /* nodal solution with BC values (rheology input of the next assembly) */
     CKD(ctx, ust_memcpy_d2d(ctx->u, ctx->uprev, n * sizeof(double), S(ctx)));
     CKD(ctx, ust_k_axpy_f64(1.0, ctx->bcval_d, ctx->u, n, S(ctx)));

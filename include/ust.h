@@ -36,11 +36,15 @@ typedef struct {
     int max_outer;           /* "-scr_ksp_max_it" */
     int max_inner;           /* "-A11_ksp_max_it" */
     int remove_nullspace;    /* -1: when ||G 1|| / ||1|| < 1e-7 ||D||_inf (ksp_pressure_nullspace.c:139-158); 0/1 force */
-    int stall_window;        /* learned PC: fall back to plain if the residual does not halve in this many its */
+    int stall_window;        /* learned PC: fall back to plain if the residual falls by less than 10 % over this
+                                many iterations */
     int inner_relax;         /* 1: the inner tolerance of each Schur iteration is inner_rtol * ||h_hat|| / ||r_k||
                                 (capped at inner_relax_cap), relaxed as the outer residual falls; 0: fixed, as
                                 the reference */
     double inner_relax_cap;  /* largest relaxed inner tolerance (default 1e-2) */
+    double stokes_rtol;      /* > 0: after the back-solve, while the FP64 Stokes residual exceeds this, restart the
+                                Schur CG from the current pressure on its true residual with a 10x tighter outer
+                                tolerance (at most 3 restarts); 0: stop at outer_rtol, as the reference */
     int verbose;
 } ust_solve_opts;
 
@@ -54,6 +58,9 @@ typedef struct {
     double stokes_residual;  /* ||[f-Ku-Gp; h-G^T u]|| / ||[f;h]|| in FP64 */
     double momentum_residual, continuity_residual;
     double t_assemble_ms, t_setup_ms, t_solve_ms;
+    int restarts;            /* Schur CG restarts taken for stokes_rtol */
+    int nullspace_removed;   /* the constant pressure was projected out (ksp_pressure_nullspace.c test) */
+    double nullspace_ratio;  /* ||G 1|| / ||1|| / ||D||_inf of the assembly (removed when < 1e-7) */
 } ust_solve_stats;
 
 void ust_solve_opts_default(ust_solve_opts *o);

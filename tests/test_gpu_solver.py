@@ -154,3 +154,20 @@ def test_graph_replays_match_eager_launches(monkeypatch):
     (ub, pb), stb, stb32 = out[1]
     assert sta["inner_its"] == stb["inner_its"] and sta32["inner_its"] == stb32["inner_its"]
     assert np.array_equal(ua, ub) and np.array_equal(pa, pb)
+
+
+def test_stokes_residual_bar_restarts(oracle):
+    """stokes_rtol: a loose outer tolerance that misses the FP64 Stokes-residual bar is tightened by restarts
+    from the current pressure until the bar holds; the restarted solution matches a tight direct run."""
+    import earthbyte_underworld_amd as ust
+    s = _spheres_3d(nel=(12, 12, 12), ppc=8)
+    s.set_bcs(*ust.free_slip_walls(s))
+    s.assemble(gravity=1.0)
+    st = s.solve(outer_rtol=1e-3, inner_rtol=1e-10, stokes_rtol=1e-7)
+    u1, p1 = s.solution()
+    assert st["restarts"] >= 1 and st["stokes_residual"] <= 1e-7, st
+    tight = s.solve(outer_rtol=1e-10, inner_rtol=1e-11)
+    u0, p0 = s.solution()
+    assert tight["restarts"] == 0
+    # at a 1e-7 Stokes residual the pressure is the less accurate field (inf-sup amplification)
+    assert rel_err(u1, u0) < 1e-5 and rel_err(p1, p0) < 1e-4
