@@ -197,7 +197,7 @@ static int levels_structure(ust_context *ctx, const ust_solve_opts *o, int need3
     if (maxlev > UST_MAX_LEVELS) maxlev = UST_MAX_LEVELS;
     ust_grid g = ctx->grid;
     int nlev = 1;
-    while (nlev < maxlev && grid_ndof(&g) > 600) {
+    while (nlev < maxlev && grid_ndof(&g) > 1100) { /* the coarsest level takes the dense FP64 solve */
         int ok = 1;
         for (int d = 0; d < dim; d++)
             if (g.nel[d] % 2 || g.nel[d] < 2) ok = 0;
