@@ -73,6 +73,18 @@ __global__ __launch_bounds__(256) void kv_dot_stage1(const T *x, const T *y, siz
     }
 }
 
+// x . y with x in FP64 and y in FP32 (the PCG's r . z with the FP32 V-cycle output z)
+__global__ __launch_bounds__(256) void kv_dot_mixed_stage1(const double *x, const float *y, size_t n,
+                                                           double *partial) {
+    __shared__ double sh[256 / 64];
+    double s = 0.0;
+    GRID_STRIDE(i, n) s += x[i] * (double)y[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
 __global__ __launch_bounds__(256) void kv_reduce_stage2(const double *partial, int np, double *out) {
     __shared__ double sh[4];
     double s = 0.0;
@@ -166,6 +178,34 @@ __global__ void kv_cg_xr_guarded(double *x, double *r, const double *p, const do
     }
 }
 
+__global__ void kv_cg_p_guarded_zf(const float *z, double *p, const double *c, size_t n) {
+    if (c[1] != 0.0) return;
+    const double beta = c[3];
+    GRID_STRIDE(i, n) p[i] = (double)z[i] + beta * p[i];
+}
+
+// x += a p, r -= a q and the block partials of r . r (zero partials once converged, so the stage-2 sum is 0
+// and the check kernel ignores it)
+__global__ __launch_bounds__(256) void kv_cg_xr_rr_guarded(double *x, double *r, const double *p, const double *q,
+                                                           const double *rz, const double *pq, const double *c,
+                                                           size_t n, double *partial) {
+    __shared__ double sh[256 / 64];
+    double s = 0.0;
+    if (c[1] == 0.0) {
+        const double a = rz[0] / pq[0];
+        GRID_STRIDE(i, n) {
+            x[i] += a * p[i];
+            const double ri = r[i] - a * q[i];
+            r[i] = ri;
+            s += ri * ri;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
 __global__ void kv_cg_check(const double *rr, double *c) {
     if (c[1] != 0.0) return;
     c[2] += 1.0;
@@ -188,6 +228,18 @@ template <typename T>
 __global__ void kv_cheby_first(const T *b, const T *invd, T *d, T *x, T c2, size_t n) {
     GRID_STRIDE(i, n) {
         const T dn = c2 * invd[i] * b[i];
+        d[i] = dn;
+        x[i] = dn;
+    }
+}
+
+// first Chebyshev step of the FP32 V-cycle on an FP64 right-hand side: b32 = (float) b, d = c2 invd b32, x = d
+__global__ void kv_cheby_first_from64(const double *b, float *b32, const float *invd, float *d, float *x, float c2,
+                                      size_t n) {
+    GRID_STRIDE(i, n) {
+        const float bi = (float)b[i];
+        const float dn = c2 * invd[i] * bi;
+        b32[i] = bi;
         d[i] = dn;
         x[i] = dn;
     }
@@ -301,6 +353,30 @@ int ust_k_cg_p_guarded_f64(const double *z, double *p, const double *ctl, size_t
 int ust_k_cg_xr_guarded_f64(double *x, double *r, const double *p, const double *q, const double *rz,
                             const double *pq, const double *ctl, size_t n, void *stream) {
     LV(kv_cg_xr_guarded, n, x, r, p, q, rz, pq, ctl, n);
+}
+int ust_k_dot_mixed(const double *x, const float *y, size_t n, double *partial, double *out, void *stream) {
+    const unsigned g = rgrid(n);
+    hipLaunchKernelGGL(kv_dot_mixed_stage1, dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
+    hipLaunchKernelGGL(kv_reduce_stage2, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, out);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+int ust_k_cg_p_guarded_zf(const float *z, double *p, const double *ctl, size_t n, void *stream) {
+    LV(kv_cg_p_guarded_zf, n, z, p, ctl, n);
+}
+int ust_k_cg_xr_rr_guarded_f64(double *x, double *r, const double *p, const double *q, const double *rz,
+                               const double *pq, const double *ctl, size_t n, double *partial, double *rr,
+                               void *stream) {
+    const unsigned g = rgrid(n);
+    hipLaunchKernelGGL(kv_cg_xr_rr_guarded, dim3(g), dim3(256), 0, ust_stream(stream), x, r, p, q, rz, pq, ctl, n,
+                       partial);
+    hipLaunchKernelGGL(kv_reduce_stage2, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, rr);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+int ust_k_cheby_first_from_f64(const double *b, float *b32, const float *invd, float *d, float *x, float c2,
+                               size_t n, void *stream) {
+    LV(kv_cheby_first_from64, n, b, b32, invd, d, x, c2, n);
 }
 int ust_k_cg_check(const double *rr, double *ctl, void *stream) {
     hipLaunchKernelGGL(kv_cg_check, dim3(1), dim3(1), 0, ust_stream(stream), rr, ctl);

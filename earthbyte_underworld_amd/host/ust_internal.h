@@ -99,6 +99,7 @@ struct ust_context {
     void *pcg_exec[2];                /* captured inner PCG iteration (FP64 / FP32 V-cycle), NULL if stale */
     int use_graphs;                   /* 0: launch the inner iteration kernel by kernel (UST_NO_GRAPH=1) */
     int pcg_hint;                     /* iterations of the last inner solve: first convergence check */
+    const double *b64_src;            /* set while a level-0 FP32 V-cycle starts from an FP64 right-hand side */
     float *wr, *wz, *wp, *wq, *wx;    /* FP32 velocity work (mixed inner PCG) */
     double *va, *vb;                  /* FP64 velocity work (Schur / IR) */
     double *pr, *pz, *pd, *pq, *ph, *ppc, *pprev; /* pressure work */

@@ -104,7 +104,10 @@ static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void *x, 
     size_t n = (size_t)L->ndof;
     for (int k = 0; k < L->cheb_deg; k++) {
         if (k == 0 && x_zero) {
-            if (f32)
+            if (f32 && ctx->b64_src)
+                CKD(ctx, ust_k_cheby_first_from_f64(ctx->b64_src, (float *)b, L->invd32, L->d32, (float *)x,
+                                                    (float)L->c2[0], n, S(ctx)));
+            else if (f32)
                 CKD(ctx, ust_k_cheby_first_f32((const float *)b, L->invd32, L->d32, (float *)x, (float)L->c2[0], n,
                                                S(ctx)));
             else
@@ -135,6 +138,7 @@ static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x) {
     }
     ust_level *C = &ctx->lev[l + 1];
     if ((rc = lev_smooth(ctx, l, f32, b, x, 1))) return rc;
+    ctx->b64_src = NULL;
     if (f32) {
         if ((rc = lev_residual(ctx, l, 1, x, b, L->ax32))) return rc;
         CKD(ctx, ust_k_restrict_f32(&L->g, L->mask, C->mask, L->ax32, C->b32, S(ctx)));
@@ -352,20 +356,30 @@ static int pcg_iteration(ust_context *ctx, int pc32) {
     double *r = ctx->vr, *z = ctx->vz, *p = ctx->vp, *q = ctx->vq, *sc = ctx->scal;
     int rc;
     if (pc32) {
+        /* z stays FP32 (wz): the first smoothing step rounds r into b32 itself, r . z and p = z + beta p
+         * read z as FP32 and widen it per element */
         ust_level *L0 = &ctx->lev[0];
-        CKD(ctx, ust_k_copy_f64_to_f32(r, L0->b32, n, S(ctx)));
-        if ((rc = vcycle(ctx, 0, 1, L0->b32, ctx->wz))) return rc;
-        CKD(ctx, ust_k_copy_f32_to_f64(ctx->wz, z, n, S(ctx)));
+        if (ctx->nlev > 1 && L0->cheb_deg > 0) {
+            ctx->b64_src = r;
+        } else {
+            CKD(ctx, ust_k_copy_f64_to_f32(r, L0->b32, n, S(ctx)));
+        }
+        rc = vcycle(ctx, 0, 1, L0->b32, ctx->wz);
+        ctx->b64_src = NULL;
+        if (rc) return rc;
+        CKD(ctx, ust_k_dot_mixed(r, ctx->wz, n, ctx->partial, sc + SC_RZ, S(ctx)));
+        CKD(ctx, ust_k_cg_beta(sc + SC_RZ, sc + SC_RZOLD, sc + SC_CG, S(ctx)));
+        CKD(ctx, ust_k_cg_p_guarded_zf(ctx->wz, p, sc + SC_CG, n, S(ctx)));
     } else {
         if ((rc = vcycle(ctx, 0, 0, r, z))) return rc;
+        CKD(ctx, ust_k_dot_f64(r, z, n, ctx->partial, sc + SC_RZ, S(ctx)));
+        CKD(ctx, ust_k_cg_beta(sc + SC_RZ, sc + SC_RZOLD, sc + SC_CG, S(ctx)));
+        CKD(ctx, ust_k_cg_p_guarded_f64(z, p, sc + SC_CG, n, S(ctx)));
     }
-    CKD(ctx, ust_k_dot_f64(r, z, n, ctx->partial, sc + SC_RZ, S(ctx)));
-    CKD(ctx, ust_k_cg_beta(sc + SC_RZ, sc + SC_RZOLD, sc + SC_CG, S(ctx)));
-    CKD(ctx, ust_k_cg_p_guarded_f64(z, p, sc + SC_CG, n, S(ctx)));
     if ((rc = lev_apply(ctx, 0, 0, p, q))) return rc;
     CKD(ctx, ust_k_dot_f64(p, q, n, ctx->partial, sc + SC_PQ, S(ctx)));
-    CKD(ctx, ust_k_cg_xr_guarded_f64(ctx->vx, r, p, q, sc + SC_RZ, sc + SC_PQ, sc + SC_CG, n, S(ctx)));
-    CKD(ctx, ust_k_dot_f64(r, r, n, ctx->partial, sc + SC_RR, S(ctx)));
+    CKD(ctx, ust_k_cg_xr_rr_guarded_f64(ctx->vx, r, p, q, sc + SC_RZ, sc + SC_PQ, sc + SC_CG, n, ctx->partial,
+                                        sc + SC_RR, S(ctx)));
     CKD(ctx, ust_k_cg_check(sc + SC_RR, sc + SC_CG, S(ctx)));
     return UST_OK;
 }

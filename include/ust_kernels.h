@@ -256,6 +256,13 @@ int ust_k_cg_p_guarded_f64(const double *z, double *p, const double *ctl, size_t
 int ust_k_cg_xr_guarded_f64(double *x, double *r, const double *p, const double *q, const double *rz,
                             const double *pq, const double *ctl, size_t n, void *stream);
 int ust_k_cg_check(const double *rr, double *ctl, void *stream);
+/* fused forms for the FP64 PCG around the FP32 V-cycle: r . z with z in FP32; p = z + beta p with z in FP32;
+ * x += a p, r -= a q together with r . r (two-stage, into rr) */
+int ust_k_dot_mixed(const double *x, const float *y, size_t n, double *partial, double *out, void *stream);
+int ust_k_cg_p_guarded_zf(const float *z, double *p, const double *ctl, size_t n, void *stream);
+int ust_k_cg_xr_rr_guarded_f64(double *x, double *r, const double *p, const double *q, const double *rz,
+                               const double *pq, const double *ctl, size_t n, double *partial, double *rr,
+                               void *stream);
 
 /* Chebyshev smoother step (Jacobi-preconditioned).  r = b - y (y = A x given), then
  * d = c1*d + c2*invd.*r ; x += d  (c1 = 0 on the first step) */
@@ -267,6 +274,9 @@ int ust_k_cheby_step_f32(const float *b, const float *Ax, const float *invd, flo
 int ust_k_cheby_first_f64(const double *b, const double *invd, double *d, double *x, double c2, size_t n,
                           void *stream);
 int ust_k_cheby_first_f32(const float *b, const float *invd, float *d, float *x, float c2, size_t n, void *stream);
+/* the same first step on an FP64 right-hand side, which is also rounded into b32 for the rest of the V-cycle */
+int ust_k_cheby_first_from_f64(const double *b, float *b32, const float *invd, float *d, float *x, float c2,
+                               size_t n, void *stream);
 /* r = b - Ax */
 int ust_k_residual_f64(const double *b, const double *Ax, double *r, size_t n, void *stream);
 int ust_k_residual_f32(const float *b, const float *Ax, float *r, size_t n, void *stream);
