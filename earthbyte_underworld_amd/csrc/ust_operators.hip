@@ -423,37 +423,48 @@ __global__ void k_dense_bc_identity(const unsigned char *__restrict__ mask, doub
     if (mask[i]) A[(size_t)i * n + i] = 1.0;
 }
 
-// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one workgroup.
-__global__ __launch_bounds__(1024) void k_dense_spd_inverse(double *__restrict__ A, int n) {
-    extern __shared__ double colk[];
-    for (int k = 0; k < n; k++) {
-        __syncthreads();
-        const double piv = A[(size_t)k * n + k];
-        for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = A[(size_t)i * n + k];
-        __syncthreads();
-        // scale row k: A[k][j] /= piv (j != k), A[k][k] = 1/piv
-        for (int j = threadIdx.x; j < n; j += blockDim.x)
-            A[(size_t)k * n + j] = (j == k) ? 1.0 / piv : A[(size_t)k * n + j] / piv;
-        __syncthreads();
-        for (size_t t = threadIdx.x; t < (size_t)n * n; t += blockDim.x) {
-            const int i = (int)(t / n), j = (int)(t % n);
-            if (i == k) continue;
-            const double aij = (j == k) ? 0.0 : A[t];
-            A[t] = aij - colk[i] * A[(size_t)k * n + j];
-        }
-    }
+// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one launch per pivot k over all n^2 entries.
+// Pivot k reads the snapshot (row k, column k) of the matrix before the step -- r = A[k][:], c = A[:][k],
+// p = A[k][k] -- and sets
+//   A[i][j] -= c_i r_j / p  (i, j != k),   A[k][j] = r_j / p,   A[i][k] = -c_i / p,   A[k][k] = 1 / p;
+// after the last pivot A holds A^-1.  The threads that write row and column k+1 also write the snapshot the
+// next launch reads (two snapshot pairs, alternating), so no entry is read after another thread wrote it.
+__global__ void k_gj_snapshot0(const double *__restrict__ A, int n, double *__restrict__ row,
+                               double *__restrict__ col) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    row[i] = A[i];
+    col[i] = A[(size_t)i * n];
 }
 
+__global__ __launch_bounds__(256) void k_gj_step(double *__restrict__ A, int n, int k,
+                                                 const double *__restrict__ row, const double *__restrict__ col,
+                                                 double *__restrict__ row1, double *__restrict__ col1) {
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= n || j >= n) return;
+    const double rp = 1.0 / row[k];
+    double v;
+    if (i == k) v = j == k ? rp : row[j] * rp;
+    else if (j == k) v = -col[i] * rp;
+    else v = A[(size_t)i * n + j] - col[i] * (row[j] * rp);
+    A[(size_t)i * n + j] = v;
+    if (i == k + 1) row1[j] = v;
+    if (j == k + 1) col1[i] = v;
+}
+
+// y = Ainv x: one workgroup (4 wavefronts) per row
 template <typename T>
-__global__ void k_dense_matvec(const double *__restrict__ Ainv, int n, const T *__restrict__ x, T *__restrict__ y) {
-    // one wavefront per row
-    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= n) return;
+__global__ __launch_bounds__(256) void k_dense_matvec(const double *__restrict__ Ainv, int n,
+                                                      const T *__restrict__ x, T *__restrict__ y) {
+    __shared__ double part[4];
+    const int row = blockIdx.x;
     double s = 0.0;
-    for (int j = lane; j < n; j += 64) s += Ainv[(size_t)row * n + j] * (double)x[j];
+    for (int j = threadIdx.x; j < n; j += 256) s += Ainv[(size_t)row * n + j] * (double)x[j];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) y[row] = (T)s;
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) y[row] = (T)((part[0] + part[1]) + (part[2] + part[3]));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -656,25 +667,30 @@ int ust_k_dense_assemble(const ust_grid *gr, const double *WK, const unsigned ch
     return UST_OK;
 }
 
-int ust_k_dense_spd_inverse(double *A, int n, void *stream) {
-    if (!A || n <= 0 || n > 4096) return UST_EINVAL;
-    hipLaunchKernelGGL(k_dense_spd_inverse, dim3(1), dim3(1024), (size_t)n * sizeof(double), ust_stream(stream), A, n);
+int ust_k_dense_spd_inverse(double *A, int n, double *work, void *stream) {
+    if (!A || !work || n <= 0 || n > 4096) return UST_EINVAL;
+    hipStream_t st = ust_stream(stream);
+    double *row[2] = {work, work + 2 * (size_t)n}, *col[2] = {work + n, work + 3 * (size_t)n};
+    hipLaunchKernelGGL(k_gj_snapshot0, dim3(ust_grid_size(n, UST_BLOCK)), dim3(UST_BLOCK), 0, st, A, n, row[0],
+                       col[0]);
+    const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((n + 3) / 4));
+    for (int k = 0; k < n; k++)
+        hipLaunchKernelGGL(k_gj_step, grid, dim3(256), 0, st, A, n, k, row[k & 1], col[k & 1], row[(k + 1) & 1],
+                           col[(k + 1) & 1]);
     UST_LAUNCH_CHECK();
     return UST_OK;
 }
 
 int ust_k_dense_matvec_f64(const double *Ainv, int n, const double *x, double *y, void *stream) {
     if (!Ainv || !x || !y || n <= 0) return UST_EINVAL;
-    hipLaunchKernelGGL(k_dense_matvec<double>, dim3(ust_grid_size(n, 4)), dim3(256), 0, ust_stream(stream), Ainv, n,
-                       x, y);
+    hipLaunchKernelGGL(k_dense_matvec<double>, dim3(n), dim3(256), 0, ust_stream(stream), Ainv, n, x, y);
     UST_LAUNCH_CHECK();
     return UST_OK;
 }
 
 int ust_k_dense_matvec_f32(const double *Ainv, int n, const float *x, float *y, void *stream) {
     if (!Ainv || !x || !y || n <= 0) return UST_EINVAL;
-    hipLaunchKernelGGL(k_dense_matvec<float>, dim3(ust_grid_size(n, 4)), dim3(256), 0, ust_stream(stream), Ainv, n, x,
-                       y);
+    hipLaunchKernelGGL(k_dense_matvec<float>, dim3(n), dim3(256), 0, ust_stream(stream), Ainv, n, x, y);
     UST_LAUNCH_CHECK();
     return UST_OK;
 }

@@ -262,7 +262,8 @@ static int levels_structure(ust_context *ctx, const ust_solve_opts *o, int need3
     }
     if (ctx->coarse_dense) {
         ctx->nc = ctx->lev[nlev - 1].ndof;
-        DALLOC(ctx, ctx->Ainv, (size_t)ctx->nc * ctx->nc * sizeof(double));
+        /* the inverse plus the 4 nc pivot snapshots of its Gauss-Jordan construction */
+        DALLOC(ctx, ctx->Ainv, ((size_t)ctx->nc * ctx->nc + 4 * (size_t)ctx->nc) * sizeof(double));
     }
     ctx->levels_prec32 = need32;
     ctx->levels_struct = 1;
@@ -289,7 +290,7 @@ int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
     ust_level *Lc = &ctx->lev[nlev - 1];
     if (ctx->coarse_dense) { /* coarsest: dense FP64 inverse */
         CKD(ctx, ust_k_dense_assemble(&Lc->g, Lc->WK64, Lc->mask, ctx->Ainv, S(ctx)));
-        CKD(ctx, ust_k_dense_spd_inverse(ctx->Ainv, ctx->nc, S(ctx)));
+        CKD(ctx, ust_k_dense_spd_inverse(ctx->Ainv, ctx->nc, ctx->Ainv + (size_t)ctx->nc * ctx->nc, S(ctx)));
     }
     int deg = o->smoother_degree > 0 ? o->smoother_degree : 2;
     if (deg > UST_MAX_CHEBY) deg = UST_MAX_CHEBY;
