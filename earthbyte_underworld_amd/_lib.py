@@ -142,6 +142,10 @@ _SIGS = {
     "ust_memcpy_d2h": (I, [vp, vp, C.c_size_t, vp]),
     "ust_memcpy_d2d": (I, [vp, vp, C.c_size_t, vp]),
     "ust_k_fill_hash_f64": (I, [vp, C.c_size_t, C.c_ulonglong, vp]),
+    "ust_k_dense_assemble": (I, [C.POINTER(Grid), vp, vp, vp, vp]),
+    "ust_k_dense_spd_inverse": (I, [vp, I, vp, vp]),
+    "ust_k_dense_matvec_f64": (I, [vp, I, vp, vp, vp]),
+    "ust_k_dense_matvec_f32": (I, [vp, I, vp, vp, vp]),
     "ust_dev_error_string": (C.c_char_p, []),
 }
 

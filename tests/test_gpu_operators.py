@@ -151,3 +151,45 @@ def test_fused_residual_and_chebyshev_step(dim, nel):
     for a in (ws, du, db, dr):
         a.free()
     s.close()
+
+
+@pytest.mark.parametrize("dim,nel", [(3, (6, 6, 6)), (2, (20, 20))])
+def test_dense_coarse_operator_and_inverse(oracle, dim, nel):
+    """The coarsest multigrid level's dense FP64 operator equals the assembled K on free dofs (identity on
+    Dirichlet dofs); its Gauss-Jordan inverse inverts it; the dense matvec applies it."""
+    import ctypes as C
+    from earthbyte_underworld_amd import _lib as L
+    s = _build(dim, nel, 12 if dim == 2 else 8, 3)
+    lib = s.lib
+    o = OracleProblem(s)
+    gs = o.system(s.bc_mask)
+    K, _ = gs.scipy()
+    g, WK, WG, WF, mask = L.Grid(), C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert lib.ust_get_device_views(s.ctx, C.byref(g), C.byref(WK), C.byref(WG), C.byref(WF), C.byref(mask)) == 0
+    n = s.ndof
+    dA = _Dev(lib, np.zeros(n * n + 4 * n))
+    assert lib.ust_k_dense_assemble(C.byref(g), WK, mask, dA.p, None) == 0
+    A = dA.get()[:n * n].reshape(n, n)
+    free = gs.eq >= 0
+    bc = ~free
+    Kd = K.toarray()
+    assert rel_err(A[np.ix_(free, free)], Kd) < 1e-13
+    assert np.array_equal(A[np.ix_(bc, bc)], np.eye(int(bc.sum())))
+    assert not A[np.ix_(free, bc)].any() and not A[np.ix_(bc, free)].any()
+    ptr = C.c_void_p(dA.p.value + n * n * 8)
+    assert lib.ust_k_dense_spd_inverse(dA.p, n, ptr, None) == 0
+    Ainv = dA.get()[:n * n].reshape(n, n)
+    cond = np.linalg.cond(A)
+    assert np.abs(Ainv @ A - np.eye(n)).max() < 1e-15 * cond * 10
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=n)
+    dx, dy = _Dev(lib, x), _Dev(lib, np.zeros(n))
+    assert lib.ust_k_dense_matvec_f64(dA.p, n, dx.p, dy.p, None) == 0
+    assert rel_err(dy.get(), Ainv @ x) < 1e-14
+    dx32, dy32 = _Dev(lib, x.astype(np.float32)), _Dev(lib, np.zeros(n, np.float32))
+    assert lib.ust_k_dense_matvec_f32(dA.p, n, dx32.p, dy32.p, None) == 0
+    assert rel_err(dy32.get().astype(np.float64), Ainv @ x.astype(np.float32).astype(np.float64)) < 1e-6
+    print(f"cond {cond:.3e}, |Ainv A - I| {np.abs(Ainv @ A - np.eye(n)).max():.3e}")
+    for a in (dA, dx, dy, dx32, dy32):
+        a.free()
+    s.close()
