@@ -224,7 +224,7 @@ struct EpiOps {
 };
 
 template <int MODE, typename T>
-__device__ __forceinline__ EpiOps<T> epilogue_load(const Epi<T> &ep, const unsigned char *mask, size_t i) {
+__device__ __forceinline__ EpiOps<T> epilogue_load(const Epi<T> &ep, const unsigned char *mask, unsigned i) {
     EpiOps<T> o;
     o.m = mask[i];
     o.b = o.invd = o.d = o.x = (T)0;
@@ -239,7 +239,7 @@ __device__ __forceinline__ EpiOps<T> epilogue_load(const Epi<T> &ep, const unsig
 
 // v: the product (K u)_i before masking
 template <int MODE, typename T>
-__device__ __forceinline__ void epilogue_store(const Epi<T> &ep, const EpiOps<T> &o, size_t i, T v) {
+__device__ __forceinline__ void epilogue_store(const Epi<T> &ep, const EpiOps<T> &o, unsigned i, T v) {
     v = o.m ? (T)0 : v;
     if (MODE == EPI_Y) {
         ep.y[i] = v;
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_tiled(GridDev g, const T *__
         const int c = t % 3, n = t / 3;
         const int gx = tx0 + n % NX, gy = ty0 + (n / NX) % NY, gz = tz0 + n / (NX * NY);
         stok[r] = t < NN * 3 && gx < g.nnx && gy < g.nny && gz < g.nnz;
-        const size_t dof = stok[r] ? ((size_t)gx + (size_t)g.nnx * ((size_t)gy + (size_t)g.nny * gz)) * 3 + c : 0;
+        const unsigned dof = stok[r] ? ((unsigned)gx + (unsigned)g.nnx * ((unsigned)gy + (unsigned)g.nny * gz)) * 3u + c : 0u;
         stm[r] = mask[dof];
         stv[r] = u[dof];
     }
@@ -334,10 +334,11 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_tiled(GridDev g, const T *__
         }
         __syncthreads();
     }
-    // writeout: every load of the thread (mask, epilogue operands) first, then the stores
+    // writeout: every load of the thread (mask, epilogue operands) first, then the stores.  Dof indices are
+    // 32-bit (the host rejects larger grids), which keeps the per-thread index registers down.
     T *slot = work + (size_t)tile * NN * 3;
     int kind[NST];  // 0 none, 1 tile-face node (workspace slot), 2 node of this tile only (written here)
-    size_t wdof[NST];
+    unsigned wdof[NST];
     EpiOps<T> ops[NST];
 #pragma unroll
     for (int r = 0; r < NST; r++) {
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_tiled(GridDev g, const T *__
         const bool in = t < NN * 3 && gx < g.nnx && gy < g.nny && gz < g.nnz;
         const bool face = on_tile_face(gx, TX, g.tnx) || on_tile_face(gy, TY, g.tny) || on_tile_face(gz, TZ, g.tnz);
         kind[r] = in ? (face ? 1 : 2) : 0;
-        wdof[r] = kind[r] == 2 ? ((size_t)gx + (size_t)g.nnx * ((size_t)gy + (size_t)g.nny * gz)) * 3 + c : 0;
+        wdof[r] = kind[r] == 2 ? ((unsigned)gx + (unsigned)g.nnx * ((unsigned)gy + (unsigned)g.nny * gz)) * 3u + c : 0u;
         ops[r] = epilogue_load<MODE, T>(ep, mask, wdof[r]);
     }
 #pragma unroll
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const T *__res
         const int c = t % 2, n = t / 2;
         const int gx = tx0 + n % NX, gy = ty0 + n / NX;
         stok[r] = t < NN * 2 && gx < g.nnx && gy < g.nny;
-        const size_t dof = stok[r] ? ((size_t)gx + (size_t)g.nnx * gy) * 2 + c : 0;
+        const unsigned dof = stok[r] ? ((unsigned)gx + (unsigned)g.nnx * gy) * 2u + c : 0u;
         stm[r] = mask[dof];
         stv[r] = u[dof];
     }
@@ -426,7 +427,7 @@ __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const T *__res
     }
     T *slot = work + (size_t)tile * NN * 2;
     int kind[NST];
-    size_t wdof[NST];
+    unsigned wdof[NST];
     EpiOps<T> ops[NST];
 #pragma unroll
     for (int r = 0; r < NST; r++) {
@@ -436,7 +437,7 @@ __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const T *__res
         const bool in = t < NN * 2 && gx < g.nnx && gy < g.nny;
         const bool face = on_tile_face(gx, TX, g.tnx) || on_tile_face(gy, TY, g.tny);
         kind[r] = in ? (face ? 1 : 2) : 0;
-        wdof[r] = kind[r] == 2 ? ((size_t)gx + (size_t)g.nnx * gy) * 2 + c : 0;
+        wdof[r] = kind[r] == 2 ? ((unsigned)gx + (unsigned)g.nnx * gy) * 2u + c : 0u;
         ops[r] = epilogue_load<MODE, T>(ep, mask, wdof[r]);
     }
 #pragma unroll
@@ -519,7 +520,7 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned 
     T acc[DIM];
 #pragma unroll
     for (int c = 0; c < DIM; c++) {
-        ops[c] = epilogue_load<MODE, T>(ep, mask, (size_t)n * DIM + c);
+        ops[c] = epilogue_load<MODE, T>(ep, mask, n * DIM + c);
         acc[c] = (T)0;
     }
     for (int a = 0; a < nzs; a++)
@@ -532,7 +533,7 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned 
                 for (int c = 0; c < DIM; c++) acc[c] += src[c];
             }
 #pragma unroll
-    for (int c = 0; c < DIM; c++) epilogue_store<MODE, T>(ep, ops[c], (size_t)n * DIM + c, acc[c]);
+    for (int c = 0; c < DIM; c++) epilogue_store<MODE, T>(ep, ops[c], n * DIM + c, acc[c]);
 }
 
 static inline size_t work_len(const GridDev &g) {
@@ -547,6 +548,7 @@ static int apply_tiled(const ust_grid *gr, const T *W, const unsigned char *mask
     if (!W || !mask || !u || !ep.y || !work) return UST_EINVAL;
     if (MODE != EPI_Y && !ep.b) return UST_EINVAL;
     if ((MODE == EPI_CHEBY || MODE == EPI_CHEBY0) && (!ep.invd || !ep.d || ep.y != u)) return UST_EINVAL;
+    if ((size_t)g.nnx * g.nny * g.nnz * g.dim >= (1ull << 32)) return UST_EINVAL;  // 32-bit dof indices
     hipStream_t st = ust_stream(stream);
     if (g.dim == 3) {
         hipLaunchKernelGGL((k_applyK3_tiled<T, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep, work);

@@ -85,7 +85,7 @@ struct ust_context {
     double *h; /* continuity RHS after lifting */
     /* multigrid hierarchy */
     int levels_ready, levels_prec32, nlev;
-    int levels_struct, levels_deg, levels_req, coarse_dense;
+    int levels_struct, levels_deg, levels_cdeg, levels_ccdeg, levels_req, coarse_dense;
     double levels_ratio;
     float *WK32_0; /* FP32 copy of the fine weights for ust_apply_K_f32 without a hierarchy */
     int wk32_0_valid;

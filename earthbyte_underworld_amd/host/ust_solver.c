@@ -177,6 +177,11 @@ static int estimate_lmax(ust_context *ctx, int l, double *lmax) {
     return UST_OK;
 }
 
+static int clamp_degree(int d, int dflt) {
+    if (d <= 0) d = dflt;
+    return d > UST_MAX_CHEBY ? UST_MAX_CHEBY : d;
+}
+
 static void cheby_coeffs(ust_level *L, double lmax, double ratio, int deg) {
     double hi = lmax, lo = hi / ratio;
     double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta, rho = 1.0 / sigma;
@@ -268,6 +273,8 @@ static int levels_structure(ust_context *ctx, const ust_solve_opts *o, int need3
     ctx->levels_prec32 = need32;
     ctx->levels_struct = 1;
     ctx->levels_deg = o->smoother_degree;
+    ctx->levels_cdeg = o->coarse_degree;
+    ctx->levels_ccdeg = o->coarsest_degree;
     ctx->levels_req = o->mg_levels;
     return UST_OK;
 }
@@ -275,6 +282,7 @@ static int levels_structure(ust_context *ctx, const ust_solve_opts *o, int need3
 int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
     int rc;
     if (!ctx->levels_struct || (need32 && !ctx->levels_prec32) || ctx->levels_deg != o->smoother_degree ||
+        ctx->levels_cdeg != o->coarse_degree || ctx->levels_ccdeg != o->coarsest_degree ||
         ctx->levels_req != o->mg_levels) {
         if ((rc = levels_structure(ctx, o, need32 || ctx->levels_prec32))) return rc;
     }
@@ -292,17 +300,17 @@ int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
         CKD(ctx, ust_k_dense_assemble(&Lc->g, Lc->WK64, Lc->mask, ctx->Ainv, S(ctx)));
         CKD(ctx, ust_k_dense_spd_inverse(ctx->Ainv, ctx->nc, ctx->Ainv + (size_t)ctx->nc * ctx->nc, S(ctx)));
     }
-    int deg = o->smoother_degree > 0 ? o->smoother_degree : 2;
-    if (deg > UST_MAX_CHEBY) deg = UST_MAX_CHEBY;
+    int deg = clamp_degree(o->smoother_degree, 2);
+    int cdeg = clamp_degree(o->coarse_degree, deg), ccdeg = clamp_degree(o->coarsest_degree, UST_MAX_CHEBY);
     double ratio = o->cheby_ratio > 1.0 ? o->cheby_ratio : 12.0;
     for (int l = 0; l < nlev; l++) {
         if (l == nlev - 1 && ctx->coarse_dense) break;
         double lm;
         if ((rc = estimate_lmax(ctx, l, &lm))) return rc;
         if (l == nlev - 1)  /* grid too coarse-resistant for a dense solve: a long Chebyshev polynomial */
-            cheby_coeffs(&ctx->lev[l], 1.1 * lm, 100.0, UST_MAX_CHEBY);
+            cheby_coeffs(&ctx->lev[l], 1.1 * lm, 100.0, ccdeg);
         else
-            cheby_coeffs(&ctx->lev[l], 1.1 * lm, ratio, deg);
+            cheby_coeffs(&ctx->lev[l], 1.1 * lm, ratio, l == 0 ? deg : cdeg);
     }
     ctx->levels_ready = 1;
     ctx->levels_ratio = o->cheby_ratio;

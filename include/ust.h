@@ -46,6 +46,10 @@ typedef struct {
                                 Schur CG from the current pressure on its true residual with a 10x tighter outer
                                 tolerance (at most 3 restarts); 0: stop at outer_rtol, as the reference */
     int verbose;
+    int coarse_degree;       /* Chebyshev degree on the multigrid levels between the finest and the coarsest
+                                (0: smoother_degree) */
+    int coarsest_degree;     /* Chebyshev degree on the coarsest level when it is too large for the dense
+                                solve (0: 24) */
 } ust_solve_opts;
 
 typedef struct {
