@@ -2,7 +2,7 @@
 // weights of each element (the MatMult of the A11 block, auglag-driver-DGTGD.c:191-403, that every
 // inner Krylov iteration and every multigrid smoothing step performs, PETScMGSolver.c:641-716).
 //
-// One 256-thread workgroup owns a tile of elements (8x8x4 in 3D, 16x16 in 2D), one element per
+// One 256-thread workgroup owns a tile of elements (16x4x4 in 3D, 16x16 in 2D), one element per
 // thread.  The tile's nodal velocities are staged in LDS, each thread applies its element with sum
 // factorisation (gradients at the Lagrange points from edge differences, stress, and the transposed
 // interpolation back to the vertices, ~0.8 kflop per 3D element), element results are reduced in

@@ -40,8 +40,12 @@ __host__ __device__ constexpr int sgn(int a) { return 2 * a - 1; }
 // Element tiles of the operator kernels: one 256-thread workgroup per tile, one element per thread.
 // The K weights are stored tile-blocked, W[(tile*NQ + q)*256 + local], so that every load of a
 // quadrature weight by a wavefront is one contiguous 64-element run.
-#define UST_TX3 8
-#define UST_TY3 8
+// 3D tiles are 16 x 4 x 4 elements: one z layer (64 elements) per wavefront; the long x extent keeps
+// the tile faces normal to x, whose nodes are gathered with a stride in memory, few.
+#ifndef UST_TX3
+#define UST_TX3 16
+#define UST_TY3 4
+#endif
 #define UST_TZ3 4
 #define UST_TX2 16
 #define UST_TY2 16

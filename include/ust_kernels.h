@@ -86,7 +86,7 @@ int ust_k_assign_materials(const ust_grid *g, const int *cell_off, int np, const
 int ust_k_noise_scale(float *coh, int np, float amp, unsigned long long seed, void *stream);
 
 /* Length (in elements of the weight type) of a K-weight array of grid g.  The K weights are
- * stored tile-blocked: W[(tile*nq + q)*256 + local], tiles of 8x8x4 (3D) or 16x16 (2D) elements
+ * stored tile-blocked: W[(tile*nq + q)*256 + local], tiles of 16x4x4 (3D) or 16x16 (2D) elements
  * in lexicographic order, local index lexicographic inside the tile; partial tiles are padded. */
 size_t ust_k_weights_len(const ust_grid *g);
 /* tile-blocked position of weight q of element e (the host-side view of the same layout) */

@@ -31,9 +31,9 @@ def _build(dim, nel, ppc, seed, walls="freeslip"):
     return s
 
 
-# single partial tile; several tiles with partial edge tiles; exact tile multiples (8x8x4 / 16x16 tiles)
+# single partial tile; several tiles with partial edge tiles; exact tile multiples (16x4x4 / 16x16 tiles)
 @pytest.mark.parametrize("dim,nel", [(2, (12, 10)), (3, (6, 5, 4)), (2, (33, 17)), (2, (32, 16)),
-                                     (3, (17, 9, 5)), (3, (16, 16, 8))])
+                                     (3, (17, 9, 5)), (3, (16, 16, 8)), (3, (32, 8, 8))])
 def test_apply_K_matches_csr(oracle, dim, nel):
     s = _build(dim, nel, 12 if dim == 2 else 8, 5)
     o = OracleProblem(s)
