@@ -309,15 +309,19 @@ __global__ void k_coarsen_weights(GridDev gf, GridDev gc, const double *__restri
 // restriction b_c = P^T r_f : coarse dof (node (I,J,K), component c) gathers fine nodes 2I+{-1,0,1}
 // (weights 1/2, 1, 1/2 per axis) over the free fine dofs.  One thread per coarse dof; the 3^DIM loads are
 // unrolled and issued together (out-of-range neighbours read a clamped in-range address with weight 0).
+// With xc set, the coarse level's first Jacobi-Chebyshev step from a zero guess follows in the same pass:
+// dc = xc = c2 invdc bc.
 template <int DIM, typename T>
 __global__ __launch_bounds__(256) void k_restrict(GridDev gf, GridDev gc, const unsigned char *__restrict__ maskf,
                                                   const unsigned char *__restrict__ maskc, const T *__restrict__ rf,
-                                                  T *__restrict__ bc) {
+                                                  T *__restrict__ bc, const T *__restrict__ invdc, T *__restrict__ dc,
+                                                  T *__restrict__ xc, T c2) {
     const unsigned ndc = (unsigned)gc.nnx * gc.nny * gc.nnz * DIM;  // 32-bit index arithmetic (< 2^31 dofs)
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ndc) return;
     if (maskc[t]) {
         bc[t] = (T)0;
+        if (xc) dc[t] = xc[t] = (T)0;
         return;
     }
     const int c = (int)(t % DIM);
@@ -349,6 +353,11 @@ __global__ __launch_bounds__(256) void k_restrict(GridDev gf, GridDev gc, const 
         }
     }
     bc[t] = acc;
+    if (xc) {
+        const T x0 = c2 * invdc[t] * acc;
+        dc[t] = x0;
+        xc[t] = x0;
+    }
 }
 
 // prolongation x_f += P x_c : even index injects, odd index averages its two coarse neighbours
@@ -614,20 +623,30 @@ int ust_k_coarsen_weights(const ust_grid *gf, const double *WKf, double *WKc, vo
 }
 
 #define TRANSFER_IMPL(T, SUF)                                                                                    \
-    int ust_k_restrict_##SUF(const ust_grid *gf, const unsigned char *maskf, const unsigned char *maskc,       \
-                             const T *rf, T *bc, void *stream) {                                               \
+    static int restrict_##SUF(const ust_grid *gf, const unsigned char *maskf, const unsigned char *maskc,      \
+                              const T *rf, T *bc, const T *invdc, T *dc, T *xc, T c2, void *stream) {         \
         ust_grid gcr;                                                                                          \
         if (coarse_grid(gf, &gcr) != UST_OK || !maskf || !maskc || !rf || !bc) return UST_EINVAL;              \
         GridDev f = grid_dev(gf), c = grid_dev(&gcr);                                                          \
         const int nnc = c.nnx * c.nny * c.nnz;                                                                 \
         if (f.dim == 3)                                                                                        \
             hipLaunchKernelGGL((k_restrict<3, T>), dim3(ust_grid_size(nnc * 3, UST_BLOCK)), dim3(UST_BLOCK), 0,\
-                               ust_stream(stream), f, c, maskf, maskc, rf, bc);                               \
+                               ust_stream(stream), f, c, maskf, maskc, rf, bc, invdc, dc, xc, c2);            \
         else                                                                                                   \
             hipLaunchKernelGGL((k_restrict<2, T>), dim3(ust_grid_size(nnc * 2, UST_BLOCK)), dim3(UST_BLOCK), 0,    \
-                               ust_stream(stream), f, c, maskf, maskc, rf, bc);                               \
+                               ust_stream(stream), f, c, maskf, maskc, rf, bc, invdc, dc, xc, c2);            \
         UST_LAUNCH_CHECK();                                                                                    \
         return UST_OK;                                                                                         \
+    }                                                                                                          \
+    int ust_k_restrict_##SUF(const ust_grid *gf, const unsigned char *maskf, const unsigned char *maskc,       \
+                             const T *rf, T *bc, void *stream) {                                               \
+        return restrict_##SUF(gf, maskf, maskc, rf, bc, nullptr, nullptr, nullptr, (T)0, stream);              \
+    }                                                                                                          \
+    int ust_k_restrict_cheby_first_##SUF(const ust_grid *gf, const unsigned char *maskf,                       \
+                                         const unsigned char *maskc, const T *rf, T *bc, const T *invdc,       \
+                                         T *dc, T *xc, T c2, void *stream) {                                   \
+        if (!invdc || !dc || !xc) return UST_EINVAL;                                                           \
+        return restrict_##SUF(gf, maskf, maskc, rf, bc, invdc, dc, xc, c2, stream);                            \
     }                                                                                                          \
     int ust_k_prolong_add_##SUF(const ust_grid *gf, const unsigned char *maskf, const T *xc, T *xf,           \
                                 void *stream) {                                                                \

@@ -85,14 +85,42 @@ __global__ __launch_bounds__(256) void kv_dot_mixed_stage1(const double *x, cons
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-__global__ __launch_bounds__(256) void kv_reduce_stage2(const double *partial, int np, double *out) {
+// sum of the stage-1 partials (one 256-thread block); the total is valid in thread 0
+__device__ __forceinline__ double block_sum_partials(const double *partial, int np) {
     __shared__ double sh[4];
     double s = 0.0;
     for (int i = threadIdx.x; i < np; i += blockDim.x) s += partial[i];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) out[0] = sh[0] + sh[1] + sh[2] + sh[3];
+    return sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ __launch_bounds__(256) void kv_reduce_stage2(const double *partial, int np, double *out) {
+    const double s = block_sum_partials(partial, np);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// stage 2 of r . z fused with the CG beta update (kv_cg_beta)
+__global__ __launch_bounds__(256) void kv_reduce_beta(const double *partial, int np, double *rz, double *rz_old,
+                                                      double *c) {
+    const double s = block_sum_partials(partial, np);
+    if (threadIdx.x != 0) return;
+    rz[0] = s;
+    if (c[1] != 0.0) return;
+    c[3] = s / rz_old[0];
+    rz_old[0] = s;
+}
+
+// stage 2 of r . r fused with the CG convergence check (kv_cg_check)
+__global__ __launch_bounds__(256) void kv_reduce_check(const double *partial, int np, double *rr, double *c) {
+    const double s = block_sum_partials(partial, np);
+    if (threadIdx.x != 0) return;
+    rr[0] = s;
+    if (c[1] != 0.0) return;
+    c[2] += 1.0;
+    if (!(s == s)) c[1] = 2.0;
+    else if (s <= c[0]) c[1] = 1.0;
 }
 
 __global__ __launch_bounds__(256) void kv_absmax_stage1(const double *x, size_t n, double *partial) {
@@ -354,6 +382,22 @@ int ust_k_cg_xr_guarded_f64(double *x, double *r, const double *p, const double 
                             const double *pq, const double *ctl, size_t n, void *stream) {
     LV(kv_cg_xr_guarded, n, x, r, p, q, rz, pq, ctl, n);
 }
+int ust_k_dot_mixed_beta(const double *x, const float *y, size_t n, double *partial, double *rz, double *rz_old,
+                         double *ctl, void *stream) {
+    const unsigned g = rgrid(n);
+    hipLaunchKernelGGL(kv_dot_mixed_stage1, dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
+    hipLaunchKernelGGL(kv_reduce_beta, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, rz, rz_old, ctl);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+int ust_k_dot_beta_f64(const double *x, const double *y, size_t n, double *partial, double *rz, double *rz_old,
+                       double *ctl, void *stream) {
+    const unsigned g = rgrid(n);
+    hipLaunchKernelGGL((kv_dot_stage1<double, true>), dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
+    hipLaunchKernelGGL(kv_reduce_beta, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, rz, rz_old, ctl);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
 int ust_k_dot_mixed(const double *x, const float *y, size_t n, double *partial, double *out, void *stream) {
     const unsigned g = rgrid(n);
     hipLaunchKernelGGL(kv_dot_mixed_stage1, dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
@@ -365,12 +409,12 @@ int ust_k_cg_p_guarded_zf(const float *z, double *p, const double *ctl, size_t n
     LV(kv_cg_p_guarded_zf, n, z, p, ctl, n);
 }
 int ust_k_cg_xr_rr_guarded_f64(double *x, double *r, const double *p, const double *q, const double *rz,
-                               const double *pq, const double *ctl, size_t n, double *partial, double *rr,
+                               const double *pq, double *ctl, size_t n, double *partial, double *rr,
                                void *stream) {
     const unsigned g = rgrid(n);
     hipLaunchKernelGGL(kv_cg_xr_rr_guarded, dim3(g), dim3(256), 0, ust_stream(stream), x, r, p, q, rz, pq, ctl, n,
                        partial);
-    hipLaunchKernelGGL(kv_reduce_stage2, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, rr);
+    hipLaunchKernelGGL(kv_reduce_check, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, rr, ctl);
     UST_LAUNCH_CHECK();
     return UST_OK;
 }

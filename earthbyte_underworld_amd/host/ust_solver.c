@@ -96,13 +96,13 @@ static int lev_residual(ust_context *ctx, int l, int f32, const void *x, const v
     return UST_OK;
 }
 
-/* Jacobi-Chebyshev smoothing of A x = b (PETScMGSolver.c:674-709 smoother role).  x_zero: x is not read
- * (taken as zero); the first step is then d = c2 D^-1 b, x = d, without a product.  Every other step is
- * one fused product + update pass. */
+/* Jacobi-Chebyshev smoothing of A x = b (PETScMGSolver.c:674-709 smoother role).  x_zero 1: x is not read
+ * (taken as zero); the first step is then d = c2 D^-1 b, x = d, without a product.  x_zero 2: that first
+ * step was already applied (by the restriction).  Every other step is one fused product + update pass. */
 static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void *x, int x_zero) {
     ust_level *L = &ctx->lev[l];
     size_t n = (size_t)L->ndof;
-    for (int k = 0; k < L->cheb_deg; k++) {
+    for (int k = x_zero == 2 ? 1 : 0; k < L->cheb_deg; k++) {
         if (k == 0 && x_zero) {
             if (f32 && ctx->b64_src)
                 CKD(ctx, ust_k_cheby_first_from_f64(ctx->b64_src, (float *)b, L->invd32, L->d32, (float *)x,
@@ -124,8 +124,9 @@ static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void *x, 
     return UST_OK;
 }
 
-/* one V-cycle: x = M^-1 b on level l (x is overwritten, not read) */
-static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x) {
+/* one V-cycle: x = M^-1 b on level l (x is overwritten, not read).  first_done: the first smoothing step
+ * from the zero guess was already applied to x (fused into the restriction that produced b). */
+static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x, int first_done) {
     ust_level *L = &ctx->lev[l];
     int rc;
     if (l == ctx->nlev - 1) {
@@ -134,20 +135,30 @@ static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x) {
             else CKD(ctx, ust_k_dense_matvec_f64(ctx->Ainv, ctx->nc, (const double *)b, (double *)x, S(ctx)));
             return UST_OK;
         }
-        return lev_smooth(ctx, l, f32, b, x, 1);
+        return lev_smooth(ctx, l, f32, b, x, first_done ? 2 : 1);
     }
     ust_level *C = &ctx->lev[l + 1];
-    if ((rc = lev_smooth(ctx, l, f32, b, x, 1))) return rc;
+    if ((rc = lev_smooth(ctx, l, f32, b, x, first_done ? 2 : 1))) return rc;
     ctx->b64_src = NULL;
+    /* the coarse level starts with a smoothing step from zero unless it is the dense coarse solve */
+    int fuse = C->cheb_deg > 0 && !(l + 1 == ctx->nlev - 1 && ctx->coarse_dense);
     if (f32) {
         if ((rc = lev_residual(ctx, l, 1, x, b, L->ax32))) return rc;
-        CKD(ctx, ust_k_restrict_f32(&L->g, L->mask, C->mask, L->ax32, C->b32, S(ctx)));
-        if ((rc = vcycle(ctx, l + 1, 1, C->b32, C->x32))) return rc;
+        if (fuse)
+            CKD(ctx, ust_k_restrict_cheby_first_f32(&L->g, L->mask, C->mask, L->ax32, C->b32, C->invd32, C->d32,
+                                                    C->x32, (float)C->c2[0], S(ctx)));
+        else
+            CKD(ctx, ust_k_restrict_f32(&L->g, L->mask, C->mask, L->ax32, C->b32, S(ctx)));
+        if ((rc = vcycle(ctx, l + 1, 1, C->b32, C->x32, fuse))) return rc;
         CKD(ctx, ust_k_prolong_add_f32(&L->g, L->mask, C->x32, (float *)x, S(ctx)));
     } else {
         if ((rc = lev_residual(ctx, l, 0, x, b, L->ax64))) return rc;
-        CKD(ctx, ust_k_restrict_f64(&L->g, L->mask, C->mask, L->ax64, C->b64, S(ctx)));
-        if ((rc = vcycle(ctx, l + 1, 0, C->b64, C->x64))) return rc;
+        if (fuse)
+            CKD(ctx, ust_k_restrict_cheby_first_f64(&L->g, L->mask, C->mask, L->ax64, C->b64, C->invd64, C->d64,
+                                                    C->x64, C->c2[0], S(ctx)));
+        else
+            CKD(ctx, ust_k_restrict_f64(&L->g, L->mask, C->mask, L->ax64, C->b64, S(ctx)));
+        if ((rc = vcycle(ctx, l + 1, 0, C->b64, C->x64, fuse))) return rc;
         CKD(ctx, ust_k_prolong_add_f64(&L->g, L->mask, C->x64, (double *)x, S(ctx)));
     }
     return lev_smooth(ctx, l, f32, b, x, 0);
@@ -373,23 +384,21 @@ static int pcg_iteration(ust_context *ctx, int pc32) {
         } else {
             CKD(ctx, ust_k_copy_f64_to_f32(r, L0->b32, n, S(ctx)));
         }
-        rc = vcycle(ctx, 0, 1, L0->b32, ctx->wz);
+        rc = vcycle(ctx, 0, 1, L0->b32, ctx->wz, 0);
         ctx->b64_src = NULL;
         if (rc) return rc;
-        CKD(ctx, ust_k_dot_mixed(r, ctx->wz, n, ctx->partial, sc + SC_RZ, S(ctx)));
-        CKD(ctx, ust_k_cg_beta(sc + SC_RZ, sc + SC_RZOLD, sc + SC_CG, S(ctx)));
+        CKD(ctx, ust_k_dot_mixed_beta(r, ctx->wz, n, ctx->partial, sc + SC_RZ, sc + SC_RZOLD, sc + SC_CG, S(ctx)));
         CKD(ctx, ust_k_cg_p_guarded_zf(ctx->wz, p, sc + SC_CG, n, S(ctx)));
     } else {
-        if ((rc = vcycle(ctx, 0, 0, r, z))) return rc;
-        CKD(ctx, ust_k_dot_f64(r, z, n, ctx->partial, sc + SC_RZ, S(ctx)));
-        CKD(ctx, ust_k_cg_beta(sc + SC_RZ, sc + SC_RZOLD, sc + SC_CG, S(ctx)));
+        if ((rc = vcycle(ctx, 0, 0, r, z, 0))) return rc;
+        CKD(ctx, ust_k_dot_beta_f64(r, z, n, ctx->partial, sc + SC_RZ, sc + SC_RZOLD, sc + SC_CG, S(ctx)));
         CKD(ctx, ust_k_cg_p_guarded_f64(z, p, sc + SC_CG, n, S(ctx)));
     }
     if ((rc = lev_apply(ctx, 0, 0, p, q))) return rc;
     CKD(ctx, ust_k_dot_f64(p, q, n, ctx->partial, sc + SC_PQ, S(ctx)));
+    /* x, r update, r . r and the convergence check */
     CKD(ctx, ust_k_cg_xr_rr_guarded_f64(ctx->vx, r, p, q, sc + SC_RZ, sc + SC_PQ, sc + SC_CG, n, ctx->partial,
                                         sc + SC_RR, S(ctx)));
-    CKD(ctx, ust_k_cg_check(sc + SC_RR, sc + SC_CG, S(ctx)));
     return UST_OK;
 }
 

@@ -193,6 +193,14 @@ int ust_k_prolong_add_f64(const ust_grid *gf, const unsigned char *maskf, const 
                           void *stream);
 int ust_k_restrict_f32(const ust_grid *gf, const unsigned char *maskf, const unsigned char *maskc,
                        const float *rf, float *bc, void *stream);
+/* restriction followed, in the same pass, by the coarse level's first Jacobi-Chebyshev step from a zero
+ * guess: dc = xc = c2 invdc bc */
+int ust_k_restrict_cheby_first_f64(const ust_grid *gf, const unsigned char *maskf, const unsigned char *maskc,
+                                   const double *rf, double *bc, const double *invdc, double *dc, double *xc,
+                                   double c2, void *stream);
+int ust_k_restrict_cheby_first_f32(const ust_grid *gf, const unsigned char *maskf, const unsigned char *maskc,
+                                   const float *rf, float *bc, const float *invdc, float *dc, float *xc, float c2,
+                                   void *stream);
 int ust_k_prolong_add_f32(const ust_grid *gf, const unsigned char *maskf, const float *xc, float *xf,
                           void *stream);
 
@@ -257,12 +265,17 @@ int ust_k_cg_p_guarded_f64(const double *z, double *p, const double *ctl, size_t
 int ust_k_cg_xr_guarded_f64(double *x, double *r, const double *p, const double *q, const double *rz,
                             const double *pq, const double *ctl, size_t n, void *stream);
 int ust_k_cg_check(const double *rr, double *ctl, void *stream);
+/* r . z (two-stage, into rz) fused with the beta update of ust_k_cg_beta; z in FP32 or FP64 */
+int ust_k_dot_mixed_beta(const double *x, const float *y, size_t n, double *partial, double *rz, double *rz_old,
+                         double *ctl, void *stream);
+int ust_k_dot_beta_f64(const double *x, const double *y, size_t n, double *partial, double *rz, double *rz_old,
+                       double *ctl, void *stream);
 /* fused forms for the FP64 PCG around the FP32 V-cycle: r . z with z in FP32; p = z + beta p with z in FP32;
- * x += a p, r -= a q together with r . r (two-stage, into rr) */
+ * x += a p, r -= a q together with r . r (two-stage, into rr) and the convergence check of ust_k_cg_check */
 int ust_k_dot_mixed(const double *x, const float *y, size_t n, double *partial, double *out, void *stream);
 int ust_k_cg_p_guarded_zf(const float *z, double *p, const double *ctl, size_t n, void *stream);
 int ust_k_cg_xr_rr_guarded_f64(double *x, double *r, const double *p, const double *q, const double *rz,
-                               const double *pq, const double *ctl, size_t n, double *partial, double *rr,
+                               const double *pq, double *ctl, size_t n, double *partial, double *rr,
                                void *stream);
 
 /* Chebyshev smoother step (Jacobi-preconditioned).  r = b - y (y = A x given), then
