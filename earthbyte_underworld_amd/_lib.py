@@ -130,6 +130,10 @@ _SIGS = {
     "ust_k_apply_K_cheby_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, D, D, I, vp, vp]),
     "ust_k_apply_K_residual_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp]),
     "ust_k_apply_K_cheby_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, I, vp, vp]),
+    "ust_k_apply_K_cheby_prolong_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, D, vp, vp, vp]),
+    "ust_k_apply_K_cheby_prolong_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, C.c_float, vp, vp, vp]),
+    "ust_k_prolong_add_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp]),
+    "ust_k_prolong_add_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp]),
     "ust_k_particle_weights": (I, [C.POINTER(Grid), vp, I, vp, vp, vp, vp, vp, vp, vp, I, vp, vp, vp, I, D, vp,
                                    vp, vp, vp, vp, vp]),
     "ust_event_create": (I, [C.POINTER(vp)]),

@@ -206,7 +206,10 @@ __device__ __forceinline__ bool on_tile_face(int gc, int T, int tn) {
 //               staged only by that tile, before its update, and tile-face nodes are updated in
 //               phase 2, after every tile has staged them.
 //   EPI_CHEBY0  the same with c1 = 0 (the previous d is not read)
-enum { EPI_Y = 0, EPI_RES = 1, EPI_CHEBY = 2, EPI_CHEBY0 = 3 };
+//   EPI_PCHEBY0 (3D) the multigrid prolongation x += P xc of the coarse-grid correction xc, then the
+//               EPI_CHEBY0 step on the corrected x, in one pass: the staged velocities are x + P xc,
+//               interpolated from the tile's coarse node block held in LDS.
+enum { EPI_Y = 0, EPI_RES = 1, EPI_CHEBY = 2, EPI_CHEBY0 = 3, EPI_PCHEBY0 = 4 };
 template <typename T>
 struct Epi {
     T *y;  // EPI_Y / EPI_RES: output; EPI_CHEBY*: x, the input u
@@ -214,7 +217,26 @@ struct Epi {
     const T *invd;
     T *d;
     T c1, c2;
+    const T *xc;  // EPI_PCHEBY0: coarse-grid correction (nodes of the grid with half the elements per axis)
 };
+
+__host__ __device__ constexpr bool cheby_mode(int m) { return m == EPI_CHEBY || m == EPI_CHEBY0 || m == EPI_PCHEBY0; }
+
+// (P xc) at fine node (i, j, k): per axis the coarse node i/2 (even i) or the mean of (i-1)/2 and (i+1)/2
+// (odd i), trilinear over the axes -- the weights of k_prolong_add.  Branch-free: always 8 terms, the
+// second node of an even axis carries weight 0.  cval(I, J, K) returns a coarse value.
+template <typename T, typename F>
+__device__ __forceinline__ T prolong_value(int i, int j, int k, F cval) {
+    const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
+    const int I1 = I0 + (i & 1), J1 = J0 + (j & 1), K1 = K0 + (k & 1);
+    const T wx = (i & 1) ? (T)0.5 : (T)1, wy = (j & 1) ? (T)0.5 : (T)1, wz = (k & 1) ? (T)0.5 : (T)1;
+    const T ox = (i & 1) ? (T)0.5 : (T)0, oy = (j & 1) ? (T)0.5 : (T)0, oz = (k & 1) ? (T)0.5 : (T)0;
+    const T a0 = wx * cval(I0, J0, K0) + ox * cval(I1, J0, K0);
+    const T a1 = wx * cval(I0, J1, K0) + ox * cval(I1, J1, K0);
+    const T a2 = wx * cval(I0, J0, K1) + ox * cval(I1, J0, K1);
+    const T a3 = wx * cval(I0, J1, K1) + ox * cval(I1, J1, K1);
+    return wz * (wy * a0 + oy * a1) + oz * (wy * a2 + oy * a3);
+}
 
 // operands the epilogue reads, loaded ahead of the stores so that a thread's loads overlap
 template <typename T>
@@ -229,7 +251,7 @@ __device__ __forceinline__ EpiOps<T> epilogue_load(const Epi<T> &ep, const unsig
     o.m = mask[i];
     o.b = o.invd = o.d = o.x = (T)0;
     if (MODE != EPI_Y) o.b = ep.b[i];
-    if (MODE == EPI_CHEBY || MODE == EPI_CHEBY0) {
+    if (cheby_mode(MODE)) {
         o.invd = ep.invd[i];
         o.x = ep.y[i];
     }
@@ -260,8 +282,21 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_tiled(GridDev g, const T *__
     constexpr int NX = TX + 1, NY = TY + 1, NZ = TZ + 1, NN = NX * NY * NZ;
     __shared__ T su[NN * 3];
     __shared__ double sy[NN * 3];  // FP64 accumulation in LDS for both precisions (ds_add_f64)
+    // EPI_PCHEBY0: the coarse nodes under the tile (tile origins are even, so fine node 2I lies on coarse I)
+    constexpr int CX = TX / 2 + 1, CY = TY / 2 + 1, CZ = TZ / 2 + 1, NC = CX * CY * CZ * 3;
+    static_assert(NC <= UST_TILE, "one coarse value per thread");
+    __shared__ T sc[MODE == EPI_PCHEBY0 ? NC : 1];
     const int tile = blockIdx.x;
     const int tx0 = (tile % g.tnx) * TX, ty0 = ((tile / g.tnx) % g.tny) * TY, tz0 = (tile / (g.tnx * g.tny)) * TZ;
+    T cv = (T)0;
+    if (MODE == EPI_PCHEBY0) {
+        const int t = threadIdx.x, c = t % 3, n = t / 3;
+        const int I = tx0 / 2 + n % CX, J = ty0 / 2 + (n / CX) % CY, K = tz0 / 2 + n / (CX * CY);
+        const int cnx = g.nx / 2 + 1, cny = g.ny / 2 + 1, cnz = g.nz / 2 + 1;
+        const bool ok = t < NC && I < cnx && J < cny && K < cnz;
+        cv = ep.xc[ok ? ((unsigned)I + (unsigned)cnx * ((unsigned)J + (unsigned)cny * K)) * 3u + c : 0u];
+        cv = ok ? cv : (T)0;
+    }
     // Loads are issued without branches, staged velocities first, then the element weights (which do not
     // depend on them), so the LDS stores below wait only for the staging loads and the weights arrive
     // while the tile is being staged.  Out-of-tile slots read dof 0 and are discarded; partial tiles are
@@ -287,6 +322,20 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_tiled(GridDev g, const T *__
     if (early) {
 #pragma unroll
         for (int q = 0; q < 27; q++) wq[q] = W[((size_t)tile * 27 + q) * UST_TILE + threadIdx.x];
+    }
+    if (MODE == EPI_PCHEBY0) {
+        if (threadIdx.x < NC) sc[threadIdx.x] = cv;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NST; r++) {
+            const int t = threadIdx.x + UST_TILE * r;
+            const int c = t % 3, n = t / 3;
+            const int ix = n % NX, iy = (n / NX) % NY, iz = n / (NX * NY);
+            if (stok[r] && !stm[r])
+                stv[r] += prolong_value<T>(ix, iy, iz, [&](int I, int J, int K) {
+                    return sc[(I + CX * (J + CY * K)) * 3 + c];
+                });
+        }
     }
 #pragma unroll
     for (int r = 0; r < NST; r++) stv[r] = (stok[r] && !stm[r]) ? stv[r] : (T)0;
@@ -354,6 +403,7 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_tiled(GridDev g, const T *__
 #pragma unroll
     for (int r = 0; r < NST; r++) {
         const int t = threadIdx.x + UST_TILE * r;
+        if (MODE == EPI_PCHEBY0 && kind[r] == 2 && !ops[r].m) ops[r].x = su[t];  // x + P xc, as staged
         if (kind[r] == 1) slot[t] = (T)sy[t];
         else if (kind[r] == 2) epilogue_store<MODE, T>(ep, ops[r], wdof[r], (T)sy[t]);
     }
@@ -532,6 +582,15 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned 
 #pragma unroll
                 for (int c = 0; c < DIM; c++) acc[c] += src[c];
             }
+    if (MODE == EPI_PCHEBY0) {  // 3D only (host-checked): x + P xc on the free components
+        const int cnx = g.nx / 2 + 1, cny = g.ny / 2 + 1;
+#pragma unroll
+        for (int c = 0; c < DIM; c++)
+            if (!ops[c].m)
+                ops[c].x += prolong_value<T>(gx, gy, gz, [&](int I, int J, int K) {
+                    return ep.xc[((unsigned)I + (unsigned)cnx * ((unsigned)J + (unsigned)cny * K)) * 3u + c];
+                });
+    }
 #pragma unroll
     for (int c = 0; c < DIM; c++) epilogue_store<MODE, T>(ep, ops[c], n * DIM + c, acc[c]);
 }
@@ -547,7 +606,8 @@ static int apply_tiled(const ust_grid *gr, const T *W, const unsigned char *mask
     GridDev g = grid_dev(gr);
     if (!W || !mask || !u || !ep.y || !work) return UST_EINVAL;
     if (MODE != EPI_Y && !ep.b) return UST_EINVAL;
-    if ((MODE == EPI_CHEBY || MODE == EPI_CHEBY0) && (!ep.invd || !ep.d || ep.y != u)) return UST_EINVAL;
+    if (cheby_mode(MODE) && (!ep.invd || !ep.d || ep.y != u)) return UST_EINVAL;
+    if (MODE == EPI_PCHEBY0 && (g.dim != 3 || !ep.xc || g.nx % 2 || g.ny % 2 || g.nz % 2)) return UST_EINVAL;
     if ((size_t)g.nnx * g.nny * g.nnz * g.dim >= (1ull << 32)) return UST_EINVAL;  // 32-bit dof indices
     hipStream_t st = ust_stream(stream);
     if (g.dim == 3) {
@@ -587,7 +647,26 @@ static int apply_cheby(const ust_grid *g, const T *W, const unsigned char *mask,
                  : apply_tiled<T, EPI_CHEBY>(g, W, mask, x, ep, work, stream);
 }
 
+template <typename T>
+static int apply_cheby_prolong(const ust_grid *g, const T *W, const unsigned char *mask, T *x, const T *b,
+                               const T *invd, T *d, T c2, const T *xc, T *work, void *stream) {
+    Epi<T> ep = {x, b, invd, d, (T)0, c2, xc};
+    return apply_tiled<T, EPI_PCHEBY0>(g, W, mask, x, ep, work, stream);
+}
+
 extern "C" size_t ust_k_apply_work_len(const ust_grid *gr) { return work_len(grid_dev(gr)); }
+
+extern "C" int ust_k_apply_K_cheby_prolong_f64(const ust_grid *g, const double *WK, const unsigned char *mask,
+                                               double *x, const double *b, const double *invd, double *d, double c2,
+                                               const double *xc, double *work, void *stream) {
+    return apply_cheby_prolong<double>(g, WK, mask, x, b, invd, d, c2, xc, work, stream);
+}
+
+extern "C" int ust_k_apply_K_cheby_prolong_f32(const ust_grid *g, const float *WK, const unsigned char *mask,
+                                               float *x, const float *b, const float *invd, float *d, float c2,
+                                               const float *xc, float *work, void *stream) {
+    return apply_cheby_prolong<float>(g, WK, mask, x, b, invd, d, c2, xc, work, stream);
+}
 
 extern "C" int ust_k_apply_K_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
                                  double *y, double *work, void *stream) {

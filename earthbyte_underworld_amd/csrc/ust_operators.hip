@@ -388,21 +388,14 @@ __global__ void k_prolong_add(GridDev gf, GridDev gc, const unsigned char *__res
 }
 
 // ------------------------------------------------------------------------------------------
-// dense coarse operator (FP64): element matrices scattered into A, identity on BC dofs
+// dense coarse operator (FP64): assembled entry by entry, identity on BC dofs
 // ------------------------------------------------------------------------------------------
+// K_e[(a,i),(b,j)] of element e (vertices a, b; components i, j) from its quadrature weights
 template <int DIM>
-__global__ void k_dense_assemble(GridDev g, const double *__restrict__ WK, const unsigned char *__restrict__ mask,
-                                 double *__restrict__ A, int n) {
-    constexpr int NV = 1 << DIM, ND = NV * DIM, NQ = DIM == 3 ? 27 : 9;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.nel * ND * ND) return;
-    const int e = t / (ND * ND), rc = t % (ND * ND), r = rc / ND, c = rc % ND;
-    int nodes[NV];
-    elem_nodes<DIM>(g, e, nodes);
-    const int R = nodes[r / DIM] * DIM + r % DIM, C = nodes[c / DIM] * DIM + c % DIM;
-    if (mask[R] || mask[C]) return;
+__device__ __forceinline__ double elem_entry(const GridDev &g, const double *__restrict__ WK, int e, int a, int i,
+                                             int b, int j) {
+    constexpr int NV = 1 << DIM, NQ = DIM == 3 ? 27 : 9;
     const double s[3] = {2.0 / g.hx, 2.0 / g.hy, 2.0 / g.hz};
-    const int a = r / DIM, i = r % DIM, b = c / DIM, j = c % DIM;
     double v = 0.0;
     for (int q = 0; q < NQ; q++) {
         const int qd[3] = {q % 3, (q / 3) % 3, q / 9};
@@ -423,7 +416,37 @@ __global__ void k_dense_assemble(GridDev g, const double *__restrict__ WK, const
         m += ga[j] * gb[i];
         v += WK[widx(g, e, q)] * m;
     }
-    unsafeAtomicAdd(&A[(size_t)R * n + C], v);
+    return v;
+}
+
+// One thread per entry (R, C): the sum over the elements holding both nodes, in a fixed element order, so
+// the matrix (and its inverse) is bitwise reproducible.  Rows and columns of Dirichlet dofs are zero here;
+// k_dense_bc_identity puts 1 on their diagonal.
+template <int DIM>
+__global__ void k_dense_assemble(GridDev g, const double *__restrict__ WK, const unsigned char *__restrict__ mask,
+                                 double *__restrict__ A, int n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * n) return;
+    const int R = (int)(t / n), C = (int)(t % n);
+    const int na = R / DIM, i = R % DIM, nb = C / DIM, j = C % DIM;
+    const int ax = na % g.nnx, ay = (na / g.nnx) % g.nny, az = na / (g.nnx * g.nny);
+    const int bx = nb % g.nnx, by = (nb / g.nnx) % g.nny, bz = nb / (g.nnx * g.nny);
+    double v = 0.0;
+    if (!mask[R] && !mask[C] && abs(ax - bx) <= 1 && abs(ay - by) <= 1 && abs(az - bz) <= 1) {
+        // elements (ex, ey, ez) containing both nodes: ex in [max(ax,bx)-1, min(ax,bx)] within the grid
+        const int x0 = max(max(ax, bx) - 1, 0), x1 = min(min(ax, bx), g.nx - 1);
+        const int y0 = max(max(ay, by) - 1, 0), y1 = min(min(ay, by), g.ny - 1);
+        const int z0 = DIM == 3 ? max(max(az, bz) - 1, 0) : 0, z1 = DIM == 3 ? min(min(az, bz), g.nz - 1) : 0;
+        for (int ez = z0; ez <= z1; ez++)
+            for (int ey = y0; ey <= y1; ey++)
+                for (int ex = x0; ex <= x1; ex++) {
+                    const int e = ex + g.nx * (ey + g.ny * ez);
+                    const int a = (ax - ex) + 2 * (ay - ey) + (DIM == 3 ? 4 * (az - ez) : 0);
+                    const int b = (bx - ex) + 2 * (by - ey) + (DIM == 3 ? 4 * (bz - ez) : 0);
+                    v += elem_entry<DIM>(g, WK, e, a, i, b, j);
+                }
+    }
+    A[t] = v;
 }
 
 __global__ void k_dense_bc_identity(const unsigned char *__restrict__ mask, double *__restrict__ A, int n) {
@@ -671,10 +694,7 @@ int ust_k_dense_assemble(const ust_grid *gr, const double *WK, const unsigned ch
     const int n = g.nnx * g.nny * g.nnz * g.dim;
     if (!WK || !mask || !A || n > 4096) return UST_EINVAL;
     hipStream_t st = ust_stream(stream);
-    hipLaunchKernelGGL(k_fill<double>, dim3(ust_grid_size((size_t)n * n, UST_BLOCK)), dim3(UST_BLOCK), 0, st, A, 0.0,
-                       (size_t)n * n);
-    const int nd = (1 << g.dim) * g.dim;
-    const size_t work = (size_t)g.nel * nd * nd;
+    const size_t work = (size_t)n * n;
     if (g.dim == 3)
         hipLaunchKernelGGL(k_dense_assemble<3>, dim3(ust_grid_size(work, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WK,
                            mask, A, n);

@@ -99,11 +99,20 @@ static int lev_residual(ust_context *ctx, int l, int f32, const void *x, const v
 /* Jacobi-Chebyshev smoothing of A x = b (PETScMGSolver.c:674-709 smoother role).  x_zero 1: x is not read
  * (taken as zero); the first step is then d = c2 D^-1 b, x = d, without a product.  x_zero 2: that first
  * step was already applied (by the restriction).  Every other step is one fused product + update pass. */
-static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void *x, int x_zero) {
+static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void *x, int x_zero, const void *xc) {
     ust_level *L = &ctx->lev[l];
     size_t n = (size_t)L->ndof;
     for (int k = x_zero == 2 ? 1 : 0; k < L->cheb_deg; k++) {
-        if (k == 0 && x_zero) {
+        if (k == 0 && xc) { /* prolongation of the coarse correction fused into the first step */
+            if (f32)
+                CKD(ctx, ust_k_apply_K_cheby_prolong_f32(&L->g, L->WK32, L->mask, (float *)x, (const float *)b,
+                                                         L->invd32, L->d32, (float)L->c2[0], (const float *)xc,
+                                                         (float *)ctx->kwork, S(ctx)));
+            else
+                CKD(ctx, ust_k_apply_K_cheby_prolong_f64(&L->g, L->WK64, L->mask, (double *)x, (const double *)b,
+                                                         L->invd64, L->d64, L->c2[0], (const double *)xc,
+                                                         (double *)ctx->kwork, S(ctx)));
+        } else if (k == 0 && x_zero) {
             if (f32 && ctx->b64_src)
                 CKD(ctx, ust_k_cheby_first_from_f64(ctx->b64_src, (float *)b, L->invd32, L->d32, (float *)x,
                                                     (float)L->c2[0], n, S(ctx)));
@@ -135,13 +144,15 @@ static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x, int 
             else CKD(ctx, ust_k_dense_matvec_f64(ctx->Ainv, ctx->nc, (const double *)b, (double *)x, S(ctx)));
             return UST_OK;
         }
-        return lev_smooth(ctx, l, f32, b, x, first_done ? 2 : 1);
+        return lev_smooth(ctx, l, f32, b, x, first_done ? 2 : 1, NULL);
     }
     ust_level *C = &ctx->lev[l + 1];
-    if ((rc = lev_smooth(ctx, l, f32, b, x, first_done ? 2 : 1))) return rc;
+    if ((rc = lev_smooth(ctx, l, f32, b, x, first_done ? 2 : 1, NULL))) return rc;
     ctx->b64_src = NULL;
     /* the coarse level starts with a smoothing step from zero unless it is the dense coarse solve */
     int fuse = C->cheb_deg > 0 && !(l + 1 == ctx->nlev - 1 && ctx->coarse_dense);
+    /* 3D: the prolongation of the correction rides on the first post-smoothing pass */
+    int pfuse = L->g.dim == 3 && L->cheb_deg > 0;
     if (f32) {
         if ((rc = lev_residual(ctx, l, 1, x, b, L->ax32))) return rc;
         if (fuse)
@@ -150,6 +161,7 @@ static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x, int 
         else
             CKD(ctx, ust_k_restrict_f32(&L->g, L->mask, C->mask, L->ax32, C->b32, S(ctx)));
         if ((rc = vcycle(ctx, l + 1, 1, C->b32, C->x32, fuse))) return rc;
+        if (pfuse) return lev_smooth(ctx, l, 1, b, x, 0, C->x32);
         CKD(ctx, ust_k_prolong_add_f32(&L->g, L->mask, C->x32, (float *)x, S(ctx)));
     } else {
         if ((rc = lev_residual(ctx, l, 0, x, b, L->ax64))) return rc;
@@ -159,9 +171,10 @@ static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x, int 
         else
             CKD(ctx, ust_k_restrict_f64(&L->g, L->mask, C->mask, L->ax64, C->b64, S(ctx)));
         if ((rc = vcycle(ctx, l + 1, 0, C->b64, C->x64, fuse))) return rc;
+        if (pfuse) return lev_smooth(ctx, l, 0, b, x, 0, C->x64);
         CKD(ctx, ust_k_prolong_add_f64(&L->g, L->mask, C->x64, (double *)x, S(ctx)));
     }
-    return lev_smooth(ctx, l, f32, b, x, 0);
+    return lev_smooth(ctx, l, f32, b, x, 0, NULL);
 }
 
 /* lambda_max of D^-1 A by power iteration (FP64) */

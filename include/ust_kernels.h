@@ -163,6 +163,14 @@ int ust_k_apply_K_cheby_f64(const ust_grid *g, const double *WK, const unsigned 
 int ust_k_apply_K_cheby_f32(const ust_grid *g, const float *WK, const unsigned char *mask, float *x,
                             const float *b, const float *invd, float *d, float c1, float c2, int first,
                             float *work, void *stream);
+/* 3D: the multigrid prolongation x += P xc (xc on the grid with half the elements per axis; Dirichlet dofs
+ * of x untouched) followed by one first Chebyshev step (c1 = 0) on the corrected x, in one pass */
+int ust_k_apply_K_cheby_prolong_f64(const ust_grid *g, const double *WK, const unsigned char *mask, double *x,
+                                    const double *b, const double *invd, double *d, double c2, const double *xc,
+                                    double *work, void *stream);
+int ust_k_apply_K_cheby_prolong_f32(const ust_grid *g, const float *WK, const unsigned char *mask, float *x,
+                                    const float *b, const float *invd, float *d, float c2, const float *xc,
+                                    float *work, void *stream);
 /* same operator, one element per thread with per-dof atomic adds (reference variant for checks) */
 int ust_k_apply_K_ref_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
                           double *y, void *stream);

@@ -193,3 +193,56 @@ def test_dense_coarse_operator_and_inverse(oracle, dim, nel):
     for a in (dA, dx, dy, dx32, dy32):
         a.free()
     s.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_prolongation_fused_chebyshev_step(prec):
+    """x += P xc followed by the first Chebyshev step, fused in one pass (3D, partial tiles on every axis),
+    against the separate prolongation and step."""
+    import ctypes as C
+    from earthbyte_underworld_amd import _lib as L
+    s = _build(3, (34, 10, 10), 6, 4)
+    lib = s.lib
+    g, WK, WG, WF, mask = L.Grid(), C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert lib.ust_get_device_views(s.ctx, C.byref(g), C.byref(WK), C.byref(WG), C.byref(WF), C.byref(mask)) == 0
+    dt = np.float64 if prec == "f64" else np.float32
+    n = s.ndof
+    fm = s.bc_mask.reshape(-1) != 0
+    nn = (35, 11, 11)
+    cn = (18, 6, 6)
+    # coarse Dirichlet flags from the coincident fine nodes; the correction vanishes there (as in the V-cycle)
+    I, J, K = np.meshgrid(np.arange(cn[0]), np.arange(cn[1]), np.arange(cn[2]), indexing="ij")
+    fnode = (2 * I + nn[0] * (2 * J + nn[1] * 2 * K)).transpose(2, 1, 0).reshape(-1)
+    cmask = fm.reshape(-1, 3)[fnode].reshape(-1)
+    rng = np.random.default_rng(8)
+    xc = rng.normal(size=cmask.size)
+    xc[cmask] = 0.0
+    x0, b = rng.normal(size=n), rng.normal(size=n)
+    b[fm] = 0.0
+    invd, d0 = rng.uniform(0.5, 2.0, n), rng.normal(size=n)
+    wl = lib.ust_k_weights_len(C.byref(g))
+    if prec == "f64":
+        W = WK
+        keep = []
+    else:
+        Wd = _Dev(lib, np.zeros(wl, np.float32))
+        assert lib.ust_k_copy_f64_to_f32(WK, Wd.p, wl, None) == 0
+        W = Wd.p
+        keep = [Wd]
+    ws = _Dev(lib, np.zeros(lib.ust_k_apply_work_len(C.byref(g)), dt))
+    c2 = 1.7
+    dxc, db, di = _Dev(lib, xc.astype(dt)), _Dev(lib, b.astype(dt)), _Dev(lib, invd.astype(dt))
+    xa, da = _Dev(lib, x0.astype(dt)), _Dev(lib, d0.astype(dt))
+    prolong = getattr(lib, f"ust_k_prolong_add_{prec}")
+    cheby = getattr(lib, f"ust_k_apply_K_cheby_{prec}")
+    fused = getattr(lib, f"ust_k_apply_K_cheby_prolong_{prec}")
+    assert prolong(C.byref(g), mask, dxc.p, xa.p, None) == 0
+    assert cheby(C.byref(g), W, mask, xa.p, db.p, di.p, da.p, 0.0, c2, 1, ws.p, None) == 0
+    xb, dd = _Dev(lib, x0.astype(dt)), _Dev(lib, d0.astype(dt))
+    assert fused(C.byref(g), W, mask, xb.p, db.p, di.p, dd.p, c2, dxc.p, ws.p, None) == 0
+    tol = 1e-13 if prec == "f64" else 2e-6
+    assert rel_err(xb.get().astype(np.float64), xa.get().astype(np.float64)) < tol
+    assert rel_err(dd.get().astype(np.float64), da.get().astype(np.float64)) < tol
+    for a in keep + [ws, dxc, db, di, xa, da, xb, dd]:
+        a.free()
+    s.close()
