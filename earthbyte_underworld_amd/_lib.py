@@ -44,6 +44,11 @@ class SolveOpts(C.Structure):
                 ("coarse_degree", C.c_int), ("coarsest_degree", C.c_int)]
 
 
+class UzawaOpts(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_int), ("min_iterations", C.c_int),
+                ("use_absolute_tolerance", C.c_int), ("preconditioner", C.c_int)]
+
+
 class SolveStats(C.Structure):
     _fields_ = [("outer_its", C.c_int), ("inner_its", C.c_int), ("inner_solves", C.c_int),
                 ("refinements", C.c_int), ("fallback_used", C.c_int), ("mg_levels", C.c_int),
@@ -103,6 +108,8 @@ _SIGS = {
     "ust_apply_G": (I, [vp, vp, vp]),
     "ust_apply_Gt": (I, [vp, vp, vp]),
     "ust_solve": (I, [vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "ust_uzawa_opts_default": (None, [C.POINTER(UzawaOpts)]),
+    "ust_solve_uzawa": (I, [vp, C.POINTER(UzawaOpts), C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ust_get_solution": (I, [vp, vp, vp]),
     "ust_get_solution_dev": (I, [vp, vp, vp]),
     "ust_velocity_solve": (I, [vp, C.POINTER(SolveOpts), vp, vp, C.POINTER(I)]),

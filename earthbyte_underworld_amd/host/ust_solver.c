@@ -673,6 +673,103 @@ int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st)
     return UST_OK;
 }
 
+void ust_uzawa_opts_default(ust_uzawa_opts *o) {
+    /* Stokes_SLE_UzawaSolver.c:258-262 */
+    o->tolerance = 1e-5;
+    o->max_iterations = 1000;
+    o->min_iterations = 1;
+    o->use_absolute_tolerance = 0;
+    o->preconditioner = 1;
+}
+
+/* Stokes_SLE_UzawaSolver.c:407-983 (no compressibility matrix, D = G^T): steps 1-5 of the reference. */
+int ust_solve_uzawa(ust_context *ctx, const ust_uzawa_opts *uzawa, const ust_solve_opts *vel, ust_solve_stats *st) {
+    ust_uzawa_opts uo;
+    ust_solve_opts o;
+    if (uzawa) uo = *uzawa;
+    else ust_uzawa_opts_default(&uo);
+    if (vel) o = *vel;
+    else ust_solve_opts_default(&o);
+    memset(st, 0, sizeof(*st));
+    if (!ctx->assembled) UST_FAIL(ctx, UST_ESTATE, "ust_assemble must run before ust_solve_uzawa");
+    if (!(uo.tolerance > 0.0) || uo.max_iterations < 1) UST_FAIL(ctx, UST_EINVAL, "bad Uzawa tolerance or iterations");
+    int need32 = o.precision == UST_PREC_MIXED, rc;
+    size_t n = (size_t)ctx->ndof, ne = (size_t)ctx->nel;
+    double t0 = wall_ms();
+    if ((rc = ust_levels_setup(ctx, &o, need32))) return rc;
+    if ((rc = ensure_work(ctx, need32))) return rc;
+    double t1 = wall_ms();
+    st->mg_levels = ctx->nlev;
+    const double scale = 1.0 / sqrt((double)ne); /* qReciprocalGlobalProblemScale */
+    double ff;
+    CKD(ctx, dotv(ctx, ctx->f, ctx->f, n, SC_T0, &ff));
+    if (sqrt(ff) / sqrt((double)n) <= 1e-99) /* Stokes_SLE_UzawaSolver.c:493-511 */
+        UST_FAIL(ctx, UST_EINVAL, "the momentum force vector is zero");
+    double *u = ctx->uprev, *q = ctx->p, *r = ctx->pr, *z = ctx->pz, *sdir = ctx->pd, *qt = ctx->pq;
+    double *vstar = ctx->va, *ft = ctx->vb;
+    /* step 1: the scale of the transformed right-hand side, G^T K^-1 f - h (velocity solve at tolerance) */
+    o.inner_rtol = uo.tolerance;
+    if ((rc = a11_solve(ctx, &o, ctx->f, vstar, st))) return rc;
+    CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, vstar, qt, 0, S(ctx)));
+    CKD(ctx, ust_k_axpy_f64(-1.0, ctx->h, qt, ne, S(ctx)));
+    double qq;
+    CKD(ctx, dotv(ctx, qt, qt, ne, SC_T0, &qq));
+    const double rhs_scale = sqrt(qq) * scale;
+    if (!isfinite(rhs_scale)) UST_FAIL(ctx, UST_ENOTCONV, "Uzawa right-hand side scale is not finite");
+    /* step 2: all further velocity solves at 0.1 tolerance */
+    o.inner_rtol = 0.1 * uo.tolerance;
+    /* step 3: u0 = K^-1 (f - G q0), r = G^T u0 - h */
+    CKD(ctx, ust_k_apply_G(&ctx->grid, ctx->WG, ctx->mask_d, q, ft, S(ctx)));
+    CKD(ctx, ust_k_residual_f64(ctx->f, ft, ft, n, S(ctx)));
+    if ((rc = a11_solve(ctx, &o, ft, u, st))) return rc;
+    CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, u, r, 0, S(ctx)));
+    CKD(ctx, ust_k_axpy_f64(-1.0, ctx->h, r, ne, S(ctx)));
+    /* step 4: preconditioned CG on the pressure */
+    int it = 0;
+    double zr_prev = 1.0, abs_res = 0.0, rel_res = 0.0;
+    do {
+        if (uo.preconditioner) CKD(ctx, ust_k_div_f64(r, ctx->approxS, z, ne, S(ctx)));
+        else CKD(ctx, ust_memcpy_d2d(z, r, ne * sizeof(double), S(ctx)));
+        double zr, sks;
+        CKD(ctx, dotv(ctx, z, r, ne, SC_T0, &zr));
+        if (it == 0) CKD(ctx, ust_memcpy_d2d(sdir, z, ne * sizeof(double), S(ctx)));
+        else CKD(ctx, ust_k_axpby_f64(1.0, z, zr / zr_prev, sdir, ne, S(ctx)));
+        /* K v* = G s */
+        CKD(ctx, ust_k_apply_G(&ctx->grid, ctx->WG, ctx->mask_d, sdir, ft, S(ctx)));
+        if ((rc = a11_solve(ctx, &o, ft, vstar, st))) return rc;
+        CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, vstar, qt, 0, S(ctx)));
+        CKD(ctx, dotv(ctx, sdir, qt, ne, SC_T0, &sks));
+        const double alpha = zr / sks;
+        if (!isfinite(zr) || !isfinite(sks) || !isfinite(alpha))
+            UST_FAIL(ctx, UST_ENOTCONV, "Uzawa step has an illegal value (Stokes_SLE_UzawaSolver.c:829-833)");
+        CKD(ctx, ust_k_axpy_f64(alpha, sdir, q, ne, S(ctx)));
+        CKD(ctx, ust_k_axpy_f64(-alpha, vstar, u, n, S(ctx)));
+        CKD(ctx, ust_k_axpy_f64(-alpha, qt, r, ne, S(ctx)));
+        zr_prev = zr;
+        double rr;
+        CKD(ctx, dotv(ctx, r, r, ne, SC_T0, &rr));
+        abs_res = sqrt(rr) * scale;
+        rel_res = abs_res / rhs_scale;
+        if (o.verbose) fprintf(stderr, "[ust] uzawa %d abs %.6e rel %.6e\n", it, abs_res, rel_res);
+        if (!isfinite(abs_res)) UST_FAIL(ctx, UST_ENOTCONV, "Uzawa residual is not finite");
+        if (it >= uo.max_iterations) {
+            st->outer_its = it;
+            UST_FAIL(ctx, UST_ENOTCONV, "Uzawa reached maxIterations without converging");
+        }
+        it++;
+    } while ((uo.use_absolute_tolerance ? abs_res : rel_res) > uo.tolerance || it < uo.min_iterations);
+    st->outer_its = it;
+    /* step 5: residuals, then the nodal velocity with its BC values */
+    if ((rc = stokes_residual(ctx, st))) return rc;
+    double t2 = wall_ms();
+    CKD(ctx, ust_memcpy_d2d(ctx->u, u, n * sizeof(double), S(ctx)));
+    CKD(ctx, ust_k_axpy_f64(1.0, ctx->bcval_d, ctx->u, n, S(ctx)));
+    CKD(ctx, ust_stream_sync(S(ctx)));
+    st->t_setup_ms = t1 - t0;
+    st->t_solve_ms = t2 - t1;
+    return UST_OK;
+}
+
 int ust_velocity_solve(ust_context *ctx, const ust_solve_opts *opts, const double *b, double *u, int *its) {
     ust_solve_opts o;
     if (opts) o = *opts;

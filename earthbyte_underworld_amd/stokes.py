@@ -304,6 +304,22 @@ class StokesSystem:
         self._check(self.lib.ust_solve(self.ctx, C.byref(o), C.byref(st)))
         return st.as_dict()
 
+    def solve_uzawa(self, tolerance=1e-5, maxIterations=1000, minIterations=1, useAbsoluteTolerance=False,
+                    preconditioner=True, opts=None, **kw):
+        """Stokes_SLE_UzawaSolver (its XML parameter names); the velocity solves use `opts` / `kw` (precision,
+        multigrid); starts from the current pressure."""
+        uo = L.UzawaOpts()
+        self.lib.ust_uzawa_opts_default(C.byref(uo))
+        uo.tolerance = tolerance
+        uo.max_iterations = maxIterations
+        uo.min_iterations = minIterations
+        uo.use_absolute_tolerance = int(bool(useAbsoluteTolerance))
+        uo.preconditioner = int(bool(preconditioner))
+        o = opts if opts is not None else default_opts(**kw)
+        st = L.SolveStats()
+        self._check(self.lib.ust_solve_uzawa(self.ctx, C.byref(uo), C.byref(o), C.byref(st)))
+        return st.as_dict()
+
     def solution(self):
         u = np.zeros(self.ndof)
         p = np.zeros(self.nel)

@@ -149,6 +149,27 @@ int ust_apply_Gt(ust_context *ctx, const double *u, double *q);
 int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *stats);
 int ust_get_solution(ust_context *ctx, double *u_nodal, double *p);
 int ust_get_solution_dev(ust_context *ctx, const double **u_nodal_dev, const double **p_dev);
+/* ---- Uzawa solve (Stokes_SLE_UzawaSolver, Stokes_SLE_UzawaSolver.c:407-983) ----
+ * Preconditioned CG on the pressure, Schur complement G^T K^-1 G applied through one velocity solve per
+ * iteration; starts from the context's current pressure (the reference's pSolnVec), as the reference does.
+ * The velocity solves use the multigrid PCG of `vel` (precision, levels, smoother; its tolerances are set
+ * here: `tolerance` for the first solve, 0.1 `tolerance` for all others, Stokes_SLE_UzawaSolver.c:566, 600).
+ * The pressure null space is not removed (as in the reference).  Errors as the reference's firewalls: a zero
+ * momentum force vector (UST_EINVAL), a non-finite step or residual, and more than max_iterations
+ * iterations (UST_ENOTCONV). */
+typedef struct {
+    double tolerance;            /* "tolerance" (default 1e-5) */
+    int max_iterations;          /* "maxIterations" (1000) */
+    int min_iterations;          /* "minIterations" (1) */
+    int use_absolute_tolerance;  /* "useAbsoluteTolerance" (0): 1 stops on ||r|| / sqrt(n_p), 0 on that over
+                                    the same norm of the Uzawa right-hand side G^T K^-1 f - h */
+    int preconditioner;          /* 1: the UzawaPreconditionerTerm matrix approxS ("Preconditioner"), 0: none */
+} ust_uzawa_opts;
+
+void ust_uzawa_opts_default(ust_uzawa_opts *o);
+/* stats: outer_its = Uzawa iterations; momentum_residual = ||f - K u - G p|| / ||[f; h]|| as for ust_solve */
+int ust_solve_uzawa(ust_context *ctx, const ust_uzawa_opts *uzawa, const ust_solve_opts *vel, ust_solve_stats *stats);
+
 /* K u = b velocity solve alone (one A11 solve) on free dofs; b, u nodal */
 int ust_velocity_solve(ust_context *ctx, const ust_solve_opts *opts, const double *b, double *u, int *its);
 
