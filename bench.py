@@ -186,6 +186,7 @@ def main():
     ap.add_argument("--mlp", default=MLP_WEIGHTS)
     ap.add_argument("--relax", type=int, default=1, help="relaxed inner tolerance on the mixed+learned path")
     ap.add_argument("--kernel-only", action="store_true", help="dominant kernel + copy only (counter runs)")
+    ap.add_argument("--uzawa", action="store_true", help="also time the Uzawa solver (reference defaults)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is design-only.",
@@ -211,10 +212,9 @@ def main():
         log("learned Schur scaling loaded from", os.path.relpath(args.mlp, ROOT))
     else:
         log("no trained MLP found: the mixed run uses the plain approxS scaling")
-    # headline path: Chebyshev degree 1 smoothing (cheaper V-cycles outweigh the extra iterations there);
-    # the plain path keeps the default degree 2
-    # fine level degree 1, coarser levels degree 2: the coarse levels are cheap and their extra smoothing
-    # cuts the inner iterations
+    # headline path: Chebyshev degree 1 on the finest level (cheaper V-cycles outweigh the extra iterations
+    # there), degree 2 on the coarser levels (short passes whose extra smoothing cuts the inner iterations);
+    # the plain path keeps the default degree 2 everywhere
     opts_mixed = solve_opts(ust, L.PREC_MIXED, learned, args.relax, degree=1, coarse_degree=2)
     opts_plain = solve_opts(ust, L.PREC_FP64, False)
 
@@ -249,6 +249,25 @@ def main():
         plain = {"solves_per_hour": 3600.0 / tp, "s_per_solve": tp, "outer_its": stp["outer_its"],
                  "inner_its": stp["inner_its"], "stokes_residual": stp["stokes_residual"]}
         log(f"plain FP64: {tp:.2f}s, {stp['outer_its']} outer its, residual {stp['stokes_residual']:.2e}")
+
+    uzawa = None
+    if args.uzawa:
+        # Stokes_SLE_UzawaSolver with the reference's defaults (tolerance 1e-5 on the relative Uzawa residual,
+        # approxS preconditioner), from p = 0, velocity solves in FP64 and in the mixed precision path
+        uzawa = {}
+        for name, vo in (("fp64", opts_plain), ("mixed", opts_mixed)):
+            s.assemble(gravity=1.0)
+            s.set_solution(np.zeros(s.ndof), np.zeros(s.nel))
+            lib.ust_stream_sync(stream)
+            t0 = time.perf_counter()
+            stu = s.solve_uzawa(tolerance=1e-5, opts=vo)
+            lib.ust_stream_sync(stream)
+            tu = time.perf_counter() - t0
+            uzawa[name] = {"s_per_solve": round(tu, 3), "solves_per_hour": round(3600.0 / tu, 2),
+                           "uzawa_its": stu["outer_its"], "inner_its": stu["inner_its"],
+                           "stokes_residual": stu["stokes_residual"]}
+            log(f"uzawa ({name} velocity solves): {tu:.2f}s, {stu['outer_its']} its, "
+                f"residual {stu['stokes_residual']:.2e}")
 
     # ---- assembly throughput (particle kernel + derived element terms) ----
     t_asm = time_on_stream(lib, stream, lambda: lib.ust_assemble(ctx, C.c_double(1.0)), 3)
@@ -298,6 +317,7 @@ def main():
         "assembly_ms": round(t_asm, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "uzawa": uzawa,
         "vs_cpu": round(value / cpu["value"], 2) if cpu else None,
     }
     print(json.dumps(out), flush=True)
