@@ -82,6 +82,7 @@ _SIGS = {
     "ust_swarm_cohesion_noise": (I, [vp, C.c_float, C.c_ulonglong]),
     "ust_swarm_download": (I, [vp, vp, vp, vp, vp]),
     "ust_swarm_count": (I, [vp]),
+    "ust_swarm_advect": (I, [vp, D, I, I, C.POINTER(I), C.POINTER(I)]),
     "ust_set_temperature": (I, [vp, vp]),
     "ust_set_solution": (I, [vp, vp, vp]),
     "ust_set_previous_solution_exists": (I, [vp, I]),

@@ -754,4 +754,74 @@ int ust_swarm_download(ust_context *ctx, int *cell_off, double *xi, double *w, u
     return UST_OK;
 }
 
+int ust_swarm_advect(ust_context *ctx, double dt, int order, int allow_fallback, int *n_removed, int *n_empty) {
+    if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
+    if (order != 1 && order != 2) UST_FAIL(ctx, UST_EINVAL, "order must be 1 or 2 (got %d)", order);
+    if (!isfinite(dt)) UST_FAIL(ctx, UST_EINVAL, "dt is not finite");
+    const int np = ctx->np, nel = ctx->nel, dim = ctx->grid.dim;
+    const size_t n = (size_t)np;
+    size_t wb = ust_k_sort_by_cell_work_bytes(np, nel);
+    void *key = NULL, *key_s = NULL, *perm = NULL, *perm_t = NULL, *nxi = NULL, *flag = NULL, *work = NULL;
+    void *off = NULL, *oxi = NULL, *ow = NULL, *omat = NULL, *ocoh = NULL;
+    int rc = UST_OK, hflag = 0;
+    if (!rc) rc = ust_dev_malloc(&key, n * sizeof(unsigned) + 8);
+    if (!rc) rc = ust_dev_malloc(&key_s, n * sizeof(unsigned) + 8);
+    if (!rc) rc = ust_dev_malloc(&perm, n * sizeof(unsigned) + 8);
+    if (!rc) rc = ust_dev_malloc(&perm_t, n * sizeof(unsigned) + 8);
+    if (!rc) rc = ust_dev_malloc(&nxi, 3 * n * sizeof(double) + 8);
+    if (!rc) rc = ust_dev_malloc(&flag, sizeof(double));
+    if (!rc) rc = ust_dev_malloc(&work, wb + 8);
+    if (!rc) rc = ust_dev_malloc(&off, (size_t)(nel + 1) * sizeof(int));
+    double *n0 = (double *)nxi, *n1 = n0 + n, *n2 = n0 + 2 * n;
+    if (!rc) rc = ust_k_fill_f64((double *)flag, 0.0, 1, ctx->stream); /* all-zero bits: the int flag reads 0 */
+    if (!rc)
+        rc = ust_k_advect(&ctx->grid, ctx->cell_off, np, ctx->xi0, ctx->xi1, ctx->xi2, ctx->u, dt, order,
+                          allow_fallback, (unsigned *)key, n0, n1, n2, (int *)flag, ctx->stream);
+    if (!rc) rc = ust_memcpy_d2h(&hflag, flag, sizeof(int), ctx->stream);
+    if (!rc && hflag) {
+        snprintf(ctx->err, sizeof(ctx->err),
+                 "a corrector step found no velocity (point outside the box) and allow_fallback is off "
+                 "(TimeIntegrand.c:324-327)");
+        rc = UST_ENOTCONV;
+    }
+    if (!rc)
+        rc = ust_k_sort_by_cell((const unsigned *)key, (unsigned *)key_s, (unsigned *)perm, (unsigned *)perm_t, np,
+                                nel, work, wb, ctx->stream);
+    if (!rc) rc = ust_k_cell_offsets((const unsigned *)key_s, np, nel, (int *)off, ctx->stream);
+    int *off_h = (int *)malloc((size_t)(nel + 1) * sizeof(int));
+    if (!off_h && !rc) rc = UST_ENOMEM;
+    if (!rc) rc = ust_memcpy_d2h(off_h, off, (size_t)(nel + 1) * sizeof(int), ctx->stream);
+    int kept = 0, empty = 0, max_ppc = 0;
+    if (!rc) {
+        kept = off_h[nel];
+        for (int e = 0; e < nel; e++) {
+            const int c = off_h[e + 1] - off_h[e];
+            if (c == 0) empty++;
+            if (c > max_ppc) max_ppc = c;
+        }
+    }
+    free(off_h);
+    const size_t k = (size_t)kept;
+    if (!rc) rc = ust_dev_malloc(&oxi, 3 * k * sizeof(double) + 8);
+    if (!rc) rc = ust_dev_malloc(&ow, k * sizeof(double) + 8);
+    if (!rc && ctx->mat) rc = ust_dev_malloc(&omat, k + 8);
+    if (!rc && ctx->coh) rc = ust_dev_malloc(&ocoh, k * sizeof(float) + 8);
+    double *o0 = (double *)oxi, *o1 = o0 + k, *o2 = o0 + 2 * k;
+    if (!rc)
+        rc = ust_k_swarm_gather(dim, kept, (const unsigned *)perm, (const unsigned *)key_s, (const int *)off, n0, n1,
+                                n2, ctx->mat, ctx->coh, o0, o1, o2, (double *)ow, (unsigned char *)omat,
+                                (float *)ocoh, ctx->stream);
+    if (!rc)
+        rc = ust_swarm_upload_dev(ctx, kept, (const int *)off, o0, o1, dim == 3 ? o2 : NULL, (const double *)ow,
+                                  (const unsigned char *)omat, (const float *)ocoh, max_ppc);
+    else if (!ctx->err[0] || rc != UST_ENOTCONV)
+        snprintf(ctx->err, sizeof(ctx->err), "swarm advection failed (%d): %s", rc, ust_dev_error_string());
+    void *tmp[] = {key, key_s, perm, perm_t, nxi, flag, work, off, oxi, ow, omat, ocoh};
+    for (size_t i = 0; i < sizeof(tmp) / sizeof(tmp[0]); i++) ust_dev_free(tmp[i]);
+    if (rc) return rc;
+    if (n_removed) *n_removed = np - kept;
+    if (n_empty) *n_empty = empty;
+    return UST_OK;
+}
+
 int ust_swarm_count(ust_context *ctx) { return ctx->np; }

@@ -320,6 +320,14 @@ class StokesSystem:
         self._check(self.lib.ust_solve_uzawa(self.ctx, C.byref(uo), C.byref(o), C.byref(st)))
         return st.as_dict()
 
+    def advect(self, dt, order=2, allowFallbackToFirstOrder=False):
+        """SwarmAdvector step through the current velocity solution, then the re-sort by element; returns
+        (particles removed, elements left empty)."""
+        nr, ne = C.c_int(), C.c_int()
+        self._check(self.lib.ust_swarm_advect(self.ctx, float(dt), int(order), int(bool(allowFallbackToFirstOrder)),
+                                              C.byref(nr), C.byref(ne)))
+        return nr.value, ne.value
+
     def solution(self):
         u = np.zeros(self.ndof)
         p = np.zeros(self.nel)

@@ -105,6 +105,13 @@ int ust_swarm_assign_materials(ust_context *ctx, int nshape, const ust_shape *sh
 int ust_swarm_cohesion_noise(ust_context *ctx, float amplitude, unsigned long long seed);
 /* copy the swarm back (any pointer may be NULL): cell_off (nel+1), xi (np*3), w (np), material (np) */
 int ust_swarm_download(ust_context *ctx, int *cell_off, double *xi, double *w, unsigned char *material);
+/* Advect the swarm through the current nodal velocity (the last solve's) by dt and re-sort it by element
+ * (SwarmAdvector + TimeIntegrator + ParticleMovementHandler + ConstantWeights): order 1 (Euler) or 2
+ * (midpoint, the TimeIntegrator default); allow_fallback: a corrector point outside the box takes the first-
+ * order step instead of failing (allowFallbackToFirstOrder).  Particles that end outside the box are removed;
+ * *n_removed counts them, *n_empty the elements left without particles (their element terms are then zero:
+ * population control is not part of this step). */
+int ust_swarm_advect(ust_context *ctx, double dt, int order, int allow_fallback, int *n_removed, int *n_empty);
 int ust_swarm_count(ust_context *ctx);
 /* per-particle cohesion factors (np floats; 1 where none were set) */
 int ust_get_cohesion_scale(ust_context *ctx, float *coh);

@@ -140,6 +140,27 @@ int ust_k_element_f(const ust_grid *g, const double *WF, double gravity, int fir
 /* approxS_e = sum_v G_e[v]^2 / K_e[v][v] (UzawaPreconditionerTerm.c:200-212) */
 int ust_k_uzawa_diag(const ust_grid *g, const double *WK, const double *WG, double *approxS, void *stream);
 
+/* --- swarm advection and re-sort (SwarmAdvector.c:288-313, TimeIntegrand.c:221-330) --- */
+/* Per particle: position from (cell, xi), Euler (order 1) or midpoint (order 2) step through the Q1 velocity
+ * u (nodal, interleaved, BC values included), new element key (nel: left the box) and local coordinates in
+ * it (n0..n2).  A corrector outside the box falls back to the first-order step when `fallback`, else sets
+ * bit 0 of *flag and leaves the particle in place. */
+int ust_k_advect(const ust_grid *g, const int *cell_off, int np, const double *xi0, const double *xi1,
+                 const double *xi2, const double *u, double dt, int order, int fallback, unsigned *key, double *n0,
+                 double *n1, double *n2, int *flag, void *stream);
+/* stable sort of particle indices by key (perm), work of ust_k_sort_by_cell_work_bytes bytes */
+size_t ust_k_sort_by_cell_work_bytes(int np, int nel);
+int ust_k_sort_by_cell(const unsigned *key, unsigned *key_sorted, unsigned *perm, unsigned *perm_tmp, int np, int nel,
+                       void *work, size_t work_bytes, void *stream);
+/* cell_off[0..nel] from sorted keys (empty cells included); cell_off[nel] = particles kept */
+int ust_k_cell_offsets(const unsigned *key_sorted, int np, int nel, int *cell_off, void *stream);
+/* the re-sorted swarm: local coordinates, ConstantWeights 2^dim / count (ConstantWeights.c:187), material and
+ * noise factor (optional) gathered through perm */
+int ust_k_swarm_gather(int dim, int np, const unsigned *perm, const unsigned *key_sorted, const int *cell_off,
+                       const double *n0, const double *n1, const double *n2, const unsigned char *mat,
+                       const float *coh, double *o0, double *o1, double *o2, double *w, unsigned char *omat,
+                       float *ocoh, void *stream);
+
 /* --- operators.  mask: per dof (n_nodes*dim), 1 = Dirichlet.  y is overwritten. --- */
 /* y = K u on the free dofs (input and output BC dofs treated as zero).  work: device workspace of
  * ust_k_apply_work_len(g) elements of the operator's type (per-tile node partial sums); one
