@@ -187,6 +187,7 @@ def main():
     ap.add_argument("--relax", type=int, default=1, help="relaxed inner tolerance on the mixed+learned path")
     ap.add_argument("--kernel-only", action="store_true", help="dominant kernel + copy only (counter runs)")
     ap.add_argument("--uzawa", action="store_true", help="also time the Uzawa solver (reference defaults)")
+    ap.add_argument("--advect", action="store_true", help="also time one swarm advection step + re-sort")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is design-only.",
@@ -286,6 +287,22 @@ def main():
         except Exception as ex:  # the CPU leg never blocks the GPU result
             log("cpu baseline failed:", ex)
 
+    advect = None
+    if args.advect:
+        # one SwarmAdvector step (midpoint rule) through the last solution, CFL 0.5, and the re-sort (last:
+        # nothing after it uses the benchmark swarm)
+        u, _ = s.solution()
+        dt = 0.5 / n / max(float(np.max(np.abs(u))), 1e-300)
+        npart = lib.ust_swarm_count(ctx)
+        lib.ust_stream_sync(stream)
+        t0 = time.perf_counter()
+        nr, ne = s.advect(dt, order=2, allowFallbackToFirstOrder=True)
+        lib.ust_stream_sync(stream)
+        ta = time.perf_counter() - t0
+        advect = {"ms": round(ta * 1e3, 2), "particles": int(npart), "removed": nr, "empty_cells": ne,
+                  "particles_per_s": round(npart / ta, 1)}
+        log(f"advect: {ta * 1e3:.1f} ms for {npart} particles, removed {nr}, empty cells {ne}")
+
     if t_mixed is None:  # --no-mixed: the plain solve alone (profiling)
         log("plain FP64 only:", json.dumps(plain))
         return
@@ -318,6 +335,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "uzawa": uzawa,
+        "advect": advect,
         "vs_cpu": round(value / cpu["value"], 2) if cpu else None,
     }
     print(json.dumps(out), flush=True)
