@@ -12,6 +12,8 @@ LIB_PATH = os.path.join(_PKG, "libust.so")
 UST_OK = 0
 PREC_FP64, PREC_MIXED = 0, 1
 SCHUR_UW, SCHUR_LEARNED, SCHUR_SCALED = 0, 1, 2
+MAT_K, MAT_G = 0, 1
+VEC_F, VEC_H, VEC_U, VEC_P = 0, 1, 2, 3
 RH_MATERIALVISCOSITY, RH_ARRHENIUS, RH_VISCOSITYLIMITER, RH_VONMISES, RH_DRUCKERPRAGER = 1, 2, 3, 4, 5
 SHAPE_EVERYWHERE, SHAPE_BELOWCOSINEPLANE, SHAPE_SPHERE, SHAPE_BOX = 0, 1, 2, 3
 TERM_CONSTITUTIVE, TERM_GRADIENT, TERM_UZAWA_PC, TERM_BUOYANCY = 1, 2, 3, 4
@@ -83,6 +85,8 @@ _SIGS = {
     "ust_swarm_download": (I, [vp, vp, vp, vp, vp]),
     "ust_swarm_count": (I, [vp]),
     "ust_swarm_advect": (I, [vp, D, I, I, C.POINTER(I), C.POINTER(I)]),
+    "ust_write_petsc_matrix": (I, [vp, I, I, C.c_char_p]),
+    "ust_write_petsc_vector": (I, [vp, I, I, C.c_char_p]),
     "ust_set_temperature": (I, [vp, vp]),
     "ust_set_solution": (I, [vp, vp, vp]),
     "ust_set_previous_solution_exists": (I, [vp, I]),

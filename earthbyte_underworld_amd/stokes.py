@@ -320,6 +320,16 @@ class StokesSystem:
         self._check(self.lib.ust_solve_uzawa(self.ctx, C.byref(uo), C.byref(o), C.byref(st)))
         return st.as_dict()
 
+    def write_petsc_matrix(self, which, path, free_only=False):
+        """PETSc binary dump of "K" or "G" (the reference's -matdumpdir / .matrix fixture format)."""
+        code = {"K": L.MAT_K, "G": L.MAT_G}[which]
+        self._check(self.lib.ust_write_petsc_matrix(self.ctx, code, int(bool(free_only)), str(path).encode()))
+
+    def write_petsc_vector(self, which, path, free_only=False):
+        """PETSc binary dump of "f", "h", "u" or "p"."""
+        code = {"f": L.VEC_F, "h": L.VEC_H, "u": L.VEC_U, "p": L.VEC_P}[which]
+        self._check(self.lib.ust_write_petsc_vector(self.ctx, code, int(bool(free_only)), str(path).encode()))
+
     def advect(self, dt, order=2, allowFallbackToFirstOrder=False):
         """SwarmAdvector step through the current velocity solution, then the re-sort by element; returns
         (particles removed, elements left empty)."""

@@ -156,6 +156,19 @@ int ust_apply_Gt(ust_context *ctx, const double *u, double *q);
 int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *stats);
 int ust_get_solution(ust_context *ctx, double *u_nodal, double *p);
 int ust_get_solution_dev(ust_context *ctx, const double **u_nodal_dev, const double **p_dev);
+/* ---- PETSc binary dumps (-matdumpdir / writeMatVec.c role, auglag-driver-DGTGD.c:101-113) ----
+ * Big-endian PETSc binary AIJ matrices and vectors, the format of the reference's .matrix fixtures.  K and G
+ * are formed from the element matrices of the current assembly (structural Q1/P0 pattern); free_only drops
+ * the Dirichlet velocity dofs and numbers the rest in node-major order (the reference's equation numbers). */
+#define UST_MAT_K 0
+#define UST_MAT_G 1
+#define UST_VEC_F 0 /* momentum right-hand side (lifted) */
+#define UST_VEC_H 1 /* continuity right-hand side */
+#define UST_VEC_U 2 /* nodal velocity solution (BC values included) */
+#define UST_VEC_P 3 /* element pressure */
+int ust_write_petsc_matrix(ust_context *ctx, int which, int free_only, const char *path);
+int ust_write_petsc_vector(ust_context *ctx, int which, int free_only, const char *path);
+
 /* ---- Uzawa solve (Stokes_SLE_UzawaSolver, Stokes_SLE_UzawaSolver.c:407-983) ----
  * Preconditioned CG on the pressure, Schur complement G^T K^-1 G applied through one velocity solve per
  * iteration; starts from the context's current pressure (the reference's pSolnVec), as the reference does.

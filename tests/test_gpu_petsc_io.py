@@ -1,0 +1,88 @@
+"""PETSc binary dumps of the GPU-assembled operators against the reference's own .matrix fixtures and the
+oracle's free-dof operators.
+
+Bar: the full K of the fixture configurations within 1e-12 relative (Frobenius) of the reference's files;
+free-dof K, G, f, h within 1e-12 of the oracle's; the files readable as PETSc binary (class ids, sizes).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests.helpers import OracleProblem, sinusoidal_temperature
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+G2 = 1.0 / np.sqrt(3.0)
+
+
+def _read_vec(path):
+    raw = open(path, "rb").read()
+    cid, n = np.frombuffer(raw[:8], ">i4")
+    assert cid == 1211214
+    return np.frombuffer(raw[8:8 + 8 * n], ">f8").astype(np.float64)
+
+
+def _gauss_system(nel, rheologies, T=None):
+    """2x2 Gauss points with weight 1 per element (the GaussSwarm of the reference's assembly tests)"""
+    import earthbyte_underworld_amd as ust
+    s = ust.StokesSystem(2, nel, (0.0, 0.0), (1.0, 1.0))
+    ne = nel[0] * nel[1]
+    xi = np.zeros((4 * ne, 3))
+    for k, (a, b) in enumerate([(-G2, -G2), (G2, -G2), (-G2, G2), (G2, G2)]):
+        xi[k::4, 0], xi[k::4, 1] = a, b
+    s.set_materials([ust.RheologyMaterial(rheologies)])
+    s.upload_swarm(np.arange(0, 4 * ne + 1, 4).astype(np.int32), xi, np.ones(4 * ne), np.zeros(4 * ne, np.uint8))
+    if T is not None:
+        s.set_temperature(T(s))
+    s.assemble()
+    return s
+
+
+@pytest.mark.parametrize("name,nel,kind", [("testMaterialViscosity2D.matrix", (10, 10), "mv"),
+                                           ("testIsoviscousSiffness.matrix", (3, 3), "mv"),
+                                           ("testArrhenius2D.matrix", (10, 10), "arr"),
+                                           ("testArrheniusWithLimiter.matrix", (10, 10), "lim")])
+def test_K_dump_matches_reference_fixture(oracle, tmp_path, name, nel, kind):
+    import earthbyte_underworld_amd as ust
+    if kind == "mv":
+        rh = [ust.MaterialViscosity(eta0=1.0)]
+    else:
+        rh = [ust.Arrhenius(eta0=1e-6, activationEnergy=27.63102112)]
+        if kind == "lim":
+            rh.append(ust.ViscosityLimiter(maxViscosity=10e4, minViscosity=10))
+    s = _gauss_system(nel, rh, None if kind == "mv" else sinusoidal_temperature)
+    out = tmp_path / "K.matrix"
+    s.write_petsc_matrix("K", out)
+    K = oracle.read_petsc_matrix(str(out)).toarray()
+    E = oracle.read_petsc_matrix(os.path.join(GOLDEN, name)).toarray()
+    assert K.shape == E.shape
+    assert np.linalg.norm(K - E) / np.linalg.norm(E) < 1e-12
+    s.close()
+
+
+def test_free_dof_dumps_match_oracle(oracle, tmp_path):
+    import earthbyte_underworld_amd as ust
+    from tests.test_gpu_solver import _spheres_3d
+    s = _spheres_3d(nel=(6, 5, 4), ppc=8)
+    mask, _ = ust.free_slip_walls(s)
+    s.set_bcs(mask)
+    s.assemble(gravity=1.0)
+    o = OracleProblem(s)
+    gs = o.system(mask)
+    K0, G0 = gs.scipy()
+    for which, ref in (("K", K0), ("G", G0)):
+        s.write_petsc_matrix(which, tmp_path / f"{which}.matrix", free_only=True)
+        A = oracle.read_petsc_matrix(str(tmp_path / f"{which}.matrix"))
+        assert A.shape == ref.shape
+        assert abs(A - ref).max() / abs(ref).max() < 1e-12
+    for which, ref in (("f", gs.f), ("h", gs.h)):
+        s.write_petsc_vector(which, tmp_path / f"{which}.vec", free_only=True)
+        v = _read_vec(tmp_path / f"{which}.vec")
+        assert v.shape == ref.shape
+        assert np.linalg.norm(v - ref) / np.linalg.norm(ref) < 1e-12
+    # full-size velocity vector: every dof
+    s.write_petsc_vector("u", tmp_path / "u.vec")
+    assert _read_vec(tmp_path / "u.vec").shape == (s.ndof,)
+    s.close()
