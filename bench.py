@@ -51,12 +51,13 @@ def time_on_stream(lib, stream, fn, reps):
     return ms.value / reps
 
 
-def solve_opts(ust, precision, learned, relax=0, degree=2, coarse_degree=0):
+def solve_opts(ust, precision, learned, relax=0, degree=2, coarse_degree=0, cheby_ratio=12.0):
     from earthbyte_underworld_amd import _lib as L
     # stokes_rtol: the metric's 1e-6 bar on the FP64 Stokes residual is enforced, not assumed
     return ust.default_opts(precision=precision, schur_pc=L.SCHUR_LEARNED if learned else L.SCHUR_UW,
                             outer_rtol=1e-7, inner_rtol=1e-8, max_outer=2000, inner_relax=relax,
-                            smoother_degree=degree, coarse_degree=coarse_degree, stokes_rtol=1e-6)
+                            smoother_degree=degree, coarse_degree=coarse_degree, cheby_ratio=cheby_ratio,
+                            stokes_rtol=1e-6)
 
 
 def cpu_baseline(size, ppc, nsph, seed, cpu_size, threads):
@@ -216,7 +217,9 @@ def main():
     # headline path: Chebyshev degree 1 on the finest level (cheaper V-cycles outweigh the extra iterations
     # there), degree 2 on the coarser levels (short passes whose extra smoothing cuts the inner iterations);
     # the plain path keeps the default degree 2 everywhere
-    opts_mixed = solve_opts(ust, L.PREC_MIXED, learned, args.relax, degree=1, coarse_degree=2)
+    # Chebyshev interval [lambda_max / 30, 1.1 lambda_max]: a wider interval than the default 12 suits the
+    # degree-2 coarse smoothing here (5,217 -> 5,048 inner iterations at 96^3; flat from 30 to 150)
+    opts_mixed = solve_opts(ust, L.PREC_MIXED, learned, args.relax, degree=1, coarse_degree=2, cheby_ratio=30.0)
     opts_plain = solve_opts(ust, L.PREC_FP64, False)
 
     def step(opts):
@@ -324,7 +327,8 @@ def main():
                                f"log10 eta U[-2,4], free-slip, BSSCR to 1e-6 relative Stokes residual",
                    "elements": s.nel, "particles": int(np_), "velocity_dofs": s.ndof,
                    "learned_schur_pc": learned, "inner_tolerance_relaxed": bool(args.relax),
-                   "smoother_degree": {"mixed": {"finest": 1, "coarser": 2}, "plain": 2}},
+                   "smoother_degree": {"mixed": {"finest": 1, "coarser": 2}, "plain": 2},
+                   "chebyshev_ratio": {"mixed": 30.0, "plain": 12.0}},
         "stokes_residual": worst_res,
         "outer_its": stats[-1]["outer_its"],
         "inner_its": stats[-1]["inner_its"],
