@@ -144,6 +144,10 @@ _SIGS = {
     "ust_k_apply_K_cheby_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, I, vp, vp]),
     "ust_k_apply_K_cheby_prolong_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, D, vp, vp, vp]),
     "ust_k_apply_K_cheby_prolong_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, C.c_float, vp, vp, vp]),
+    "ust_k_restrict_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp]),
+    "ust_k_restrict_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp]),
+    "ust_k_restrict_cheby_first_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp, D, vp]),
+    "ust_k_restrict_cheby_first_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp, C.c_float, vp]),
     "ust_k_prolong_add_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp]),
     "ust_k_prolong_add_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp]),
     "ust_k_particle_weights": (I, [C.POINTER(Grid), vp, I, vp, vp, vp, vp, vp, vp, vp, I, vp, vp, vp, I, D, vp,

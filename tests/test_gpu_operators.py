@@ -246,3 +246,71 @@ def test_prolongation_fused_chebyshev_step(prec):
     for a in keep + [ws, dxc, db, di, xa, da, xb, dd]:
         a.free()
     s.close()
+
+
+def _restrict_numpy(r, fmask, cmask, nnf, dim):
+    """b_c = P^T r over the free fine dofs: fine nodes 2I-1..2I+1 per axis, weights 1/2 off the centre"""
+    nnc = tuple((n - 1) // 2 + 1 for n in nnf)
+    rf = np.where(fmask, 0.0, r).reshape(tuple(reversed(nnf)) + (dim,))  # [k][j][i][c]
+    out = np.zeros(tuple(reversed(nnc)) + (dim,))
+    offs = [(-1, 0.5), (0, 1.0), (1, 0.5)]
+    zoffs = offs if dim == 3 else [(0, 1.0)]
+    for dz, wz in zoffs:
+        for dy, wy in offs:
+            for dx, wx in offs:
+                K, J, I = np.meshgrid(*[np.arange(n) for n in reversed(nnc)], indexing="ij")
+                k, j, i = 2 * K + dz, 2 * J + dy, 2 * I + dx
+                ok = (i >= 0) & (i < nnf[0]) & (j >= 0) & (j < nnf[1]) & (k >= 0) & (k < nnf[2])
+                v = rf[np.clip(k, 0, nnf[2] - 1), np.clip(j, 0, nnf[1] - 1), np.clip(i, 0, nnf[0] - 1)]
+                out += np.where(ok[..., None], wx * wy * wz * v, 0.0)
+    out = out.reshape(-1)
+    out[cmask] = 0.0
+    return out
+
+
+@pytest.mark.parametrize("dim,nel", [(3, (18, 10, 6)), (3, (16, 16, 16)), (2, (20, 14))])
+def test_restriction_is_adjoint_of_prolongation(dim, nel):
+    """The restriction against the formula, <R r, xc> = <r, P xc> with the prolongation
+    kernel, and the fused first smoothing step dc = xc = c2 invd R r."""
+    import ctypes as C
+    from earthbyte_underworld_amd import _lib as L
+    s = _build(dim, nel, 6 if dim == 3 else 9, 2)
+    lib = s.lib
+    g, WK, WG, WF, mask = L.Grid(), C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert lib.ust_get_device_views(s.ctx, C.byref(g), C.byref(WK), C.byref(WG), C.byref(WF), C.byref(mask)) == 0
+    nnf = tuple(n + 1 for n in nel) + ((1,) if dim == 2 else ())
+    nnc = tuple((n - 1) // 2 + 1 for n in nnf)
+    fm = s.bc_mask.reshape(-1) != 0
+    K_, J_, I_ = np.meshgrid(*[np.arange(n) for n in reversed(nnc)], indexing="ij")  # nnc is padded to 3 axes
+    fnode = (2 * I_ + nnf[0] * (2 * J_ + nnf[1] * 2 * K_)).reshape(-1)
+    cm = fm.reshape(-1, dim)[fnode].reshape(-1)
+    rng = np.random.default_rng(5)
+    r = rng.normal(size=fm.size)
+    xc = rng.normal(size=cm.size)
+    xc[cm] = 0.0
+    ref = _restrict_numpy(r, fm, cm, nnf, dim)
+    dcm = _Dev(lib, cm.astype(np.uint8))
+    for prec, dt, tol in (("f64", np.float64, 1e-14), ("f32", np.float32, 1e-6)):
+        dr, db = _Dev(lib, r.astype(dt)), _Dev(lib, np.zeros(cm.size, dt))
+        assert getattr(lib, f"ust_k_restrict_{prec}")(C.byref(g), mask, dcm.p, dr.p, db.p, None) == 0
+        b = db.get().astype(np.float64)
+        assert np.linalg.norm(b - ref) <= tol * np.linalg.norm(ref)
+        dxc, dxf = _Dev(lib, xc.astype(dt)), _Dev(lib, np.zeros(fm.size, dt))
+        assert getattr(lib, f"ust_k_prolong_add_{prec}")(C.byref(g), mask, dxc.p, dxf.p, None) == 0
+        Px = dxf.get().astype(np.float64)
+        lhs, rhs = b @ xc, np.where(fm, 0.0, r) @ Px
+        assert abs(lhs - rhs) <= 10 * tol * np.linalg.norm(b) * np.linalg.norm(xc)
+        invd = rng.uniform(0.5, 2.0, cm.size).astype(dt)
+        di, dd, dx2, db2 = _Dev(lib, invd), _Dev(lib, np.zeros(cm.size, dt)), _Dev(lib, np.zeros(cm.size, dt)), \
+            _Dev(lib, np.zeros(cm.size, dt))
+        c2 = 1.3
+        assert getattr(lib, f"ust_k_restrict_cheby_first_{prec}")(C.byref(g), mask, dcm.p, dr.p, db2.p, di.p, dd.p,
+                                                                  dx2.p, c2, None) == 0
+        b2 = db2.get()
+        assert np.array_equal(b2, db.get())
+        x0 = (dt(c2) * invd * b2).astype(np.float64)
+        assert np.allclose(dd.get(), x0, rtol=tol, atol=0) and np.allclose(dx2.get(), x0, rtol=tol, atol=0)
+        for a in (dr, db, dxc, dxf, di, dd, dx2, db2):
+            a.free()
+    dcm.free()
+    s.close()
