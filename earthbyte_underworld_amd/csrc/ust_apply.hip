@@ -531,7 +531,8 @@ static inline unsigned face_node_count(const GridDev &g) {
     return fz * g.nnx * g.nny + fy * g.nnx * (g.nnz - fz) + fx * (g.nny - fy) * (g.nnz - fz);
 }
 
-// one thread per tile-face node, all components (32-bit index arithmetic: grids stay below 2^31 dofs)
+// one thread per tile-face dof: consecutive threads take the components of a node, so the slot reads of a
+// wavefront are contiguous (32-bit index arithmetic: grids stay below 2^31 dofs)
 template <typename T, int DIM, int MODE>
 __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned char *__restrict__ mask,
                                                       const T *__restrict__ work, Epi<T> ep) {
@@ -541,7 +542,9 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned 
     const unsigned nZ = (unsigned)fz * g.nnx * g.nny;
     const unsigned nY = (unsigned)fy * g.nnx * (g.nnz - fz);
     const unsigned nX = (unsigned)fx * (g.nny - fy) * (g.nnz - fz);
-    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned tt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = (int)(tt % DIM);
+    unsigned t = tt / DIM;
     int gx, gy, gz;
     if (t < nZ) {
         const unsigned r = t / g.nnx;
@@ -566,33 +569,22 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned 
     const int nxs = node_tiles(gx, TX, g.tnx, tx, lxs);
     const int nys = node_tiles(gy, TY, g.tny, ty, lys);
     const int nzs = DIM == 3 ? node_tiles(gz, TZ, g.tnz, tz, lzs) : 1;
-    EpiOps<T> ops[DIM];
-    T acc[DIM];
-#pragma unroll
-    for (int c = 0; c < DIM; c++) {
-        ops[c] = epilogue_load<MODE, T>(ep, mask, n * DIM + c);
-        acc[c] = (T)0;
-    }
+    EpiOps<T> ops = epilogue_load<MODE, T>(ep, mask, n * DIM + c);
+    T acc = (T)0;
     for (int a = 0; a < nzs; a++)
         for (int b = 0; b < nys; b++)
             for (int d = 0; d < nxs; d++) {
                 const unsigned tile = (unsigned)tx[d] + (unsigned)g.tnx * ((unsigned)ty[b] + (unsigned)g.tny * tz[a]);
                 const unsigned ln = lxs[d] + NX * (lys[b] + NY * lzs[a]);
-                const T *src = work + ((size_t)tile * NN + ln) * DIM;
-#pragma unroll
-                for (int c = 0; c < DIM; c++) acc[c] += src[c];
+                acc += work[((size_t)tile * NN + ln) * DIM + c];
             }
-    if (MODE == EPI_PCHEBY0) {  // 3D only (host-checked): x + P xc on the free components
+    if (MODE == EPI_PCHEBY0 && !ops.m) {  // 3D only (host-checked): x + P xc on a free component
         const int cnx = g.nx / 2 + 1, cny = g.ny / 2 + 1;
-#pragma unroll
-        for (int c = 0; c < DIM; c++)
-            if (!ops[c].m)
-                ops[c].x += prolong_value<T>(gx, gy, gz, [&](int I, int J, int K) {
-                    return ep.xc[((unsigned)I + (unsigned)cnx * ((unsigned)J + (unsigned)cny * K)) * 3u + c];
-                });
+        ops.x += prolong_value<T>(gx, gy, gz, [&](int I, int J, int K) {
+            return ep.xc[((unsigned)I + (unsigned)cnx * ((unsigned)J + (unsigned)cny * K)) * 3u + c];
+        });
     }
-#pragma unroll
-    for (int c = 0; c < DIM; c++) epilogue_store<MODE, T>(ep, ops[c], n * DIM + c, acc[c]);
+    epilogue_store<MODE, T>(ep, ops, n * DIM + c, acc);
 }
 
 static inline size_t work_len(const GridDev &g) {
@@ -613,13 +605,13 @@ static int apply_tiled(const ust_grid *gr, const T *W, const unsigned char *mask
     if (g.dim == 3) {
         hipLaunchKernelGGL((k_applyK3_tiled<T, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep, work);
         const unsigned nf = face_node_count<3>(g);
-        if (nf) hipLaunchKernelGGL((k_gather_tiles<T, 3, MODE>), dim3(ust_grid_size(nf, 256)), dim3(256), 0, st, g,
-                                   mask, work, ep);
+        if (nf) hipLaunchKernelGGL((k_gather_tiles<T, 3, MODE>), dim3(ust_grid_size((size_t)nf * 3, 256)), dim3(256),
+                                   0, st, g, mask, work, ep);
     } else {
         hipLaunchKernelGGL((k_applyK2_tiled<T, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep, work);
         const unsigned nf = face_node_count<2>(g);
-        if (nf) hipLaunchKernelGGL((k_gather_tiles<T, 2, MODE>), dim3(ust_grid_size(nf, 256)), dim3(256), 0, st, g,
-                                   mask, work, ep);
+        if (nf) hipLaunchKernelGGL((k_gather_tiles<T, 2, MODE>), dim3(ust_grid_size((size_t)nf * 2, 256)), dim3(256),
+                                   0, st, g, mask, work, ep);
     }
     UST_LAUNCH_CHECK();
     return UST_OK;
