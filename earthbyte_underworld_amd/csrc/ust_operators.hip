@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void k_restrict(GridDev gf, GridDev gc, const 
                 const int i = 2 * I + dx;
                 const bool okx = i >= 0 && i < gf.nnx;
                 const int ic = okx ? i : 2 * I;
-                const size_t fd = ((size_t)ic + (size_t)gf.nnx * ((size_t)jc + (size_t)gf.nny * kc)) * DIM + c;
+                const unsigned fd = ((unsigned)ic + (unsigned)gf.nnx * ((unsigned)jc + (unsigned)gf.nny * kc)) * DIM + c;
                 const T v = rf[fd];
                 const bool use = okx && oky && okz && !maskf[fd];
                 const T wt = (T)((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0));
