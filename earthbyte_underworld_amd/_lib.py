@@ -85,6 +85,7 @@ _SIGS = {
     "ust_swarm_download": (I, [vp, vp, vp, vp, vp]),
     "ust_swarm_count": (I, [vp]),
     "ust_swarm_advect": (I, [vp, D, I, I, C.POINTER(I), C.POINTER(I)]),
+    "ust_swarm_dvc_weights": (I, [vp, C.POINTER(I), I]),
     "ust_write_petsc_matrix": (I, [vp, I, I, C.c_char_p]),
     "ust_write_petsc_vector": (I, [vp, I, I, C.c_char_p]),
     "ust_set_temperature": (I, [vp, vp]),

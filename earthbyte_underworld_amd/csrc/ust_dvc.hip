@@ -1,0 +1,246 @@
+// Discrete Voronoi cell (DVC) integration weights on gfx950.
+//
+// DVCWeights (PICellerator/Weights/src/DVCWeights.c): every element carries a fixed sub-grid of nx*ny(*nz)
+// cells over its local box [-1,1]^dim; the particles grow their Voronoi regions over it from their seed cells,
+// one particle after another, round after round, until no particle claims a cell.  A particle claims an unowned
+// cell on its boundary, and takes an owned one when it is strictly nearer the cell centre than the owner
+// (_ClaimCells :261-317 with _DistanceTest :604-610); its next boundary is the set of neighbours of the cells it
+// has just claimed that it does not own (_UpdateBchain :395-446).  Then each particle moves to the centroid of
+// its cells and weighs their volume (_GetCentroids :203-231, _Calculate3D/2D :763-925).
+//
+// Within one particle's turn every boundary cell is decided on its own (owner and distances only), and the next
+// boundary is a set, so the reference's lists can be held as sets: one bit per sub-cell.  Only the particle
+// order is sequential.  One 64-lane wave owns one element: lane L handles sub-cells 64 j + L, a turn is one
+// ballot per 64 sub-cells, and the boundary sets of all particles of the element stay in LDS as bitmasks.
+#include "ust_common.h"
+
+namespace {
+
+constexpr int DVC_LANES = 64;
+
+struct DvcShape {
+    int n[3];     // sub-grid resolution (nz = 1 in 2D)
+    int ncell;    // n[0] n[1] n[2]
+    int words;    // 64-bit words of a sub-cell set
+    int max_np;   // particles per element the LDS layout holds
+};
+
+// (x_o + x_p - 2 x_c).(x_o - x_p) in the reference's evaluation order, without contraction into fused
+// multiply-adds: > 0 when particle p is strictly nearer the centre c than the owner o
+template <int DIM>
+__device__ __forceinline__ double dvc_distance_test(const double *c, const double *po, const double *pp) {
+#pragma clang fp contract(off)
+    double s = (po[0] + pp[0] - c[0] - c[0]) * (po[0] - pp[0]);
+    s = s + (po[1] + pp[1] - c[1] - c[1]) * (po[1] - pp[1]);
+    if (DIM == 3) s = s + (po[2] + pp[2] - c[2] - c[2]) * (po[2] - pp[2]);
+    return s;
+}
+
+// bit c of a sub-cell set
+__device__ __forceinline__ bool in_set(const unsigned long long *set, int c) {
+    return (set[c >> 6] >> (c & 63)) & 1ull;
+}
+
+// a face neighbour of sub-cell c is in the set `claimed` (the neighbour relation is symmetric, so c borders
+// the claimed cells exactly when one of its neighbours is claimed)
+template <int DIM>
+__device__ __forceinline__ bool touches(const DvcShape &s, const unsigned long long *claimed, int c) {
+    const int nx = s.n[0], ny = s.n[1], nxy = nx * ny;
+    const int i = c % nx, j = (c / nx) % ny, k = c / nxy;
+    bool t = false;
+    if (j > 0) t = t || in_set(claimed, c - nx);
+    if (j < ny - 1) t = t || in_set(claimed, c + nx);
+    if (i < nx - 1) t = t || in_set(claimed, c + 1);
+    if (i > 0) t = t || in_set(claimed, c - 1);
+    if (DIM == 3) {
+        if (k < s.n[2] - 1) t = t || in_set(claimed, c + nxy);
+        if (k > 0) t = t || in_set(claimed, c - nxy);
+    }
+    return t;
+}
+
+// flag bits: 1 element without particles, 2 more particles than the LDS layout holds, 4 round limit reached
+template <int DIM>
+__global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int *cell_off, int nel, double *xi0,
+                                                           double *xi1, double *xi2, double *w, int set_centroids,
+                                                           int *flag) {
+    extern __shared__ __align__(16) unsigned char dvc_lds[];
+    const int e = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (e >= nel) return;
+    const int off = cell_off[e];
+    const int np = cell_off[e + 1] - off;
+    if (np <= 0 || np > s.max_np) {
+        if (lane == 0) atomicOr(flag, np <= 0 ? 1 : 2);
+        return;
+    }
+    const int W = s.words, ncell = s.ncell;
+    // LDS: centres per axis | particle coordinates (np x DIM) | boundary sets (np x W) | claimed set (W) | owner
+    double *axc = (double *)dvc_lds;                                  // n[0] + n[1] + n[2]
+    double *pc = axc + (s.n[0] + s.n[1] + s.n[2]);                    // max_np * DIM
+    unsigned long long *bset = (unsigned long long *)(pc + (size_t)s.max_np * DIM);  // max_np * W
+    unsigned long long *cset = bset + (size_t)s.max_np * W;           // W
+    short *owner = (short *)(cset + W);                                // ncell
+
+    const double d[3] = {2.0 / s.n[0], 2.0 / s.n[1], 2.0 / s.n[2]};
+    if (lane < DIM) {  // centres accumulated as the grid constructor does: -1 - d/2, then + d per cell
+        const int n = lane == 0 ? s.n[0] : (lane == 1 ? s.n[1] : s.n[2]);
+        const double dl = lane == 0 ? d[0] : (lane == 1 ? d[1] : d[2]);
+        double *a = axc + (lane == 0 ? 0 : (lane == 1 ? s.n[0] : s.n[0] + s.n[1]));
+        double x = -1.0 - dl / 2.0;
+        for (int i = 0; i < n; i++) {
+            x = x + dl;
+            a[i] = x;
+        }
+    }
+    for (int p = lane; p < np; p += DVC_LANES) {
+        pc[p * DIM + 0] = xi0[off + p];
+        pc[p * DIM + 1] = xi1[off + p];
+        if (DIM == 3) pc[p * DIM + 2] = xi2[off + p];
+    }
+    for (int c = lane; c < ncell; c += DVC_LANES) owner[c] = -1;
+    __syncthreads();
+
+    const double *cxs = axc, *cys = axc + s.n[0], *czs = axc + s.n[0] + s.n[1];
+    const int nx = s.n[0], nxy = s.n[0] * s.n[1];
+
+    // the new boundary of particle p from the set in cset (all lanes; cset and owner final)
+    auto update = [&](int p) {
+        for (int j = 0; j < W; j++) {
+            const int c = 64 * j + lane;
+            bool b = false;
+            if (c < ncell) b = owner[c] != p && touches<DIM>(s, cset, c);
+            const unsigned long long m = __ballot(b);
+            if (lane == 0) bset[(size_t)p * W + j] = m;
+        }
+    };
+
+    // seeds, each followed at once by its boundary (DVCWeights.c:676-696)
+    for (int p = 0; p < np; p++) {
+        if (lane == 0) {
+            int idx[3] = {0, 0, 0};
+#pragma unroll
+            for (int a = 0; a < DIM; a++) {
+                const double t = (pc[p * DIM + a] - (-1.0)) / d[a];
+                // C truncation toward zero; k == n moves to n - 1; outside [-1, 1] (and NaN): the nearest cell
+                int k = (t >= 0.0) ? (t < (double)s.n[a] ? (int)t : s.n[a] - 1) : 0;
+                idx[a] = k;
+            }
+            const int seed = idx[0] + idx[1] * nx + (DIM == 3 ? idx[2] * nxy : 0);
+            owner[seed] = (short)p;
+            for (int j = 0; j < W; j++) cset[j] = 0ull;
+            cset[seed >> 6] = 1ull << (seed & 63);
+        }
+        __syncthreads();
+        update(p);
+        __syncthreads();
+    }
+
+    // rounds over the particles in order until nothing is claimed (DVCWeights.c:698-710).  Every claim changes
+    // an owner and an owner only changes to a strictly nearer particle, so the rounds are bounded; the limit
+    // below keeps the loop finite whatever the rounding does.
+    const long long max_rounds = (long long)ncell * np + 2;
+    long long rounds = 0;
+    int total = 1;
+    while (total != 0) {
+        if (++rounds > max_rounds) {
+            if (lane == 0) atomicOr(flag, 4);
+            break;
+        }
+        total = 0;
+        for (int p = 0; p < np; p++) {
+            const unsigned long long *bp = bset + (size_t)p * W;
+            const double *pp = pc + p * DIM;
+            int claimed = 0;
+            for (int j = 0; j < W; j++) {
+                const int c = 64 * j + lane;
+                bool win = false;
+                if ((bp[j] >> lane) & 1ull) {
+                    const int o = owner[c];
+                    if (o == -1) {
+                        win = true;
+                    } else if (o != p) {
+                        const int ci = c % nx, cj = (c / nx) % s.n[1], ck = c / nxy;
+                        const double cc[3] = {cxs[ci], cys[cj], DIM == 3 ? czs[ck] : 0.0};
+                        win = dvc_distance_test<DIM>(cc, pc + o * DIM, pp) > 0.0;
+                    }
+                    if (win) owner[c] = (short)p;
+                }
+                const unsigned long long m = __ballot(win);
+                if (lane == 0) cset[j] = m;
+                claimed += __popcll(m);
+            }
+            __syncthreads();
+            update(p);
+            __syncthreads();
+            total += claimed;
+        }
+    }
+
+    // centroids summed in sub-cell order, weight = count * cell volume (DVCWeights.c:203-231)
+    const double vol = DIM == 3 ? d[0] * d[1] * d[2] : d[0] * d[1];
+    for (int p = lane; p < np; p += DVC_LANES) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        int count = 0;
+        int c = 0;
+        for (int k = 0; k < (DIM == 3 ? s.n[2] : 1); k++)
+            for (int j = 0; j < s.n[1]; j++)
+                for (int i = 0; i < nx; i++, c++)
+                    if (owner[c] == p) {
+                        sx += cxs[i];
+                        sy += cys[j];
+                        if (DIM == 3) sz += czs[k];
+                        count++;
+                    }
+        w[off + p] = (double)count * vol;
+        if (set_centroids) {
+            xi0[off + p] = count ? sx / count : 0.0;
+            xi1[off + p] = count ? sy / count : 0.0;
+            if (DIM == 3) xi2[off + p] = count ? sz / count : 0.0;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t ust_k_dvc_lds_bytes(int dim, const int *res, int max_np) {
+    const int nz = dim == 3 ? res[2] : 1;
+    const size_t ncell = (size_t)res[0] * res[1] * nz;
+    const size_t words = (ncell + 63) / 64;
+    return (size_t)(res[0] + res[1] + nz) * 8 + (size_t)max_np * dim * 8 + (size_t)max_np * words * 8 +
+           words * 8 + ncell * sizeof(short);
+}
+
+int ust_k_dvc_weights(int dim, const int *res, int nel, const int *cell_off, int max_np, double *xi0, double *xi1,
+                      double *xi2, double *w, int set_centroids, int *flag, void *stream) {
+    if ((dim != 2 && dim != 3) || !res || nel < 0 || !cell_off || !xi0 || !xi1 || !w || !flag || max_np < 1 ||
+        max_np > 32767)
+        return UST_EINVAL;
+    if (dim == 3 && !xi2) return UST_EINVAL;
+    DvcShape s;
+    s.n[0] = res[0];
+    s.n[1] = res[1];
+    s.n[2] = dim == 3 ? res[2] : 1;
+    for (int a = 0; a < 3; a++)
+        if (s.n[a] < 1 || s.n[a] > 64) return UST_EINVAL;
+    s.ncell = s.n[0] * s.n[1] * s.n[2];
+    if (s.ncell > 4096) return UST_EINVAL;
+    s.words = (s.ncell + 63) / 64;
+    s.max_np = max_np;
+    const size_t lds = ust_k_dvc_lds_bytes(dim, res, max_np);
+    if (lds > 65536) return UST_EINVAL;
+    if (nel == 0) return UST_OK;
+    hipStream_t st = ust_stream(stream);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_dvc_weights<3>, dim3(nel), dim3(DVC_LANES), lds, st, s, cell_off, nel, xi0, xi1, xi2, w,
+                           set_centroids, flag);
+    else
+        hipLaunchKernelGGL(k_dvc_weights<2>, dim3(nel), dim3(DVC_LANES), lds, st, s, cell_off, nel, xi0, xi1, xi2, w,
+                           set_centroids, flag);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+}  // extern "C"

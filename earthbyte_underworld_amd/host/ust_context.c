@@ -825,3 +825,40 @@ int ust_swarm_advect(ust_context *ctx, double dt, int order, int allow_fallback,
 }
 
 int ust_swarm_count(ust_context *ctx) { return ctx->np; }
+
+int ust_swarm_dvc_weights(ust_context *ctx, const int *res, int set_centroids) {
+    if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
+    if (!res) UST_FAIL(ctx, UST_EINVAL, "resolution missing");
+    const int dim = ctx->grid.dim;
+    int ncell = 1;
+    for (int a = 0; a < dim; a++) {
+        if (res[a] < 1 || res[a] > 64) UST_FAIL(ctx, UST_EINVAL, "DVC resolution %d out of 1..64", res[a]);
+        ncell *= res[a];
+    }
+    if (ncell > 4096) UST_FAIL(ctx, UST_EINVAL, "DVC sub-grid of %d cells exceeds 4096", ncell);
+    if (ctx->max_ppc < 1 || ctx->max_ppc > 32767)
+        UST_FAIL(ctx, UST_EINVAL, "%d particles per element: DVC holds 1..32767", ctx->max_ppc);
+    const size_t lds = ust_k_dvc_lds_bytes(dim, res, ctx->max_ppc);
+    if (lds > 65536)
+        UST_FAIL(ctx, UST_EINVAL, "DVC needs %zu bytes of LDS per element (%d particles, %d sub-cells): limit 65536",
+                 lds, ctx->max_ppc, ncell);
+    void *flag = NULL;
+    int hflag = 0;
+    CKD(ctx, ust_dev_malloc(&flag, sizeof(double)));
+    int rc = ust_k_fill_f64((double *)flag, 0.0, 1, ctx->stream); /* all-zero bits: the int flag reads 0 */
+    if (!rc)
+        rc = ust_k_dvc_weights(dim, res, ctx->nel, ctx->cell_off, ctx->max_ppc, ctx->xi0, ctx->xi1, ctx->xi2, ctx->w,
+                               set_centroids, (int *)flag, ctx->stream);
+    if (!rc) rc = ust_memcpy_d2h(&hflag, flag, sizeof(int), ctx->stream);
+    ust_dev_free(flag);
+    if (rc) UST_FAIL(ctx, rc, "DVC weights failed (%d): %s", rc, ust_dev_error_string());
+    /* the weights and points changed even when an element failed: nothing assembled stays valid */
+    ctx->assembled = 0;
+    ctx->cache_valid = 0;
+    if (hflag & 1)
+        UST_FAIL(ctx, UST_EINVAL, "under resolved cell: an element has no particles, add more particles "
+                                  "(DVCWeights.c:787)");
+    if (hflag & 2) UST_FAIL(ctx, UST_EINVAL, "an element holds more particles than the swarm's maximum per cell");
+    if (hflag & 4) UST_FAIL(ctx, UST_ENOTCONV, "DVC region growth did not settle within its round limit");
+    return UST_OK;
+}

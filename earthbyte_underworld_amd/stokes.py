@@ -338,6 +338,14 @@ class StokesSystem:
                                               C.byref(nr), C.byref(ne)))
         return nr.value, ne.value
 
+    def dvc_weights(self, resolution=10, set_centroids=True):
+        """DVCWeights as the weights calculator: discrete Voronoi regions of each element's particles on a
+        resolution^dim sub-grid (an int, or one per axis as resolutionX/Y/Z); weights become the region volumes
+        and, with set_centroids (the reference's behaviour), the local coordinates their centroids."""
+        res = [int(resolution)] * 3 if np.isscalar(resolution) else [int(r) for r in resolution] + [1] * 3
+        arr = (C.c_int * 3)(*res[:3])
+        self._check(self.lib.ust_swarm_dvc_weights(self.ctx, arr, int(bool(set_centroids))))
+
     def solution(self):
         u = np.zeros(self.ndof)
         p = np.zeros(self.nel)

@@ -112,6 +112,13 @@ int ust_swarm_download(ust_context *ctx, int *cell_off, double *xi, double *w, u
  * *n_removed counts them, *n_empty the elements left without particles (their element terms are then zero:
  * population control is not part of this step). */
 int ust_swarm_advect(ust_context *ctx, double dt, int order, int allow_fallback, int *n_removed, int *n_empty);
+/* DVCWeights as the swarm's weights calculator (PICellerator/Weights/src/DVCWeights.c:763-925, replacing
+ * WeightsCalculator_CalculateAll's per-cell _DVCWeights_Calculate): the particles of each element grow discrete
+ * Voronoi regions on a res[0] x res[1] (x res[2]) sub-grid of the local element (XML resolutionX/Y/Z, default
+ * 10), each weight becomes its region's volume and, when set_centroids (the reference's behaviour), the local
+ * coordinates move to the region centroid.  An element without particles is an error ("under resolved cell",
+ * DVCWeights.c:787). */
+int ust_swarm_dvc_weights(ust_context *ctx, const int *res, int set_centroids);
 int ust_swarm_count(ust_context *ctx);
 /* per-particle cohesion factors (np floats; 1 where none were set) */
 int ust_get_cohesion_scale(ust_context *ctx, float *coh);

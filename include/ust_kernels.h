@@ -161,6 +161,17 @@ int ust_k_swarm_gather(int dim, int np, const unsigned *perm, const unsigned *ke
                        const float *coh, double *o0, double *o1, double *o2, double *w, unsigned char *omat,
                        float *ocoh, void *stream);
 
+/* --- DVC integration weights (PICellerator/Weights/src/DVCWeights.c:203-925) --- */
+/* LDS bytes of one element's workgroup for sub-grid res (dim entries) and max_np particles per element */
+size_t ust_k_dvc_lds_bytes(int dim, const int *res, int max_np);
+/* Per element of a cell-sorted swarm: discrete Voronoi regions of its particles on the res sub-grid of
+ * [-1,1]^dim, weight w = region volume, xi moved to the region centroid when set_centroids.  Bits of *flag:
+ * 1 an element without particles, 2 an element with more than max_np, 4 the round limit (never reached when
+ * ownership only passes to strictly nearer particles).  res entries 1..64, at most 4096 sub-cells, LDS of
+ * ust_k_dvc_lds_bytes <= 64 KiB. */
+int ust_k_dvc_weights(int dim, const int *res, int nel, const int *cell_off, int max_np, double *xi0, double *xi1,
+                      double *xi2, double *w, int set_centroids, int *flag, void *stream);
+
 /* --- operators.  mask: per dof (n_nodes*dim), 1 = Dirichlet.  y is overwritten. --- */
 /* y = K u on the free dofs (input and output BC dofs treated as zero).  work: device workspace of
  * ust_k_apply_work_len(g) elements of the operator's type (per-tile node partial sums); one
