@@ -23,6 +23,30 @@ struct DvcShape {
     int ncell;    // n[0] n[1] n[2]
     int words;    // 64-bit words of a sub-cell set
     int max_np;   // particles per element the LDS layout holds
+    int q64, r64; // 64 = q64 * n[0] + r64: a lane's next sub-cell is 64 further on
+};
+
+// the (i, j, k) of sub-cell 64 w + lane, advanced one word at a time without divisions
+struct CellWalk {
+    int i, j, k;
+    __device__ __forceinline__ CellWalk(const DvcShape &s, int lane) {
+        i = lane % s.n[0];
+        const int t = lane / s.n[0];
+        j = t % s.n[1];
+        k = t / s.n[1];
+    }
+    __device__ __forceinline__ void next(const DvcShape &s) {
+        i += s.r64;
+        j += s.q64;
+        if (i >= s.n[0]) {
+            i -= s.n[0];
+            j++;
+        }
+        while (j >= s.n[1]) {
+            j -= s.n[1];
+            k++;
+        }
+    }
 };
 
 // (x_o + x_p - 2 x_c).(x_o - x_p) in the reference's evaluation order, without contraction into fused
@@ -44,9 +68,10 @@ __device__ __forceinline__ bool in_set(const unsigned long long *set, int c) {
 // a face neighbour of sub-cell c is in the set `claimed` (the neighbour relation is symmetric, so c borders
 // the claimed cells exactly when one of its neighbours is claimed)
 template <int DIM>
-__device__ __forceinline__ bool touches(const DvcShape &s, const unsigned long long *claimed, int c) {
+__device__ __forceinline__ bool touches(const DvcShape &s, const unsigned long long *claimed, int c,
+                                        const CellWalk &cw) {
     const int nx = s.n[0], ny = s.n[1], nxy = nx * ny;
-    const int i = c % nx, j = (c / nx) % ny, k = c / nxy;
+    const int i = cw.i, j = cw.j, k = cw.k;
     bool t = false;
     if (j > 0) t = t || in_set(claimed, c - nx);
     if (j < ny - 1) t = t || in_set(claimed, c + nx);
@@ -104,35 +129,42 @@ __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int
     const double *cxs = axc, *cys = axc + s.n[0], *czs = axc + s.n[0] + s.n[1];
     const int nx = s.n[0], nxy = s.n[0] * s.n[1];
 
-    // the new boundary of particle p from the set in cset (all lanes; cset and owner final)
-    auto update = [&](int p) {
-        for (int j = 0; j < W; j++) {
-            const int c = 64 * j + lane;
-            bool b = false;
-            if (c < ncell) b = owner[c] != p && touches<DIM>(s, cset, c);
-            const unsigned long long m = __ballot(b);
+    // the new boundary of particle p from the set in cset, whose non-zero words lie in [cmin, cmax] (all
+    // lanes; cset and owner final).  A boundary cell lies within one neighbour step (at most nx, or nx*ny in
+    // 3D, sub-cells) of a claimed cell, so words outside that reach are empty without looking.
+    const int reach = DIM == 3 ? nxy : nx;
+    auto update = [&](int p, int cmin, int cmax) {
+        const int lo = 64 * cmin - reach, hi = 64 * cmax + 63 + reach;
+        CellWalk cw(s, lane);
+        for (int j = 0; j < W; j++, cw.next(s)) {
+            unsigned long long m = 0ull;
+            if (cmax >= 0 && 64 * j + 63 >= lo && 64 * j <= hi) {
+                const int c = 64 * j + lane;
+                bool b = false;
+                if (c < ncell) b = owner[c] != p && touches<DIM>(s, cset, c, cw);
+                m = __ballot(b);
+            }
             if (lane == 0) bset[(size_t)p * W + j] = m;
         }
     };
 
     // seeds, each followed at once by its boundary (DVCWeights.c:676-696)
     for (int p = 0; p < np; p++) {
-        if (lane == 0) {
-            int idx[3] = {0, 0, 0};
+        int idx[3] = {0, 0, 0};  // every lane computes the seed (the same value)
 #pragma unroll
-            for (int a = 0; a < DIM; a++) {
-                const double t = (pc[p * DIM + a] - (-1.0)) / d[a];
-                // C truncation toward zero; k == n moves to n - 1; outside [-1, 1] (and NaN): the nearest cell
-                int k = (t >= 0.0) ? (t < (double)s.n[a] ? (int)t : s.n[a] - 1) : 0;
-                idx[a] = k;
-            }
-            const int seed = idx[0] + idx[1] * nx + (DIM == 3 ? idx[2] * nxy : 0);
+        for (int a = 0; a < DIM; a++) {
+            const double t = (pc[p * DIM + a] - (-1.0)) / d[a];
+            // C truncation toward zero; k == n moves to n - 1; outside [-1, 1] (and NaN): the nearest cell
+            idx[a] = (t >= 0.0) ? (t < (double)s.n[a] ? (int)t : s.n[a] - 1) : 0;
+        }
+        const int seed = idx[0] + idx[1] * nx + (DIM == 3 ? idx[2] * nxy : 0);
+        if (lane == 0) {
             owner[seed] = (short)p;
             for (int j = 0; j < W; j++) cset[j] = 0ull;
             cset[seed >> 6] = 1ull << (seed & 63);
         }
         __syncthreads();
-        update(p);
+        update(p, seed >> 6, seed >> 6);
         __syncthreads();
     }
 
@@ -151,27 +183,35 @@ __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int
         for (int p = 0; p < np; p++) {
             const unsigned long long *bp = bset + (size_t)p * W;
             const double *pp = pc + p * DIM;
-            int claimed = 0;
-            for (int j = 0; j < W; j++) {
-                const int c = 64 * j + lane;
-                bool win = false;
-                if ((bp[j] >> lane) & 1ull) {
-                    const int o = owner[c];
-                    if (o == -1) {
-                        win = true;
-                    } else if (o != p) {
-                        const int ci = c % nx, cj = (c / nx) % s.n[1], ck = c / nxy;
-                        const double cc[3] = {cxs[ci], cys[cj], DIM == 3 ? czs[ck] : 0.0};
-                        win = dvc_distance_test<DIM>(cc, pc + o * DIM, pp) > 0.0;
+            int claimed = 0, cmin = W, cmax = -1;
+            CellWalk cw(s, lane);
+            for (int j = 0; j < W; j++, cw.next(s)) {
+                const unsigned long long bw = bp[j];
+                unsigned long long m = 0ull;
+                if (bw != 0ull) {
+                    bool win = false;
+                    if ((bw >> lane) & 1ull) {
+                        const int c = 64 * j + lane;
+                        const int o = owner[c];
+                        if (o == -1) {
+                            win = true;
+                        } else if (o != p) {
+                            const double cc[3] = {cxs[cw.i], cys[cw.j], DIM == 3 ? czs[cw.k] : 0.0};
+                            win = dvc_distance_test<DIM>(cc, pc + o * DIM, pp) > 0.0;
+                        }
+                        if (win) owner[c] = (short)p;
                     }
-                    if (win) owner[c] = (short)p;
+                    m = __ballot(win);
                 }
-                const unsigned long long m = __ballot(win);
                 if (lane == 0) cset[j] = m;
+                if (m) {
+                    cmin = cmin < j ? cmin : j;
+                    cmax = j;
+                }
                 claimed += __popcll(m);
             }
             __syncthreads();
-            update(p);
+            update(p, cmin, cmax);
             __syncthreads();
             total += claimed;
         }
@@ -229,6 +269,8 @@ int ust_k_dvc_weights(int dim, const int *res, int nel, const int *cell_off, int
     if (s.ncell > 4096) return UST_EINVAL;
     s.words = (s.ncell + 63) / 64;
     s.max_np = max_np;
+    s.q64 = 64 / s.n[0];
+    s.r64 = 64 % s.n[0];
     const size_t lds = ust_k_dvc_lds_bytes(dim, res, max_np);
     if (lds > 65536) return UST_EINVAL;
     if (nel == 0) return UST_OK;
