@@ -29,9 +29,9 @@ struct DvcShape {
 // the (i, j, k) of sub-cell 64 w + lane, advanced one word at a time without divisions
 struct CellWalk {
     int i, j, k;
-    __device__ __forceinline__ CellWalk(const DvcShape &s, int lane) {
-        i = lane % s.n[0];
-        const int t = lane / s.n[0];
+    __device__ __forceinline__ CellWalk(const DvcShape &s, int c) {
+        i = c % s.n[0];
+        const int t = c / s.n[0];
         j = t % s.n[1];
         k = t / s.n[1];
     }
@@ -134,18 +134,20 @@ __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int
     // 3D, sub-cells) of a claimed cell, so words outside that reach are empty without looking.
     const int reach = DIM == 3 ? nxy : nx;
     auto update = [&](int p, int cmin, int cmax) {
-        const int lo = 64 * cmin - reach, hi = 64 * cmax + 63 + reach;
-        CellWalk cw(s, lane);
-        for (int j = 0; j < W; j++, cw.next(s)) {
-            unsigned long long m = 0ull;
-            if (cmax >= 0 && 64 * j + 63 >= lo && 64 * j <= hi) {
+        unsigned long long mine = 0ull;  // lane t < W ends with word t of the new boundary
+        if (cmax >= 0) {
+            const int lo = 64 * cmin - reach, hi = 64 * cmax + 63 + reach;
+            const int j0 = lo < 0 ? 0 : lo >> 6, j1 = (hi >> 6) < W - 1 ? (hi >> 6) : W - 1;
+            CellWalk cw(s, 64 * j0 + lane);
+            for (int j = j0; j <= j1; j++, cw.next(s)) {
                 const int c = 64 * j + lane;
                 bool b = false;
                 if (c < ncell) b = owner[c] != p && touches<DIM>(s, cset, c, cw);
-                m = __ballot(b);
+                const unsigned long long m = __ballot(b);
+                if (lane == j) mine = m;
             }
-            if (lane == 0) bset[(size_t)p * W + j] = m;
         }
+        if (lane < W) bset[(size_t)p * W + lane] = mine;
     };
 
     // seeds, each followed at once by its boundary (DVCWeights.c:676-696)
@@ -184,32 +186,36 @@ __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int
             const unsigned long long *bp = bset + (size_t)p * W;
             const double *pp = pc + p * DIM;
             int claimed = 0, cmin = W, cmax = -1;
-            CellWalk cw(s, lane);
-            for (int j = 0; j < W; j++, cw.next(s)) {
+            // the words of p's boundary that hold cells, one ballot; only those are visited
+            const unsigned long long bmine = lane < W ? bp[lane] : 0ull;
+            unsigned long long live = __ballot(bmine != 0ull);
+            unsigned long long cmine = 0ull;  // lane t < W ends with word t of the claimed set
+            while (live) {
+                const int j = __builtin_ctzll(live);
+                live &= live - 1ull;
                 const unsigned long long bw = bp[j];
-                unsigned long long m = 0ull;
-                if (bw != 0ull) {
-                    bool win = false;
-                    if ((bw >> lane) & 1ull) {
-                        const int c = 64 * j + lane;
-                        const int o = owner[c];
-                        if (o == -1) {
-                            win = true;
-                        } else if (o != p) {
-                            const double cc[3] = {cxs[cw.i], cys[cw.j], DIM == 3 ? czs[cw.k] : 0.0};
-                            win = dvc_distance_test<DIM>(cc, pc + o * DIM, pp) > 0.0;
-                        }
-                        if (win) owner[c] = (short)p;
+                const int c = 64 * j + lane;
+                bool win = false;
+                if ((bw >> lane) & 1ull) {
+                    const int o = owner[c];
+                    if (o == -1) {
+                        win = true;
+                    } else if (o != p) {
+                        const CellWalk cw(s, c);
+                        const double cc[3] = {cxs[cw.i], cys[cw.j], DIM == 3 ? czs[cw.k] : 0.0};
+                        win = dvc_distance_test<DIM>(cc, pc + o * DIM, pp) > 0.0;
                     }
-                    m = __ballot(win);
+                    if (win) owner[c] = (short)p;
                 }
-                if (lane == 0) cset[j] = m;
+                const unsigned long long m = __ballot(win);
+                if (lane == j) cmine = m;
                 if (m) {
                     cmin = cmin < j ? cmin : j;
-                    cmax = j;
+                    cmax = cmax > j ? cmax : j;
                 }
                 claimed += __popcll(m);
             }
+            if (lane < W) cset[lane] = cmine;
             __syncthreads();
             update(p, cmin, cmax);
             __syncthreads();
