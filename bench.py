@@ -189,6 +189,7 @@ def main():
     ap.add_argument("--kernel-only", action="store_true", help="dominant kernel + copy only (counter runs)")
     ap.add_argument("--uzawa", action="store_true", help="also time the Uzawa solver (reference defaults)")
     ap.add_argument("--advect", action="store_true", help="also time one swarm advection step + re-sort")
+    ap.add_argument("--dvc", action="store_true", help="also time DVC weights (10^3 sub-cells) on the swarm")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is design-only.",
@@ -306,6 +307,22 @@ def main():
                   "particles_per_s": round(npart / ta, 1)}
         log(f"advect: {ta * 1e3:.1f} ms for {npart} particles, removed {nr}, empty cells {ne}")
 
+    dvc = None
+    if args.dvc:
+        # DVCWeights on the benchmark swarm (last: it moves the points to their region centroids)
+        npart = lib.ust_swarm_count(ctx)
+        lib.ust_stream_sync(stream)
+        t0 = time.perf_counter()
+        try:
+            s.dvc_weights(10)
+            lib.ust_stream_sync(stream)
+            td = time.perf_counter() - t0
+            dvc = {"ms": round(td * 1e3, 2), "particles": int(npart), "resolution": 10,
+                   "particles_per_s": round(npart / td, 1)}
+            log(f"dvc weights: {td * 1e3:.1f} ms for {npart} particles")
+        except Exception as ex:  # e.g. an element emptied by the advection leg
+            log("dvc weights failed:", ex)
+
     if t_mixed is None:  # --no-mixed: the plain solve alone (profiling)
         log("plain FP64 only:", json.dumps(plain))
         return
@@ -340,6 +357,7 @@ def main():
         "cpu_baseline": cpu,
         "uzawa": uzawa,
         "advect": advect,
+        "dvc": dvc,
         "vs_cpu": round(value / cpu["value"], 2) if cpu else None,
     }
     print(json.dumps(out), flush=True)
