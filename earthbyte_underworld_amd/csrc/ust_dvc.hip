@@ -60,28 +60,24 @@ __device__ __forceinline__ double dvc_distance_test(const double *c, const doubl
     return s;
 }
 
-// bit c of a sub-cell set
-__device__ __forceinline__ bool in_set(const unsigned long long *set, int c) {
-    return (set[c >> 6] >> (c & 63)) & 1ull;
+// 64-bit value of lane `src` (0 when src is outside the wave)
+__device__ __forceinline__ unsigned long long lane_word(unsigned long long x, int src) {
+    const int lo = __shfl((int)(unsigned)x, src & 63), hi = __shfl((int)(unsigned)(x >> 32), src & 63);
+    const unsigned long long v = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+    return (src >= 0 && src < 64) ? v : 0ull;
 }
 
-// a face neighbour of sub-cell c is in the set `claimed` (the neighbour relation is symmetric, so c borders
-// the claimed cells exactly when one of its neighbours is claimed)
-template <int DIM>
-__device__ __forceinline__ bool touches(const DvcShape &s, const unsigned long long *claimed, int c,
-                                        const CellWalk &cw) {
-    const int nx = s.n[0], ny = s.n[1], nxy = nx * ny;
-    const int i = cw.i, j = cw.j, k = cw.k;
-    bool t = false;
-    if (j > 0) t = t || in_set(claimed, c - nx);
-    if (j < ny - 1) t = t || in_set(claimed, c + nx);
-    if (i < nx - 1) t = t || in_set(claimed, c + 1);
-    if (i > 0) t = t || in_set(claimed, c - 1);
-    if (DIM == 3) {
-        if (k < s.n[2] - 1) t = t || in_set(claimed, c + nxy);
-        if (k > 0) t = t || in_set(claimed, c - nxy);
-    }
-    return t;
+// a sub-cell set held one word per lane (lane t: sub-cells 64 t .. 64 t + 63), shifted by s sub-cells:
+// bit c of up(x, s) is bit c - s of x, bit c of down(x, s) is bit c + s of x
+__device__ __forceinline__ unsigned long long set_up(unsigned long long x, int s, int lane) {
+    const int q = s >> 6, r = s & 63;
+    const unsigned long long a = lane_word(x, lane - q);
+    return r ? (a << r) | (lane_word(x, lane - q - 1) >> (64 - r)) : a;
+}
+__device__ __forceinline__ unsigned long long set_down(unsigned long long x, int s, int lane) {
+    const int q = s >> 6, r = s & 63;
+    const unsigned long long a = lane_word(x, lane + q);
+    return r ? (a >> r) | (lane_word(x, lane + q + 1) << (64 - r)) : a;
 }
 
 // flag bits: 1 element without particles, 2 more particles than the LDS layout holds, 4 round limit reached
@@ -100,12 +96,11 @@ __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int
         return;
     }
     const int W = s.words, ncell = s.ncell;
-    // LDS: centres per axis | particle coordinates (np x DIM) | boundary sets (np x W) | claimed set (W) | owner
+    // LDS: centres per axis | particle coordinates (np x DIM) | boundary sets (np x W) | owner
     double *axc = (double *)dvc_lds;                                  // n[0] + n[1] + n[2]
     double *pc = axc + (s.n[0] + s.n[1] + s.n[2]);                    // max_np * DIM
     unsigned long long *bset = (unsigned long long *)(pc + (size_t)s.max_np * DIM);  // max_np * W
-    unsigned long long *cset = bset + (size_t)s.max_np * W;           // W
-    short *owner = (short *)(cset + W);                                // ncell
+    short *owner = (short *)(bset + (size_t)s.max_np * W);            // ncell
 
     const double d[3] = {2.0 / s.n[0], 2.0 / s.n[1], 2.0 / s.n[2]};
     if (lane < DIM) {  // centres accumulated as the grid constructor does: -1 - d/2, then + d per cell
@@ -129,23 +124,43 @@ __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int
     const double *cxs = axc, *cys = axc + s.n[0], *czs = axc + s.n[0] + s.n[1];
     const int nx = s.n[0], nxy = s.n[0] * s.n[1];
 
-    // the new boundary of particle p from the set in cset, whose non-zero words lie in [cmin, cmax] (all
-    // lanes; cset and owner final).  A boundary cell lies within one neighbour step (at most nx, or nx*ny in
-    // 3D, sub-cells) of a claimed cell, so words outside that reach are empty without looking.
-    const int reach = DIM == 3 ? nxy : nx;
-    auto update = [&](int p, int cmin, int cmax) {
-        unsigned long long mine = 0ull;  // lane t < W ends with word t of the new boundary
-        if (cmax >= 0) {
-            const int lo = 64 * cmin - reach, hi = 64 * cmax + 63 + reach;
-            const int j0 = lo < 0 ? 0 : lo >> 6, j1 = (hi >> 6) < W - 1 ? (hi >> 6) : W - 1;
-            CellWalk cw(s, 64 * j0 + lane);
-            for (int j = j0; j <= j1; j++, cw.next(s)) {
-                const int c = 64 * j + lane;
-                bool b = false;
-                if (c < ncell) b = owner[c] != p && touches<DIM>(s, cset, c, cw);
-                const unsigned long long m = __ballot(b);
-                if (lane == j) mine = m;
+    // per lane t < W, the sub-cells of word t that have a neighbour in -x, +x, -y, +y, and that exist
+    unsigned long long m_im = 0ull, m_ip = 0ull, m_jm = 0ull, m_jp = 0ull, m_val = 0ull;
+    if (lane < W) {
+        CellWalk cw(s, 64 * lane);
+        for (int b = 0; b < 64 && 64 * lane + b < ncell; b++) {
+            const unsigned long long bit = 1ull << b;
+            m_val |= bit;
+            if (cw.i > 0) m_im |= bit;
+            if (cw.i < nx - 1) m_ip |= bit;
+            if (cw.j > 0) m_jm |= bit;
+            if (cw.j < s.n[1] - 1) m_jp |= bit;
+            if (++cw.i == nx) {
+                cw.i = 0;
+                if (++cw.j == s.n[1]) {
+                    cw.j = 0;
+                    cw.k++;
+                }
             }
+        }
+    }
+
+    // the new boundary of particle p from the set it has just claimed (lane t < W holds word t in `cw_set`;
+    // owner final): the face neighbours of the claimed cells, by whole-word shifts, less the cells p owns
+    auto update = [&](int p, unsigned long long cw_set) {
+        unsigned long long nb = (set_up(cw_set, 1, lane) & m_im) | (set_down(cw_set, 1, lane) & m_ip) |
+                                (set_up(cw_set, nx, lane) & m_jm) | (set_down(cw_set, nx, lane) & m_jp);
+        if (DIM == 3) nb |= set_up(cw_set, nxy, lane) | set_down(cw_set, nxy, lane);
+        nb &= m_val;
+        unsigned long long live = __ballot(nb != 0ull);
+        unsigned long long mine = 0ull;  // lane t < W ends with word t of the new boundary
+        while (live) {
+            const int j = __builtin_ctzll(live);
+            live &= live - 1ull;
+            const unsigned long long nj = lane_word(nb, j);
+            const bool b = ((nj >> lane) & 1ull) && owner[64 * j + lane] != p;
+            const unsigned long long m = __ballot(b);
+            if (lane == j) mine = m;
         }
         if (lane < W) bset[(size_t)p * W + lane] = mine;
     };
@@ -160,13 +175,9 @@ __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int
             idx[a] = (t >= 0.0) ? (t < (double)s.n[a] ? (int)t : s.n[a] - 1) : 0;
         }
         const int seed = idx[0] + idx[1] * nx + (DIM == 3 ? idx[2] * nxy : 0);
-        if (lane == 0) {
-            owner[seed] = (short)p;
-            for (int j = 0; j < W; j++) cset[j] = 0ull;
-            cset[seed >> 6] = 1ull << (seed & 63);
-        }
+        if (lane == 0) owner[seed] = (short)p;
         __syncthreads();
-        update(p, seed >> 6, seed >> 6);
+        update(p, lane == (seed >> 6) ? 1ull << (seed & 63) : 0ull);
         __syncthreads();
     }
 
@@ -185,7 +196,7 @@ __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int
         for (int p = 0; p < np; p++) {
             const unsigned long long *bp = bset + (size_t)p * W;
             const double *pp = pc + p * DIM;
-            int claimed = 0, cmin = W, cmax = -1;
+            int claimed = 0;
             // the words of p's boundary that hold cells, one ballot; only those are visited
             const unsigned long long bmine = lane < W ? bp[lane] : 0ull;
             unsigned long long live = __ballot(bmine != 0ull);
@@ -209,15 +220,10 @@ __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int
                 }
                 const unsigned long long m = __ballot(win);
                 if (lane == j) cmine = m;
-                if (m) {
-                    cmin = cmin < j ? cmin : j;
-                    cmax = cmax > j ? cmax : j;
-                }
                 claimed += __popcll(m);
             }
-            if (lane < W) cset[lane] = cmine;
             __syncthreads();
-            update(p, cmin, cmax);
+            update(p, cmine);
             __syncthreads();
             total += claimed;
         }
@@ -256,7 +262,7 @@ size_t ust_k_dvc_lds_bytes(int dim, const int *res, int max_np) {
     const size_t ncell = (size_t)res[0] * res[1] * nz;
     const size_t words = (ncell + 63) / 64;
     return (size_t)(res[0] + res[1] + nz) * 8 + (size_t)max_np * dim * 8 + (size_t)max_np * words * 8 +
-           words * 8 + ncell * sizeof(short);
+           ncell * sizeof(short);
 }
 
 int ust_k_dvc_weights(int dim, const int *res, int nel, const int *cell_off, int max_np, double *xi0, double *xi1,
