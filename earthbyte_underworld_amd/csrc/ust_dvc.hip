@@ -24,16 +24,21 @@ struct DvcShape {
     int words;    // 64-bit words of a sub-cell set
     int max_np;   // particles per element the LDS layout holds
     int q64, r64; // 64 = q64 * n[0] + r64: a lane's next sub-cell is 64 further on
+    float inv[2]; // 1 / n[0], 1 / n[1]
 };
+
+// floor(c / n) for 0 <= c < 4096 and 1 <= n <= 64: (c + 1/2) / n is at least 1/128 away from an integer and the
+// float product errs by less than 4096 * 2^-22 < 1/1000, so the truncation is exact
+__device__ __forceinline__ int small_div(int c, float inv) { return (int)(((float)c + 0.5f) * inv); }
 
 // the (i, j, k) of sub-cell 64 w + lane, advanced one word at a time without divisions
 struct CellWalk {
     int i, j, k;
     __device__ __forceinline__ CellWalk(const DvcShape &s, int c) {
-        i = c % s.n[0];
-        const int t = c / s.n[0];
-        j = t % s.n[1];
-        k = t / s.n[1];
+        const int t = small_div(c, s.inv[0]);
+        i = c - t * s.n[0];
+        k = small_div(t, s.inv[1]);
+        j = t - k * s.n[1];
     }
     __device__ __forceinline__ void next(const DvcShape &s) {
         i += s.r64;
@@ -283,6 +288,8 @@ int ust_k_dvc_weights(int dim, const int *res, int nel, const int *cell_off, int
     s.max_np = max_np;
     s.q64 = 64 / s.n[0];
     s.r64 = 64 % s.n[0];
+    s.inv[0] = 1.0f / (float)s.n[0];
+    s.inv[1] = 1.0f / (float)s.n[1];
     const size_t lds = ust_k_dvc_lds_bytes(dim, res, max_np);
     if (lds > 65536) return UST_EINVAL;
     if (nel == 0) return UST_OK;
