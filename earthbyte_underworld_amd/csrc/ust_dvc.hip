@@ -85,8 +85,9 @@ __device__ __forceinline__ unsigned long long set_down(unsigned long long x, int
     return r ? (a >> r) | (lane_word(x, lane + q + 1) << (64 - r)) : a;
 }
 
-// flag bits: 1 element without particles, 2 more particles than the LDS layout holds, 4 round limit reached
-template <int DIM>
+// flag bits: 1 element without particles, 2 more particles than the LDS layout holds, 4 round limit reached.
+// OT: the owner grid's type, signed char when at most 127 particles share an element (less LDS, more waves)
+template <int DIM, typename OT>
 __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int *cell_off, int nel, double *xi0,
                                                            double *xi1, double *xi2, double *w, int set_centroids,
                                                            int *flag) {
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int
     double *axc = (double *)dvc_lds;                                  // n[0] + n[1] + n[2]
     double *pc = axc + (s.n[0] + s.n[1] + s.n[2]);                    // max_np * DIM
     unsigned long long *bset = (unsigned long long *)(pc + (size_t)s.max_np * DIM);  // max_np * W
-    short *owner = (short *)(bset + (size_t)s.max_np * W);            // ncell
+    OT *owner = (OT *)(bset + (size_t)s.max_np * W);                  // ncell
 
     const double d[3] = {2.0 / s.n[0], 2.0 / s.n[1], 2.0 / s.n[2]};
     if (lane < DIM) {  // centres accumulated as the grid constructor does: -1 - d/2, then + d per cell
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int
             idx[a] = (t >= 0.0) ? (t < (double)s.n[a] ? (int)t : s.n[a] - 1) : 0;
         }
         const int seed = idx[0] + idx[1] * nx + (DIM == 3 ? idx[2] * nxy : 0);
-        if (lane == 0) owner[seed] = (short)p;
+        if (lane == 0) owner[seed] = (OT)p;
         __syncthreads();
         update(p, lane == (seed >> 6) ? 1ull << (seed & 63) : 0ull);
         __syncthreads();
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(DVC_LANES) void k_dvc_weights(DvcShape s, const int
                         const double cc[3] = {cxs[cw.i], cys[cw.j], DIM == 3 ? czs[cw.k] : 0.0};
                         win = dvc_distance_test<DIM>(cc, pc + o * DIM, pp) > 0.0;
                     }
-                    if (win) owner[c] = (short)p;
+                    if (win) owner[c] = (OT)p;
                 }
                 const unsigned long long m = __ballot(win);
                 if (lane == j) cmine = m;
@@ -267,7 +268,7 @@ size_t ust_k_dvc_lds_bytes(int dim, const int *res, int max_np) {
     const size_t ncell = (size_t)res[0] * res[1] * nz;
     const size_t words = (ncell + 63) / 64;
     return (size_t)(res[0] + res[1] + nz) * 8 + (size_t)max_np * dim * 8 + (size_t)max_np * words * 8 +
-           ncell * sizeof(short);
+           ncell * (max_np <= 127 ? sizeof(signed char) : sizeof(short));
 }
 
 int ust_k_dvc_weights(int dim, const int *res, int nel, const int *cell_off, int max_np, double *xi0, double *xi1,
@@ -294,12 +295,19 @@ int ust_k_dvc_weights(int dim, const int *res, int nel, const int *cell_off, int
     if (lds > 65536) return UST_EINVAL;
     if (nel == 0) return UST_OK;
     hipStream_t st = ust_stream(stream);
-    if (dim == 3)
-        hipLaunchKernelGGL(k_dvc_weights<3>, dim3(nel), dim3(DVC_LANES), lds, st, s, cell_off, nel, xi0, xi1, xi2, w,
-                           set_centroids, flag);
+    const bool narrow = max_np <= 127;
+    if (dim == 3 && narrow)
+        hipLaunchKernelGGL((k_dvc_weights<3, signed char>), dim3(nel), dim3(DVC_LANES), lds, st, s, cell_off, nel,
+                           xi0, xi1, xi2, w, set_centroids, flag);
+    else if (dim == 3)
+        hipLaunchKernelGGL((k_dvc_weights<3, short>), dim3(nel), dim3(DVC_LANES), lds, st, s, cell_off, nel, xi0,
+                           xi1, xi2, w, set_centroids, flag);
+    else if (narrow)
+        hipLaunchKernelGGL((k_dvc_weights<2, signed char>), dim3(nel), dim3(DVC_LANES), lds, st, s, cell_off, nel,
+                           xi0, xi1, xi2, w, set_centroids, flag);
     else
-        hipLaunchKernelGGL(k_dvc_weights<2>, dim3(nel), dim3(DVC_LANES), lds, st, s, cell_off, nel, xi0, xi1, xi2, w,
-                           set_centroids, flag);
+        hipLaunchKernelGGL((k_dvc_weights<2, short>), dim3(nel), dim3(DVC_LANES), lds, st, s, cell_off, nel, xi0,
+                           xi1, xi2, w, set_centroids, flag);
     UST_LAUNCH_CHECK();
     return UST_OK;
 }

@@ -43,12 +43,15 @@ def test_dvc_reference_fixture(dim):
     (2, (9, 7), (7, 5), "ragged"),
     (3, (5, 4, 3), (10, 10, 10), None),
     (3, (4, 3, 3), (8, 6, 5), "ragged"),
+    (2, (3, 2), (10, 10), "wide"),
 ])
 def test_dvc_matches_oracle(dim, nel, res, counts):
     from oracle import dvc_oracle as D
     s = _system(dim, nel)
     ne = int(np.prod(nel))
-    if counts == "ragged":  # 1..70 particles per element, some more than 64 (two particle passes per lane)
+    if counts == "wide":  # more than 127 particles in an element: the 16-bit owner grid
+        s.random_particle_layout(seed=13, cell_counts=[1, 130, 5, 64, 65, 128])
+    elif counts == "ragged":  # 1..70 particles per element, some more than 64 (two particle passes per lane)
         rng = np.random.default_rng(5)
         cc = rng.integers(1, 71, ne)
         cc[0], cc[-1] = 1, 70
