@@ -10,8 +10,12 @@
 //
 // Within one particle's turn every boundary cell is decided on its own (owner and distances only), and the next
 // boundary is a set, so the reference's lists can be held as sets: one bit per sub-cell.  Only the particle
-// order is sequential.  One 64-lane wave owns one element: lane L handles sub-cells 64 j + L, a turn is one
-// ballot per 64 sub-cells, and the boundary sets of all particles of the element stay in LDS as bitmasks.
+// order is sequential.  One 64-lane wave owns one element and a sub-cell set is 64-bit words, word j covering
+// sub-cells 64 j .. 64 j + 63.  In a turn, lane L decides sub-cell 64 j + L of every boundary word j that holds
+// cells, one ballot per word, which leaves the claimed set one word per lane; the next boundary is that set
+// dilated by whole-word shifts across lanes (+-1, +-nx, +-nx*ny sub-cells, masked at the sub-grid's x and y
+// faces), less the cells the particle owns.  The boundary sets of all particles of the element, the owner grid
+// and the particle coordinates stay in LDS.
 #include "ust_common.h"
 
 namespace {
@@ -23,7 +27,6 @@ struct DvcShape {
     int ncell;    // n[0] n[1] n[2]
     int words;    // 64-bit words of a sub-cell set
     int max_np;   // particles per element the LDS layout holds
-    int q64, r64; // 64 = q64 * n[0] + r64: a lane's next sub-cell is 64 further on
     float inv[2]; // 1 / n[0], 1 / n[1]
 };
 
@@ -31,7 +34,7 @@ struct DvcShape {
 // float product errs by less than 4096 * 2^-22 < 1/1000, so the truncation is exact
 __device__ __forceinline__ int small_div(int c, float inv) { return (int)(((float)c + 0.5f) * inv); }
 
-// the (i, j, k) of sub-cell 64 w + lane, advanced one word at a time without divisions
+// the (i, j, k) of sub-cell c (x fastest) without integer divisions
 struct CellWalk {
     int i, j, k;
     __device__ __forceinline__ CellWalk(const DvcShape &s, int c) {
@@ -39,18 +42,6 @@ struct CellWalk {
         i = c - t * s.n[0];
         k = small_div(t, s.inv[1]);
         j = t - k * s.n[1];
-    }
-    __device__ __forceinline__ void next(const DvcShape &s) {
-        i += s.r64;
-        j += s.q64;
-        if (i >= s.n[0]) {
-            i -= s.n[0];
-            j++;
-        }
-        while (j >= s.n[1]) {
-            j -= s.n[1];
-            k++;
-        }
     }
 };
 
@@ -287,8 +278,6 @@ int ust_k_dvc_weights(int dim, const int *res, int nel, const int *cell_off, int
     if (s.ncell > 4096) return UST_EINVAL;
     s.words = (s.ncell + 63) / 64;
     s.max_np = max_np;
-    s.q64 = 64 / s.n[0];
-    s.r64 = 64 % s.n[0];
     s.inv[0] = 1.0f / (float)s.n[0];
     s.inv[1] = 1.0f / (float)s.n[1];
     const size_t lds = ust_k_dvc_lds_bytes(dim, res, max_np);
