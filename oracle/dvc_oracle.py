@@ -6,7 +6,7 @@ Follows PICellerator/Weights/src/DVCWeights.c step by step, with the reference's
 and claimed lists per particle, a `done` flag for de-duplication):
 - the fixed sub-grid over the local element [-1,1]^dim, x fastest, cell centres accumulated as x0 - dx/2 + dx
   + dx ... (_DVCWeights_ConstructGrid / _ConstructGrid2D, :500-595), neighbours S/N = -+nx, E/W = +-1,
-  U/D = +-nx*ny, -2 outside;
+  U/D = +-nx*ny, -2 outside (pinned by DVCWeightsSuite_testConstructGrid{,2D}.txt);
 - the seed cell of particle i: k = (int)((x - (-1))/dx) per axis, k == n moved to n-1 (:676-684), owned by i
   (a later particle seeded in the same cell takes it), then the boundary of i at once (:685-695);
 - rounds over the particles in order until no particle claims a cell (:700-710): a boundary cell with no owner
@@ -39,6 +39,29 @@ def _axis_centres(n):
     return d, out
 
 
+def sub_grid(res):
+    """The fixed sub-grid of _DVCWeights_ConstructGrid / _ConstructGrid2D: cell centres (ncell, dim) and the
+    neighbour table (ncell, 2 dim) in the order N, S, E, W[, U, D], -2 where there is none."""
+    dim = len(res)
+    nx, ny = res[0], res[1]
+    nz = res[2] if dim == 3 else 1
+    _, cxs = _axis_centres(nx)
+    _, cys = _axis_centres(ny)
+    czs = _axis_centres(nz)[1] if dim == 3 else None
+    cen, nbr = [], []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = i + nx * j + nx * ny * k
+                cen.append((cxs[i], cys[j], czs[k]) if dim == 3 else (cxs[i], cys[j]))
+                row = [c + nx if j < ny - 1 else -2, c - nx if j > 0 else -2,
+                       c + 1 if i < nx - 1 else -2, c - 1 if i > 0 else -2]
+                if dim == 3:
+                    row += [c + nx * ny if k < nz - 1 else -2, c - nx * ny if k > 0 else -2]
+                nbr.append(row)
+    return np.array(cen), np.array(nbr)
+
+
 def dvc_cell(pts, res):
     """DVC centroids and weights of the particles of one element.
 
@@ -52,23 +75,17 @@ def dvc_cell(pts, res):
     nx, ny = res[0], res[1]
     nz = res[2] if dim == 3 else 1
     ncell = nx * ny * nz
-    dx, cxs = _axis_centres(nx)
-    dy, cys = _axis_centres(ny)
-    if dim == 3:
-        dz, czs = _axis_centres(nz)
-    centre = []
-    for k in range(nz):
-        for j in range(ny):
-            for i in range(nx):
-                centre.append((cxs[i], cys[j], czs[k]) if dim == 3 else (cxs[i], cys[j]))
+    dx = _axis_centres(nx)[0]
+    dy = _axis_centres(ny)[0]
+    dz = _axis_centres(nz)[0] if dim == 3 else None
+    cen_arr, nbr_arr = sub_grid(res)
+    centre = [tuple(float(v) for v in c) for c in cen_arr]
+    # _UpdateBchain visits the neighbours in the order S, N, E, W, U, D
+    order = [1, 0, 2, 3, 4, 5][:2 * dim]
+    nbr_list = [[int(r[o]) for o in order] for r in nbr_arr]
 
     def neighbours(c):
-        i, j, k = c % nx, (c // nx) % ny, c // (nx * ny)
-        out = [c - nx if j > 0 else -2, c + nx if j < ny - 1 else -2,
-               c + 1 if i < nx - 1 else -2, c - 1 if i > 0 else -2]
-        if dim == 3:
-            out += [c + nx * ny if k < nz - 1 else -2, c - nx * ny if k > 0 else -2]
-        return out
+        return nbr_list[c]
 
     owner = [-1] * ncell
     done = [0] * ncell

@@ -1,8 +1,9 @@
 """The DVC weights oracle (oracle/dvc_oracle.py) pinned against the reference's own fixtures
-(DVCWeightsSuite_testGetCentroids{,2D}.txt: 10 particles on a 10 x 10 (x 10) sub-grid, centroids and weights
-printed to 6 decimals; particle coordinates from PICellerator/Weights/tests/DVCWeightsSuite.c:168-206 and
-:389-417), the weights summing to the element volume, a single particle owning the whole grid, and particles
-whose discrete Voronoi regions are known in closed form (half and quarter splits)."""
+(DVCWeightsSuite_testConstructGrid{,2D}.txt: the 10 x 10 (x 10) sub-grid's neighbour table and cell centres;
+DVCWeightsSuite_testGetCentroids{,2D}.txt: 10 particles on that sub-grid, centroids and weights printed to 6
+decimals; particle coordinates from PICellerator/Weights/tests/DVCWeightsSuite.c:168-206 and :389-417), the
+weights summing to the element volume, a single particle owning the whole grid, and particles whose discrete
+Voronoi regions are known in closed form (half and quarter splits)."""
 import os
 import re
 
@@ -45,6 +46,20 @@ def _read_fixture(name, dim):
     w = re.findall(r"weight = ([-+0-9.eE]+)", txt)
     coords = re.findall(r"coords: \([^)]*\) = \(" + ",".join([num] * dim) + r"\)", txt)
     return (np.array(coords, dtype=float), np.array(cen, dtype=float), np.array(w, dtype=float))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_reference_grid_fixture(dim):
+    # DVCWeightsSuite.c:103-147 / :300-336: exact neighbours, centres to the printed 6 decimals
+    name = "DVCWeightsSuite_testConstructGrid%s.txt" % ("2D" if dim == 2 else "")
+    txt = open(os.path.join(HERE, "golden", name)).read()
+    keys = ["N", "S", "E", "W", "U", "D"][:2 * dim]
+    vals = re.findall(r"Values: \(" + ", ".join(k + r": (-?\d+)" for k in keys) + r"\)", txt)
+    coords = re.findall(r"Coords: \(" + ",".join([r"\s*([-+0-9.eE]+)"] * dim) + r"\)", txt)
+    cen, nbr = D.sub_grid((10,) * dim)
+    assert len(vals) == len(coords) == 10 ** dim == len(nbr)
+    np.testing.assert_array_equal(nbr, np.array(vals, dtype=int))
+    np.testing.assert_allclose(cen, np.array(coords, dtype=float), atol=5.01e-7)
 
 
 @pytest.mark.parametrize("dim", [2, 3])
