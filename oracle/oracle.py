@@ -290,3 +290,15 @@ def read_petsc_matrix(path):
     v = np.frombuffer(raw[off:off + 8 * nz], ">f8").astype(np.float64)
     rp = np.concatenate([[0], np.cumsum(rl)])
     return sp.csr_matrix((v, ci, rp), shape=(M, N))
+
+
+def dvc_swarm(dim, res, cell_off, xi, set_centroids=True):
+    """DVCWeights over a cell-sorted swarm with the C restatement (orc_dvc_swarm): new (xi, w)."""
+    xi = np.array(xi, dtype=np.float64, copy=True, order="C")
+    off = np.ascontiguousarray(cell_off, dtype=np.int32)
+    w = np.zeros(xi.shape[0])
+    r = (C.c_int * 3)(*(list(res) + [1, 1, 1])[:3])
+    rc = lib().orc_dvc_swarm(int(dim), r, int(len(off) - 1), _p(off), _p(xi), _p(w), int(bool(set_centroids)))
+    if rc:
+        raise RuntimeError(f"under resolved cell {-rc - 1}: no particles")
+    return xi, w

@@ -1123,3 +1123,134 @@ int orc_bsscr_solve(const ORC_Mesh *m, const int *eq, int neq, const unsigned ch
     mg_free(&mg);
     return 0;
 }
+
+/* ---- DVCWeights (PICellerator/Weights/src/DVCWeights.c) ----
+ * The reference's own data structures: a boundary list per particle, one list of the cells just claimed, a
+ * `done` flag per sub-cell for de-duplication.  Sub-grid and centres: _DVCWeights_ConstructGrid (:500-556,
+ * centres accumulated from x0 - dx/2); seeds and rounds: _DVCWeights_CreateVoronoi (:668-713); claims:
+ * _ClaimCells (:261-317) with _DistanceTest (:607-610); boundaries: _UpdateBchain (:395-446, neighbours in the
+ * order S, N, E, W, U, D); centroids and weights: _GetCentroids (:203-231). */
+static void dvc_axis_centres(int n, double *d, double *c) {
+    const double dx = 2.0 / n;
+    double x = -1.0 - dx / 2.0;
+    for (int i = 0; i < n; i++) {
+        x = x + dx;
+        c[i] = x;
+    }
+    *d = dx;
+}
+
+int orc_dvc_cell(int dim, const int *res, int np, const double *pts, double *cen, double *w) {
+    if (np < 1) return -1;
+    const int nx = res[0], ny = res[1], nz = dim == 3 ? res[2] : 1, nxy = nx * ny, ncell = nxy * nz;
+    double d[3] = {0.0, 0.0, 0.0}, ax[3][64];
+    dvc_axis_centres(nx, &d[0], ax[0]);
+    dvc_axis_centres(ny, &d[1], ax[1]);
+    if (dim == 3) dvc_axis_centres(nz, &d[2], ax[2]);
+    const int nn[3] = {nx, ny, nz};
+    int *owner = (int *)malloc(sizeof(int) * ncell);
+    char *done = (char *)calloc(ncell, 1);
+    int *bound = (int *)malloc(sizeof(int) * (size_t)np * ncell); /* a boundary has at most ncell cells */
+    int *nbound = (int *)calloc(np, sizeof(int));
+    int *claimed = (int *)malloc(sizeof(int) * ncell);
+    for (int c = 0; c < ncell; c++) owner[c] = -1;
+
+#define DVC_CENTRE(c, a) ax[a][(a) == 0 ? (c) % nx : ((a) == 1 ? ((c) / nx) % ny : (c) / nxy)]
+    /* the new boundary of p from the nclaimed cells in `claimed` */
+#define DVC_UPDATE(p, nclaimed)                                                                           \
+    do {                                                                                                  \
+        int *b = bound + (size_t)(p) * ncell, cnt = 0;                                                    \
+        for (int q = 0; q < (nclaimed); q++) {                                                            \
+            const int c0 = claimed[q], i0 = c0 % nx, j0 = (c0 / nx) % ny, k0 = c0 / nxy;                  \
+            const int nb[6] = {j0 > 0 ? c0 - nx : -2, j0 < ny - 1 ? c0 + nx : -2, i0 < nx - 1 ? c0 + 1 : -2, \
+                               i0 > 0 ? c0 - 1 : -2, k0 < nz - 1 ? c0 + nxy : -2, k0 > 0 ? c0 - nxy : -2};  \
+            for (int t = 0; t < 2 * dim; t++) {                                                           \
+                const int c1 = nb[t];                                                                     \
+                if (c1 != -2 && owner[c1] != (p) && !done[c1]) {                                          \
+                    b[cnt++] = c1;                                                                        \
+                    done[c1] = 1;                                                                         \
+                }                                                                                         \
+            }                                                                                             \
+        }                                                                                                 \
+        for (int q = 0; q < cnt; q++) done[b[q]] = 0;                                                     \
+        nbound[p] = cnt;                                                                                  \
+    } while (0)
+
+    for (int p = 0; p < np; p++) {
+        int idx[3] = {0, 0, 0};
+        for (int a = 0; a < dim; a++) {
+            const double t = (pts[p * dim + a] - (-1.0)) / d[a];
+            idx[a] = t >= 0.0 ? (t < (double)nn[a] ? (int)t : nn[a] - 1) : 0;
+        }
+        const int seed = idx[0] + idx[1] * nx + idx[2] * nxy;
+        owner[seed] = p;
+        claimed[0] = seed;
+        DVC_UPDATE(p, 1);
+    }
+    int total = 1;
+    while (total != 0) {
+        total = 0;
+        for (int p = 0; p < np; p++) {
+            const int *b = bound + (size_t)p * ncell;
+            int nc = 0;
+            for (int q = 0; q < nbound[p]; q++) {
+                const int c = b[q], o = owner[c];
+                int win = o == -1;
+                if (!win && o != p) {
+                    double s = 0.0;
+                    for (int a = 0; a < dim; a++) {
+                        const double x0 = DVC_CENTRE(c, a), x1 = pts[o * dim + a], x2 = pts[p * dim + a];
+                        s = s + (x1 + x2 - x0 - x0) * (x1 - x2);
+                    }
+                    win = s > 0.0;
+                }
+                if (win) {
+                    owner[c] = p;
+                    claimed[nc++] = c;
+                }
+            }
+            total += nc;
+            DVC_UPDATE(p, nc);
+        }
+    }
+#undef DVC_UPDATE
+    const double vol = dim == 3 ? d[0] * d[1] * d[2] : d[0] * d[1];
+    int *count = (int *)calloc(np, sizeof(int));
+    for (int q = 0; q < np * dim; q++) cen[q] = 0.0;
+    for (int c = 0; c < ncell; c++) {
+        const int p = owner[c];
+        for (int a = 0; a < dim; a++) cen[p * dim + a] += DVC_CENTRE(c, a);
+        count[p]++;
+    }
+#undef DVC_CENTRE
+    for (int p = 0; p < np; p++) {
+        w[p] = count[p] * vol;
+        for (int a = 0; a < dim; a++) cen[p * dim + a] = count[p] ? cen[p * dim + a] / count[p] : 0.0;
+    }
+    free(count);
+    free(owner);
+    free(done);
+    free(bound);
+    free(nbound);
+    free(claimed);
+    return 0;
+}
+
+int orc_dvc_swarm(int dim, const int *res, int nel, const int *cell_off, double *xi, double *w, int set_centroids) {
+    for (int e = 0; e < nel; e++)
+        if (cell_off[e + 1] <= cell_off[e]) return -(e + 1);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int e = 0; e < nel; e++) {
+        const int a = cell_off[e], n = cell_off[e + 1] - a;
+        double *pts = (double *)malloc(sizeof(double) * n * dim), *cen = (double *)malloc(sizeof(double) * n * dim);
+        for (int p = 0; p < n; p++)
+            for (int k = 0; k < dim; k++) pts[p * dim + k] = xi[(size_t)(a + p) * 3 + k];
+        orc_dvc_cell(dim, res, n, pts, cen, w + a);
+        if (set_centroids)
+            for (int p = 0; p < n; p++)
+                for (int k = 0; k < dim; k++) xi[(size_t)(a + p) * 3 + k] = cen[p * dim + k];
+        free(pts);
+        free(cen);
+    }
+    return 0;
+}

@@ -158,6 +158,14 @@ int orc_velocity_solve(const ORC_Mesh *m, const int *eq, int neq, const unsigned
 
 void orc_set_threads(int n);
 
+/* ---- DVCWeights (PICellerator/Weights/src/DVCWeights.c:203-925) ---- */
+/* One element: np particles pts (np x dim, local coordinates in [-1,1]) on the res sub-grid; writes the region
+ * centroids cen (np x dim) and volumes w (np).  Returns 0, or -1 when np < 1 ("under resolved cell"). */
+int orc_dvc_cell(int dim, const int *res, int np, const double *pts, double *cen, double *w);
+/* A cell-sorted swarm, elements in parallel (OpenMP): xi (np x 3, in/out: centroids when set_centroids) and w.
+ * Returns 0, or -(e + 1) for the first element e without particles. */
+int orc_dvc_swarm(int dim, const int *res, int nel, const int *cell_off, double *xi, double *w, int set_centroids);
+
 #ifdef __cplusplus
 }
 #endif

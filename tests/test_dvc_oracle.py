@@ -110,3 +110,20 @@ def test_later_particle_takes_shared_seed_and_ties_stay():
 def test_empty_cell_is_an_error():
     with pytest.raises(D.DVCError):
         D.dvc_swarm(2, np.array([0, 3, 3]), np.zeros((3, 3)), (4, 4))
+
+
+@pytest.mark.parametrize("dim,res", [(2, (10, 10)), (3, (10, 10, 10)), (3, (7, 5, 6))])
+def test_c_restatement_matches_python(dim, res):
+    # the C restatement (the CPU baseline of the DVC kernel) against the list algorithm above, bit for bit
+    from oracle import oracle as O
+    rng = np.random.default_rng(dim * 100 + res[0])
+    counts = rng.integers(1, 40, 24)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    xi = np.zeros((off[-1], 3))
+    xi[:, :dim] = rng.uniform(-1.0, 1.0, (off[-1], dim))
+    xi_c, w_c = O.dvc_swarm(dim, res, off, xi)
+    xi_p, w_p = D.dvc_swarm(dim, off, xi, res)
+    assert np.array_equal(w_c, w_p)
+    assert np.array_equal(xi_c, xi_p)
+    xi_c, w_c = O.dvc_swarm(3, (10, 10, 10), np.array([0, 10], np.int32), np.c_[PTS3])
+    np.testing.assert_allclose(w_c, _read_fixture("DVCWeightsSuite_testGetCentroids.txt", 3)[2], atol=5.01e-7)
