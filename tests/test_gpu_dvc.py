@@ -104,3 +104,19 @@ def test_dvc_empty_element_is_an_error():
     with pytest.raises(ust.UstError, match="under resolved"):
         s.dvc_weights(10)
     s.close()
+
+
+def test_dvc_matches_c_oracle_16cube():
+    # a realistic element count: 16^3 elements x 30 particles (MINSTD layout), 10^3 sub-cells, against the C
+    # restatement of the same list algorithm
+    from oracle import oracle as O
+    s = _system(3, (16, 16, 16))
+    s.random_particle_layout(particlesPerCell=30, seed=13)
+    off, xi, w, _ = s.swarm()
+    xi_ref, w_ref = O.dvc_swarm(3, (10, 10, 10), off, xi)
+    s.dvc_weights(10)
+    _, xi1, w1, _ = s.swarm()
+    print(f"16^3 x 30: {off[-1]} particles, {int(np.sum(w1 != w_ref))} weights differ")
+    assert np.array_equal(w1, w_ref)
+    assert np.array_equal(xi1, xi_ref)
+    s.close()
