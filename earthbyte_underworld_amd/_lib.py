@@ -57,7 +57,7 @@ class SolveStats(C.Structure):
                 ("stokes_residual", C.c_double), ("momentum_residual", C.c_double),
                 ("continuity_residual", C.c_double), ("t_assemble_ms", C.c_double), ("t_setup_ms", C.c_double),
                 ("t_solve_ms", C.c_double), ("restarts", C.c_int), ("nullspace_removed", C.c_int),
-                ("nullspace_ratio", C.c_double)]
+                ("nullspace_ratio", C.c_double), ("converged", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -84,6 +84,7 @@ _SIGS = {
     "ust_swarm_cohesion_noise": (I, [vp, C.c_float, C.c_ulonglong]),
     "ust_swarm_download": (I, [vp, vp, vp, vp, vp]),
     "ust_swarm_count": (I, [vp]),
+    "ust_swarm_download_integration_points": (I, [vp, vp, vp]),
     "ust_swarm_advect": (I, [vp, D, I, I, C.POINTER(I), C.POINTER(I)]),
     "ust_swarm_dvc_weights": (I, [vp, C.POINTER(I), I]),
     "ust_write_petsc_matrix": (I, [vp, I, I, C.c_char_p]),

@@ -109,9 +109,14 @@ static void free_mesh_data(ust_context *ctx) {
     ctx->assembled = 0;
 }
 
+static void free_integration_points(ust_context *ctx) {
+    DFREE(ctx->ixi0); DFREE(ctx->ixi1); DFREE(ctx->ixi2);
+}
+
 static void free_swarm(ust_context *ctx) {
     DFREE(ctx->cell_off);
     DFREE(ctx->xi0); DFREE(ctx->xi1); DFREE(ctx->xi2); DFREE(ctx->w);
+    free_integration_points(ctx);
     DFREE(ctx->mat); DFREE(ctx->coh); DFREE(ctx->eta); DFREE(ctx->yielded);
     ctx->np = 0;
 }
@@ -403,7 +408,10 @@ int ust_assemble(ust_context *ctx, double gravity) {
             if (t == UST_RH_VONMISES || t == UST_RH_DRUCKERPRAGER) needV = 1;
         }
     const int ppc_mean = (ctx->np + ctx->nel - 1) / ctx->nel;
-    CKD(ctx, ust_k_particle_weights(&ctx->grid, ctx->cell_off, ppc_mean, ctx->xi0, ctx->xi1, ctx->xi2, ctx->w,
+    /* the integration points: the material points themselves, or the points a weights calculator moved */
+    const int sep = ctx->ixi0 != NULL;
+    const double *q0 = sep ? ctx->ixi0 : ctx->xi0, *q1 = sep ? ctx->ixi1 : ctx->xi1, *q2 = sep ? ctx->ixi2 : ctx->xi2;
+    CKD(ctx, ust_k_particle_weights(&ctx->grid, ctx->cell_off, ppc_mean, q0, q1, q2, ctx->w,
                                     ctx->mat, ctx->coh, ctx->mats_d, ctx->nmat, needT ? ctx->T : NULL,
                                     needV ? ctx->u : NULL, needV ? ctx->p : NULL, ctx->prev_solution, gravity,
                                     ctx->eta, ctx->yielded, ctx->WK, ctx->WG, ctx->WF, ctx->stream));
@@ -826,6 +834,32 @@ int ust_swarm_advect(ust_context *ctx, double dt, int order, int allow_fallback,
 
 int ust_swarm_count(ust_context *ctx) { return ctx->np; }
 
+int ust_swarm_download_integration_points(ust_context *ctx, double *xi, double *w) {
+    if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
+    size_t n = (size_t)ctx->np;
+    if (xi) {
+        double *tmp = (double *)malloc(n * sizeof(double));
+        if (!tmp) UST_FAIL(ctx, UST_ENOMEM, "host allocation failed");
+        const int sep = ctx->ixi0 != NULL;
+        const double *src[3] = {sep ? ctx->ixi0 : ctx->xi0, sep ? ctx->ixi1 : ctx->xi1, sep ? ctx->ixi2 : ctx->xi2};
+        for (int d = 0; d < 3; d++) {
+            if (d < ctx->grid.dim) {
+                int rc = ust_memcpy_d2h(tmp, src[d], n * sizeof(double), ctx->stream);
+                if (rc) {
+                    free(tmp);
+                    UST_FAIL(ctx, rc, "download failed: %s", ust_dev_error_string());
+                }
+            } else {
+                memset(tmp, 0, n * sizeof(double));
+            }
+            for (size_t i = 0; i < n; i++) xi[3 * i + d] = tmp[i];
+        }
+        free(tmp);
+    }
+    if (w) CKD(ctx, ust_memcpy_d2h(w, ctx->w, n * sizeof(double), ctx->stream));
+    return UST_OK;
+}
+
 int ust_swarm_dvc_weights(ust_context *ctx, const int *res, int set_centroids) {
     if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
     if (!res) UST_FAIL(ctx, UST_EINVAL, "resolution missing");
@@ -842,13 +876,33 @@ int ust_swarm_dvc_weights(ust_context *ctx, const int *res, int set_centroids) {
     if (lds > 65536)
         UST_FAIL(ctx, UST_EINVAL, "DVC needs %zu bytes of LDS per element (%d particles, %d sub-cells): limit 65536",
                  lds, ctx->max_ppc, ncell);
+    /* DVCWeights moves the integration points (DVCWeights.c:836-843), not the material points they were
+     * mapped from: the centroids go to the context's integration-point coordinates, seeded from the
+     * material points as CoincidentMapper maps them, and the material points stay where they are */
+    const size_t n = (size_t)ctx->np;
+    double *d0 = ctx->xi0, *d1 = ctx->xi1, *d2 = ctx->xi2;
+    if (set_centroids) {
+        if (!ctx->ixi0) {
+            DALLOC(ctx, ctx->ixi0, n * sizeof(double));
+            DALLOC(ctx, ctx->ixi1, n * sizeof(double));
+            if (dim == 3) DALLOC(ctx, ctx->ixi2, n * sizeof(double));
+        }
+        CKD(ctx, ust_memcpy_d2d(ctx->ixi0, ctx->xi0, n * sizeof(double), ctx->stream));
+        CKD(ctx, ust_memcpy_d2d(ctx->ixi1, ctx->xi1, n * sizeof(double), ctx->stream));
+        if (dim == 3) CKD(ctx, ust_memcpy_d2d(ctx->ixi2, ctx->xi2, n * sizeof(double), ctx->stream));
+        d0 = ctx->ixi0;
+        d1 = ctx->ixi1;
+        d2 = ctx->ixi2;
+    } else {
+        free_integration_points(ctx); /* weights only: the integration points coincide with the material points */
+    }
     void *flag = NULL;
     int hflag = 0;
     CKD(ctx, ust_dev_malloc(&flag, sizeof(double)));
     int rc = ust_k_fill_f64((double *)flag, 0.0, 1, ctx->stream); /* all-zero bits: the int flag reads 0 */
     if (!rc)
-        rc = ust_k_dvc_weights(dim, res, ctx->nel, ctx->cell_off, ctx->max_ppc, ctx->xi0, ctx->xi1, ctx->xi2, ctx->w,
-                               set_centroids, (int *)flag, ctx->stream);
+        rc = ust_k_dvc_weights(dim, res, ctx->nel, ctx->cell_off, ctx->max_ppc, d0, d1, d2, ctx->w, set_centroids,
+                               (int *)flag, ctx->stream);
     if (!rc) rc = ust_memcpy_d2h(&hflag, flag, sizeof(int), ctx->stream);
     ust_dev_free(flag);
     if (rc) UST_FAIL(ctx, rc, "DVC weights failed (%d): %s", rc, ust_dev_error_string());

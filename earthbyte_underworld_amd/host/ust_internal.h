@@ -54,10 +54,14 @@ struct ust_context {
     int nmat;
     ust_material mats_h[UST_MAX_MATERIALS];
     ust_material *mats_d;
-    /* swarm (device SoA) */
+    /* swarm (device SoA).  xi0..2 are the material points' local coordinates (MaterialPointsSwarm: what
+     * advection moves and material shapes are evaluated at); w is the integration weight.  The integration
+     * points coincide with the material points (CoincidentMapper) unless a weights calculator moved them:
+     * then ixi0..2 hold their own local coordinates (DVCWeights' centroids), NULL otherwise. */
     int np, max_ppc;
     int *cell_off;
     double *xi0, *xi1, *xi2, *w;
+    double *ixi0, *ixi1, *ixi2;
     unsigned char *mat;
     float *coh;
     double *eta;

@@ -509,44 +509,70 @@ static int remove_mean(ust_context *ctx, double *v) {
     return UST_OK;
 }
 
-/* preconditioned CG on the Schur complement from the current p; returns its and the final ratio */
+/* preconditioned CG on the Schur complement from the current p.  *conv: the outer tolerance was met; *stalled:
+ * the stall test (less than 10 % residual reduction over the last stall_window iterations) ended it. */
 static int schur_cg(ust_context *ctx, const ust_solve_opts *o, int ns, const double *pc, int stall_window,
-                    double hn, ust_solve_stats *st, int *stalled) {
+                    double hn, ust_solve_stats *st, int *stalled, int *conv) {
     size_t ne = (size_t)ctx->nel;
-    double rn2, hist[4096];
+    double rn2;
     int rc;
     *stalled = 0;
-    CKD(ctx, dotv(ctx, ctx->pr, ctx->pr, ne, SC_O_RR, &rn2));
-    for (int it = 0; it < o->max_outer; it++) {
+    *conv = 0;
+    /* residual norms of the last stall_window + 1 iterations (a ring: any iteration count keeps the test) */
+    double *ring = NULL;
+    if (stall_window > 0) {
+        ring = (double *)malloc(sizeof(double) * (size_t)(stall_window + 1));
+        if (!ring) UST_FAIL(ctx, UST_ENOMEM, "host allocation failed");
+    }
+    rc = dotv(ctx, ctx->pr, ctx->pr, ne, SC_O_RR, &rn2);
+    for (int it = 0; !rc && it < o->max_outer; it++) {
         double rn = sqrt(rn2);
-        if (it < 4096) hist[it] = rn;
         if (o->verbose) fprintf(stderr, "[ust] scr it %d |r|/|h| = %.3e\n", st->outer_its, rn / hn);
-        if (rn <= o->outer_rtol * hn) break;
-        /* stalled: less than 10 % residual reduction over the last stall_window iterations (a slowly converging
-         * learned run keeps going; a scaling that does not precondition is abandoned) */
-        if (stall_window > 0 && it >= stall_window && it < 4096 && rn > 0.9 * hist[it - stall_window]) {
-            *stalled = 1;
+        if (rn <= o->outer_rtol * hn) {
+            *conv = 1;
             break;
         }
-        CKD(ctx, ust_k_div_f64(ctx->pr, pc, ctx->pz, ne, S(ctx)));
-        CKD(ctx, ust_k_dot_f64(ctx->pr, ctx->pz, ne, ctx->partial, ctx->scal + SC_O_RZ, S(ctx)));
-        CKD(ctx, ust_k_cg_update_p_f64(ctx->pz, ctx->pd, ctx->scal + SC_O_RZ, ctx->scal + SC_O_RZOLD, it == 0, ne,
-                                       S(ctx)));
-        CKD(ctx, ust_memcpy_d2d(ctx->scal + SC_O_RZOLD, ctx->scal + SC_O_RZ, sizeof(double), S(ctx)));
+        /* stalled: less than 10 % residual reduction over the last stall_window iterations (a slowly converging
+         * learned run keeps going; a scaling that does not precondition is abandoned) */
+        if (ring) {
+            if (it >= stall_window && rn > 0.9 * ring[(it - stall_window) % (stall_window + 1)]) {
+                *stalled = 1;
+                break;
+            }
+            ring[it % (stall_window + 1)] = rn;
+        }
+        if ((rc = ust_k_div_f64(ctx->pr, pc, ctx->pz, ne, S(ctx)))) break;
+        if ((rc = ust_k_dot_f64(ctx->pr, ctx->pz, ne, ctx->partial, ctx->scal + SC_O_RZ, S(ctx)))) break;
+        if ((rc = ust_k_cg_update_p_f64(ctx->pz, ctx->pd, ctx->scal + SC_O_RZ, ctx->scal + SC_O_RZOLD, it == 0, ne,
+                                        S(ctx))))
+            break;
+        if ((rc = ust_memcpy_d2d(ctx->scal + SC_O_RZOLD, ctx->scal + SC_O_RZ, sizeof(double), S(ctx)))) break;
         double rtol = o->inner_rtol;
         if (o->inner_relax && rn > 0.0) {
             rtol = o->inner_rtol * hn / rn;
             if (rtol > o->inner_relax_cap) rtol = o->inner_relax_cap;
             if (rtol < o->inner_rtol) rtol = o->inner_rtol;
         }
-        if ((rc = schur_apply(ctx, o, ctx->pd, ctx->pq, rtol, st))) return rc;
-        CKD(ctx, ust_k_dot_f64(ctx->pd, ctx->pq, ne, ctx->partial, ctx->scal + SC_O_PQ, S(ctx)));
-        CKD(ctx, ust_k_cg_update_xr_f64(ctx->p, ctx->pr, ctx->pd, ctx->pq, ctx->scal + SC_O_RZOLD, ctx->scal + SC_O_PQ,
-                                        ne, S(ctx)));
-        CKD(ctx, dotv(ctx, ctx->pr, ctx->pr, ne, SC_O_RR, &rn2));
+        if ((rc = schur_apply(ctx, o, ctx->pd, ctx->pq, rtol, st))) {
+            free(ring);
+            return rc;
+        }
+        if ((rc = ust_k_dot_f64(ctx->pd, ctx->pq, ne, ctx->partial, ctx->scal + SC_O_PQ, S(ctx)))) break;
+        if ((rc = ust_k_cg_update_xr_f64(ctx->p, ctx->pr, ctx->pd, ctx->pq, ctx->scal + SC_O_RZOLD,
+                                         ctx->scal + SC_O_PQ, ne, S(ctx))))
+            break;
+        if ((rc = dotv(ctx, ctx->pr, ctx->pr, ne, SC_O_RR, &rn2))) {
+            free(ring);
+            return rc;
+        }
         st->outer_its++;
-        if (!(rn2 == rn2)) UST_FAIL(ctx, UST_ENOTCONV, "Schur CG produced NaN");
+        if (!(rn2 == rn2)) {
+            free(ring);
+            UST_FAIL(ctx, UST_ENOTCONV, "Schur CG produced NaN");
+        }
     }
+    free(ring);
+    if (rc) UST_FAIL(ctx, rc, "Schur CG step failed (%d): %s", rc, ust_dev_error_string());
     return UST_OK;
 }
 
@@ -624,7 +650,7 @@ int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st)
     double hn = sqrt(hh) > 0 ? sqrt(hh) : 1.0;
     CKD(ctx, ust_k_fill_f64(ctx->p, 0.0, ne, S(ctx)));
     CKD(ctx, ust_memcpy_d2d(ctx->pr, ctx->ph, ne * sizeof(double), S(ctx)));
-    int stalled = 0;
+    int stalled = 0, conv = 0;
     const double *pc = ctx->approxS;
     const double *pc_last = ctx->approxS; /* the preconditioner the Schur CG finished with */
     if (o.schur_pc == UST_SCHUR_LEARNED || o.schur_pc == UST_SCHUR_SCALED) {
@@ -636,17 +662,17 @@ int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st)
         }
         pc = ctx->ppc;
         pc_last = pc;
-        if ((rc = schur_cg(ctx, &o, ns, pc, o.stall_window, hn, st, &stalled))) return rc;
+        if ((rc = schur_cg(ctx, &o, ns, pc, o.stall_window, hn, st, &stalled, &conv))) return rc;
         if (stalled) {
             pc_last = ctx->approxS;
             /* plain BSSCR takes over from the current iterate: r = h_hat - S p */
             st->fallback_used = 1;
             if ((rc = schur_apply(ctx, &o, ctx->p, ctx->pq, o.inner_rtol, st))) return rc;
             CKD(ctx, ust_k_sub_f64(ctx->ph, ctx->pq, ctx->pr, ne, S(ctx)));
-            if ((rc = schur_cg(ctx, &o, ns, ctx->approxS, 0, hn, st, &stalled))) return rc;
+            if ((rc = schur_cg(ctx, &o, ns, ctx->approxS, 0, hn, st, &stalled, &conv))) return rc;
         }
     } else {
-        if ((rc = schur_cg(ctx, &o, ns, pc, 0, hn, st, &stalled))) return rc;
+        if ((rc = schur_cg(ctx, &o, ns, pc, 0, hn, st, &stalled, &conv))) return rc;
     }
     double bs_scale = 0.3;
     if ((rc = back_solve(ctx, &o, ns, bs_scale, st))) return rc;
@@ -658,11 +684,14 @@ int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st)
             o.outer_rtol *= 0.1;
             if ((rc = schur_apply(ctx, &o, ctx->p, ctx->pq, o.inner_rtol, st))) return rc;
             CKD(ctx, ust_k_sub_f64(ctx->ph, ctx->pq, ctx->pr, ne, S(ctx)));
-            if ((rc = schur_cg(ctx, &o, ns, pc_last, 0, hn, st, &stalled))) return rc;
+            if ((rc = schur_cg(ctx, &o, ns, pc_last, 0, hn, st, &stalled, &conv))) return rc;
         }
         bs_scale *= 0.1;
         if ((rc = back_solve(ctx, &o, ns, bs_scale, st))) return rc;
     }
+    /* converged: the Stokes-residual bar when one is set, else the outer tolerance of the last Schur CG (max_outer
+     * reached without it, as KSP's iteration limit, returns UST_OK with converged = 0) */
+    st->converged = o.stokes_rtol > 0.0 ? st->stokes_residual <= o.stokes_rtol : conv;
     double t2 = wall_ms();
     /* nodal solution with BC values (rheology input of the next assembly) */
     CKD(ctx, ust_memcpy_d2d(ctx->u, ctx->uprev, n * sizeof(double), S(ctx)));
@@ -759,6 +788,7 @@ int ust_solve_uzawa(ust_context *ctx, const ust_uzawa_opts *uzawa, const ust_sol
         it++;
     } while ((uo.use_absolute_tolerance ? abs_res : rel_res) > uo.tolerance || it < uo.min_iterations);
     st->outer_its = it;
+    st->converged = 1;
     /* step 5: residuals, then the nodal velocity with its BC values */
     if ((rc = stokes_residual(ctx, st))) return rc;
     double t2 = wall_ms();

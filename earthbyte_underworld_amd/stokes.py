@@ -198,6 +198,15 @@ class StokesSystem:
         self._check(self.lib.ust_swarm_download(self.ctx, _ptr(off), _ptr(xi), _ptr(w), _ptr(mat)))
         return off, xi, w, mat
 
+    def integration_points(self):
+        """(xi, w) of the integration points: the material points' local coordinates unless DVC weights with
+        set_centroids moved them to the Voronoi centroids."""
+        n = self.lib.ust_swarm_count(self.ctx)
+        xi = np.zeros((n, 3))
+        w = np.zeros(n)
+        self._check(self.lib.ust_swarm_download_integration_points(self.ctx, _ptr(xi), _ptr(w)))
+        return xi, w
+
     def set_temperature(self, T):
         T = np.ascontiguousarray(T, dtype=np.float64)
         assert T.shape[0] == self.nnodes

@@ -65,6 +65,8 @@ typedef struct {
     int restarts;            /* Schur CG restarts taken for stokes_rtol */
     int nullspace_removed;   /* the constant pressure was projected out (ksp_pressure_nullspace.c test) */
     double nullspace_ratio;  /* ||G 1|| / ||1|| / ||D||_inf of the assembly (removed when < 1e-7) */
+    int converged;           /* 1: the stopping test held (stokes_rtol when set, else outer_rtol); 0: max_outer
+                                ended the Schur CG first (the solve still returns UST_OK, as KSP at its limit) */
 } ust_solve_stats;
 
 void ust_solve_opts_default(ust_solve_opts *o);
@@ -103,7 +105,8 @@ int ust_swarm_assign_materials(ust_context *ctx, int nshape, const ust_shape *sh
                                const unsigned char *cell_material);
 /* per-particle cohesion factor 1 + amplitude*u, u uniform in (-1,1) (seeded) */
 int ust_swarm_cohesion_noise(ust_context *ctx, float amplitude, unsigned long long seed);
-/* copy the swarm back (any pointer may be NULL): cell_off (nel+1), xi (np*3), w (np), material (np) */
+/* copy the swarm back (any pointer may be NULL): cell_off (nel+1), material-point local coordinates xi (np*3),
+ * integration weights w (np), material (np) */
 int ust_swarm_download(ust_context *ctx, int *cell_off, double *xi, double *w, unsigned char *material);
 /* Advect the swarm through the current nodal velocity (the last solve's) by dt and re-sort it by element
  * (SwarmAdvector + TimeIntegrator + ParticleMovementHandler + ConstantWeights): order 1 (Euler) or 2
@@ -115,11 +118,17 @@ int ust_swarm_advect(ust_context *ctx, double dt, int order, int allow_fallback,
 /* DVCWeights as the swarm's weights calculator (PICellerator/Weights/src/DVCWeights.c:763-925, replacing
  * WeightsCalculator_CalculateAll's per-cell _DVCWeights_Calculate): the particles of each element grow discrete
  * Voronoi regions on a res[0] x res[1] (x res[2]) sub-grid of the local element (XML resolutionX/Y/Z, default
- * 10), each weight becomes its region's volume and, when set_centroids (the reference's behaviour), the local
- * coordinates move to the region centroid.  An element without particles is an error ("under resolved cell",
- * DVCWeights.c:787). */
+ * 10), each weight becomes its region's volume and, when set_centroids (the reference's behaviour), the
+ * integration points move to the region centroids (DVCWeights.c:836-843).  The material points stay where they
+ * are: the context keeps the integration points' own coordinates, which the assembly uses, until the swarm
+ * changes (upload, layout or advection map the integration points onto the material points again, with
+ * ConstantWeights; call this again after each advection, as the reference recalculates the weights after each
+ * remap).  An element without particles is an error ("under resolved cell", DVCWeights.c:787). */
 int ust_swarm_dvc_weights(ust_context *ctx, const int *res, int set_centroids);
 int ust_swarm_count(ust_context *ctx);
+/* the integration points (IntegrationPointsSwarm: local coordinates xi (np*3) and weights w (np), either may
+ * be NULL): the material points' coordinates unless ust_swarm_dvc_weights moved them to the centroids */
+int ust_swarm_download_integration_points(ust_context *ctx, double *xi, double *w);
 /* per-particle cohesion factors (np floats; 1 where none were set) */
 int ust_get_cohesion_scale(ust_context *ctx, float *coh);
 

@@ -266,7 +266,9 @@ int orc_find_element(const ORC_Mesh *m, const double *x) {
 
 /* RandomParticleLayout: cells in order, ppc per cell, 3 draws per particle even in 2D
  * (RandomParticleLayout.c:195-213) inside the cell box (Swarm_GetCellMinMaxCoords). */
-void orc_layout_random(const ORC_Mesh *m, int ppc, long seed, double *coords) {
+/* RandomParticleLayout with a per-cell particle count (cellParticleCountTbl, RandomParticleLayout.c:195-213):
+ * the cells in order, the particles of a cell in order, three draws per particle in the cell's box. */
+void orc_layout_random_counts(const ORC_Mesh *m, const int *cell_off, long seed, double *coords) {
     int nel = orc_nelements(m), nodes[8];
     orc_rng_seed(seed);
     for (int e = 0; e < nel; e++) {
@@ -278,9 +280,17 @@ void orc_layout_random(const ORC_Mesh *m, int ppc, long seed, double *coords) {
             mn[d] = v0[d];
             mx[d] = v1[d];
         }
-        for (int p = 0; p < ppc; p++)
-            for (int d = 0; d < 3; d++) coords[3 * ((size_t)e * ppc + p) + d] = orc_rng_random_minmax(mn[d], mx[d]);
+        for (int p = cell_off[e]; p < cell_off[e + 1]; p++)
+            for (int d = 0; d < 3; d++) coords[3 * (size_t)p + d] = orc_rng_random_minmax(mn[d], mx[d]);
     }
+}
+
+void orc_layout_random(const ORC_Mesh *m, int ppc, long seed, double *coords) {
+    int nel = orc_nelements(m);
+    int *off = (int *)malloc(sizeof(int) * (size_t)(nel + 1));
+    for (int e = 0; e <= nel; e++) off[e] = e * ppc;
+    orc_layout_random_counts(m, off, seed, coords);
+    free(off);
 }
 
 /* CoincidentMapper (CoincidentMapper.c:150-216): xi by Newton; ConstantWeights

@@ -66,6 +66,7 @@ double orc_rng_random_minmax(double mn, double mx);
 long   orc_rng_state(void);
 /* ppc particles per cell, cells in element order, 3 draws per particle. coords: np*3 */
 void orc_layout_random(const ORC_Mesh *m, int ppc, long seed, double *coords);
+void orc_layout_random_counts(const ORC_Mesh *m, const int *cell_off, long seed, double *coords);
 
 /* ---- mesh / element type ---- */
 int  orc_nnodes(const ORC_Mesh *m);

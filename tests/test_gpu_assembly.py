@@ -36,6 +36,54 @@ def test_layout_matches_oracle(oracle):
     assert np.array_equal(w, wo)
 
 
+def test_layout_matches_oracle_3d(oracle):
+    """The same in 3D (three draws per particle, x then y then z, cells in x-fastest order) on a non-cubic
+    box with uneven counts."""
+    import earthbyte_underworld_amd as ust
+    nel, box = (4, 3, 5), (2.0, 1.0, 0.7)
+    s = ust.StokesSystem(3, nel, (0.0, -1.0, 0.2), box)
+    counts = np.random.default_rng(2).integers(1, 12, int(np.prod(nel)))
+    s.random_particle_layout(seed=13, cell_counts=counts)
+    off, xi, w, _ = s.swarm()
+    m = oracle.make_mesh(3, nel, (0.0, -1.0, 0.2), box)
+    lib = oracle.lib()
+    coords = np.zeros((int(counts.sum()), 3))
+    # the oracle's layout per cell count: draw the cell boxes in order with the reference's per-cell loop
+    lib.orc_layout_random_counts(oracle.C.byref(m), oracle._p(off.astype(np.int32)), 13, oracle._p(coords))
+    xo, wo = oracle.map_particles(m, coords, off)
+    assert np.array_equal(np.diff(off), counts)
+    assert np.max(np.abs(xi - xo)) < 1e-14
+    assert np.array_equal(w, wo)
+
+
+def test_layout_constant_weights_suite():
+    """ConstantWeightsSuite.c:217-262 on the GPU layout: 1000 seeds of RandomParticleLayout (20 particles
+    per cell, 2x2 elements on [0,4]^2) + ConstantWeights, element-0 integrals of the suite's six functions
+    give the suite's error statistics.  The device's Lagrange weights integrate the quadratic-exact functions
+    to the same values as the particle sum (detJ = 1 on these 2x2 elements)."""
+    import earthbyte_underworld_amd as ust
+    from tests import weights_suite as W
+    s = ust.StokesSystem(2, W.NEL, W.BOX[0], W.BOX[1])
+    s.set_materials([ust.RheologyMaterial([ust.MaterialViscosity(eta0=1.0)], density=0.0)])
+    xs, ws = [], []
+    lag = np.array([-1.0, 0.0, 1.0])
+    qx, qy = lag[np.arange(9) % 3], lag[np.arange(9) // 3]
+    for seed in range(W.SEEDS):
+        s.random_particle_layout(particlesPerCell=W.PPC, seed=seed)
+        off, xi, w, _ = s.swarm()
+        xs.append(xi[:W.PPC])
+        ws.append(w[:W.PPC])
+        if seed < 50:
+            s.assemble(gravity=0.0)
+            WK = s.weights()[0][:, 0]
+            for name in ("ConstantFunction", "LinearFunction", "QuadraticFunction"):
+                f = W.FUNCTIONS[name][0]
+                ref = float(np.sum(w[:W.PPC] * f(xi[:W.PPC, 0], xi[:W.PPC, 1])))
+                assert abs(float(np.sum(WK * f(qx, qy))) - ref) < 1e-13 * max(1.0, abs(ref))
+    stats = W.element_integrals(np.array(xs), np.array(ws), np.ones((W.SEEDS, W.PPC)))
+    assert W.check(stats) == []
+
+
 # particles per cell across the lane-segment widths of the assembly kernel (8, 12, 16, 21, 32, 64 and a
 # count above 64 that loops)
 @pytest.mark.parametrize("dim,nel,ppc", [(2, (8, 6), 20), (3, (5, 4, 3), 30), (3, (9, 5, 3), 3), (3, (9, 5, 3), 6),

@@ -31,7 +31,7 @@ def test_dvc_reference_fixture(dim):
     xi[:, :dim] = pts
     s.upload_swarm(np.array([0, 10]), xi, np.full(10, 0.1))
     s.dvc_weights(10)
-    _, xi1, w1, _ = s.swarm()
+    xi1, w1 = s.integration_points()
     print(f"dim {dim}: weights {w1}")
     np.testing.assert_allclose(w1, w_ref, atol=5.01e-7)
     np.testing.assert_allclose(xi1[:, :dim], cen_ref, atol=5.01e-7)
@@ -44,6 +44,8 @@ def test_dvc_reference_fixture(dim):
     (3, (5, 4, 3), (10, 10, 10), None),
     (3, (4, 3, 3), (8, 6, 5), "ragged"),
     (2, (3, 2), (10, 10), "wide"),
+    (3, (3, 2, 2), (10, 10, 10), "wide3"),
+    (3, (2, 2, 2), (8, 6, 5), "wide3"),
 ])
 def test_dvc_matches_oracle(dim, nel, res, counts):
     from oracle import dvc_oracle as D
@@ -51,6 +53,8 @@ def test_dvc_matches_oracle(dim, nel, res, counts):
     ne = int(np.prod(nel))
     if counts == "wide":  # more than 127 particles in an element: the 16-bit owner grid
         s.random_particle_layout(seed=13, cell_counts=[1, 130, 5, 64, 65, 128])
+    elif counts == "wide3":  # the same in 3D (z-shift dilation on 16-bit owners, LDS near its limit)
+        s.random_particle_layout(seed=13, cell_counts=([130, 200, 7, 64, 128, 1, 150, 90, 33, 127, 129, 3])[:ne])
     elif counts == "ragged":  # 1..70 particles per element, some more than 64 (two particle passes per lane)
         rng = np.random.default_rng(5)
         cc = rng.integers(1, 71, ne)
@@ -61,7 +65,9 @@ def test_dvc_matches_oracle(dim, nel, res, counts):
     off, xi, w, mat = s.swarm()
     xi_ref, w_ref = D.dvc_swarm(dim, off, xi, res)
     s.dvc_weights(res)
-    off1, xi1, w1, _ = s.swarm()
+    off1, xm, w1, _ = s.swarm()
+    xi1, w1i = s.integration_points()
+    assert np.array_equal(xm, xi) and np.array_equal(w1i, w1)  # the material points stay put
     print(f"dim {dim} nel {nel} res {res}: {off[-1]} particles, weight range {w1.min():.4g}..{w1.max():.4g}")
     assert np.array_equal(off1, off)
     assert np.array_equal(w1, w_ref)
@@ -83,6 +89,7 @@ def test_dvc_weights_only_keeps_points_and_solve_follows():
     off, xi, w, _ = s.swarm()
     s.dvc_weights((10, 10), set_centroids=False)
     _, xi1, w1, _ = s.swarm()
+    assert np.array_equal(s.integration_points()[0], xi)
     _, w_ref = D.dvc_swarm(2, off, xi, (10, 10), set_centroids=False)
     assert np.array_equal(xi1, xi)
     assert np.array_equal(w1, w_ref)
@@ -115,8 +122,67 @@ def test_dvc_matches_c_oracle_16cube():
     off, xi, w, _ = s.swarm()
     xi_ref, w_ref = O.dvc_swarm(3, (10, 10, 10), off, xi)
     s.dvc_weights(10)
-    _, xi1, w1, _ = s.swarm()
+    xi1, w1 = s.integration_points()
     print(f"16^3 x 30: {off[-1]} particles, {int(np.sum(w1 != w_ref))} weights differ")
     assert np.array_equal(w1, w_ref)
     assert np.array_equal(xi1, xi_ref)
     s.close()
+
+
+def test_dvc_moves_integration_points_not_material_points():
+    """DVCWeights moves the integration points to the Voronoi centroids (DVCWeights.c:836-843); the material
+    points keep their coordinates.  So DVC followed by an advection step moves the material points exactly as
+    the advection alone does, and with a zero velocity the material points stay where they are.  The
+    assembly integrates with the centroids and the DVC weights (against the oracle on the same points)."""
+    import earthbyte_underworld_amd as ust
+    from oracle import dvc_oracle as D
+    from tests.helpers import OracleProblem, max_rel_err
+
+    def build():
+        s = _system(2, (10, 8))
+        s.random_particle_layout(particlesPerCell=9, seed=21)
+        s.assign_materials([ust.Everywhere(0), ust.Sphere(1, (0.5, 0.5), 0.25)])
+        s.set_bcs(*ust.free_slip_walls(s))
+        xy = s.node_coords()
+        u = np.zeros(s.ndof)
+        u[0::2] = 0.3 * np.sin(np.pi * xy[:, 1])
+        u[1::2] = -0.2 * np.sin(np.pi * xy[:, 0])
+        s.set_solution(u, None)
+        return s
+
+    a, b = build(), build()
+    off, xi0, w0, mat0 = a.swarm()
+    a.dvc_weights(10)
+    xi_c, w_c = a.integration_points()
+    xi_ref, w_ref = D.dvc_swarm(2, off, xi0, (10, 10))
+    assert np.array_equal(xi_c, xi_ref) and np.array_equal(w_c, w_ref)
+    assert np.array_equal(a.swarm()[1], xi0)
+    # the assembly uses the centroids and DVC weights (K_e against the oracle on exactly those points)
+    a.assemble(gravity=1.0)
+    o = OracleProblem(a)
+    o_ke = o.Ke  # built from the swarm download: material points with the DVC weights
+    from oracle import oracle as O
+    from tests.helpers import oracle_mesh, to_oracle_materials
+    m = oracle_mesh(a)
+    eta, _ = O.eval_viscosity(m, to_oracle_materials(a.materials), off, xi_ref, mat0.astype(np.int32))
+    Ke, _, _, _ = O.assemble_elements(m, off, xi_ref, w_ref, eta)
+    assert max_rel_err(a.element_matrices(), Ke) < 1e-12
+    assert max_rel_err(o_ke, Ke) > 1e-6  # the material points would give another K_e
+    # advection moves the material points as if DVC had not run
+    a.advect(0.05, order=2)
+    b.advect(0.05, order=2)
+    offa, xa, wa, ma = a.swarm()
+    offb, xb, wb, mb = b.swarm()
+    assert np.array_equal(offa, offb) and np.array_equal(xa, xb) and np.array_equal(ma, mb)
+    assert np.array_equal(wa, wb)  # ConstantWeights again after the re-sort
+    assert np.array_equal(a.integration_points()[0], xa)  # and the integration points coincide again
+    # zero velocity: DVC then advection leaves the material points where they were
+    c = build()
+    c.set_solution(np.zeros(c.ndof), None)
+    offc, xc0, _, _ = c.swarm()
+    c.dvc_weights(10)
+    c.advect(0.05, order=2)
+    offc1, xc1, _, _ = c.swarm()
+    assert np.array_equal(offc1, offc) and np.max(np.abs(xc1 - xc0)) < 1e-14
+    for s in (a, b, c):
+        s.close()

@@ -67,6 +67,39 @@ def test_minstd_known_answer(oracle):
     assert int(s[0]) == 16807 and int(s[-1]) == 1043618065
 
 
+def _constant_weights_suite_stats(O, layout_shift=0, swap_axes=False):
+    from tests import weights_suite as W
+    m = O.make_mesh(2, W.NEL, W.BOX[0], W.BOX[1])
+    off = np.arange(0, 4 * W.PPC + 1, W.PPC).astype(np.int32)
+    xs, ws, ds = [], [], []
+    for seed in range(W.SEEDS):
+        c = O.layout_random(m, W.PPC, seed)
+        if layout_shift:  # a planted mistake: the cell takes the draws of the next particle on
+            c = np.roll(c.reshape(-1), -layout_shift).reshape(c.shape)
+        if swap_axes:
+            c = c[:, [1, 0, 2]]
+        xi, w = O.map_particles(m, c, off)
+        xs.append(xi[:W.PPC])
+        ws.append(w[:W.PPC])
+        ds.append([O.shape_global_derivs(m, 0, xi[p])[0] for p in range(W.PPC)])
+    return W, W.element_integrals(np.array(xs), np.array(ws), np.array(ds))
+
+
+def test_layout_constant_weights_suite(oracle):
+    """RandomParticleLayout + CoincidentMapper + ConstantWeights reproduce the element-integral error
+    statistics of ConstantWeightsSuite.c:217-262 (1000 seeds, 20 particles in element 0 of 2x2 on [0,4]^2)."""
+    W, stats = _constant_weights_suite_stats(oracle)
+    assert W.check(stats) == []
+
+
+@pytest.mark.parametrize("shift,swap", [(1, False), (3, False), (0, True)])
+def test_layout_constant_weights_suite_catches_mistakes(oracle, shift, swap):
+    """The statistics reject a layout off by one draw, off by one particle (2 draws per particle in 2D
+    shifts every later particle), or with the axes exchanged."""
+    W, stats = _constant_weights_suite_stats(oracle, shift, swap)
+    assert len(W.check(stats)) >= 3
+
+
 def test_layout_golden(oracle):
     """First particles of the seed-13 layout on the Rayleigh-Taylor box (golden vector)."""
     gold = np.load(os.path.join(GOLDEN, "layout_seed13.npz"))
