@@ -1,10 +1,12 @@
 """Benchmark: 3D PIC Stokes solves/hour to 1e-6 residual (mixed precision + learned Schur scaling vs plain
 FP64), plus assembly GB/s, on one MI355X.
 
-Workload: 96^3 Q1/P0 elements on the unit cube, 30 particles/cell (26.5M particles, seeded MINSTD layout),
-background eta 1 / rho 0 with 40 seeded spheres (radius U[0.03,0.08], log10 eta U[-2,4], rho 1), free-slip
-walls, gravity 1.  One step = particle assembly (fused rheology) + a BSSCR solve to a relative FP64 Stokes
-residual <= 1e-6, inputs (swarm, materials, BCs) resident in HBM before the timed region.
+Default workload (the metric's configuration): 128x128x64 Q1/P0 elements on [0,2]x[0,2]x[0,1], 20 particles/cell
+(21.0M particles, seeded MINSTD layout), log10 eta from a seeded Gaussian random field (correlation length 0.1)
+mapped to [-3, 3] (contrast 1e6) and sampled at every particle (61 materials of 0.1 decade), density
+proportional to -log10 eta, free-slip walls, gravity 1.  One step = particle assembly (fused rheology) + a
+BSSCR solve to a relative FP64 Stokes residual <= 1e-6, inputs (swarm, materials, BCs) resident in HBM before
+the timed region.  --workload spheres runs the 96^3 spheres configuration instead.
 
 Prints one JSON line.
 """
@@ -60,107 +62,166 @@ def solve_opts(ust, precision, learned, relax=0, degree=2, coarse_degree=0, cheb
                             stokes_rtol=1e-6)
 
 
-def cpu_baseline(size, ppc, nsph, seed, cpu_size, threads):
-    """The FP64 oracle (same algorithm: particle assembly + BSSCR with GMG) on a cpu_size^3 sample of the
-    same configuration, scaled to the full element count."""
-    from oracle import oracle as O
-    from earthbyte_underworld_amd.configs import sphere_set
+def _grf_oracle_sample(O, n, ppc, seed, threads):
+    """The random-field configuration at n x n x n/2 elements on the oracle: the same layout rule, the field
+    sampled at the nodes and interpolated at every particle, the same 61 materials.  Returns assembly and solve
+    times and iteration counts."""
+    from earthbyte_underworld_amd.configs import gaussian_random_field
     O.set_threads(threads)
-    n = cpu_size
-    m = O.make_mesh(3, (n, n, n))
-    ne = n ** 3
-    t0 = time.time()
+    nel, box = (n, n, n // 2), (2.0, 2.0, 1.0)
+    m = O.make_mesh(3, nel, (0, 0, 0), box)
+    ne = int(np.prod(nel))
     coords = O.layout_random(m, ppc, 13)
     off = np.arange(0, ne * ppc + 1, ppc).astype(np.int32)
     xi, w = O.map_particles(m, coords, off)
-    centres, radii, logs = sphere_set(nsph, seed)
-    mat = np.zeros(len(w), np.int32)
-    for k in range(nsph):
-        mat[np.sum((coords - centres[k]) ** 2, axis=1) <= radii[k] ** 2] = k + 1
-    mats = [O.make_material([(O.RH_MATERIALVISCOSITY, (1.0,))], 0.0)] + \
-           [O.make_material([(O.RH_MATERIALVISCOSITY, (10.0 ** logs[k],))], 1.0) for k in range(nsph)]
-    t_setup = time.time() - t0
-    t0 = time.time()
+    nodes = tuple(k + 1 for k in nel)
+    f = gaussian_random_field(nodes, tuple(b * (k + 1) / k for b, k in zip(box, nel)), 0.1, seed)
+    f = (f - f.min()) / (f.max() - f.min()) * 6.0 - 3.0
+    g = coords / (np.array(box) / np.array(nel))
+    i0 = np.minimum(np.floor(g).astype(int), np.array(nel) - 1)
+    t = g - i0
+    v = np.zeros(len(w))
+    for a in range(8):
+        b = [(a >> d) & 1 for d in range(3)]
+        v += np.prod([t[:, d] if b[d] else 1 - t[:, d] for d in range(3)], axis=0) * \
+            f[i0[:, 0] + b[0], i0[:, 1] + b[1], i0[:, 2] + b[2]]
+    mat = np.clip(np.rint((v + 3.0) / 6.0 * 60), 0, 60).astype(np.int32)
+    mats = [O.make_material([(O.RH_MATERIALVISCOSITY, (10.0 ** lv,))], -lv / 3.0) for lv in np.linspace(-3, 3, 61)]
+    t0 = time.perf_counter()
     eta, _ = O.eval_viscosity(m, mats, off, xi, mat)
     rho = np.array([mats[k].density for k in mat])
     Ke, Ge, fe, aS = O.assemble_elements(m, off, xi, w, eta, rho=rho, gravity=1.0)
-    nn = (n + 1) ** 3
+    nn = int(np.prod(nodes))
     idx = np.arange(nn)
-    ijk = [idx % (n + 1), (idx // (n + 1)) % (n + 1), idx // (n + 1) ** 2]
+    ijk = [idx % nodes[0], (idx // nodes[0]) % nodes[1], idx // (nodes[0] * nodes[1])]
     mask = np.zeros((nn, 3), np.uint8)
     for a in range(3):
-        mask[(ijk[a] == 0) | (ijk[a] == n), a] = 1
+        mask[(ijk[a] == 0) | (ijk[a] == nel[a]), a] = 1
     gs = O.GlobalSystem(m, mask.reshape(-1), np.zeros(nn * 3), Ke, Ge, fe)
-    t_asm = time.time() - t0
-    t0 = time.time()
-    u, p, st = gs.solve(aS, inner_rtol=1e-8, outer_rtol=1e-7, smooth_its=1)
-    t_solve = time.time() - t0
-    scale = (size / cpu_size) ** 3
-    t_full = (t_asm + t_solve) * scale
-    return {"value": 3600.0 / t_full, "unit": "solves/hour", "cores": threads, "kind": "port",
-            "sample": f"FP64 C oracle (OpenMP {threads} threads): particle assembly + BSSCR/GMG solve of the same "
-                      f"configuration at {cpu_size}^3 elements ({t_asm:.2f}s assembly + {t_solve:.2f}s solve, "
-                      f"{st.outer_its} outer / {st.inner_its_total} inner its, residual "
-                      f"{st.stokes_rel_residual:.1e}), scaled by the element-count ratio {scale:.0f} "
-                      f"(iteration growth with resolution ignored: an upper bound on the CPU rate)"}
+    t_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, _, st = gs.solve(aS, inner_rtol=1e-8, outer_rtol=1e-7, smooth_its=1)
+    t_solve = time.perf_counter() - t0
+    return {"n": n, "elements": ne, "threads": threads, "t_asm": t_asm, "t_solve": t_solve, "outer": st.outer_its,
+            "inner": st.inner_its_total, "residual": st.stokes_rel_residual}
 
 
-def dominant_kernel(lib, ctx, s, stream, reps=50):
-    """The fused FP32 Jacobi-Chebyshev smoothing pass of the inner solves' V-cycle on the finest grid
-    (k_applyK3_tiled<float, EPI_CHEBY> + k_gather_tiles): its average time over reps launches on the library
-    stream (HIP events) and its algorithmic HBM bytes."""
+def cpu_baseline(nel_full, ppc, seed, outer_full, threads, sizes=(8, 12, 16, 24), single_sizes=(8, 12, 16)):
+    """The FP64 CPU oracle (the same algorithm: particle assembly, CSR operators, BSSCR with Galerkin GMG and
+    SSOR) built with -march=native on this host, timed on samples of the configuration at n x n x n/2.  The
+    full-size estimate separates the two factors: the cost of one outer (Schur CG) iteration, fitted as
+    c N^b over the samples (it contains the inner solves, so the growth of inner iterations is in b), times the
+    outer iteration count of the full-size problem, which the plain FP64 GPU run of the same BSSCR measured;
+    plus the assembly, fitted the same way.  Done with all threads and with one.  The sample sizes halve down
+    to a small coarsest grid (as the full grid does), so that no sample is dominated by its dense coarse solve;
+    the symmetric Gauss-Seidel smoother is sequential, as PETSc's SOR is within one rank, so threads speed up
+    only the products and dots."""
+    from oracle import oracle as O
+    O.use_native()
+    out = {}
+    for label, nth, szs in (("all", threads, sizes), ("single", 1, single_sizes)):
+        runs = [_grf_oracle_sample(O, n, ppc, seed, nth) for n in szs]
+        N = np.array([r["elements"] for r in runs], float)
+        per_outer = np.array([r["t_solve"] / r["outer"] for r in runs])
+        b, lc = np.polyfit(np.log(N), np.log(per_outer), 1)
+        ba, la = np.polyfit(np.log(N), np.log([max(r["t_asm"], 1e-6) for r in runs]), 1)
+        t_full = np.exp(lc) * nel_full ** b * outer_full + np.exp(la) * nel_full ** ba
+        out[label] = {"solves_per_hour": 3600.0 / t_full, "s_per_solve": t_full, "per_outer_exponent": round(b, 3),
+                      "assembly_exponent": round(ba, 3), "samples": runs}
+        log(f"cpu baseline ({label}, {nth} threads): {t_full:.0f} s per full-size solve "
+            f"(per-outer-iteration cost ~ N^{b:.2f}, {outer_full} outer iterations)")
+    a = out["all"]
+    largest = a["samples"][-1]
+    return {"value": a["solves_per_hour"], "unit": "solves/hour", "cores": threads, "kind": "port",
+            "single_thread": {"value": out["single"]["solves_per_hour"], "cores": 1,
+                              "s_per_solve": out["single"]["s_per_solve"]},
+            "s_per_solve": a["s_per_solve"],
+            "sample": f"FP64 C oracle (gcc -O3 -march=native, OpenMP {threads} threads) on the random-field "
+                      f"configuration at n x n x n/2 elements, n = {list(sizes)} (largest: {largest['elements']} "
+                      f"elements, {largest['t_asm']:.2f}s assembly + {largest['t_solve']:.2f}s solve, "
+                      f"{largest['outer']} outer / {largest['inner']} inner its); full size = fitted cost per "
+                      f"outer iteration (~N^{a['per_outer_exponent']}) x the {outer_full} outer iterations the "
+                      f"plain FP64 GPU solve of the full problem took, + fitted assembly",
+            "fits": {k: {kk: vv for kk, vv in v.items() if kk != "samples"} for k, v in out.items()},
+            "samples": {k: v["samples"] for k, v in out.items()}}
+
+
+def fine_level_kernels(lib, ctx, s, stream, reps=50):
+    """The three operator passes every inner iteration of the headline solve makes on the finest grid: the FP32
+    multigrid residual r = b - K x (EPI_RES), the FP32 prolongation + first post-smoothing step (EPI_PCHEBY0),
+    and the FP64 product q = K p of the PCG.  Each is launched reps times through the kernel ABI on the library
+    stream and timed with HIP events there; algorithmic HBM bytes per launch are the weights (27 per element at
+    the operator's weight precision) plus the vectors each pass must read and write once.  Returns the roofline
+    object of the pass that takes the most time and the figures of all three."""
     from earthbyte_underworld_amd import _lib as L
     g = L.Grid()
     WK, WG, WF, mask = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
     lib.ust_get_device_views(ctx, C.byref(g), C.byref(WK), C.byref(WG), C.byref(WF), C.byref(mask))
     wl = lib.ust_k_weights_len(C.byref(g))
     n = s.ndof
+    nc = int(np.prod([g.nel[d] // 2 + 1 for d in range(3)])) * 3
+    work = lib.ust_k_apply_work_len(C.byref(g))
     bufs = {}
-    for name, nbytes in (("W", wl * 4), ("x", n * 4), ("b", n * 4), ("invd", n * 4), ("d", n * 4),
-                         ("work", lib.ust_k_apply_work_len(C.byref(g)) * 4)):
+    for name, nbytes in (("W32", wl * 4), ("x", n * 4), ("b", n * 4), ("r", n * 4), ("invd", n * 4), ("d", n * 4),
+                         ("xc", nc * 4), ("work", work * 8), ("u64", n * 8), ("y64", n * 8)):
         bufs[name] = C.c_void_p()
         assert lib.ust_dev_malloc(C.byref(bufs[name]), nbytes) == 0
-    lib.ust_k_copy_f64_to_f32(WK, bufs["W"], wl, stream)
+    lib.ust_k_copy_f64_to_f32(WK, bufs["W32"], wl, stream)
     rng = np.random.default_rng(0)
-    for name in ("x", "b", "d"):
-        a = rng.normal(size=n).astype(np.float32)
-        lib.ust_memcpy_h2d(bufs[name], a.ctypes.data_as(C.c_void_p), n * 4, stream)
+    for name, cnt, dt in (("x", n, np.float32), ("b", n, np.float32), ("d", n, np.float32), ("xc", nc, np.float32),
+                          ("u64", n, np.float64)):
+        a = (1e-3 * rng.normal(size=cnt)).astype(dt)
+        lib.ust_memcpy_h2d(bufs[name], a.ctypes.data_as(C.c_void_p), a.nbytes, stream)
     a = np.full(n, 1e-3, np.float32)  # a contracting step keeps x bounded over the repetitions
     lib.ust_memcpy_h2d(bufs["invd"], a.ctypes.data_as(C.c_void_p), n * 4, stream)
-    ms = time_on_stream(lib, stream, lambda: lib.ust_k_apply_K_cheby_f32(
-        C.byref(g), bufs["W"], mask, bufs["x"], bufs["b"], bufs["invd"], bufs["d"], C.c_float(0.1),
-        C.c_float(0.5), 0, bufs["work"], stream), reps)
+    B = bufs
+    runs = {
+        "k_applyK3_tiled<float,EPI_RES> + k_gather_tiles (FP32 multigrid residual)": (
+            lambda: lib.ust_k_apply_K_residual_f32(C.byref(g), B["W32"], mask, B["x"], B["b"], B["r"], B["work"],
+                                                   stream),
+            s.nel * 27 * 4 + n * (3 * 4 + 1)),
+        "k_applyK3_tiled<float,EPI_PCHEBY0> + k_gather_tiles (FP32 prolongation + Chebyshev step)": (
+            lambda: lib.ust_k_apply_K_cheby_prolong_f32(C.byref(g), B["W32"], mask, B["x"], B["b"], B["invd"], B["d"],
+                                                        C.c_float(0.5), B["xc"], B["work"], stream),
+            s.nel * 27 * 4 + n * (5 * 4 + 1) + nc * 4),
+        "k_applyK3_tiled<double,EPI_Y> + k_gather_tiles (FP64 PCG product)": (
+            lambda: lib.ust_k_apply_K_f64(C.byref(g), WK, mask, B["u64"], B["y64"], B["work"], stream),
+            s.nel * 27 * 8 + n * (2 * 8 + 1)),
+    }
+    kernels = {}
+    for name, (fn, nbytes) in runs.items():
+        ms = time_on_stream(lib, stream, fn, reps)
+        achieved = nbytes / (ms * 1e-3) / 1e9
+        kernels[name] = {"avg_launch_us": round(ms * 1e3, 2), "algorithmic_bytes_per_launch": int(nbytes),
+                         "achieved_GBps": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4)}
     for p in bufs.values():
         lib.ust_dev_free(p)
-    # weights 27 x 4 B per element; per dof: x read + write, b, invd, d read + write (4 B each), mask 1 B
-    nbytes = s.nel * 27 * 4 + n * (6 * 4 + 1)
-    achieved = nbytes / (ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_applyK3_tiled<float,EPI_CHEBY> + k_gather_tiles (fused FP32 K u + Chebyshev step)",
-            "avg_launch_us": round(ms * 1e3, 2), "algorithmic_bytes_per_launch": int(nbytes)}
+    top = max(kernels, key=lambda k: kernels[k]["avg_launch_us"])
+    k = kernels[top]
+    roofline = {"bound": "hbm", "achieved": k["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": k["frac"], "traffic": None, "kernel": top, "avg_launch_us": k["avg_launch_us"],
+                "algorithmic_bytes_per_launch": k["algorithmic_bytes_per_launch"]}
+    return roofline, kernels
 
 
-def measured_traffic():
-    """HBM bytes per launch of the dominant kernel pair from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), or None when it has not been measured."""
-    path = os.path.join(ROOT, "profiles", "k_applyK3_tiled_f32_cheby_96cube_traffic.json")
+def measured_traffic(kernel, workload_tag):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy) of a kernel from the committed
+    counter summary profiles/operator_traffic_<workload_tag>.json, or None when it has not been measured."""
+    path = os.path.join(ROOT, "profiles", f"operator_traffic_{workload_tag}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        return json.load(f).get(kernel, {}).get("hbm_bytes_per_launch")
 
 
 def kernel_only(args):
-    """Runs only the dominant kernel (50 launches) and a 256 MiB device copy (20 launches) for counter
-    collection under rocprofv3 --pmc."""
-    from earthbyte_underworld_amd.configs import spheres_3d
-    n = args.size
-    s = spheres_3d(nel=(n, n, n), ppc=args.ppc, nsph=args.nsph, seed=args.seed)
+    """Runs only the fine-level operator passes (50 launches each) and a 256 MiB device copy (20 launches) for
+    counter collection under rocprofv3 --pmc."""
+    s, _, _ = build_workload(args)
     s.assemble(gravity=1.0)
     lib, ctx = s.lib, s.ctx
     stream = lib.ust_get_stream(ctx)
-    r = dominant_kernel(lib, ctx, s, stream)
+    r, kern = fine_level_kernels(lib, ctx, s, stream)
     nb = 256 << 20
     a, b = C.c_void_p(), C.c_void_p()
     lib.ust_dev_malloc(C.byref(a), nb)
@@ -168,7 +229,32 @@ def kernel_only(args):
     for _ in range(20):
         lib.ust_memcpy_d2d(b, a, nb, stream)
     lib.ust_stream_sync(stream)
-    print(json.dumps({"kernel_only": r, "copy_bytes": nb}), flush=True)
+    print(json.dumps({"kernel_only": kern, "copy_bytes": nb}), flush=True)
+
+
+def build_workload(args):
+    """The benchmark's system and a description of it.  grf (default): the random-field configuration on
+    held-out field seed 5000 (the learned scaling is trained on seeds 1000+); spheres: 96^3 x 30."""
+    from earthbyte_underworld_amd.configs import random_field_3d, spheres_3d
+    if args.workload == "grf":
+        n = args.size or 128
+        ppc = args.ppc or 20
+        seed = 5000 if args.seed < 0 else args.seed
+        log(f"building {n}x{n}x{n // 2} x {ppc} ppc random-field workload (seed {seed})")
+        s = random_field_3d(nel=(n, n, n // 2), ppc=ppc, seed=seed)
+        desc = (f"3D Gaussian random field: {n}x{n}x{n // 2} Q1/P0 on [0,2]x[0,2]x[0,1], {ppc} particles/cell, "
+                f"log10 eta in [-3,3] (contrast 1e6) per particle, correlation length 0.1, field seed {seed}, "
+                f"free-slip, BSSCR to 1e-6 relative Stokes residual")
+    else:
+        n = args.size or 96
+        ppc = args.ppc or 30
+        seed = 2024 if args.seed < 0 else args.seed
+        log(f"building {n}^3 x {ppc} ppc spheres workload")
+        s = spheres_3d(nel=(n, n, n), ppc=ppc, nsph=args.nsph, seed=seed)
+        desc = (f"3D spheres: {n}^3 Q1/P0, {ppc} particles/cell, {args.nsph} spheres log10 eta U[-2,4], "
+                f"free-slip, BSSCR to 1e-6 relative Stokes residual")
+    args.ppc, args.seed = ppc, seed
+    return s, desc, n
 
 
 def main():
@@ -176,14 +262,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--size", type=int, default=96)
-    ap.add_argument("--ppc", type=int, default=30)
+    ap.add_argument("--workload", choices=["grf", "spheres"], default="grf",
+                    help="grf: 128x128x64 Gaussian random field (the metric's configuration); spheres: 96^3")
+    ap.add_argument("--size", type=int, default=0, help="elements per axis (grf: n x n x n/2); 0: the default")
+    ap.add_argument("--ppc", type=int, default=0, help="particles per cell; 0: 20 (grf) / 30 (spheres)")
     ap.add_argument("--nsph", type=int, default=40)
-    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--seed", type=int, default=-1, help="field / sphere-set seed; -1: 5000 (grf) / 2024 (spheres)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="plain FP64 solve only (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-size", type=int, default=16)
     ap.add_argument("--mlp", default=MLP_WEIGHTS)
     ap.add_argument("--relax", type=int, default=1, help="relaxed inner tolerance on the mixed+learned path")
     ap.add_argument("--kernel-only", action="store_true", help="dominant kernel + copy only (counter runs)")
@@ -201,11 +288,7 @@ def main():
         return
     import earthbyte_underworld_amd as ust
     from earthbyte_underworld_amd import _lib as L
-    from earthbyte_underworld_amd.configs import spheres_3d
-
-    n = args.size
-    log(f"building {n}^3 x {args.ppc} ppc spheres workload")
-    s = spheres_3d(nel=(n, n, n), ppc=args.ppc, nsph=args.nsph, seed=args.seed)
+    s, workload, n = build_workload(args)
     lib, ctx = s.lib, s.ctx
     stream = lib.ust_get_stream(ctx)
     learned = os.path.exists(args.mlp)
@@ -278,15 +361,18 @@ def main():
     t_asm = time_on_stream(lib, stream, lambda: lib.ust_assemble(ctx, C.c_double(1.0)), 3)
     np_ = lib.ust_swarm_count(ctx)
     asm_bytes = np_ * (3 * 8 + 8 + 1 + 8) + s.nel * (27 + 8 + 8) * 8
-    roofline = dominant_kernel(lib, ctx, s, stream)
-    roofline["traffic"] = measured_traffic()
+    roofline, kernels = fine_level_kernels(lib, ctx, s, stream)
+    roofline["traffic"] = measured_traffic(roofline["kernel"], f"{args.workload}{n}")
 
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         try:
-            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
-            cpu = cpu_baseline(n, args.ppc, args.nsph, args.seed, args.cpu_size, threads)
+            if args.workload != "grf":
+                raise RuntimeError("the CPU baseline samples the random-field configuration only")
+            if plain is None:
+                raise RuntimeError("the CPU baseline needs the plain FP64 run's outer iteration count")
+            cpu = cpu_baseline(s.nel, args.ppc, args.seed, plain["outer_its"], threads)
             log("cpu baseline", cpu["sample"])
         except Exception as ex:  # the CPU leg never blocks the GPU result
             log("cpu baseline failed:", ex)
@@ -339,9 +425,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 V-cycle inside f64 inner PCG; f64 outer CG, residuals, coarse solve",
-        "data": "synthetic (seeded swarm and sphere set)",
-        "config": {"workload": f"3D spheres: {n}^3 Q1/P0, {args.ppc} particles/cell, {args.nsph} spheres "
-                               f"log10 eta U[-2,4], free-slip, BSSCR to 1e-6 relative Stokes residual",
+        "data": "synthetic (seeded swarm and viscosity field)",
+        "config": {"workload": workload,
                    "elements": s.nel, "particles": int(np_), "velocity_dofs": s.ndof,
                    "learned_schur_pc": learned, "inner_tolerance_relaxed": bool(args.relax),
                    "smoother_degree": {"mixed": {"finest": 1, "coarser": 2}, "plain": 2},
@@ -354,6 +439,7 @@ def main():
         "assembly_GBps": round(asm_bytes / (t_asm * 1e-3) / 1e9, 1),
         "assembly_ms": round(t_asm, 3),
         "roofline": roofline,
+        "fine_level_kernels": kernels,
         "cpu_baseline": cpu,
         "uzawa": uzawa,
         "advect": advect,

@@ -82,6 +82,7 @@ _SIGS = {
     "ust_swarm_random": (I, [vp, vp, I, C.c_long]),
     "ust_swarm_assign_materials": (I, [vp, I, vp, vp]),
     "ust_swarm_cohesion_noise": (I, [vp, C.c_float, C.c_ulonglong]),
+    "ust_swarm_assign_materials_field": (I, [vp, vp, D, D, I, I]),
     "ust_swarm_download": (I, [vp, vp, vp, vp, vp]),
     "ust_swarm_count": (I, [vp]),
     "ust_swarm_download_integration_points": (I, [vp, vp, vp]),

@@ -9,8 +9,8 @@ viscoplastic_2d         512x256 box [0,4]x[0,1], Poisson(25) particles/cell clip
                         eta = A exp(20/(T+T0)) with T0 = 1.037 (1e4 contrast over T in [0,1]), linear T,
                         Drucker-Prager (cohesion 1e3, friction 0.6, 5e-2 cohesion noise), vx = +-1 on the sides
 random_field_3d         128x128x64 box [0,2]x[0,2]x[0,1], 20 particles/cell, log10 eta from a seeded Gaussian
-                        random field (correlation length 0.1) mapped to [-3,3] in 61 materials of 0.1 decade,
-                        rho proportional to -log10 eta
+                        random field (correlation length 0.1) mapped to [-3,3], sampled at every particle and
+                        binned to 61 materials of 0.1 decade, rho proportional to -log10 eta
 slab_3d                 256x256x128 box [0,2]x[0,2]x[0,1], 20 particles/cell, layered slab + inclusions with a
                         1e6 viscosity contrast
 """
@@ -92,23 +92,24 @@ def gaussian_random_field(shape, lengths, corr=0.1, seed=0):
     return (f - f.mean()) / f.std()
 
 
-def random_field_materials(logeta_cells, nbins=61, lo=-3.0, hi=3.0):
-    """Quantise log10 eta to nbins materials (MaterialViscosity each, density = -log10 eta / 3)."""
+def random_field_materials(nbins=61, lo=-3.0, hi=3.0):
+    """nbins materials of log10 eta spaced evenly over [lo, hi] (0.1 decade for 61 over [-3, 3]), MaterialViscosity
+    each, density = -log10 eta / 3."""
     levels = np.linspace(lo, hi, nbins)
-    idx = np.clip(np.rint((logeta_cells - lo) / (hi - lo) * (nbins - 1)), 0, nbins - 1).astype(np.uint8)
-    mats = [S.RheologyMaterial([S.MaterialViscosity(eta0=10.0 ** lv)], density=-lv / 3.0) for lv in levels]
-    return mats, idx
+    return [S.RheologyMaterial([S.MaterialViscosity(eta0=10.0 ** lv)], density=-lv / 3.0) for lv in levels]
 
 
-def random_field_3d(nel=(128, 128, 64), ppc=20, seed=0, box=(2.0, 2.0, 1.0), corr=0.1):
+def random_field_3d(nel=(128, 128, 64), ppc=20, seed=0, box=(2.0, 2.0, 1.0), corr=0.1, nbins=61):
+    """log10 eta from a seeded Gaussian random field sampled at the nodes, interpolated at every particle and
+    mapped to [-3, 3] (contrast 1e6); each particle takes the material of its 0.1-decade bin, so the viscosity
+    varies inside the elements as the field does."""
     s = S.StokesSystem(3, nel, (0.0, 0.0, 0.0), box)
     s.random_particle_layout(particlesPerCell=ppc, seed=13)
-    f = gaussian_random_field(tuple(nel), box, corr, seed)  # indexed [i, j, k]
+    nodes = tuple(n + 1 for n in nel)
+    f = gaussian_random_field(nodes, tuple(b * (n + 1) / n for b, n in zip(box, nel)), corr, seed)  # [i, j, k]
     f = (f - f.min()) / (f.max() - f.min()) * 6.0 - 3.0
-    cells = f.transpose(2, 1, 0).reshape(-1)  # element id = i + nx (j + ny k)
-    mats, idx = random_field_materials(cells)
-    s.set_materials(mats)
-    s.assign_materials([], cell_material=idx)
+    s.set_materials(random_field_materials(nbins))
+    s.assign_materials_field(f.transpose(2, 1, 0).reshape(-1), -3.0, 3.0, nbins)  # node id = i + nnx (j + nny k)
     s.set_bcs(*S.free_slip_walls(s))
     return s
 

@@ -106,6 +106,31 @@ __global__ void k_assign_materials(GridDev g, BoxDev box, const int *__restrict_
     mat[p] = (unsigned char)m;
 }
 
+// material of each particle from a nodal field sampled at the particle (Q1 interpolation in its element),
+// quantised to nbins materials first_mat.. over [lo, hi]
+template <int DIM>
+__global__ void k_assign_materials_field(GridDev g, const int *__restrict__ cell_off, int np,
+                                         const double *__restrict__ xi0, const double *__restrict__ xi1,
+                                         const double *__restrict__ xi2, const float *__restrict__ field, double lo,
+                                         double hi, int nbins, int first_mat, unsigned char *__restrict__ mat) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int e = find_cell(cell_off, g.nel, p);
+    const int ix = e % g.nx, iy = (e / g.nx) % g.ny, iz = e / (g.nx * g.ny);
+    const double xi[3] = {xi0[p], xi1[p], DIM == 3 ? xi2[p] : 0.0};
+    double v = 0.0;
+    for (int a = 0; a < (1 << DIM); a++) {
+        const int bx = a & 1, by = (a >> 1) & 1, bz = (a >> 2) & 1;
+        double N = 0.5 * (1.0 + (2 * bx - 1) * xi[0]) * 0.5 * (1.0 + (2 * by - 1) * xi[1]);
+        if (DIM == 3) N *= 0.5 * (1.0 + (2 * bz - 1) * xi[2]);
+        const size_t n = (size_t)(ix + bx) + (size_t)g.nnx * ((size_t)(iy + by) + (size_t)g.nny * (iz + bz));
+        v += N * (double)field[n];
+    }
+    double t = rint((v - lo) / (hi - lo) * (double)(nbins - 1));
+    t = t < 0.0 ? 0.0 : (t > (double)(nbins - 1) ? (double)(nbins - 1) : t);
+    mat[p] = (unsigned char)(first_mat + (int)t);
+}
+
 __device__ __forceinline__ unsigned long long mix64s(unsigned long long z) {
     z += 0x9E3779B97F4A7C15ULL;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -170,6 +195,25 @@ extern "C" int ust_k_noise_scale(float *coh, int np, float amp, unsigned long lo
     if (np == 0) return UST_OK;
     hipLaunchKernelGGL(k_noise_scale, dim3(ust_grid_size(np, 256)), dim3(256), 0, ust_stream(stream), coh, np, amp,
                        seed);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+extern "C" int ust_k_assign_materials_field(const ust_grid *gr, const int *cell_off, int np, const double *xi0,
+                                            const double *xi1, const double *xi2, const float *field_nodal, double lo,
+                                            double hi, int nbins, int first_material, unsigned char *mat,
+                                            void *stream) {
+    GridDev g = grid_dev(gr);
+    if (!cell_off || !xi0 || !xi1 || !mat || !field_nodal || (g.dim == 3 && !xi2) || np < 0 || nbins < 1 ||
+        first_material < 0 || first_material + nbins > 256 || !(hi > lo))
+        return UST_EINVAL;
+    if (np == 0) return UST_OK;
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_assign_materials_field<3>, dim3(ust_grid_size(np, 256)), dim3(256), 0, ust_stream(stream),
+                           g, cell_off, np, xi0, xi1, xi2, field_nodal, lo, hi, nbins, first_material, mat);
+    else
+        hipLaunchKernelGGL(k_assign_materials_field<2>, dim3(ust_grid_size(np, 256)), dim3(256), 0, ust_stream(stream),
+                           g, cell_off, np, xi0, xi1, xi2, field_nodal, lo, hi, nbins, first_material, mat);
     UST_LAUNCH_CHECK();
     return UST_OK;
 }

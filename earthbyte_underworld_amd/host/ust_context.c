@@ -724,6 +724,28 @@ int ust_swarm_assign_materials(ust_context *ctx, int nshape, const ust_shape *sh
     return UST_OK;
 }
 
+int ust_swarm_assign_materials_field(ust_context *ctx, const float *field_nodal, double lo, double hi, int nbins,
+                                     int first_material) {
+    if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
+    if (!field_nodal || nbins < 1 || !(hi > lo)) UST_FAIL(ctx, UST_EINVAL, "bad material field");
+    if (first_material < 0 || first_material + nbins > ctx->nmat)
+        UST_FAIL(ctx, UST_EINVAL, "materials %d..%d not defined (only %d)", first_material, first_material + nbins - 1,
+                 ctx->nmat);
+    if (!ctx->mat) DALLOC(ctx, ctx->mat, (size_t)ctx->np);
+    void *d_f = NULL;
+    const size_t nb = (size_t)ctx->nnodes * sizeof(float);
+    int rc = ust_dev_malloc(&d_f, nb);
+    if (!rc) rc = ust_memcpy_h2d(d_f, field_nodal, nb, ctx->stream);
+    if (!rc)
+        rc = ust_k_assign_materials_field(&ctx->grid, ctx->cell_off, ctx->np, ctx->xi0, ctx->xi1, ctx->xi2,
+                                          (const float *)d_f, lo, hi, nbins, first_material, ctx->mat, ctx->stream);
+    if (!rc) rc = ust_stream_sync(ctx->stream);
+    ust_dev_free(d_f);
+    if (rc) UST_FAIL(ctx, rc, "material field assignment failed: %s", ust_dev_error_string());
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
 int ust_swarm_cohesion_noise(ust_context *ctx, float amplitude, unsigned long long seed) {
     if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
     if (!ctx->coh) DALLOC(ctx, ctx->coh, (size_t)ctx->np * sizeof(float));

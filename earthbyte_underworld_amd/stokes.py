@@ -186,6 +186,14 @@ class StokesSystem:
         cm = None if cell_material is None else np.ascontiguousarray(cell_material, dtype=np.uint8)
         self._check(self.lib.ust_swarm_assign_materials(self.ctx, len(shapes), arr, _ptr(cm)))
 
+    def assign_materials_field(self, field_nodal, lo, hi, nbins, first_material=0):
+        """Material per particle from a nodal field interpolated at the particle and quantised to nbins
+        materials first_material.. over [lo, hi]."""
+        f = np.ascontiguousarray(field_nodal, dtype=np.float32)
+        assert f.shape[0] == self.nnodes
+        self._check(self.lib.ust_swarm_assign_materials_field(self.ctx, _ptr(f), float(lo), float(hi), int(nbins),
+                                                              int(first_material)))
+
     def cohesion_noise(self, amplitude, seed):
         self._check(self.lib.ust_swarm_cohesion_noise(self.ctx, float(amplitude), int(seed)))
 

@@ -103,6 +103,10 @@ int ust_swarm_random(ust_context *ctx, const int *cell_off, int ppc, long seed);
 /* material index per particle from cell_material (host, nel; may be NULL) and shapes in order */
 int ust_swarm_assign_materials(ust_context *ctx, int nshape, const ust_shape *shapes,
                                const unsigned char *cell_material);
+/* material per particle from a nodal field (host, float per node) sampled at the particle and quantised to
+ * nbins materials first_material.. over [lo, hi] (ust_k_assign_materials_field) */
+int ust_swarm_assign_materials_field(ust_context *ctx, const float *field_nodal, double lo, double hi, int nbins,
+                                     int first_material);
 /* per-particle cohesion factor 1 + amplitude*u, u uniform in (-1,1) (seeded) */
 int ust_swarm_cohesion_noise(ust_context *ctx, float amplitude, unsigned long long seed);
 /* copy the swarm back (any pointer may be NULL): cell_off (nel+1), material-point local coordinates xi (np*3),

@@ -82,6 +82,13 @@ int ust_k_random_layout(const ust_grid *g, const int *cell_off, int np, long see
 int ust_k_assign_materials(const ust_grid *g, const int *cell_off, int np, const double *xi0, const double *xi1,
                            const double *xi2, const ust_shape *shapes_dev, int nshape,
                            const unsigned char *cell_material, unsigned char *mat, void *stream);
+/* material of each particle from a nodal field (float, node-major) interpolated at the particle by the Q1
+ * shape functions of its element: first_material + round((v - lo) / (hi - lo) * (nbins - 1)), clamped to the
+ * nbins materials (a continuous property field expressed as RheologyMaterials; no reference counterpart: the
+ * reference builds such set-ups from many Shape components) */
+int ust_k_assign_materials_field(const ust_grid *g, const int *cell_off, int np, const double *xi0,
+                                 const double *xi1, const double *xi2, const float *field_nodal, double lo, double hi,
+                                 int nbins, int first_material, unsigned char *mat, void *stream);
 /* coh[p] = 1 + amp * u, u uniform in (-1, 1) from a counter hash of (seed, p) */
 int ust_k_noise_scale(float *coh, int np, float amp, unsigned long long seed, void *stream);
 

@@ -56,10 +56,21 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+_NATIVE = False
+
+
+def use_native():
+    """Use the oracle built for this host's CPU (-march=native; built here on first use), for timing."""
+    global _NATIVE, _LIB
+    subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True)
+    _NATIVE = True
+    _LIB = None
+
+
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "libust_oracle.so")
+        path = os.path.join(_HERE, "libust_oracle_native.so" if _NATIVE else "libust_oracle.so")
         if not os.path.exists(path):
             build()
         _LIB = C.CDLL(path)
@@ -68,6 +79,7 @@ def lib():
         _LIB.orc_rng_state.restype = C.c_long
         _LIB.orc_rng_seed.argtypes = [C.c_long]
         _LIB.orc_layout_random.argtypes = [C.POINTER(Mesh), C.c_int, C.c_long, C.c_void_p]
+        _LIB.orc_layout_random_counts.argtypes = [C.POINTER(Mesh), C.c_void_p, C.c_long, C.c_void_p]
         _LIB.orc_element_uzawa.restype = C.c_double
         _LIB.orc_stokes_residual.restype = C.c_double
         _LIB.orc_particle_viscosity.restype = C.c_double
@@ -111,6 +123,14 @@ def layout_random(mesh, ppc, seed):
     nel = mesh_nel(mesh)
     coords = np.zeros((nel * ppc, 3))
     lib().orc_layout_random(C.byref(mesh), ppc, seed, _p(coords))
+    return coords
+
+
+def layout_random_counts(mesh, cell_off, seed):
+    """RandomParticleLayout with per-cell counts (cell_off, nel + 1) -> global coordinates (np, 3)."""
+    cell_off = np.ascontiguousarray(cell_off, dtype=np.int32)
+    coords = np.zeros((int(cell_off[-1]), 3))
+    lib().orc_layout_random_counts(C.byref(mesh), _p(cell_off), seed, _p(coords))
     return coords
 
 

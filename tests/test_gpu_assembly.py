@@ -46,10 +46,7 @@ def test_layout_matches_oracle_3d(oracle):
     s.random_particle_layout(seed=13, cell_counts=counts)
     off, xi, w, _ = s.swarm()
     m = oracle.make_mesh(3, nel, (0.0, -1.0, 0.2), box)
-    lib = oracle.lib()
-    coords = np.zeros((int(counts.sum()), 3))
-    # the oracle's layout per cell count: draw the cell boxes in order with the reference's per-cell loop
-    lib.orc_layout_random_counts(oracle.C.byref(m), oracle._p(off.astype(np.int32)), 13, oracle._p(coords))
+    coords = oracle.layout_random_counts(m, off, 13)  # the cells in order, each with its own count
     xo, wo = oracle.map_particles(m, coords, off)
     assert np.array_equal(np.diff(off), counts)
     assert np.max(np.abs(xi - xo)) < 1e-14
@@ -225,3 +222,27 @@ def test_plugin_callbacks_accumulate(oracle):
     grows = (C.POINTER(C.c_double) * 8)(*[gbuf[i].ctypes.data_as(C.POINTER(C.c_double)) for i in range(8)])
     lib.ust_GradientStiffnessMatrixTerm_AssembleElement(C.byref(term_g), None, e, None, None, grows)
     assert np.allclose(gbuf[:, 0], Ge[e], rtol=0, atol=1e-14)
+
+
+def test_material_field_assignment():
+    """ust_swarm_assign_materials_field: a nodal field interpolated at every particle by the Q1 shape functions
+    of its element and binned to materials, against the same arithmetic in numpy on the downloaded swarm."""
+    import earthbyte_underworld_amd as ust
+    nel = (6, 5, 4)
+    s = ust.StokesSystem(3, nel, (0.0, 0.0, 0.0), (2.0, 2.0, 1.0))
+    s.random_particle_layout(particlesPerCell=9, seed=13)
+    s.set_materials([ust.RheologyMaterial([ust.MaterialViscosity(eta0=10.0 ** (0.1 * k))]) for k in range(61)])
+    f = np.random.default_rng(1).uniform(-3.5, 3.5, s.nnodes).astype(np.float32)
+    s.assign_materials_field(f, -3.0, 3.0, 61)
+    off, xi, _, mat = s.swarm()
+    e = np.repeat(np.arange(s.nel), np.diff(off))
+    ix, iy, iz = e % nel[0], (e // nel[0]) % nel[1], e // (nel[0] * nel[1])
+    v = np.zeros(len(e))
+    for a in range(8):
+        b = [(a >> d) & 1 for d in range(3)]
+        N = np.prod([0.5 * (1.0 + (2 * b[d] - 1) * xi[:, d]) for d in range(3)], axis=0)
+        n = (ix + b[0]) + (nel[0] + 1) * ((iy + b[1]) + (nel[1] + 1) * (iz + b[2]))
+        v += N * f[n].astype(np.float64)
+    ref = np.clip(np.rint((v + 3.0) / 6.0 * 60.0), 0, 60).astype(np.uint8)
+    assert np.array_equal(mat, ref)
+    assert mat.min() == 0 and mat.max() == 60  # the clamp at both ends is exercised
