@@ -233,6 +233,10 @@ def test_material_field_assignment():
     s.random_particle_layout(particlesPerCell=9, seed=13)
     s.set_materials([ust.RheologyMaterial([ust.MaterialViscosity(eta0=10.0 ** (0.1 * k))]) for k in range(61)])
     f = np.random.default_rng(1).uniform(-3.5, 3.5, s.nnodes).astype(np.float32)
+    corner = lambda i, j, k: [(i + a) + (nel[0] + 1) * ((j + b) + (nel[1] + 1) * (k + c))  # noqa: E731
+                              for a in (0, 1) for b in (0, 1) for c in (0, 1)]
+    f[corner(0, 0, 0)] = 10.0  # element 0 lies above the range, the last element below it
+    f[corner(nel[0] - 1, nel[1] - 1, nel[2] - 1)] = -10.0
     s.assign_materials_field(f, -3.0, 3.0, 61)
     off, xi, _, mat = s.swarm()
     e = np.repeat(np.arange(s.nel), np.diff(off))
