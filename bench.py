@@ -148,11 +148,12 @@ def cpu_baseline(nel_full, ppc, seed, outer_full, threads, sizes=(8, 12, 16, 24)
 
 def fine_level_kernels(lib, ctx, s, stream, reps=50):
     """The three operator passes every inner iteration of the headline solve makes on the finest grid: the FP32
-    multigrid residual r = b - K x (EPI_RES), the FP32 prolongation + first post-smoothing step (EPI_PCHEBY0),
-    and the FP64 product q = K p of the PCG.  Each is launched reps times through the kernel ABI on the library
-    stream and timed with HIP events there; algorithmic HBM bytes per launch are the weights (27 per element at
-    the operator's weight precision) plus the vectors each pass must read and write once.  Returns the roofline
-    object of the pass that takes the most time and the figures of all three."""
+    multigrid residual r = b - K x (EPI_RES) and the FP32 prolongation + first post-smoothing step
+    (EPI_PCHEBY0), both with the V-cycle's bfloat16 K weights, and the FP64 product q = K p of the PCG.  Each is
+    launched reps times through the kernel ABI on the library stream and timed with HIP events there;
+    algorithmic HBM bytes per launch are the weights (27 per element at the operator's weight precision) plus
+    the vectors each pass must read and write once.  Returns the roofline object of the pass that takes the
+    most time and the figures of all three."""
     from earthbyte_underworld_amd import _lib as L
     g = L.Grid()
     WK, WG, WF, mask = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
@@ -162,11 +163,11 @@ def fine_level_kernels(lib, ctx, s, stream, reps=50):
     nc = int(np.prod([g.nel[d] // 2 + 1 for d in range(3)])) * 3
     work = lib.ust_k_apply_work_len(C.byref(g))
     bufs = {}
-    for name, nbytes in (("W32", wl * 4), ("x", n * 4), ("b", n * 4), ("r", n * 4), ("invd", n * 4), ("d", n * 4),
+    for name, nbytes in (("W16", wl * 2), ("x", n * 4), ("b", n * 4), ("r", n * 4), ("invd", n * 4), ("d", n * 4),
                          ("xc", nc * 4), ("work", work * 8), ("u64", n * 8), ("y64", n * 8)):
         bufs[name] = C.c_void_p()
         assert lib.ust_dev_malloc(C.byref(bufs[name]), nbytes) == 0
-    lib.ust_k_copy_f64_to_f32(WK, bufs["W32"], wl, stream)
+    lib.ust_k_copy_f64_to_bf16(WK, bufs["W16"], wl, stream)
     rng = np.random.default_rng(0)
     for name, cnt, dt in (("x", n, np.float32), ("b", n, np.float32), ("d", n, np.float32), ("xc", nc, np.float32),
                           ("u64", n, np.float64)):
@@ -176,15 +177,16 @@ def fine_level_kernels(lib, ctx, s, stream, reps=50):
     lib.ust_memcpy_h2d(bufs["invd"], a.ctypes.data_as(C.c_void_p), n * 4, stream)
     B = bufs
     runs = {
-        "k_applyK3_tiled<float,EPI_RES> + k_gather_tiles (FP32 multigrid residual)": (
-            lambda: lib.ust_k_apply_K_residual_f32(C.byref(g), B["W32"], mask, B["x"], B["b"], B["r"], B["work"],
-                                                   stream),
-            s.nel * 27 * 4 + n * (3 * 4 + 1)),
-        "k_applyK3_tiled<float,EPI_PCHEBY0> + k_gather_tiles (FP32 prolongation + Chebyshev step)": (
-            lambda: lib.ust_k_apply_K_cheby_prolong_f32(C.byref(g), B["W32"], mask, B["x"], B["b"], B["invd"], B["d"],
-                                                        C.c_float(0.5), B["xc"], B["work"], stream),
-            s.nel * 27 * 4 + n * (5 * 4 + 1) + nc * 4),
-        "k_applyK3_tiled<double,EPI_Y> + k_gather_tiles (FP64 PCG product)": (
+        "k_applyK3_col<float,bf16,EPI_RES> + k_gather_tiles (FP32 multigrid residual, bfloat16 weights)": (
+            lambda: lib.ust_k_apply_K_residual_f32_bf16w(C.byref(g), B["W16"], mask, B["x"], B["b"], B["r"], B["work"],
+                                                         stream),
+            s.nel * 27 * 2 + n * (3 * 4 + 1)),
+        "k_applyK3_col<float,bf16,EPI_PCHEBY0> + k_gather_tiles (FP32 prolongation + Chebyshev step, bfloat16 "
+        "weights)": (
+            lambda: lib.ust_k_apply_K_cheby_prolong_f32_bf16w(C.byref(g), B["W16"], mask, B["x"], B["b"], B["invd"],
+                                                              B["d"], C.c_float(0.5), B["xc"], B["work"], stream),
+            s.nel * 27 * 2 + n * (5 * 4 + 1) + nc * 4),
+        "k_applyK3_col<double,double,EPI_Y> + k_gather_tiles (FP64 PCG product)": (
             lambda: lib.ust_k_apply_K_f64(C.byref(g), WK, mask, B["u64"], B["y64"], B["work"], stream),
             s.nel * 27 * 8 + n * (2 * 8 + 1)),
     }

@@ -43,7 +43,7 @@ class SolveOpts(C.Structure):
                 ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("max_inner", C.c_int),
                 ("remove_nullspace", C.c_int), ("stall_window", C.c_int), ("inner_relax", C.c_int),
                 ("inner_relax_cap", C.c_double), ("stokes_rtol", C.c_double), ("verbose", C.c_int),
-                ("coarse_degree", C.c_int), ("coarsest_degree", C.c_int)]
+                ("coarse_degree", C.c_int), ("coarsest_degree", C.c_int), ("vcycle_bf16", C.c_int)]
 
 
 class UzawaOpts(C.Structure):
@@ -147,6 +147,12 @@ _SIGS = {
     "ust_k_apply_K_cheby_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, I, vp, vp]),
     "ust_k_apply_K_cheby_prolong_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, D, vp, vp, vp]),
     "ust_k_apply_K_cheby_prolong_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, C.c_float, vp, vp, vp]),
+    "ust_k_apply_K_f32_bf16w": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp]),
+    "ust_k_apply_K_residual_f32_bf16w": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp]),
+    "ust_k_apply_K_cheby_f32_bf16w": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, I, vp,
+                                          vp]),
+    "ust_k_apply_K_cheby_prolong_f32_bf16w": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, C.c_float, vp, vp, vp]),
+    "ust_k_copy_f64_to_bf16": (I, [vp, vp, C.c_size_t, vp]),
     "ust_k_restrict_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp]),
     "ust_k_restrict_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp]),
     "ust_k_restrict_cheby_first_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp, D, vp]),

@@ -2,13 +2,17 @@
 // weights of each element (the MatMult of the A11 block, auglag-driver-DGTGD.c:191-403, that every
 // inner Krylov iteration and every multigrid smoothing step performs, PETScMGSolver.c:641-716).
 //
-// One 256-thread workgroup owns a tile of elements (16x4x4 in 3D, 16x16 in 2D), one element per
-// thread.  The tile's nodal velocities are staged in LDS, each thread applies its element with sum
-// factorisation (gradients at the Lagrange points from edge differences, stress, and the transposed
-// interpolation back to the vertices, ~0.8 kflop per 3D element), element results are reduced in
-// LDS, and the tile writes its whole node block to its own slot of a workspace.  A second, node-
-// ordered pass sums the slots covering each dof (one for interior nodes, two to eight on tile faces)
-// and writes y, so y needs no clearing and no global atomics are used.
+// 3D: one 256-thread workgroup owns a 16x16 column of elements, one element per thread, and streams a
+// chunk of element layers along z.  Node planes are staged in LDS once each; per layer every thread applies
+// its element with sum factorisation (gradients at the Lagrange points from edge differences, stress, and
+// the transposed interpolation back to the vertices, ~0.8 kflop per element) and leaves its 8 vertex
+// results in LDS; each node of the plane below the layer then sums, in a fixed order, the contributions of
+// the (up to) four elements above it and the four below it (carried in registers from the previous layer),
+// and is finished: written to y through the epilogue, unless it lies on the ring of the column (shared with
+// a neighbouring column) or on a chunk-boundary plane (shared with the next chunk).  Those nodes go to the
+// workgroup's slot of a workspace, and a second, node-ordered pass sums the two to eight slots covering each
+// of them.  No atomics touch memory and the summation order is fixed, so K u is bitwise reproducible.
+// 2D: one workgroup per 16x16 tile, node block reduced in LDS, the same workspace scheme on the tile faces.
 #include "ust_common.h"
 
 // axes orthogonal to k, in increasing order
@@ -49,14 +53,14 @@ __device__ __forceinline__ void interp22T(const T R[3][3], T O[2][2]) {
 
 // physical gradient d u_i / d x_k at the 9 points orthogonal to k (constant along k):
 // gp[q1][q2] = (s_k / 2) * interp22(u(k=1) - u(k=0))
-template <typename T, int I, int K>
-__device__ __forceinline__ void grad3(const T u[8][3], T sk, T gp[3][3]) {
+template <typename T, int I, int K, typename U>
+__device__ __forceinline__ void grad3(const U &u, T sk, T gp[3][3]) {
     T D[2][2];
 #pragma unroll
     for (int b1 = 0; b1 < 2; b1++)
 #pragma unroll
         for (int b2 = 0; b2 < 2; b2++)
-            D[b1][b2] = (T)0.5 * sk * (u[node3(K, 1, b1, b2)][I] - u[node3(K, 0, b1, b2)][I]);
+            D[b1][b2] = (T)0.5 * sk * (u(node3(K, 1, b1, b2), I) - u(node3(K, 0, b1, b2), I));
     interp22<T>(D, gp);
 }
 
@@ -75,8 +79,8 @@ __device__ __forceinline__ void scatter3(const T R[3][3], T sk, T y[8][3]) {
         }
 }
 
-template <typename T, int I>
-__device__ __forceinline__ void diag3(const T u[8][3], const T W[27], const T s[3], T y[8][3]) {
+template <typename T, int I, typename U>
+__device__ __forceinline__ void diag3(const U &u, const T W[27], const T s[3], T y[8][3]) {
     T gp[3][3], R[3][3];
     grad3<T, I, I>(u, s[I], gp);
 #pragma unroll
@@ -97,8 +101,8 @@ __device__ __forceinline__ void diag3(const T u[8][3], const T W[27], const T s[
     scatter3<T, I, I>(R, s[I], y);
 }
 
-template <typename T, int I, int K>
-__device__ __forceinline__ void pair3(const T u[8][3], const T W[27], const T s[3], T y[8][3]) {
+template <typename T, int I, int K, typename U>
+__device__ __forceinline__ void pair3(const U &u, const T W[27], const T s[3], T y[8][3]) {
     T gik[3][3], gki[3][3], Rik[3][3], Rki[3][3];
     grad3<T, I, K>(u, s[K], gik);  // d u_I / d x_K : indexed by the axes orthogonal to K
     grad3<T, K, I>(u, s[I], gki);  // d u_K / d x_I : indexed by the axes orthogonal to I
@@ -120,8 +124,10 @@ __device__ __forceinline__ void pair3(const T u[8][3], const T W[27], const T s[
     scatter3<T, K, I>(Rki, s[I], y);
 }
 
-template <typename T>
-__device__ __forceinline__ void element_apply3(const T u[8][3], const T W[27], const T s[3], T y[8][3]) {
+// u(a, i): component i of the element's vertex a (read where needed, so that the velocities need not all be
+// held in registers at once)
+template <typename T, typename U>
+__device__ __forceinline__ void element_apply3(const U &u, const T W[27], const T s[3], T y[8][3]) {
     diag3<T, 0>(u, W, s, y);
     diag3<T, 1>(u, W, s, y);
     diag3<T, 2>(u, W, s, y);
@@ -274,143 +280,174 @@ __device__ __forceinline__ void epilogue_store(const Epi<T> &ep, const EpiOps<T>
     }
 }
 
-template <typename T, int MODE>
-__global__ __launch_bounds__(256, 2) void k_applyK3_tiled(GridDev g, const T *__restrict__ W,
-                                                       const unsigned char *__restrict__ mask, const T *u,
-                                                       Epi<T> ep, T *__restrict__ work) {
-    constexpr int TX = UST_TX3, TY = UST_TY3, TZ = UST_TZ3;
-    constexpr int NX = TX + 1, NY = TY + 1, NZ = TZ + 1, NN = NX * NY * NZ;
-    __shared__ T su[NN * 3];
-    __shared__ double sy[NN * 3];  // FP64 accumulation in LDS for both precisions (ds_add_f64)
-    // EPI_PCHEBY0: the coarse nodes under the tile (tile origins are even, so fine node 2I lies on coarse I)
-    constexpr int CX = TX / 2 + 1, CY = TY / 2 + 1, CZ = TZ / 2 + 1, NC = CX * CY * CZ * 3;
-    static_assert(NC <= UST_TILE, "one coarse value per thread");
-    __shared__ T sc[MODE == EPI_PCHEBY0 ? NC : 1];
+// ------------------------------------------------------------------------------------------
+// 3D column kernel.  Node plane of a 16x16 column: 17x17 nodes, 867 dofs, NR = 4 per thread.
+// ------------------------------------------------------------------------------------------
+constexpr int CPX = UST_TX3 + 1, CPY = UST_TY3 + 1, CNP = CPX * CPY, CNPD = CNP * 3;
+constexpr int CNR = (CNPD + UST_TILE - 1) / UST_TILE;
+constexpr int EBS = UST_TILE + 1;  // row stride of the element-result buffer (rows shift across LDS banks)
+
+template <typename T, typename WT, int MODE>
+__global__ __launch_bounds__(256, 2) void k_applyK3_col(GridDev g, const WT *__restrict__ W,
+                                                     const unsigned char *__restrict__ mask, const T *u, Epi<T> ep,
+                                                     T *__restrict__ work) {
+    __shared__ T su[2][CNPD];     // staged node planes k and k+1 (free dofs; Dirichlet dofs read 0)
+    __shared__ T eb[24 * EBS];    // the layer's element results: eb[(a*3 + i)*EBS + element]
+    const int ntxy = g.tnx * g.tny;
     const int tile = blockIdx.x;
-    const int tx0 = (tile % g.tnx) * TX, ty0 = ((tile / g.tnx) % g.tny) * TY, tz0 = (tile / (g.tnx * g.tny)) * TZ;
-    T cv = (T)0;
-    if (MODE == EPI_PCHEBY0) {
-        const int t = threadIdx.x, c = t % 3, n = t / 3;
-        const int I = tx0 / 2 + n % CX, J = ty0 / 2 + (n / CX) % CY, K = tz0 / 2 + n / (CX * CY);
-        const int cnx = g.nx / 2 + 1, cny = g.ny / 2 + 1, cnz = g.nz / 2 + 1;
-        const bool ok = t < NC && I < cnx && J < cny && K < cnz;
-        cv = ep.xc[ok ? ((unsigned)I + (unsigned)cnx * ((unsigned)J + (unsigned)cny * K)) * 3u + c : 0u];
-        cv = ok ? cv : (T)0;
-    }
-    // Loads are issued without branches, staged velocities first, then the element weights (which do not
-    // depend on them), so the LDS stores below wait only for the staging loads and the weights arrive
-    // while the tile is being staged.  Out-of-tile slots read dof 0 and are discarded; partial tiles are
-    // padded in W, so every load is in bounds.
-    constexpr int NST = (NN * 3 + UST_TILE - 1) / UST_TILE;
-    T stv[NST];
-    bool stok[NST];
-    unsigned char stm[NST];
+    const int txy = tile % ntxy, chunk = tile / ntxy;
+    const int x0 = (txy % g.tnx) * UST_TX3, y0 = (txy / g.tnx) * UST_TY3;
+    const int z0 = chunk * g.tz, z1 = min(z0 + g.tz, g.nz);
+    const int tid = threadIdx.x, lx = tid % UST_TX3, ly = tid / UST_TX3;
+    const bool active = x0 + lx < g.nx && y0 + ly < g.ny;
+    const unsigned pstride = (unsigned)g.nnx * (unsigned)g.nny * 3u;  // dofs of one global node plane
+    const int cnx = g.nx / 2 + 1, cny = g.ny / 2 + 1;                 // EPI_PCHEBY0: the coarse grid
+    // the plane dofs this thread stages and finishes: t = tid + 256 r -> node n = t / 3 (i = n % 17, j = n / 17),
+    // component c = t % 3; pofs = the in-plane part of the global dof index, -1 outside the plane or the grid
+    int pofs[CNR];
+    bool ring[CNR];
 #pragma unroll
-    for (int r = 0; r < NST; r++) {
-        const int t = threadIdx.x + UST_TILE * r;
-        const int c = t % 3, n = t / 3;
-        const int gx = tx0 + n % NX, gy = ty0 + (n / NX) % NY, gz = tz0 + n / (NX * NY);
-        stok[r] = t < NN * 3 && gx < g.nnx && gy < g.nny && gz < g.nnz;
-        const unsigned dof = stok[r] ? ((unsigned)gx + (unsigned)g.nnx * ((unsigned)gy + (unsigned)g.nny * gz)) * 3u + c : 0u;
-        stm[r] = mask[dof];
-        stv[r] = u[dof];
+    for (int r = 0; r < CNR; r++) {
+        const int t = tid + UST_TILE * r, n = t / 3, c = t % 3;
+        const int gx = x0 + n % CPX, gy = y0 + n / CPX;
+        const bool ok = t < CNPD && gx < g.nnx && gy < g.nny;
+        pofs[r] = ok ? (gx + g.nnx * gy) * 3 + c : -1;
+        ring[r] = on_tile_face(gx, UST_TX3, g.tnx) || on_tile_face(gy, UST_TY3, g.tny);
     }
-    // FP32: the weights are loaded here, ahead of the barrier; FP64 loads them after it, which keeps the
-    // register count within two waves per SIMD
-    constexpr bool early = sizeof(T) == 4;
-    T wq[27];
-    if (early) {
+    // staged value of plane dof r on global plane gz: u on free dofs (plus the prolongation for EPI_PCHEBY0)
+    auto stage = [&](int gz, T *v) {
+        unsigned char m[CNR];
+        T x[CNR];
 #pragma unroll
-        for (int q = 0; q < 27; q++) wq[q] = W[((size_t)tile * 27 + q) * UST_TILE + threadIdx.x];
-    }
-    if (MODE == EPI_PCHEBY0) {
-        if (threadIdx.x < NC) sc[threadIdx.x] = cv;
-        __syncthreads();
+        for (int r = 0; r < CNR; r++) {
+            const unsigned dof = pofs[r] >= 0 ? (unsigned)pofs[r] + (unsigned)gz * pstride : 0u;
+            m[r] = mask[dof];
+            x[r] = u[dof];
+        }
 #pragma unroll
-        for (int r = 0; r < NST; r++) {
-            const int t = threadIdx.x + UST_TILE * r;
-            const int c = t % 3, n = t / 3;
-            const int ix = n % NX, iy = (n / NX) % NY, iz = n / (NX * NY);
-            if (stok[r] && !stm[r])
-                stv[r] += prolong_value<T>(ix, iy, iz, [&](int I, int J, int K) {
-                    return sc[(I + CX * (J + CY * K)) * 3 + c];
+        for (int r = 0; r < CNR; r++) {
+            T val = (pofs[r] >= 0 && !m[r]) ? x[r] : (T)0;
+            if (MODE == EPI_PCHEBY0 && pofs[r] >= 0 && !m[r]) {
+                const int t = tid + UST_TILE * r, n = t / 3, c = t % 3;
+                val += prolong_value<T>(x0 + n % CPX, y0 + n / CPX, gz, [&](int I, int J, int K) {
+                    return ep.xc[((unsigned)I + (unsigned)cnx * ((unsigned)J + (unsigned)cny * K)) * 3u + c];
                 });
+            }
+            v[r] = val;
         }
-    }
+    };
+    // finish plane dof r of global plane gz with value v: the epilogue for a node of this workgroup only, the
+    // workspace slot (local plane lp) for a node it shares
+    T *slot = work + (size_t)tile * CNP * (g.tz + 1) * 3;
+    auto finish = [&](int gz, int lp, bool zshared, const T *planeval, const EpiOps<T> *ops, const T *v) {
 #pragma unroll
-    for (int r = 0; r < NST; r++) stv[r] = (stok[r] && !stm[r]) ? stv[r] : (T)0;
-#pragma unroll
-    for (int r = 0; r < NST; r++) {
-        const int t = threadIdx.x + UST_TILE * r;
-        if (t < NN * 3) {
-            su[t] = stv[r];
-            sy[t] = 0.0;
-        }
-    }
-    __syncthreads();
-    const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
-    const bool active = tx0 + lx < g.nx && ty0 + ly < g.ny && tz0 + lz < g.nz;
-    T ye[8][3];
-    if (active) {
-        if (!early) {
-#pragma unroll
-            for (int q = 0; q < 27; q++) wq[q] = W[((size_t)tile * 27 + q) * UST_TILE + threadIdx.x];
-        }
-        T ue[8][3];
-#pragma unroll
-        for (int a = 0; a < 8; a++) {
-            const int ln = (lx + (a & 1)) + NX * ((ly + ((a >> 1) & 1)) + NY * (lz + (a >> 2)));
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                ue[a][i] = su[ln * 3 + i];
-                ye[a][i] = (T)0;
+        for (int r = 0; r < CNR; r++) {
+            if (pofs[r] < 0) continue;
+            const int t = tid + UST_TILE * r;
+            if (ring[r] || zshared) {
+                slot[(size_t)lp * CNPD + t] = v[r];
+            } else {
+                EpiOps<T> o = ops[r];
+                if (MODE == EPI_PCHEBY0 && !o.m) o.x = planeval[t];  // x + P xc, as staged
+                epilogue_store<MODE, T>(ep, o, (unsigned)pofs[r] + (unsigned)gz * pstride, v[r]);
             }
         }
-        const T s[3] = {(T)(2.0 / g.hx), (T)(2.0 / g.hy), (T)(2.0 / g.hz)};
-        element_apply3<T>(ue, wq, s, ye);
+    };
+    auto load_ops = [&](int gz, bool zshared, EpiOps<T> *ops) {
+#pragma unroll
+        for (int r = 0; r < CNR; r++) {
+            const bool own = pofs[r] >= 0 && !ring[r] && !zshared;
+            ops[r] = epilogue_load<MODE, T>(ep, mask, own ? (unsigned)pofs[r] + (unsigned)gz * pstride : 0u);
+        }
+    };
+    constexpr bool early = sizeof(T) == 4;  // FP64 defers loads to keep two waves per SIMD
+    T sv[CNR], carry[CNR];
+    stage(z0, sv);
+#pragma unroll
+    for (int r = 0; r < CNR; r++) {
+        const int t = tid + UST_TILE * r;
+        if (t < CNPD) su[0][t] = sv[r];
+        carry[r] = (T)0;  // plane z0 receives nothing from below within this chunk
     }
-    // Wave w holds the element layer lz = w, whose nodes are layers w and w+1: even waves never share a
-    // node, nor do odd waves, and within one LDS add the 64 lanes hit 64 distinct nodes.  Adding in two
-    // wave-parity phases therefore fixes the order in which every node receives its contributions.
-    for (int phase = 0; phase < 2; phase++) {
-        if (active && (lz & 1) == phase) {
+    const T s[3] = {(T)(2.0 / g.hx), (T)(2.0 / g.hy), (T)(2.0 / g.hz)};
+    const size_t wbase = ((size_t)txy * g.nz) * 27 * UST_TILE + tid;
+    for (int k = z0; k < z1; k++) {
+        const int cur = (k - z0) & 1, nxt = cur ^ 1;
+        const bool zsh = k == z0 && z0 > 0;  // the chunk's first plane is shared with the chunk below
+        stage(k + 1, sv);
+        T wq[27];
+        EpiOps<T> ops[CNR];
+        if (early) {
 #pragma unroll
-            for (int a = 0; a < 8; a++) {
-                const int ln = (lx + (a & 1)) + NX * ((ly + ((a >> 1) & 1)) + NY * (lz + (a >> 2)));
+            for (int q = 0; q < 27; q++) wq[q] = wload<T, WT>(W, wbase + ((size_t)k * 27 + q) * UST_TILE);
+            load_ops(k, zsh, ops);
+        }
+        __syncthreads();  // the previous layer is done with su[nxt] and eb
 #pragma unroll
-                for (int i = 0; i < 3; i++) atomicAdd(&sy[ln * 3 + i], (double)ye[a][i]);
-            }
+        for (int r = 0; r < CNR; r++) {
+            const int t = tid + UST_TILE * r;
+            if (t < CNPD) su[nxt][t] = sv[r];
         }
         __syncthreads();
-    }
-    // writeout: every load of the thread (mask, epilogue operands) first, then the stores.  Dof indices are
-    // 32-bit (the host rejects larger grids), which keeps the per-thread index registers down.
-    T *slot = work + (size_t)tile * NN * 3;
-    int kind[NST];  // 0 none, 1 tile-face node (workspace slot), 2 node of this tile only (written here)
-    unsigned wdof[NST];
-    EpiOps<T> ops[NST];
+        {
+            if (!early) {
 #pragma unroll
-    for (int r = 0; r < NST; r++) {
-        const int t = threadIdx.x + UST_TILE * r;
-        const int c = t % 3, n = t / 3;
-        const int gx = tx0 + n % NX, gy = ty0 + (n / NX) % NY, gz = tz0 + n / (NX * NY);
-        const bool in = t < NN * 3 && gx < g.nnx && gy < g.nny && gz < g.nnz;
-        const bool face = on_tile_face(gx, TX, g.tnx) || on_tile_face(gy, TY, g.tny) || on_tile_face(gz, TZ, g.tnz);
-        kind[r] = in ? (face ? 1 : 2) : 0;
-        wdof[r] = kind[r] == 2 ? ((unsigned)gx + (unsigned)g.nnx * ((unsigned)gy + (unsigned)g.nny * gz)) * 3u + c : 0u;
-        ops[r] = epilogue_load<MODE, T>(ep, mask, wdof[r]);
-    }
+                for (int q = 0; q < 27; q++) wq[q] = wload<T, WT>(W, wbase + ((size_t)k * 27 + q) * UST_TILE);
+            }
+            T ye[8][3];
 #pragma unroll
-    for (int r = 0; r < NST; r++) {
-        const int t = threadIdx.x + UST_TILE * r;
-        if (MODE == EPI_PCHEBY0 && kind[r] == 2 && !ops[r].m) ops[r].x = su[t];  // x + P xc, as staged
-        if (kind[r] == 1) slot[t] = (T)sy[t];
-        else if (kind[r] == 2) epilogue_store<MODE, T>(ep, ops[r], wdof[r], (T)sy[t]);
+            for (int a = 0; a < 8; a++)
+#pragma unroll
+                for (int i = 0; i < 3; i++) ye[a][i] = (T)0;
+            const T *sb = &su[cur][(lx + CPX * ly) * 3], *st = &su[nxt][(lx + CPX * ly) * 3];
+            auto ue = [&](int a, int i) -> T {
+                return (a >> 2 ? st : sb)[((a & 1) + CPX * ((a >> 1) & 1)) * 3 + i];
+            };
+            element_apply3<T>(ue, wq, s, ye);
+#pragma unroll
+            for (int a = 0; a < 8; a++)
+#pragma unroll
+                for (int i = 0; i < 3; i++) eb[(a * 3 + i) * EBS + tid] = active ? ye[a][i] : (T)0;
+        }
+        if (!early) load_ops(k, zsh, ops);
+        __syncthreads();
+        // plane k: the carried contributions of layer k-1 plus those of layer k below the plane; the
+        // contributions of layer k above plane k+1 become the new carry.  Fixed order: elements (i-1, j-1),
+        // (i, j-1), (i-1, j), (i, j).
+        T v[CNR];
+#pragma unroll
+        for (int r = 0; r < CNR; r++) {
+            const int t = tid + UST_TILE * r, n = t / 3, c = t % 3, i = n % CPX, j = n / CPX;
+            T lo = (T)0, hi = (T)0;
+            if (t < CNPD) {
+#pragma unroll
+                for (int dy = 1; dy >= 0; dy--)
+#pragma unroll
+                    for (int dx = 1; dx >= 0; dx--) {
+                        const int ex = i - dx, ey = j - dy;
+                        if (ex >= 0 && ex < UST_TX3 && ey >= 0 && ey < UST_TY3) {
+                            const int a = dx + 2 * dy, e = ex + UST_TX3 * ey;
+                            lo += eb[(a * 3 + c) * EBS + e];
+                            hi += eb[((a + 4) * 3 + c) * EBS + e];
+                        }
+                    }
+            }
+            v[r] = carry[r] + lo;
+            carry[r] = hi;
+        }
+        finish(k, k - z0, zsh, su[cur], ops, v);
+    }
+    // the chunk's last plane: shared with the chunk above unless it is the top of the grid
+    {
+        const bool zsh = z1 < g.nz;
+        EpiOps<T> ops[CNR];
+        load_ops(z1, zsh, ops);
+        finish(z1, z1 - z0, zsh, su[(z1 - z0) & 1], ops, carry);
     }
 }
 
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const T *__restrict__ W,
+template <typename T, typename WT, int MODE>
+__global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const WT *__restrict__ W,
                                                        const unsigned char *__restrict__ mask, const T *u,
                                                        Epi<T> ep, T *__restrict__ work) {
     constexpr int TX = UST_TX2, TY = UST_TY2;
@@ -435,7 +472,7 @@ __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const T *__res
     }
     T wq[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) wq[q] = W[((size_t)tile * 9 + q) * UST_TILE + threadIdx.x];
+    for (int q = 0; q < 9; q++) wq[q] = wload<T, WT>(W, ((size_t)tile * 9 + q) * UST_TILE + threadIdx.x);
 #pragma unroll
     for (int r = 0; r < NST; r++) stv[r] = (stok[r] && !stm[r]) ? stv[r] : (T)0;
 #pragma unroll
@@ -532,12 +569,13 @@ static inline unsigned face_node_count(const GridDev &g) {
 }
 
 // one thread per tile-face dof: consecutive threads take the components of a node, so the slot reads of a
-// wavefront are contiguous (32-bit index arithmetic: grids stay below 2^31 dofs)
+// wavefront are contiguous (32-bit index arithmetic: grids stay below 2^31 dofs).  Tiles are g.tx x g.ty (x g.tz)
+// elements; a workgroup's slot holds its (tx+1)(ty+1)(tz+1) node block.
 template <typename T, int DIM, int MODE>
 __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned char *__restrict__ mask,
                                                       const T *__restrict__ work, Epi<T> ep) {
-    constexpr int TX = DIM == 3 ? UST_TX3 : UST_TX2, TY = DIM == 3 ? UST_TY3 : UST_TY2, TZ = DIM == 3 ? UST_TZ3 : 1;
-    constexpr int NX = TX + 1, NY = TY + 1, NN = NX * NY * (DIM == 3 ? TZ + 1 : 1);
+    const int TX = g.tx, TY = g.ty, TZ = DIM == 3 ? g.tz : 1;
+    const int NX = TX + 1, NY = TY + 1, NN = NX * NY * (DIM == 3 ? TZ + 1 : 1);
     const int fx = g.tnx - 1, fy = g.tny - 1, fz = DIM == 3 ? g.tnz - 1 : 0;
     const unsigned nZ = (unsigned)fz * g.nnx * g.nny;
     const unsigned nY = (unsigned)fy * g.nnx * (g.nnz - fz);
@@ -588,12 +626,12 @@ __global__ __launch_bounds__(256) void k_gather_tiles(GridDev g, const unsigned 
 }
 
 static inline size_t work_len(const GridDev &g) {
-    const int nn = g.dim == 3 ? (UST_TX3 + 1) * (UST_TY3 + 1) * (UST_TZ3 + 1) : (UST_TX2 + 1) * (UST_TY2 + 1);
+    const size_t nn = g.dim == 3 ? (size_t)(g.tx + 1) * (g.ty + 1) * (g.tz + 1) : (size_t)(g.tx + 1) * (g.ty + 1);
     return (size_t)g.ntiles * nn * g.dim;
 }
 
-template <typename T, int MODE>
-static int apply_tiled(const ust_grid *gr, const T *W, const unsigned char *mask, const T *u, Epi<T> ep, T *work,
+template <typename T, typename WT, int MODE>
+static int apply_tiled(const ust_grid *gr, const WT *W, const unsigned char *mask, const T *u, Epi<T> ep, T *work,
                        void *stream) {
     GridDev g = grid_dev(gr);
     if (!W || !mask || !u || !ep.y || !work) return UST_EINVAL;
@@ -603,12 +641,14 @@ static int apply_tiled(const ust_grid *gr, const T *W, const unsigned char *mask
     if ((size_t)g.nnx * g.nny * g.nnz * g.dim >= (1ull << 32)) return UST_EINVAL;  // 32-bit dof indices
     hipStream_t st = ust_stream(stream);
     if (g.dim == 3) {
-        hipLaunchKernelGGL((k_applyK3_tiled<T, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep, work);
+        hipLaunchKernelGGL((k_applyK3_col<T, WT, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep,
+                           work);
         const unsigned nf = face_node_count<3>(g);
         if (nf) hipLaunchKernelGGL((k_gather_tiles<T, 3, MODE>), dim3(ust_grid_size((size_t)nf * 3, 256)), dim3(256),
                                    0, st, g, mask, work, ep);
     } else {
-        hipLaunchKernelGGL((k_applyK2_tiled<T, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep, work);
+        hipLaunchKernelGGL((k_applyK2_tiled<T, WT, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep,
+                           work);
         const unsigned nf = face_node_count<2>(g);
         if (nf) hipLaunchKernelGGL((k_gather_tiles<T, 2, MODE>), dim3(ust_grid_size((size_t)nf * 2, 256)), dim3(256),
                                    0, st, g, mask, work, ep);
@@ -617,33 +657,33 @@ static int apply_tiled(const ust_grid *gr, const T *W, const unsigned char *mask
     return UST_OK;
 }
 
-template <typename T>
-static int apply_plain(const ust_grid *g, const T *W, const unsigned char *mask, const T *u, T *y, T *work,
+template <typename T, typename WT>
+static int apply_plain(const ust_grid *g, const WT *W, const unsigned char *mask, const T *u, T *y, T *work,
                        void *stream) {
-    Epi<T> ep = {y, nullptr, nullptr, nullptr, (T)0, (T)0};
-    return apply_tiled<T, EPI_Y>(g, W, mask, u, ep, work, stream);
+    Epi<T> ep = {y, nullptr, nullptr, nullptr, (T)0, (T)0, nullptr};
+    return apply_tiled<T, WT, EPI_Y>(g, W, mask, u, ep, work, stream);
 }
 
-template <typename T>
-static int apply_residual(const ust_grid *g, const T *W, const unsigned char *mask, const T *u, const T *b, T *r,
+template <typename T, typename WT>
+static int apply_residual(const ust_grid *g, const WT *W, const unsigned char *mask, const T *u, const T *b, T *r,
                           T *work, void *stream) {
-    Epi<T> ep = {r, b, nullptr, nullptr, (T)0, (T)0};
-    return apply_tiled<T, EPI_RES>(g, W, mask, u, ep, work, stream);
+    Epi<T> ep = {r, b, nullptr, nullptr, (T)0, (T)0, nullptr};
+    return apply_tiled<T, WT, EPI_RES>(g, W, mask, u, ep, work, stream);
 }
 
-template <typename T>
-static int apply_cheby(const ust_grid *g, const T *W, const unsigned char *mask, T *x, const T *b, const T *invd,
+template <typename T, typename WT>
+static int apply_cheby(const ust_grid *g, const WT *W, const unsigned char *mask, T *x, const T *b, const T *invd,
                        T *d, T c1, T c2, int first, T *work, void *stream) {
-    Epi<T> ep = {x, b, invd, d, c1, c2};
-    return first ? apply_tiled<T, EPI_CHEBY0>(g, W, mask, x, ep, work, stream)
-                 : apply_tiled<T, EPI_CHEBY>(g, W, mask, x, ep, work, stream);
+    Epi<T> ep = {x, b, invd, d, c1, c2, nullptr};
+    return first ? apply_tiled<T, WT, EPI_CHEBY0>(g, W, mask, x, ep, work, stream)
+                 : apply_tiled<T, WT, EPI_CHEBY>(g, W, mask, x, ep, work, stream);
 }
 
-template <typename T>
-static int apply_cheby_prolong(const ust_grid *g, const T *W, const unsigned char *mask, T *x, const T *b,
+template <typename T, typename WT>
+static int apply_cheby_prolong(const ust_grid *g, const WT *W, const unsigned char *mask, T *x, const T *b,
                                const T *invd, T *d, T c2, const T *xc, T *work, void *stream) {
     Epi<T> ep = {x, b, invd, d, (T)0, c2, xc};
-    return apply_tiled<T, EPI_PCHEBY0>(g, W, mask, x, ep, work, stream);
+    return apply_tiled<T, WT, EPI_PCHEBY0>(g, W, mask, x, ep, work, stream);
 }
 
 extern "C" size_t ust_k_apply_work_len(const ust_grid *gr) { return work_len(grid_dev(gr)); }
@@ -651,50 +691,76 @@ extern "C" size_t ust_k_apply_work_len(const ust_grid *gr) { return work_len(gri
 extern "C" int ust_k_apply_K_cheby_prolong_f64(const ust_grid *g, const double *WK, const unsigned char *mask,
                                                double *x, const double *b, const double *invd, double *d, double c2,
                                                const double *xc, double *work, void *stream) {
-    return apply_cheby_prolong<double>(g, WK, mask, x, b, invd, d, c2, xc, work, stream);
+    return apply_cheby_prolong<double, double>(g, WK, mask, x, b, invd, d, c2, xc, work, stream);
 }
 
 extern "C" int ust_k_apply_K_cheby_prolong_f32(const ust_grid *g, const float *WK, const unsigned char *mask,
                                                float *x, const float *b, const float *invd, float *d, float c2,
                                                const float *xc, float *work, void *stream) {
-    return apply_cheby_prolong<float>(g, WK, mask, x, b, invd, d, c2, xc, work, stream);
+    return apply_cheby_prolong<float, float>(g, WK, mask, x, b, invd, d, c2, xc, work, stream);
 }
 
 extern "C" int ust_k_apply_K_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
                                  double *y, double *work, void *stream) {
-    return apply_plain<double>(g, WK, mask, u, y, work, stream);
+    return apply_plain<double, double>(g, WK, mask, u, y, work, stream);
 }
 
 extern "C" int ust_k_apply_K_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
                                  float *y, float *work, void *stream) {
-    return apply_plain<float>(g, WK, mask, u, y, work, stream);
+    return apply_plain<float, float>(g, WK, mask, u, y, work, stream);
 }
 
 extern "C" int ust_k_apply_K_residual_f64(const ust_grid *g, const double *WK, const unsigned char *mask,
                                           const double *u, const double *b, double *r, double *work, void *stream) {
-    return apply_residual<double>(g, WK, mask, u, b, r, work, stream);
+    return apply_residual<double, double>(g, WK, mask, u, b, r, work, stream);
 }
 
 extern "C" int ust_k_apply_K_residual_f32(const ust_grid *g, const float *WK, const unsigned char *mask,
                                           const float *u, const float *b, float *r, float *work, void *stream) {
-    return apply_residual<float>(g, WK, mask, u, b, r, work, stream);
+    return apply_residual<float, float>(g, WK, mask, u, b, r, work, stream);
 }
 
 extern "C" int ust_k_apply_K_cheby_f64(const ust_grid *g, const double *WK, const unsigned char *mask, double *x,
                                        const double *b, const double *invd, double *d, double c1, double c2,
                                        int first, double *work, void *stream) {
-    return apply_cheby<double>(g, WK, mask, x, b, invd, d, c1, c2, first, work, stream);
+    return apply_cheby<double, double>(g, WK, mask, x, b, invd, d, c1, c2, first, work, stream);
 }
 
 extern "C" int ust_k_apply_K_cheby_f32(const ust_grid *g, const float *WK, const unsigned char *mask, float *x,
                                        const float *b, const float *invd, float *d, float c1, float c2, int first,
                                        float *work, void *stream) {
-    return apply_cheby<float>(g, WK, mask, x, b, invd, d, c1, c2, first, work, stream);
+    return apply_cheby<float, float>(g, WK, mask, x, b, invd, d, c1, c2, first, work, stream);
+}
+
+// FP32 operator with bfloat16 K weights (the multigrid V-cycle's operator; half the weight stream)
+extern "C" int ust_k_apply_K_f32_bf16w(const ust_grid *g, const unsigned short *WK, const unsigned char *mask,
+                                       const float *u, float *y, float *work, void *stream) {
+    return apply_plain<float, unsigned short>(g, WK, mask, u, y, work, stream);
+}
+
+extern "C" int ust_k_apply_K_residual_f32_bf16w(const ust_grid *g, const unsigned short *WK,
+                                                const unsigned char *mask, const float *u, const float *b, float *r,
+                                                float *work, void *stream) {
+    return apply_residual<float, unsigned short>(g, WK, mask, u, b, r, work, stream);
+}
+
+extern "C" int ust_k_apply_K_cheby_f32_bf16w(const ust_grid *g, const unsigned short *WK, const unsigned char *mask,
+                                             float *x, const float *b, const float *invd, float *d, float c1, float c2,
+                                             int first, float *work, void *stream) {
+    return apply_cheby<float, unsigned short>(g, WK, mask, x, b, invd, d, c1, c2, first, work, stream);
+}
+
+extern "C" int ust_k_apply_K_cheby_prolong_f32_bf16w(const ust_grid *g, const unsigned short *WK,
+                                                     const unsigned char *mask, float *x, const float *b,
+                                                     const float *invd, float *d, float c2, const float *xc,
+                                                     float *work, void *stream) {
+    return apply_cheby_prolong<float, unsigned short>(g, WK, mask, x, b, invd, d, c2, xc, work, stream);
 }
 
 extern "C" size_t ust_k_weights_len(const ust_grid *gr) {
     GridDev g = grid_dev(gr);
-    return (size_t)g.ntiles * (g.dim == 3 ? 27 : 9) * UST_TILE;
+    if (g.dim == 3) return (size_t)g.tnx * g.tny * g.nz * 27 * UST_TILE;
+    return (size_t)g.ntiles * 9 * UST_TILE;
 }
 
 extern "C" size_t ust_k_weights_index(const ust_grid *gr, int e, int q) {

@@ -38,25 +38,24 @@ __host__ __device__ constexpr int cfac(int a, int q) { return a ? q : 2 - q; }
 __host__ __device__ constexpr int sgn(int a) { return 2 * a - 1; }
 
 // Element tiles of the operator kernels: one 256-thread workgroup per tile, one element per thread.
-// The K weights are stored tile-blocked, W[(tile*NQ + q)*256 + local], so that every load of a
-// quadrature weight by a wavefront is one contiguous 64-element run.
-// 3D tiles are 16 x 4 x 4 elements: one z layer (64 elements) per wavefront; the long x extent keeps
-// the tile faces normal to x, whose nodes are gathered with a stride in memory, few.
-#ifndef UST_TX3
+// 2D tiles are 16 x 16 elements.  In 3D a workgroup owns a 16 x 16 column of elements and streams a chunk
+// of tz element layers along z (tz from the grid size, so that a launch has enough workgroups); the K
+// weights of a column are stored layer after layer, W[((txy*nz + ez)*27 + q)*256 + local], so that every
+// load of a quadrature weight by a wavefront is one contiguous 64-element run, for any chunk length.
 #define UST_TX3 16
-#define UST_TY3 4
-#endif
-#define UST_TZ3 4
+#define UST_TY3 16
 #define UST_TX2 16
 #define UST_TY2 16
 #define UST_TILE 256
+#define UST_TZ3_MAX 16
 
 struct GridDev {
     int dim;
     int nx, ny, nz;       // elements per axis (nz = 1 in 2D)
     int nnx, nny, nnz;    // nodes per axis
     int nel;
-    int tnx, tny, tnz;    // tiles per axis
+    int tx, ty, tz;       // tile extent in elements (3D: tz = z-chunk length of a workgroup)
+    int tnx, tny, tnz;    // tiles per axis (3D: tnz = z-chunks per column)
     int ntiles;
     double hx, hy, hz;
     double xmin, ymin, zmin;
@@ -67,9 +66,9 @@ struct GridDev {
 __host__ __device__ __forceinline__ size_t widx(const GridDev &g, int e, int q) {
     const int ex = e % g.nx, ey = (e / g.nx) % g.ny, ez = e / (g.nx * g.ny);
     if (g.dim == 3) {
-        const int t = ex / UST_TX3 + g.tnx * (ey / UST_TY3 + g.tny * (ez / UST_TZ3));
-        const int l = ex % UST_TX3 + UST_TX3 * (ey % UST_TY3 + UST_TY3 * (ez % UST_TZ3));
-        return ((size_t)t * 27 + q) * UST_TILE + l;
+        const int txy = ex / UST_TX3 + g.tnx * (ey / UST_TY3);
+        const int l = ex % UST_TX3 + UST_TX3 * (ey % UST_TY3);
+        return (((size_t)txy * g.nz + ez) * 27 + q) * UST_TILE + l;
     }
     const int t = ex / UST_TX2 + g.tnx * (ey / UST_TY2);
     const int l = ex % UST_TX2 + UST_TX2 * (ey % UST_TY2);
@@ -87,10 +86,23 @@ static inline GridDev grid_dev(const ust_grid *g) {
     d.nnz = g->dim == 3 ? d.nz + 1 : 1;
     d.nel = d.nx * d.ny * d.nz;
     if (g->dim == 3) {
+        d.tx = UST_TX3;
+        d.ty = UST_TY3;
         d.tnx = (d.nx + UST_TX3 - 1) / UST_TX3;
         d.tny = (d.ny + UST_TY3 - 1) / UST_TY3;
-        d.tnz = (d.nz + UST_TZ3 - 1) / UST_TZ3;
+        // z-chunk: long enough that few node planes sit on chunk boundaries, short enough that the launch has
+        // about 512 workgroups (two resident per CU on 256 CUs) or more
+        long long layers = (long long)d.tnx * d.tny * d.nz;
+        long long zc = layers / 512;
+        if (zc < 2) zc = 2;
+        if (zc > UST_TZ3_MAX) zc = UST_TZ3_MAX;
+        if (zc > d.nz) zc = d.nz;
+        d.tz = (int)zc;
+        d.tnz = (d.nz + d.tz - 1) / d.tz;
     } else {
+        d.tx = UST_TX2;
+        d.ty = UST_TY2;
+        d.tz = 1;
         d.tnx = (d.nx + UST_TX2 - 1) / UST_TX2;
         d.tny = (d.ny + UST_TY2 - 1) / UST_TY2;
         d.tnz = 1;
@@ -104,4 +116,22 @@ static inline GridDev grid_dev(const ust_grid *g) {
     d.zmin = g->dim == 3 ? g->xmin[2] : 0.0;
     d.ymax = g->ymax;
     return d;
+}
+
+// bfloat16 is the upper half of an FP32: widened by a shift, narrowed with round-to-nearest-even
+__device__ __forceinline__ float bf16_to_f32(unsigned short b) { return __uint_as_float(((unsigned)b) << 16); }
+__device__ __forceinline__ unsigned short f32_to_bf16(float f) {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+// K weights stored as bfloat16 are read through wload<float, unsigned short>
+template <typename T, typename WT>
+__device__ __forceinline__ T wload(const WT *__restrict__ W, size_t i) {
+    return (T)W[i];
+}
+template <>
+__device__ __forceinline__ float wload<float, unsigned short>(const unsigned short *__restrict__ W, size_t i) {
+    return bf16_to_f32(W[i]);
 }

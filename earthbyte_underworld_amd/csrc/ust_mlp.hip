@@ -9,15 +9,6 @@
 
 #define MLP_H 64
 
-__device__ __forceinline__ float bf16_to_f32(unsigned short b) { return __uint_as_float(((unsigned)b) << 16); }
-
-// round-to-nearest-even FP32 -> BF16
-__device__ __forceinline__ unsigned short f32_to_bf16(float f) {
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
 
 // element viscosity from the K weights: sum_q WK[q] = sum_p w_p detJ eta_p (sum_q L_q = 1)
 __device__ __forceinline__ double elem_logeta(const GridDev &g, const double *WK, int e) {

@@ -35,6 +35,9 @@ __global__ void kv_sign(double *x, size_t n, unsigned long long seed) {
 __global__ void kv_d2f(const double *x, float *y, size_t n) {
     GRID_STRIDE(i, n) y[i] = (float)x[i];
 }
+__global__ void kv_d2bf(const double *x, unsigned short *y, size_t n) {
+    GRID_STRIDE(i, n) y[i] = f32_to_bf16((float)x[i]);
+}
 __global__ void kv_f2d(const float *x, double *y, size_t n) {
     GRID_STRIDE(i, n) y[i] = (double)x[i];
 }
@@ -297,6 +300,7 @@ int ust_k_fill_sign_f64(double *x, size_t n, unsigned long long seed, void *stre
 }
 int ust_k_copy_f64_to_f32(const double *x, float *y, size_t n, void *stream) { LV(kv_d2f, n, x, y, n); }
 int ust_k_copy_f32_to_f64(const float *x, double *y, size_t n, void *stream) { LV(kv_f2d, n, x, y, n); }
+int ust_k_copy_f64_to_bf16(const double *x, unsigned short *y, size_t n, void *stream) { LV(kv_d2bf, n, x, y, n); }
 int ust_k_axpy_f64(double a, const double *x, double *y, size_t n, void *stream) { LV(kv_axpy, n, a, x, y, n); }
 int ust_k_axpby_f64(double a, const double *x, double b, double *y, size_t n, void *stream) {
     LV(kv_axpby, n, a, x, b, y, n);

@@ -40,6 +40,7 @@ void ust_solve_opts_default(ust_solve_opts *o) {
     o->stall_window = 30;
     o->inner_relax_cap = 1e-2;
     o->verbose = 0;
+    o->vcycle_bf16 = 1;
 }
 
 int ust_create(ust_context **out, int device) {

@@ -18,7 +18,8 @@ typedef struct {
     ust_grid g;
     int nnodes, ndof, nel;
     double *WK64;          /* level 0 borrows ctx->WK */
-    float *WK32;
+    float *WK32;           /* FP32 V-cycle weights (vcycle_bf16 = 0) */
+    unsigned short *WK16;  /* bfloat16 V-cycle weights (vcycle_bf16 = 1) */
     unsigned char *mask;   /* device */
     unsigned char *mask_h; /* host */
     double *invd64;
@@ -89,7 +90,7 @@ struct ust_context {
     double *h; /* continuity RHS after lifting */
     /* multigrid hierarchy */
     int levels_ready, levels_prec32, nlev;
-    int levels_struct, levels_deg, levels_cdeg, levels_ccdeg, levels_req, coarse_dense;
+    int levels_struct, levels_deg, levels_cdeg, levels_ccdeg, levels_req, coarse_dense, levels_w16;
     double levels_ratio;
     float *WK32_0; /* FP32 copy of the fine weights for ust_apply_K_f32 without a hierarchy */
     int wk32_0_valid;

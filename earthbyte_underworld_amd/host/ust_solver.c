@@ -63,6 +63,7 @@ void ust_levels_free(ust_context *ctx) {
             free(L->mask_h);
         }
         DFREE(L->WK32);
+        DFREE(L->WK16);
         DFREE(L->invd64);
         DFREE(L->invd32);
         DFREE(L->x64); DFREE(L->b64); DFREE(L->ax64); DFREE(L->d64);
@@ -79,7 +80,10 @@ void ust_levels_free(ust_context *ctx) {
 /* ---------------- level operations (prec: 0 = FP64, 1 = FP32) ---------------- */
 static int lev_apply(ust_context *ctx, int l, int f32, const void *x, void *y) {
     ust_level *L = &ctx->lev[l];
-    if (f32) CKD(ctx, ust_k_apply_K_f32(&L->g, L->WK32, L->mask, (const float *)x, (float *)y, (float *)ctx->kwork, S(ctx)));
+    if (f32 && L->WK16)
+        CKD(ctx, ust_k_apply_K_f32_bf16w(&L->g, L->WK16, L->mask, (const float *)x, (float *)y, (float *)ctx->kwork,
+                                         S(ctx)));
+    else if (f32) CKD(ctx, ust_k_apply_K_f32(&L->g, L->WK32, L->mask, (const float *)x, (float *)y, (float *)ctx->kwork, S(ctx)));
     else CKD(ctx, ust_k_apply_K_f64(&L->g, L->WK64, L->mask, (const double *)x, (double *)y, (double *)ctx->kwork, S(ctx)));
     return UST_OK;
 }
@@ -87,7 +91,10 @@ static int lev_apply(ust_context *ctx, int l, int f32, const void *x, void *y) {
 /* r = b - A x on level l (fused product and residual) */
 static int lev_residual(ust_context *ctx, int l, int f32, const void *x, const void *b, void *r) {
     ust_level *L = &ctx->lev[l];
-    if (f32)
+    if (f32 && L->WK16)
+        CKD(ctx, ust_k_apply_K_residual_f32_bf16w(&L->g, L->WK16, L->mask, (const float *)x, (const float *)b,
+                                                  (float *)r, (float *)ctx->kwork, S(ctx)));
+    else if (f32)
         CKD(ctx, ust_k_apply_K_residual_f32(&L->g, L->WK32, L->mask, (const float *)x, (const float *)b, (float *)r,
                                             (float *)ctx->kwork, S(ctx)));
     else
@@ -104,7 +111,11 @@ static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void *x, 
     size_t n = (size_t)L->ndof;
     for (int k = x_zero == 2 ? 1 : 0; k < L->cheb_deg; k++) {
         if (k == 0 && xc) { /* prolongation of the coarse correction fused into the first step */
-            if (f32)
+            if (f32 && L->WK16)
+                CKD(ctx, ust_k_apply_K_cheby_prolong_f32_bf16w(&L->g, L->WK16, L->mask, (float *)x, (const float *)b,
+                                                               L->invd32, L->d32, (float)L->c2[0], (const float *)xc,
+                                                               (float *)ctx->kwork, S(ctx)));
+            else if (f32)
                 CKD(ctx, ust_k_apply_K_cheby_prolong_f32(&L->g, L->WK32, L->mask, (float *)x, (const float *)b,
                                                          L->invd32, L->d32, (float)L->c2[0], (const float *)xc,
                                                          (float *)ctx->kwork, S(ctx)));
@@ -122,6 +133,10 @@ static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void *x, 
             else
                 CKD(ctx, ust_k_cheby_first_f64((const double *)b, L->invd64, L->d64, (double *)x, L->c2[0], n,
                                                S(ctx)));
+        } else if (f32 && L->WK16) {
+            CKD(ctx, ust_k_apply_K_cheby_f32_bf16w(&L->g, L->WK16, L->mask, (float *)x, (const float *)b, L->invd32,
+                                                   L->d32, (float)L->c1[k], (float)L->c2[k], k == 0,
+                                                   (float *)ctx->kwork, S(ctx)));
         } else if (f32) {
             CKD(ctx, ust_k_apply_K_cheby_f32(&L->g, L->WK32, L->mask, (float *)x, (const float *)b, L->invd32, L->d32,
                                              (float)L->c1[k], (float)L->c2[k], k == 0, (float *)ctx->kwork, S(ctx)));
@@ -277,7 +292,8 @@ static int levels_structure(ust_context *ctx, const ust_solve_opts *o, int need3
         DALLOC(ctx, L->d64, n * sizeof(double));
         if (need32) {
             DALLOC(ctx, L->invd32, n * sizeof(float));
-            DALLOC(ctx, L->WK32, ust_k_weights_len(&L->g) * sizeof(float));
+            if (o->vcycle_bf16) DALLOC(ctx, L->WK16, ust_k_weights_len(&L->g) * sizeof(unsigned short));
+            else DALLOC(ctx, L->WK32, ust_k_weights_len(&L->g) * sizeof(float));
             DALLOC(ctx, L->x32, n * sizeof(float));
             DALLOC(ctx, L->b32, n * sizeof(float));
             DALLOC(ctx, L->ax32, n * sizeof(float));
@@ -300,12 +316,14 @@ static int levels_structure(ust_context *ctx, const ust_solve_opts *o, int need3
     ctx->levels_cdeg = o->coarse_degree;
     ctx->levels_ccdeg = o->coarsest_degree;
     ctx->levels_req = o->mg_levels;
+    ctx->levels_w16 = need32 ? (o->vcycle_bf16 != 0) : -1;
     return UST_OK;
 }
 
 int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
     int rc;
-    if (!ctx->levels_struct || (need32 && !ctx->levels_prec32) || ctx->levels_deg != o->smoother_degree ||
+    if (!ctx->levels_struct || (need32 && !ctx->levels_prec32) ||
+        (need32 && ctx->levels_w16 != (o->vcycle_bf16 != 0)) || ctx->levels_deg != o->smoother_degree ||
         ctx->levels_cdeg != o->coarse_degree || ctx->levels_ccdeg != o->coarsest_degree ||
         ctx->levels_req != o->mg_levels) {
         if ((rc = levels_structure(ctx, o, need32 || ctx->levels_prec32))) return rc;
@@ -317,6 +335,7 @@ int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
         ust_level *L = &ctx->lev[l];
         if (l > 0) CKD(ctx, ust_k_coarsen_weights(&ctx->lev[l - 1].g, ctx->lev[l - 1].WK64, L->WK64, S(ctx)));
         if (L->WK32) CKD(ctx, ust_k_copy_f64_to_f32(L->WK64, L->WK32, ust_k_weights_len(&L->g), S(ctx)));
+        if (L->WK16) CKD(ctx, ust_k_copy_f64_to_bf16(L->WK64, L->WK16, ust_k_weights_len(&L->g), S(ctx)));
         CKD(ctx, ust_k_inv_diag(&L->g, L->WK64, L->mask, L->invd64, L->invd32, S(ctx)));
     }
     ust_level *Lc = &ctx->lev[nlev - 1];

@@ -50,6 +50,8 @@ typedef struct {
                                 (0: smoother_degree) */
     int coarsest_degree;     /* Chebyshev degree on the coarsest level when it is too large for the dense
                                 solve (0: 24) */
+    int vcycle_bf16;         /* mixed precision: 1 (default) the FP32 V-cycle reads bfloat16 K weights (half the
+                                weight stream of every multigrid pass), 0 FP32 weights */
 } ust_solve_opts;
 
 typedef struct {

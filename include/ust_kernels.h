@@ -92,9 +92,10 @@ int ust_k_assign_materials_field(const ust_grid *g, const int *cell_off, int np,
 /* coh[p] = 1 + amp * u, u uniform in (-1, 1) from a counter hash of (seed, p) */
 int ust_k_noise_scale(float *coh, int np, float amp, unsigned long long seed, void *stream);
 
-/* Length (in elements of the weight type) of a K-weight array of grid g.  The K weights are
- * stored tile-blocked: W[(tile*nq + q)*256 + local], tiles of 16x4x4 (3D) or 16x16 (2D) elements
- * in lexicographic order, local index lexicographic inside the tile; partial tiles are padded. */
+/* Length (in elements of the weight type) of a K-weight array of grid g.  The K weights are stored
+ * tile-blocked.  3D: columns of 16x16 elements in lexicographic order, each column layer by layer,
+ * W[((column*nz + layer)*27 + q)*256 + local]; 2D: 16x16 tiles, W[(tile*9 + q)*256 + local]; local
+ * index lexicographic inside the 16x16 block, partial blocks padded with zeros. */
 size_t ust_k_weights_len(const ust_grid *g);
 /* tile-blocked position of weight q of element e (the host-side view of the same layout) */
 size_t ust_k_weights_index(const ust_grid *g, int e, int q);
@@ -210,6 +211,18 @@ int ust_k_apply_K_cheby_prolong_f64(const ust_grid *g, const double *WK, const u
 int ust_k_apply_K_cheby_prolong_f32(const ust_grid *g, const float *WK, const unsigned char *mask, float *x,
                                     const float *b, const float *invd, float *d, float c2, const float *xc,
                                     float *work, void *stream);
+/* the FP32 operator with bfloat16 K weights (half the weight stream; the multigrid V-cycle's operator):
+ * the same four passes as the _f32 entry points above */
+int ust_k_apply_K_f32_bf16w(const ust_grid *g, const unsigned short *WK, const unsigned char *mask, const float *u,
+                            float *y, float *work, void *stream);
+int ust_k_apply_K_residual_f32_bf16w(const ust_grid *g, const unsigned short *WK, const unsigned char *mask,
+                                     const float *u, const float *b, float *r, float *work, void *stream);
+int ust_k_apply_K_cheby_f32_bf16w(const ust_grid *g, const unsigned short *WK, const unsigned char *mask, float *x,
+                                  const float *b, const float *invd, float *d, float c1, float c2, int first,
+                                  float *work, void *stream);
+int ust_k_apply_K_cheby_prolong_f32_bf16w(const ust_grid *g, const unsigned short *WK, const unsigned char *mask,
+                                          float *x, const float *b, const float *invd, float *d, float c2,
+                                          const float *xc, float *work, void *stream);
 /* same operator, one element per thread with per-dof atomic adds (reference variant for checks) */
 int ust_k_apply_K_ref_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
                           double *y, void *stream);
@@ -271,6 +284,8 @@ int ust_k_fill_hash_f64(double *x, size_t n, unsigned long long seed, void *stre
 int ust_k_fill_sign_f64(double *x, size_t n, unsigned long long seed, void *stream);
 int ust_k_copy_f64_to_f32(const double *x, float *y, size_t n, void *stream);
 int ust_k_copy_f32_to_f64(const float *x, double *y, size_t n, void *stream);
+/* y = bfloat16(float(x)), round to nearest even (the K weights of the multigrid V-cycle) */
+int ust_k_copy_f64_to_bf16(const double *x, unsigned short *y, size_t n, void *stream);
 int ust_k_axpy_f64(double a, const double *x, double *y, size_t n, void *stream);     /* y += a x */
 int ust_k_axpby_f64(double a, const double *x, double b, double *y, size_t n, void *stream);
 int ust_k_add_f32_to_f64(const float *x, double *y, size_t n, void *stream);        /* y += x */

@@ -314,3 +314,111 @@ def test_restriction_is_adjoint_of_prolongation(dim, nel):
             a.free()
     dcm.free()
     s.close()
+
+
+def _grid(dim, nel, h=(0.5, 0.7, 0.9)):
+    from earthbyte_underworld_amd import _lib as L
+    g = L.Grid()
+    g.dim = dim
+    for d in range(3):
+        g.nel[d] = nel[d] if d < dim else 1
+        g.h[d] = h[d] if d < dim else 1.0
+        g.xmin[d] = 0.0
+        g.xmax[d] = g.nel[d] * g.h[d]
+    g.ymax = g.xmax[1]
+    return g
+
+
+# z-chunk lengths 2 (partial last chunk), 2, 4 (partial), 6, 8, 16; partial 16x16 columns; exact multiples
+@pytest.mark.parametrize("nel", [(17, 9, 5), (16, 16, 2), (33, 20, 7), (100, 36, 50), (64, 64, 130), (96, 96, 96),
+                                 (128, 128, 64), (128, 128, 128)])
+def test_column_kernel_against_element_reference(nel):
+    """The streaming column kernel (every epilogue; FP64, FP32 and bfloat16 weights) against the one-element-
+    per-thread reference operator with atomics, on random weights, masks and vectors: the chunked z-streaming,
+    the ring and chunk-boundary workspace and the gather must add up to the same product."""
+    import ctypes as C
+    import earthbyte_underworld_amd as ust  # noqa: F401  (loads the library)
+    from earthbyte_underworld_amd import _lib as L
+    lib = L.load()
+    g = _grid(3, nel)
+    n = int(np.prod([k + 1 for k in nel])) * 3
+    wl = lib.ust_k_weights_len(C.byref(g))
+    rng = np.random.default_rng(sum(nel))
+    mask = (rng.random(n) < 0.1).astype(np.uint8)
+    dm = _Dev(lib, mask)
+    W64 = _Dev(lib, np.zeros(wl))
+    assert lib.ust_k_fill_hash_f64(W64.p, wl, 7, None) == 0
+    W32 = _Dev(lib, np.zeros(wl, np.float32))
+    assert lib.ust_k_copy_f64_to_f32(W64.p, W32.p, wl, None) == 0
+    W16 = _Dev(lib, np.zeros(wl, np.uint16))
+    assert lib.ust_k_copy_f64_to_bf16(W64.p, W16.p, wl, None) == 0
+    small = n < 3_000_000
+    # bfloat16 weights widened to FP32 for the reference
+    W16f = _Dev(lib, (W16.get().astype(np.uint32) << 16).view(np.float32)) if small else None
+    work = _Dev(lib, np.zeros(lib.ust_k_apply_work_len(C.byref(g))))
+    u = rng.normal(size=n)
+    free = mask == 0
+
+    def ref(W, x, dt):
+        dx, dy = _Dev(lib, x.astype(dt)), _Dev(lib, np.zeros(n, dt))
+        f = lib.ust_k_apply_K_ref_f64 if dt == np.float64 else lib.ust_k_apply_K_ref_f32
+        assert f(C.byref(g), W, dm.p, dx.p, dy.p, None) == 0
+        out = dy.get().astype(np.float64)
+        dx.free()
+        dy.free()
+        return out
+
+    y64 = ref(W64.p, u, np.float64)
+    du = _Dev(lib, u)
+    dy = _Dev(lib, np.zeros(n))
+    assert lib.ust_k_apply_K_f64(C.byref(g), W64.p, dm.p, du.p, dy.p, work.p, None) == 0
+    y = dy.get()
+    assert rel_err(y, y64) < 1e-13 and np.all(y[~free] == 0.0)
+    dy.free()
+    cases = [("f32", W32.p, lambda: ref(W32.p, u, np.float32), 1e-5, np.float32)]
+    if small:
+        cases.append(("bf16w", W16.p, lambda: ref(W16f.p, u, np.float32), 1e-5, np.float32))
+    rb = rng.normal(size=n)
+    rb[~free] = 0.0
+    invd, d0 = rng.uniform(0.5, 2.0, n), rng.normal(size=n)
+    for name, W, yref_fn, tol, dt in cases:
+        yref = yref_fn()
+        sfx = "_bf16w" if name == "bf16w" else ""
+        du32, dy32 = _Dev(lib, u.astype(dt)), _Dev(lib, np.zeros(n, dt))
+        work32 = _Dev(lib, np.zeros(lib.ust_k_apply_work_len(C.byref(g)), dt))
+        assert getattr(lib, f"ust_k_apply_K_f32{sfx}")(C.byref(g), W, dm.p, du32.p, dy32.p, work32.p, None) == 0
+        assert rel_err(dy32.get().astype(np.float64), yref) < tol, name
+        db = _Dev(lib, rb.astype(dt))
+        assert getattr(lib, f"ust_k_apply_K_residual_f32{sfx}")(C.byref(g), W, dm.p, du32.p, db.p, dy32.p, work32.p,
+                                                                None) == 0
+        assert rel_err(dy32.get().astype(np.float64), rb - yref) < tol, name
+        c1, c2 = 0.3, 0.8
+        for first in (0, 1):
+            dx, dd, di = _Dev(lib, u.astype(dt)), _Dev(lib, d0.astype(dt)), _Dev(lib, invd.astype(dt))
+            assert getattr(lib, f"ust_k_apply_K_cheby_f32{sfx}")(C.byref(g), W, dm.p, dx.p, db.p, di.p, dd.p, c1, c2,
+                                                                 first, work32.p, None) == 0
+            dn = (0.0 if first else c1 * d0) + c2 * invd * (rb - yref)  # Dirichlet dofs: b = 0 and (K x) = 0
+            assert rel_err(dd.get().astype(np.float64), dn) < tol, (name, first)
+            assert rel_err(dx.get().astype(np.float64), u + dn) < tol, (name, first)
+            for a in (dx, dd, di):
+                a.free()
+        if all(k % 2 == 0 for k in nel) and small:
+            cn = tuple(k // 2 + 1 for k in nel)
+            xc = rng.normal(size=int(np.prod(cn)) * 3)
+            dxc = _Dev(lib, xc.astype(dt))
+            xp = _Dev(lib, u.astype(dt))
+            assert lib.ust_k_prolong_add_f32(C.byref(g), dm.p, dxc.p, xp.p, None) == 0
+            x1 = xp.get().astype(np.float64)
+            y1 = ref(W16f.p if name == "bf16w" else W32.p, x1, np.float32)
+            dx, dd, di = _Dev(lib, u.astype(dt)), _Dev(lib, d0.astype(dt)), _Dev(lib, invd.astype(dt))
+            assert getattr(lib, f"ust_k_apply_K_cheby_prolong_f32{sfx}")(C.byref(g), W, dm.p, dx.p, db.p, di.p, dd.p,
+                                                                         c2, dxc.p, work32.p, None) == 0
+            dn = c2 * invd * (rb - y1)
+            assert rel_err(dd.get().astype(np.float64), dn) < 5 * tol, name
+            assert rel_err(dx.get().astype(np.float64), x1 + dn) < 5 * tol, name
+            for a in (dx, dd, di, dxc, xp):
+                a.free()
+        for a in (du32, dy32, work32, db):
+            a.free()
+    for a in [dm, W64, W32, W16, work, du] + ([W16f] if W16f else []):
+        a.free()
