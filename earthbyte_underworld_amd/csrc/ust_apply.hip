@@ -293,6 +293,11 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_col(GridDev g, const WT *__r
                                                      T *__restrict__ work) {
     __shared__ T su[2][CNPD];     // staged node planes k and k+1 (free dofs; Dirichlet dofs read 0)
     __shared__ T eb[24 * EBS];    // the layer's element results: eb[(a*3 + i)*EBS + element]
+    // EPI_PCHEBY0: a ring of two coarse node planes under the column (9x9 coarse nodes; the column's origin is
+    // even, so fine node 2I lies on coarse node I)
+    constexpr int CCX = UST_TX3 / 2 + 1, CCY = UST_TY3 / 2 + 1, NCP = CCX * CCY * 3;
+    static_assert(NCP <= UST_TILE, "one coarse value per thread");
+    __shared__ T sc[MODE == EPI_PCHEBY0 ? 2 * NCP : 1];
     const int ntxy = g.tnx * g.tny;
     const int tile = blockIdx.x;
     const int txy = tile % ntxy, chunk = tile / ntxy;
@@ -301,7 +306,7 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_col(GridDev g, const WT *__r
     const int tid = threadIdx.x, lx = tid % UST_TX3, ly = tid / UST_TX3;
     const bool active = x0 + lx < g.nx && y0 + ly < g.ny;
     const unsigned pstride = (unsigned)g.nnx * (unsigned)g.nny * 3u;  // dofs of one global node plane
-    const int cnx = g.nx / 2 + 1, cny = g.ny / 2 + 1;                 // EPI_PCHEBY0: the coarse grid
+    const int cnx = g.nx / 2 + 1, cny = g.ny / 2 + 1, cnz = g.nz / 2 + 1;  // EPI_PCHEBY0: the coarse grid
     // the plane dofs this thread stages and finishes: t = tid + 256 r -> node n = t / 3 (i = n % 17, j = n / 17),
     // component c = t % 3; pofs = the in-plane part of the global dof index, -1 outside the plane or the grid
     int pofs[CNR];
@@ -314,23 +319,32 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_col(GridDev g, const WT *__r
         pofs[r] = ok ? (gx + g.nnx * gy) * 3 + c : -1;
         ring[r] = on_tile_face(gx, UST_TX3, g.tnx) || on_tile_face(gy, UST_TY3, g.tny);
     }
-    // staged value of plane dof r on global plane gz: u on free dofs (plus the prolongation for EPI_PCHEBY0)
-    auto stage = [&](int gz, T *v) {
-        unsigned char m[CNR];
-        T x[CNR];
+    // global loads of plane gz (issued ahead of their use)
+    auto load_raw = [&](int gz, T *x, unsigned char *m) {
 #pragma unroll
         for (int r = 0; r < CNR; r++) {
             const unsigned dof = pofs[r] >= 0 ? (unsigned)pofs[r] + (unsigned)gz * pstride : 0u;
-            m[r] = mask[dof];
+            m[r] = pofs[r] >= 0 ? mask[dof] : (unsigned char)1;
             x[r] = u[dof];
         }
+    };
+    // coarse plane c under the column: thread t < NCP loads one value (0 outside the coarse grid)
+    auto coarse_load = [&](int c) -> T {
+        const int n = tid / 3, cc = tid % 3;
+        const int I = x0 / 2 + n % CCX, J = y0 / 2 + n / CCX;
+        const bool ok = tid < NCP && I < cnx && J < cny && c < cnz;
+        const T v = ep.xc[ok ? ((unsigned)I + (unsigned)cnx * ((unsigned)J + (unsigned)cny * c)) * 3u + cc : 0u];
+        return ok ? v : (T)0;
+    };
+    // staged value: u on free dofs, plus P xc from the coarse ring for EPI_PCHEBY0
+    auto staged = [&](int gz, const T *x, const unsigned char *m, T *v) {
 #pragma unroll
         for (int r = 0; r < CNR; r++) {
-            T val = (pofs[r] >= 0 && !m[r]) ? x[r] : (T)0;
-            if (MODE == EPI_PCHEBY0 && pofs[r] >= 0 && !m[r]) {
+            T val = m[r] ? (T)0 : x[r];
+            if (MODE == EPI_PCHEBY0 && !m[r]) {
                 const int t = tid + UST_TILE * r, n = t / 3, c = t % 3;
                 val += prolong_value<T>(x0 + n % CPX, y0 + n / CPX, gz, [&](int I, int J, int K) {
-                    return ep.xc[((unsigned)I + (unsigned)cnx * ((unsigned)J + (unsigned)cny * K)) * 3u + c];
+                    return sc[(K & 1) * NCP + ((I - x0 / 2) + CCX * (J - y0 / 2)) * 3 + c];
                 });
             }
             v[r] = val;
@@ -360,37 +374,87 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_col(GridDev g, const WT *__r
             ops[r] = epilogue_load<MODE, T>(ep, mask, own ? (unsigned)pofs[r] + (unsigned)gz * pstride : 0u);
         }
     };
-    constexpr bool early = sizeof(T) == 4;  // FP64 defers loads to keep two waves per SIMD
-    T sv[CNR], carry[CNR];
-    stage(z0, sv);
-#pragma unroll
-    for (int r = 0; r < CNR; r++) {
-        const int t = tid + UST_TILE * r;
-        if (t < CNPD) su[0][t] = sv[r];
-        carry[r] = (T)0;  // plane z0 receives nothing from below within this chunk
-    }
-    const T s[3] = {(T)(2.0 / g.hx), (T)(2.0 / g.hy), (T)(2.0 / g.hz)};
     const size_t wbase = ((size_t)txy * g.nz) * 27 * UST_TILE + tid;
-    for (int k = z0; k < z1; k++) {
-        const int cur = (k - z0) & 1, nxt = cur ^ 1;
-        const bool zsh = k == z0 && z0 > 0;  // the chunk's first plane is shared with the chunk below
-        stage(k + 1, sv);
-        T wq[27];
-        EpiOps<T> ops[CNR];
-        if (early) {
+    auto load_w = [&](int k, WT *w) {
 #pragma unroll
-            for (int q = 0; q < 27; q++) wq[q] = wload<T, WT>(W, wbase + ((size_t)k * 27 + q) * UST_TILE);
-            load_ops(k, zsh, ops);
+        for (int q = 0; q < 27; q++) w[q] = W[wbase + ((size_t)k * 27 + q) * UST_TILE];
+    };
+    // FP32 (and bfloat16 weights): the next layer's node plane and weights are prefetched in registers during
+    // this layer; FP64 has no registers to spare for them and loads them at the layer
+    constexpr bool wpre = sizeof(T) == 4;
+    // ---- prologue: the coarse ring (planes z0/2 and z0/2 + 1), plane z0, and the loads for layer z0 ----
+    if (MODE == EPI_PCHEBY0) {
+        const int c0 = z0 >> 1;
+        const T v0 = coarse_load(c0), v1 = coarse_load(c0 + 1);
+        if (tid < NCP) {
+            sc[(c0 & 1) * NCP + tid] = v0;
+            sc[((c0 + 1) & 1) * NCP + tid] = v1;
         }
-        __syncthreads();  // the previous layer is done with su[nxt] and eb
+        __syncthreads();
+    }
+    T xr[CNR], carry[CNR];
+    unsigned char mr[CNR];
+    {
+        T v[CNR];
+        load_raw(z0, xr, mr);
+        staged(z0, xr, mr, v);
 #pragma unroll
         for (int r = 0; r < CNR; r++) {
             const int t = tid + UST_TILE * r;
-            if (t < CNPD) su[nxt][t] = sv[r];
+            if (t < CNPD) su[0][t] = v[r];
+            carry[r] = (T)0;  // plane z0 receives nothing from below within this chunk
+        }
+    }
+    if (wpre) load_raw(z0 + 1, xr, mr);
+    WT wr[27];
+    if (wpre) load_w(z0, wr);
+    // EPI_PCHEBY0: staging an even plane p writes coarse plane p/2 + 1 into the ring (the odd plane p + 1 needs
+    // it); its value is loaded one layer ahead
+    T cv = (T)0;
+    if (MODE == EPI_PCHEBY0 && wpre && ((z0 + 1) & 1) == 0) cv = coarse_load(((z0 + 1) >> 1) + 1);
+    const T s[3] = {(T)(2.0 / g.hx), (T)(2.0 / g.hy), (T)(2.0 / g.hz)};
+    for (int k = z0; k < z1; k++) {
+        const int cur = (k - z0) & 1, nxt = cur ^ 1;
+        const bool zsh = k == z0 && z0 > 0;  // the chunk's first plane is shared with the chunk below
+        // this layer's operands, loaded during the previous one
+        T x1[CNR];
+        unsigned char m1[CNR];
+        if (wpre) {
+#pragma unroll
+            for (int r = 0; r < CNR; r++) {
+                x1[r] = xr[r];
+                m1[r] = mr[r];
+            }
+        } else {
+            load_raw(k + 1, x1, m1);
+        }
+        T wq[27];
+        if (wpre) {
+#pragma unroll
+            for (int q = 0; q < 27; q++) wq[q] = wload<T, WT>(wr, q);
+        }
+        const T cvk = (MODE == EPI_PCHEBY0 && !wpre && ((k + 1) & 1) == 0) ? coarse_load(((k + 1) >> 1) + 1) : cv;
+        // the next layer's loads
+        if (wpre && k + 2 <= z1) load_raw(k + 2, xr, mr);
+        if (wpre && k + 1 < z1) load_w(k + 1, wr);
+        if (MODE == EPI_PCHEBY0 && wpre && ((k + 2) & 1) == 0 && k + 1 < z1) cv = coarse_load(((k + 2) >> 1) + 1);
+        EpiOps<T> ops[CNR];
+        if (wpre) load_ops(k, zsh, ops);
+        __syncthreads();  // the previous layer is done with su[nxt], eb and the coarse ring slot written below
+        {
+            T v[CNR];
+            staged(k + 1, x1, m1, v);
+#pragma unroll
+            for (int r = 0; r < CNR; r++) {
+                const int t = tid + UST_TILE * r;
+                if (t < CNPD) su[nxt][t] = v[r];
+            }
+            if (MODE == EPI_PCHEBY0 && ((k + 1) & 1) == 0 && tid < NCP)
+                sc[((((k + 1) >> 1) + 1) & 1) * NCP + tid] = cvk;
         }
         __syncthreads();
         {
-            if (!early) {
+            if (!wpre) {
 #pragma unroll
                 for (int q = 0; q < 27; q++) wq[q] = wload<T, WT>(W, wbase + ((size_t)k * 27 + q) * UST_TILE);
             }
@@ -409,7 +473,7 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_col(GridDev g, const WT *__r
 #pragma unroll
                 for (int i = 0; i < 3; i++) eb[(a * 3 + i) * EBS + tid] = active ? ye[a][i] : (T)0;
         }
-        if (!early) load_ops(k, zsh, ops);
+        if (!wpre) load_ops(k, zsh, ops);
         __syncthreads();
         // plane k: the carried contributions of layer k-1 plus those of layer k below the plane; the
         // contributions of layer k above plane k+1 become the new carry.  Fixed order: elements (i-1, j-1),

@@ -166,8 +166,9 @@ static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x, int 
     ctx->b64_src = NULL;
     /* the coarse level starts with a smoothing step from zero unless it is the dense coarse solve */
     int fuse = C->cheb_deg > 0 && !(l + 1 == ctx->nlev - 1 && ctx->coarse_dense);
-    /* 3D: the prolongation of the correction rides on the first post-smoothing pass */
-    int pfuse = L->g.dim == 3 && L->cheb_deg > 0;
+    /* 3D FP32: the prolongation of the correction rides on the first post-smoothing pass (FP64 keeps the
+     * separate prolongation: the fused FP64 pass needs more registers than two waves per SIMD allow) */
+    int pfuse = L->g.dim == 3 && L->cheb_deg > 0 && f32;
     if (f32) {
         if ((rc = lev_residual(ctx, l, 1, x, b, L->ax32))) return rc;
         if (fuse)
