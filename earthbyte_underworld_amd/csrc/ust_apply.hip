@@ -12,6 +12,8 @@
 // a neighbouring column) or on a chunk-boundary plane (shared with the next chunk).  Those nodes go to the
 // workgroup's slot of a workspace, and a second, node-ordered pass sums the two to eight slots covering each
 // of them.  No atomics touch memory and the summation order is fixed, so K u is bitwise reproducible.
+// Grids with fewer than 1024 column layers (the coarse multigrid levels) use 16x4x4 tiles instead, one per
+// workgroup, which gives small grids more workgroups.
 // 2D: one workgroup per 16x16 tile, node block reduced in LDS, the same workspace scheme on the tile faces.
 #include "ust_common.h"
 
@@ -510,6 +512,147 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_col(GridDev g, const WT *__r
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// 3D tile kernel for grids too small for the column kernel to fill the GPU (the coarse multigrid levels):
+// one 256-thread workgroup per 16x4x4 tile, one z layer of 64 elements per wavefront, the node block staged
+// and reduced in LDS (FP64 accumulation, two wave-parity phases fix the summation order), tile-face nodes
+// through the workspace.
+// ------------------------------------------------------------------------------------------
+template <typename T, typename WT, int MODE>
+__global__ __launch_bounds__(256, 2) void k_applyK3_tiled(GridDev g, const WT *__restrict__ W,
+                                                       const unsigned char *__restrict__ mask, const T *u,
+                                                       Epi<T> ep, T *__restrict__ work) {
+    constexpr int TX = UST_TTX3, TY = UST_TTY3, TZ = UST_TTZ3;
+    constexpr int NX = TX + 1, NY = TY + 1, NZ = TZ + 1, NN = NX * NY * NZ;
+    __shared__ T su[NN * 3];
+    __shared__ double sy[NN * 3];  // FP64 accumulation in LDS for both precisions (ds_add_f64)
+    // EPI_PCHEBY0: the coarse nodes under the tile (tile origins are even, so fine node 2I lies on coarse I)
+    constexpr int CX = TX / 2 + 1, CY = TY / 2 + 1, CZ = TZ / 2 + 1, NC = CX * CY * CZ * 3;
+    static_assert(NC <= UST_TILE, "one coarse value per thread");
+    __shared__ T sc[MODE == EPI_PCHEBY0 ? NC : 1];
+    const int tile = blockIdx.x;
+    const int tx0 = (tile % g.tnx) * TX, ty0 = ((tile / g.tnx) % g.tny) * TY, tz0 = (tile / (g.tnx * g.tny)) * TZ;
+    T cv = (T)0;
+    if (MODE == EPI_PCHEBY0) {
+        const int t = threadIdx.x, c = t % 3, n = t / 3;
+        const int I = tx0 / 2 + n % CX, J = ty0 / 2 + (n / CX) % CY, K = tz0 / 2 + n / (CX * CY);
+        const int cnx = g.nx / 2 + 1, cny = g.ny / 2 + 1, cnz = g.nz / 2 + 1;
+        const bool ok = t < NC && I < cnx && J < cny && K < cnz;
+        cv = ep.xc[ok ? ((unsigned)I + (unsigned)cnx * ((unsigned)J + (unsigned)cny * K)) * 3u + c : 0u];
+        cv = ok ? cv : (T)0;
+    }
+    // Loads are issued without branches, staged velocities first, then the element weights (which do not
+    // depend on them), so the LDS stores below wait only for the staging loads and the weights arrive
+    // while the tile is being staged.  Out-of-tile slots read dof 0 and are discarded; partial tiles are
+    // padded in W, so every load is in bounds.
+    constexpr int NST = (NN * 3 + UST_TILE - 1) / UST_TILE;
+    T stv[NST];
+    bool stok[NST];
+    unsigned char stm[NST];
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = threadIdx.x + UST_TILE * r;
+        const int c = t % 3, n = t / 3;
+        const int gx = tx0 + n % NX, gy = ty0 + (n / NX) % NY, gz = tz0 + n / (NX * NY);
+        stok[r] = t < NN * 3 && gx < g.nnx && gy < g.nny && gz < g.nnz;
+        const unsigned dof = stok[r] ? ((unsigned)gx + (unsigned)g.nnx * ((unsigned)gy + (unsigned)g.nny * gz)) * 3u + c : 0u;
+        stm[r] = mask[dof];
+        stv[r] = u[dof];
+    }
+    // FP32: the weights are loaded here, ahead of the barrier; FP64 loads them after it, which keeps the
+    // register count within two waves per SIMD
+    constexpr bool early = sizeof(T) == 4;
+    T wq[27];
+    if (early) {
+#pragma unroll
+        for (int q = 0; q < 27; q++) wq[q] = wload<T, WT>(W, ((size_t)tile * 27 + q) * UST_TILE + threadIdx.x);
+    }
+    if (MODE == EPI_PCHEBY0) {
+        if (threadIdx.x < NC) sc[threadIdx.x] = cv;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NST; r++) {
+            const int t = threadIdx.x + UST_TILE * r;
+            const int c = t % 3, n = t / 3;
+            const int ix = n % NX, iy = (n / NX) % NY, iz = n / (NX * NY);
+            if (stok[r] && !stm[r])
+                stv[r] += prolong_value<T>(ix, iy, iz, [&](int I, int J, int K) {
+                    return sc[(I + CX * (J + CY * K)) * 3 + c];
+                });
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NST; r++) stv[r] = (stok[r] && !stm[r]) ? stv[r] : (T)0;
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = threadIdx.x + UST_TILE * r;
+        if (t < NN * 3) {
+            su[t] = stv[r];
+            sy[t] = 0.0;
+        }
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
+    const bool active = tx0 + lx < g.nx && ty0 + ly < g.ny && tz0 + lz < g.nz;
+    T ye[8][3];
+    if (active) {
+        if (!early) {
+#pragma unroll
+            for (int q = 0; q < 27; q++) wq[q] = wload<T, WT>(W, ((size_t)tile * 27 + q) * UST_TILE + threadIdx.x);
+        }
+        T ue[8][3];
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+            const int ln = (lx + (a & 1)) + NX * ((ly + ((a >> 1) & 1)) + NY * (lz + (a >> 2)));
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                ue[a][i] = su[ln * 3 + i];
+                ye[a][i] = (T)0;
+            }
+        }
+        const T s[3] = {(T)(2.0 / g.hx), (T)(2.0 / g.hy), (T)(2.0 / g.hz)};
+        element_apply3<T>([&](int a, int i) -> T { return ue[a][i]; }, wq, s, ye);
+    }
+    // Wave w holds the element layer lz = w, whose nodes are layers w and w+1: even waves never share a
+    // node, nor do odd waves, and within one LDS add the 64 lanes hit 64 distinct nodes.  Adding in two
+    // wave-parity phases therefore fixes the order in which every node receives its contributions.
+    for (int phase = 0; phase < 2; phase++) {
+        if (active && (lz & 1) == phase) {
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                const int ln = (lx + (a & 1)) + NX * ((ly + ((a >> 1) & 1)) + NY * (lz + (a >> 2)));
+#pragma unroll
+                for (int i = 0; i < 3; i++) atomicAdd(&sy[ln * 3 + i], (double)ye[a][i]);
+            }
+        }
+        __syncthreads();
+    }
+    // writeout: every load of the thread (mask, epilogue operands) first, then the stores.  Dof indices are
+    // 32-bit (the host rejects larger grids), which keeps the per-thread index registers down.
+    T *slot = work + (size_t)tile * NN * 3;
+    int kind[NST];  // 0 none, 1 tile-face node (workspace slot), 2 node of this tile only (written here)
+    unsigned wdof[NST];
+    EpiOps<T> ops[NST];
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = threadIdx.x + UST_TILE * r;
+        const int c = t % 3, n = t / 3;
+        const int gx = tx0 + n % NX, gy = ty0 + (n / NX) % NY, gz = tz0 + n / (NX * NY);
+        const bool in = t < NN * 3 && gx < g.nnx && gy < g.nny && gz < g.nnz;
+        const bool face = on_tile_face(gx, TX, g.tnx) || on_tile_face(gy, TY, g.tny) || on_tile_face(gz, TZ, g.tnz);
+        kind[r] = in ? (face ? 1 : 2) : 0;
+        wdof[r] = kind[r] == 2 ? ((unsigned)gx + (unsigned)g.nnx * ((unsigned)gy + (unsigned)g.nny * gz)) * 3u + c : 0u;
+        ops[r] = epilogue_load<MODE, T>(ep, mask, wdof[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = threadIdx.x + UST_TILE * r;
+        if (MODE == EPI_PCHEBY0 && kind[r] == 2 && !ops[r].m) ops[r].x = su[t];  // x + P xc, as staged
+        if (kind[r] == 1) slot[t] = (T)sy[t];
+        else if (kind[r] == 2) epilogue_store<MODE, T>(ep, ops[r], wdof[r], (T)sy[t]);
+    }
+}
+
 template <typename T, typename WT, int MODE>
 __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const WT *__restrict__ W,
                                                        const unsigned char *__restrict__ mask, const T *u,
@@ -705,8 +848,12 @@ static int apply_tiled(const ust_grid *gr, const WT *W, const unsigned char *mas
     if ((size_t)g.nnx * g.nny * g.nnz * g.dim >= (1ull << 32)) return UST_EINVAL;  // 32-bit dof indices
     hipStream_t st = ust_stream(stream);
     if (g.dim == 3) {
-        hipLaunchKernelGGL((k_applyK3_col<T, WT, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep,
-                           work);
+        if (g.col)
+            hipLaunchKernelGGL((k_applyK3_col<T, WT, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep,
+                               work);
+        else
+            hipLaunchKernelGGL((k_applyK3_tiled<T, WT, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u,
+                               ep, work);
         const unsigned nf = face_node_count<3>(g);
         if (nf) hipLaunchKernelGGL((k_gather_tiles<T, 3, MODE>), dim3(ust_grid_size((size_t)nf * 3, 256)), dim3(256),
                                    0, st, g, mask, work, ep);
@@ -823,7 +970,8 @@ extern "C" int ust_k_apply_K_cheby_prolong_f32_bf16w(const ust_grid *g, const un
 
 extern "C" size_t ust_k_weights_len(const ust_grid *gr) {
     GridDev g = grid_dev(gr);
-    if (g.dim == 3) return (size_t)g.tnx * g.tny * g.nz * 27 * UST_TILE;
+    if (g.dim == 3 && g.col) return (size_t)g.tnx * g.tny * g.nz * 27 * UST_TILE;
+    if (g.dim == 3) return (size_t)g.ntiles * 27 * UST_TILE;
     return (size_t)g.ntiles * 9 * UST_TILE;
 }
 

@@ -41,16 +41,23 @@ __host__ __device__ constexpr int sgn(int a) { return 2 * a - 1; }
 // 2D tiles are 16 x 16 elements.  In 3D a workgroup owns a 16 x 16 column of elements and streams a chunk
 // of tz element layers along z (tz from the grid size, so that a launch has enough workgroups); the K
 // weights of a column are stored layer after layer, W[((txy*nz + ez)*27 + q)*256 + local], so that every
-// load of a quadrature weight by a wavefront is one contiguous 64-element run, for any chunk length.
+// load of a quadrature weight by a wavefront is one contiguous 64-element run, for any chunk length.  A 3D
+// grid with fewer than UST_COL_MIN column layers (tnx*tny*nz) uses 16 x 4 x 4 tiles instead, stored tile
+// after tile, W[(tile*27 + q)*256 + local]: small grids then get more workgroups.
 #define UST_TX3 16
 #define UST_TY3 16
+#define UST_TTX3 16
+#define UST_TTY3 4
+#define UST_TTZ3 4
 #define UST_TX2 16
 #define UST_TY2 16
 #define UST_TILE 256
 #define UST_TZ3_MAX 16
+#define UST_COL_MIN 1024
 
 struct GridDev {
     int dim;
+    int col;              // 3D column layout (1) or 16x4x4 tiles (0)
     int nx, ny, nz;       // elements per axis (nz = 1 in 2D)
     int nnx, nny, nnz;    // nodes per axis
     int nel;
@@ -65,10 +72,15 @@ struct GridDev {
 // index of quadrature weight q of element e in the tile-blocked layout
 __host__ __device__ __forceinline__ size_t widx(const GridDev &g, int e, int q) {
     const int ex = e % g.nx, ey = (e / g.nx) % g.ny, ez = e / (g.nx * g.ny);
-    if (g.dim == 3) {
+    if (g.dim == 3 && g.col) {
         const int txy = ex / UST_TX3 + g.tnx * (ey / UST_TY3);
         const int l = ex % UST_TX3 + UST_TX3 * (ey % UST_TY3);
         return (((size_t)txy * g.nz + ez) * 27 + q) * UST_TILE + l;
+    }
+    if (g.dim == 3) {
+        const int t = ex / UST_TTX3 + g.tnx * (ey / UST_TTY3 + g.tny * (ez / UST_TTZ3));
+        const int l = ex % UST_TTX3 + UST_TTX3 * (ey % UST_TTY3 + UST_TTY3 * (ez % UST_TTZ3));
+        return ((size_t)t * 27 + q) * UST_TILE + l;
     }
     const int t = ex / UST_TX2 + g.tnx * (ey / UST_TY2);
     const int l = ex % UST_TX2 + UST_TX2 * (ey % UST_TY2);
@@ -85,20 +97,32 @@ static inline GridDev grid_dev(const ust_grid *g) {
     d.nny = d.ny + 1;
     d.nnz = g->dim == 3 ? d.nz + 1 : 1;
     d.nel = d.nx * d.ny * d.nz;
+    d.col = 0;
     if (g->dim == 3) {
-        d.tx = UST_TX3;
-        d.ty = UST_TY3;
-        d.tnx = (d.nx + UST_TX3 - 1) / UST_TX3;
-        d.tny = (d.ny + UST_TY3 - 1) / UST_TY3;
-        // z-chunk: long enough that few node planes sit on chunk boundaries, short enough that the launch has
-        // about 512 workgroups (two resident per CU on 256 CUs) or more
-        long long layers = (long long)d.tnx * d.tny * d.nz;
-        long long zc = layers / 512;
-        if (zc < 2) zc = 2;
-        if (zc > UST_TZ3_MAX) zc = UST_TZ3_MAX;
-        if (zc > d.nz) zc = d.nz;
-        d.tz = (int)zc;
-        d.tnz = (d.nz + d.tz - 1) / d.tz;
+        const int cnx = (d.nx + UST_TX3 - 1) / UST_TX3, cny = (d.ny + UST_TY3 - 1) / UST_TY3;
+        const long long layers = (long long)cnx * cny * d.nz;
+        d.col = layers >= UST_COL_MIN;
+        if (d.col) {
+            d.tx = UST_TX3;
+            d.ty = UST_TY3;
+            d.tnx = cnx;
+            d.tny = cny;
+            // z-chunk: long enough that few node planes sit on chunk boundaries, short enough that the launch
+            // has about 512 workgroups (two resident per CU on 256 CUs) or more
+            long long zc = layers / 512;
+            if (zc < 2) zc = 2;
+            if (zc > UST_TZ3_MAX) zc = UST_TZ3_MAX;
+            if (zc > d.nz) zc = d.nz;
+            d.tz = (int)zc;
+            d.tnz = (d.nz + d.tz - 1) / d.tz;
+        } else {
+            d.tx = UST_TTX3;
+            d.ty = UST_TTY3;
+            d.tz = UST_TTZ3;
+            d.tnx = (d.nx + UST_TTX3 - 1) / UST_TTX3;
+            d.tny = (d.ny + UST_TTY3 - 1) / UST_TTY3;
+            d.tnz = (d.nz + UST_TTZ3 - 1) / UST_TTZ3;
+        }
     } else {
         d.tx = UST_TX2;
         d.ty = UST_TY2;

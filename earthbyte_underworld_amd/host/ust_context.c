@@ -183,7 +183,8 @@ int ust_set_mesh(ust_context *ctx, int dim, const int nel[3], const double xmin[
     DALLOC(ctx, ctx->h, ne * sizeof(double));
     DALLOC(ctx, ctx->tmpv, nd * sizeof(double));
     DALLOC(ctx, ctx->tmpp, ne * sizeof(double));
-    DALLOC(ctx, ctx->kwork, ust_k_apply_work_len(&ctx->grid) * sizeof(double));
+    ctx->kwork_len = ust_k_apply_work_len(&ctx->grid);
+    DALLOC(ctx, ctx->kwork, ctx->kwork_len * sizeof(double));
     ctx->has_mesh = 1;
     ctx->prev_solution = 0;
     ctx->nullspace = 0;

@@ -80,7 +80,8 @@ struct ust_context {
     int nullspace; /* constant pressure found in the null space of G (ksp_pressure_nullspace.c:139-158 test) */
     double ns_ratio; /* ||G 1|| / ||1|| / ||D||_inf of the last assembly */
     double *tmpv, *tmpp; /* velocity- and pressure-sized scratch of the assembly */
-    void *kwork;         /* K u workspace (per-tile node partials), sized for the finest grid in FP64 */
+    void *kwork;         /* K u workspace (per-tile node partials) in FP64 elements, for every level's grid */
+    size_t kwork_len;
     /* assembly */
     int assembled;
     double gravity;

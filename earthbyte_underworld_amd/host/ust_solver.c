@@ -306,6 +306,16 @@ static int levels_structure(ust_context *ctx, const ust_solve_opts *o, int need3
                 g.h[d] *= 2.0;
             }
     }
+    /* the operator workspace serves every level: grow it if a coarse grid's tiling needs more */
+    for (int l = 0; l < nlev; l++) {
+        const size_t wl = ust_k_apply_work_len(&ctx->lev[l].g);
+        if (wl > ctx->kwork_len) {
+            DFREE(ctx->kwork);
+            ctx->kwork_len = 0;
+            DALLOC(ctx, ctx->kwork, wl * sizeof(double));
+            ctx->kwork_len = wl;
+        }
+    }
     if (ctx->coarse_dense) {
         ctx->nc = ctx->lev[nlev - 1].ndof;
         /* the inverse plus the 4 nc pivot snapshots of its Gauss-Jordan construction */

@@ -329,13 +329,15 @@ def _grid(dim, nel, h=(0.5, 0.7, 0.9)):
     return g
 
 
-# z-chunk lengths 2 (partial last chunk), 2, 4 (partial), 6, 8, 16; partial 16x16 columns; exact multiples
-@pytest.mark.parametrize("nel", [(17, 9, 5), (16, 16, 2), (33, 20, 7), (100, 36, 50), (64, 64, 130), (96, 96, 96),
-                                 (128, 128, 64), (128, 128, 128)])
+# 16x4x4 tiles (small grids: partial tiles on every axis, a single layer of tiles); column kernel with z-chunks
+# of 2, 4 (partial last chunk), 5 (partial 16x16 columns), 6, 8 and 16
+@pytest.mark.parametrize("nel", [(17, 9, 5), (16, 16, 2), (33, 20, 7), (100, 36, 50), (64, 64, 130), (40, 70, 200),
+                                 (96, 96, 96), (128, 128, 64), (128, 128, 128)])
 def test_column_kernel_against_element_reference(nel):
-    """The streaming column kernel (every epilogue; FP64, FP32 and bfloat16 weights) against the one-element-
-    per-thread reference operator with atomics, on random weights, masks and vectors: the chunked z-streaming,
-    the ring and chunk-boundary workspace and the gather must add up to the same product."""
+    """The 3D operator kernels (the streaming column kernel on large grids, 16x4x4 tiles on small ones; every
+    epilogue; FP64, FP32 and bfloat16 weights) against the one-element-per-thread reference operator with
+    atomics, on random weights, masks and vectors: the chunked z-streaming, the ring and chunk-boundary
+    workspace and the gather must add up to the same product."""
     import ctypes as C
     import earthbyte_underworld_amd as ust  # noqa: F401  (loads the library)
     from earthbyte_underworld_amd import _lib as L
