@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(_PKG, "libust.so")
 
 UST_OK = 0
 PREC_FP64, PREC_MIXED = 0, 1
-SCHUR_UW, SCHUR_LEARNED, SCHUR_SCALED = 0, 1, 2
+SCHUR_UW, SCHUR_LEARNED, SCHUR_SCALED, SCHUR_GKGDIAG = 0, 1, 2, 3
 MAT_K, MAT_G = 0, 1
 VEC_F, VEC_H, VEC_U, VEC_P = 0, 1, 2, 3
 RH_MATERIALVISCOSITY, RH_ARRHENIUS, RH_VISCOSITYLIMITER, RH_VONMISES, RH_DRUCKERPRAGER = 1, 2, 3, 4, 5
@@ -25,7 +25,17 @@ class Rheology(C.Structure):
 
 
 class Material(C.Structure):
-    _fields_ = [("nrh", C.c_int), ("rh", Rheology * 4), ("density", C.c_double)]
+    _fields_ = [("nrh", C.c_int), ("rh", Rheology * 4), ("density", C.c_double), ("alpha", C.c_double),
+                ("ref_temperature", C.c_double), ("beta", C.c_double), ("ref_depth", C.c_double)]
+
+
+class Buoyancy(C.Structure):
+    _fields_ = [("vertical_axis", C.c_int), ("use_direction", C.c_int), ("direction", C.c_double * 3),
+                ("adjust", C.c_int), ("hydrostatic", C.c_int)] + \
+        [(k, C.c_double) for k in ("hs_upper_density", "hs_upper_alpha", "hs_lower_density", "hs_lower_alpha",
+                                   "hs_height", "hs_material_boundary", "hs_T0", "hs_linear_coefficient",
+                                   "hs_exponential_coefficient1", "hs_exponential_coefficient2", "hs_v",
+                                   "hs_width", "hs_time")]
 
 
 class Shape(C.Structure):
@@ -57,7 +67,7 @@ class SolveStats(C.Structure):
                 ("stokes_residual", C.c_double), ("momentum_residual", C.c_double),
                 ("continuity_residual", C.c_double), ("t_assemble_ms", C.c_double), ("t_setup_ms", C.c_double),
                 ("t_solve_ms", C.c_double), ("restarts", C.c_int), ("nullspace_removed", C.c_int),
-                ("nullspace_ratio", C.c_double), ("converged", C.c_int)]
+                ("nullspace_ratio", C.c_double), ("converged", C.c_int), ("hhat_rescaled", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -91,6 +101,7 @@ _SIGS = {
     "ust_write_petsc_matrix": (I, [vp, I, I, C.c_char_p]),
     "ust_write_petsc_vector": (I, [vp, I, I, C.c_char_p]),
     "ust_set_temperature": (I, [vp, vp]),
+    "ust_set_buoyancy": (I, [vp, C.POINTER(Buoyancy)]),
     "ust_set_solution": (I, [vp, vp, vp]),
     "ust_set_previous_solution_exists": (I, [vp, I]),
     "ust_set_bcs": (I, [vp, vp, vp]),
@@ -119,6 +130,7 @@ _SIGS = {
     "ust_uzawa_opts_default": (None, [C.POINTER(UzawaOpts)]),
     "ust_solve_uzawa": (I, [vp, C.POINTER(UzawaOpts), C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ust_get_solution": (I, [vp, vp, vp]),
+    "ust_get_schur_preconditioner": (I, [vp, C.POINTER(SolveOpts), vp]),
     "ust_get_solution_dev": (I, [vp, vp, vp]),
     "ust_velocity_solve": (I, [vp, C.POINTER(SolveOpts), vp, vp, C.POINTER(I)]),
     "ust_picard": (I, [vp, D, D, I, C.POINTER(SolveOpts), C.POINTER(I), C.POINTER(D), C.POINTER(SolveStats)]),
@@ -159,7 +171,7 @@ _SIGS = {
     "ust_k_restrict_cheby_first_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp, C.c_float, vp]),
     "ust_k_prolong_add_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp]),
     "ust_k_prolong_add_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp]),
-    "ust_k_particle_weights": (I, [C.POINTER(Grid), vp, I, vp, vp, vp, vp, vp, vp, vp, I, vp, vp, vp, I, D, vp,
+    "ust_k_particle_weights": (I, [C.POINTER(Grid), vp, I, vp, vp, vp, vp, vp, vp, vp, I, vp, vp, vp, I, vp, vp,
                                    vp, vp, vp, vp, vp]),
     "ust_event_create": (I, [C.POINTER(vp)]),
     "ust_event_destroy": (I, [vp]),

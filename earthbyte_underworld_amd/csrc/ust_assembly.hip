@@ -9,6 +9,7 @@
 //   G_e: GradientStiffnessMatrixTerm.c:220-249 ; f_e: BuoyancyForceTerm.c:370-448 ;
 //   approxS_e: UzawaPreconditionerTerm.c:195-212
 #include <math.h>
+#include <string.h>
 
 #include "ust_common.h"
 
@@ -119,6 +120,70 @@ __device__ double particle_viscosity(const GridDev &g, const ust_material &M, co
 }
 
 // ------------------------------------------------------------------------------------------
+// per-particle density of BuoyancyForceTerm_AssembleElement (BuoyancyForceTerm.c:391-425):
+// rho0 (1 - alpha (T - Tref)), times (1 + beta (depth - depthref)) when beta != 0, less the background
+// density of the HydrostaticTerm (HydrostaticTerm.c:267-330) at the particle's height.  full = 0: rho0.
+// ------------------------------------------------------------------------------------------
+struct BuoyDev {
+    ust_buoyancy b;
+    int full;
+    double top;  // box maximum along the vertical axis
+};
+
+__device__ __forceinline__ double hydrostatic_density(const ust_buoyancy &b, double y) {
+    double thick, boundary;
+    if (b.hs_v != 0) {
+        thick = (b.hs_height - b.hs_material_boundary) * b.hs_width / (b.hs_width + b.hs_v * b.hs_time);
+        boundary = b.hs_material_boundary +
+                   (b.hs_upper_density / b.hs_lower_density) * (b.hs_height - b.hs_material_boundary - thick);
+    } else {
+        thick = b.hs_height - b.hs_material_boundary;
+        boundary = b.hs_material_boundary;
+    }
+    const double height = thick + boundary, h = b.hs_height - y;
+    const double T = h < 0 ? b.hs_T0
+                           : b.hs_T0 + b.hs_linear_coefficient * h +
+                                 b.hs_exponential_coefficient1 * (1 - exp(-b.hs_exponential_coefficient2 * h));
+    double density, alpha;
+    if (y > height) {
+        density = alpha = 0;
+    } else if (y > boundary) {
+        density = b.hs_upper_density;
+        alpha = b.hs_upper_alpha;
+    } else {
+        density = b.hs_lower_density;
+        alpha = b.hs_lower_alpha;
+    }
+    return density * (1 - alpha * T);
+}
+
+template <int DIM, bool NEED_T>
+__device__ double particle_density(const GridDev &g, const BuoyDev &bd, const ust_material &M, const int *nodes,
+                                   const double *xi, const double *T, int e) {
+    if (!bd.full) return M.density;
+    const int idx[3] = {e % g.nx, (e / g.nx) % g.ny, e / (g.nx * g.ny)};
+    const double mn[3] = {g.xmin, g.ymin, g.zmin}, h[3] = {g.hx, g.hy, g.hz};
+    const int va = bd.b.vertical_axis;
+    const double cv = mn[va] + ((double)idx[va] + 0.5 * (xi[va] + 1.0)) * h[va];
+    const double depth = fabs(bd.top - cv);
+    double temperature = 0.0;
+    if (NEED_T) {
+#pragma unroll
+        for (int a = 0; a < (1 << DIM); a++) {
+            double N = 1.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) N *= 0.5 * (1.0 + sgn((a >> d) & 1) * xi[d]);
+            temperature += N * T[nodes[a]];
+        }
+    }
+    double background = 0.0;
+    if (bd.b.hydrostatic) background = hydrostatic_density(bd.b, mn[1] + ((double)idx[1] + 0.5 * (xi[1] + 1.0)) * h[1]);
+    double density = M.density * (1.0 - M.alpha * (temperature - M.ref_temperature));
+    if (M.beta != 0) density *= (1.0 + M.beta * (depth - M.ref_depth));
+    return density - background;
+}
+
+// ------------------------------------------------------------------------------------------
 // particles -> weights.  A wavefront covers 64/S consecutive elements (S chosen near the mean particle
 // count, so few lanes idle); lane k of a segment takes particles k, k+S, ... of its element (coalesced:
 // particles are cell-sorted) and accumulates its 43 (3D) partial sums in registers.  The partials are
@@ -134,7 +199,7 @@ __global__ __launch_bounds__(64 * PW_WAVES) void k_particle_weights(
     GridDev g, const int *__restrict__ cell_off, const double *__restrict__ xi0, const double *__restrict__ xi1,
     const double *__restrict__ xi2, const double *__restrict__ w, const unsigned char *__restrict__ mat,
     const float *__restrict__ coh, const ust_material *__restrict__ mats, int nmat, const double *__restrict__ T,
-    const double *__restrict__ vel, const double *__restrict__ pres, int prev, double *__restrict__ eta_out,
+    const double *__restrict__ vel, const double *__restrict__ pres, int prev, BuoyDev bd, double *__restrict__ eta_out,
     unsigned char *__restrict__ yield_out, double *__restrict__ WK, double *__restrict__ WG,
     double *__restrict__ WF) {
     constexpr int NQ = DIM == 3 ? 27 : 9;
@@ -173,7 +238,7 @@ __global__ __launch_bounds__(64 * PW_WAVES) void k_particle_weights(
             if (NEED_VEL && yield_out && yf >= 0) yield_out[pi] = (unsigned char)yf;
             const double wd = w[pi] * detJ;
             const double et = wd * eta;
-            const double wr = wd * M.density;
+            const double wr = wd * particle_density<DIM, NEED_T>(g, bd, M, nodes, xi, T, e);
             double Lx[3], Ly[3], Lz[3];
             lagrange3(xi[0], Lx);
             lagrange3(xi[1], Ly);
@@ -319,8 +384,12 @@ __global__ void k_element_G(GridDev g, const double *__restrict__ WG, int first,
     for (int v = 0; v < ND; v++) Ge[(size_t)t * ND + v] = ge[v];
 }
 
+struct FDir {
+    double d[3];
+};
+
 template <int DIM>
-__global__ void k_element_f(GridDev g, const double *__restrict__ WF, double gravity, int first, int count,
+__global__ void k_element_f(GridDev g, const double *__restrict__ WF, double gravity, FDir dir, int first, int count,
                             double *__restrict__ fe) {
     constexpr int NV = 1 << DIM, ND = NV * DIM;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -329,7 +398,23 @@ __global__ void k_element_f(GridDev g, const double *__restrict__ WF, double gra
     for (int a = 0; a < NV; a++)
 #pragma unroll
         for (int i = 0; i < DIM; i++)
-            fe[(size_t)t * ND + a * DIM + i] = (i == 1) ? -gravity * WF[(size_t)a * g.nel + first + t] : 0.0;
+            fe[(size_t)t * ND + a * DIM + i] = dir.d[i] != 0.0 ? gravity * dir.d[i] * WF[(size_t)a * g.nel + first + t]
+                                                                : 0.0;
+}
+
+// BuoyancyForceTerm "adjust": WF *= 2^dim detJ / sum_b WG[b] (= 2^dim / the element's sum of weights)
+template <int DIM>
+__global__ void k_adjust_force(GridDev g, const double *__restrict__ WG, double *__restrict__ WF) {
+    constexpr int NV = 1 << DIM;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.nel) return;
+    const double detJ = (DIM == 3) ? 0.125 * g.hx * g.hy * g.hz : 0.25 * g.hx * g.hy;
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < NV; b++) s += WG[(size_t)b * g.nel + e];
+    const double f = s > 0.0 ? (double)NV * detJ / s : 1.0;
+#pragma unroll
+    for (int b = 0; b < NV; b++) WF[(size_t)b * g.nel + e] *= f;
 }
 
 // diag of K_e: sum_q W_q (sum_k GNx_k(a)^2 + GNx_i(a)^2)
@@ -376,15 +461,23 @@ extern "C" int ust_k_particle_weights(const ust_grid *gr, const int *cell_off, i
                                       const unsigned char *mat, const float *coh_scale,
                                       const ust_material *materials_dev, int nmat, const double *temperature,
                                       const double *velocity, const double *pressure, int prev_solution,
-                                      double gravity, double *eta_out, unsigned char *yield_out, double *WK,
-                                      double *WG, double *WF, void *stream) {
-    (void)gravity;
+                                      const ust_buoyancy *buoyancy, double *eta_out, unsigned char *yield_out,
+                                      double *WK, double *WG, double *WF, void *stream) {
     if (!gr || !cell_off || !xi0 || !xi1 || !w || !materials_dev || !WK || !WG || !WF) return UST_EINVAL;
     if (gr->dim == 3 && !xi2) return UST_EINVAL;
     if (nmat < 1 || nmat > UST_MAX_MATERIALS) return UST_EINVAL;
     GridDev g = grid_dev(gr);
     if (g.nel <= 0) return UST_OK;
-    // temperature / velocity lookups only when a rheology reads them (the caller passes NULL otherwise)
+    BuoyDev bd;
+    memset(&bd, 0, sizeof(bd));
+    if (buoyancy) {
+        if (buoyancy->vertical_axis < 0 || buoyancy->vertical_axis >= gr->dim) return UST_EINVAL;
+        bd.b = *buoyancy;
+        bd.full = 1;
+        bd.top = gr->xmax[buoyancy->vertical_axis];
+    }
+    // temperature / velocity lookups only when a rheology or the density reads them (the caller passes NULL
+    // otherwise)
     const bool needT = temperature != nullptr, needV = velocity != nullptr && pressure != nullptr;
     // segment width: the smallest candidate >= the caller's particles-per-cell hint (the mean count; a cell
     // with more particles than S loops), so few lanes idle; segments never straddle a wavefront
@@ -401,7 +494,7 @@ extern "C" int ust_k_particle_weights(const ust_grid *gr, const int *cell_off, i
 #define LAUNCH_PW(D, SS, NT, NVL)                                                                                 \
     hipLaunchKernelGGL((k_particle_weights<D, SS, NT, NVL>), dim3(grid), dim3(64 * PW_WAVES), 0, st, g, cell_off, \
                        xi0, xi1, xi2, w, mat, coh_scale, materials_dev, nmat, temperature, velocity, pressure,     \
-                       prev_solution, eta_out, yield_out, WK, WG, WF)
+                       prev_solution, bd, eta_out, yield_out, WK, WG, WF)
 #define LAUNCH_S(D, NT, NVL)                                  \
     do {                                                      \
         switch (S) {                                          \
@@ -461,17 +554,38 @@ extern "C" int ust_k_element_G(const ust_grid *gr, const double *WG, int first, 
     return UST_OK;
 }
 
-extern "C" int ust_k_element_f(const ust_grid *gr, const double *WF, double gravity, int first, int count,
-                               double *fe, void *stream) {
+extern "C" int ust_k_element_f_dir(const ust_grid *gr, const double *WF, double gravity, const double *dir,
+                                   int first, int count, double *fe, void *stream) {
     GridDev g = grid_dev(gr);
-    if (first < 0 || count < 0 || first + count > g.nel || !WF || !fe) return UST_EINVAL;
+    if (first < 0 || count < 0 || first + count > g.nel || !WF || !fe || !dir) return UST_EINVAL;
     if (count == 0) return UST_OK;
+    FDir d = {{dir[0], dir[1], g.dim == 3 ? dir[2] : 0.0}};
     if (g.dim == 3)
         hipLaunchKernelGGL(k_element_f<3>, dim3(ust_grid_size(count, UST_BLOCK)), dim3(UST_BLOCK), 0,
-                           ust_stream(stream), g, WF, gravity, first, count, fe);
+                           ust_stream(stream), g, WF, gravity, d, first, count, fe);
     else
         hipLaunchKernelGGL(k_element_f<2>, dim3(ust_grid_size(count, UST_BLOCK)), dim3(UST_BLOCK), 0,
-                           ust_stream(stream), g, WF, gravity, first, count, fe);
+                           ust_stream(stream), g, WF, gravity, d, first, count, fe);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+extern "C" int ust_k_element_f(const ust_grid *gr, const double *WF, double gravity, int first, int count,
+                               double *fe, void *stream) {
+    const double down[3] = {0.0, -1.0, 0.0};
+    return ust_k_element_f_dir(gr, WF, gravity, down, first, count, fe, stream);
+}
+
+extern "C" int ust_k_adjust_force_weights(const ust_grid *gr, const double *WG, double *WF, void *stream) {
+    GridDev g = grid_dev(gr);
+    if (!WG || !WF) return UST_EINVAL;
+    if (g.nel == 0) return UST_OK;
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_adjust_force<3>, dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           ust_stream(stream), g, WG, WF);
+    else
+        hipLaunchKernelGGL(k_adjust_force<2>, dim3(ust_grid_size(g.nel, UST_BLOCK)), dim3(UST_BLOCK), 0,
+                           ust_stream(stream), g, WG, WF);
     UST_LAUNCH_CHECK();
     return UST_OK;
 }

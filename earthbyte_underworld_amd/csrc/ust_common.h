@@ -17,6 +17,8 @@
     } while (0)
 
 void ust_set_hip_error(hipError_t e, const char *file, int line);
+// an error message for ust_dev_error_string() from a check that is not a HIP call
+void ust_set_error_text(const char *text);
 
 static inline unsigned ust_grid_size(size_t n, unsigned block) {
     size_t g = (n + block - 1) / block;

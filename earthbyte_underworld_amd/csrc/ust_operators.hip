@@ -128,7 +128,7 @@ __device__ __forceinline__ double g_entry(const GridDev &g, const double *wv, in
 
 // one thread per node: y[n, i] = sum over the <= 2^DIM elements around n of G_e[a, i] p_e, a being the
 // node's vertex index in e.  Every dof is written once (0 on Dirichlet dofs) in a fixed order.
-template <int DIM>
+template <int DIM, bool ABS>
 __global__ __launch_bounds__(256) void k_apply_G(GridDev g, const double *__restrict__ WG,
                                                  const unsigned char *__restrict__ mask, const double *__restrict__ p,
                                                  double *__restrict__ y) {
@@ -145,12 +145,12 @@ __global__ __launch_bounds__(256) void k_apply_G(GridDev g, const double *__rest
         const int ex = gx - (a & 1), ey = gy - ((a >> 1) & 1), ez = DIM == 3 ? gz - (a >> 2) : 0;
         if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
         const size_t e = (size_t)ex + (size_t)g.nx * ((size_t)ey + (size_t)g.ny * ez);
-        const double pe = p[e], wa = WG[(size_t)a * g.nel + e];
+        const double pe = ABS ? 1.0 : p[e], wa = WG[(size_t)a * g.nel + e];
 #pragma unroll
         for (int i = 0; i < DIM; i++) {
             const double h = i == 0 ? g.hx : (i == 1 ? g.hy : g.hz);
             const double gai = -(double)sgn((a >> i) & 1) / h * (wa + WG[(size_t)(a ^ (1 << i)) * g.nel + e]);
-            acc[i] += gai * pe;
+            acc[i] += ABS ? fabs(gai) : gai * pe;
         }
     }
 #pragma unroll
@@ -182,6 +182,11 @@ __global__ __launch_bounds__(256) void k_apply_Gt(GridDev g, const double *__res
             const bool bc = mask[dof] != 0;
             if (MODE == 2) {
                 if (!bc) s += fabs(g_entry<DIM>(g, wv, a, i));
+            } else if (MODE == 3) {
+                if (!bc) {
+                    const double gi = g_entry<DIM>(g, wv, a, i);
+                    s += gi * gi * u[dof];
+                }
             } else if (MODE == 0 ? !bc : bc) {
                 s += g_entry<DIM>(g, wv, a, i) * u[dof];
             }
@@ -190,16 +195,19 @@ __global__ __launch_bounds__(256) void k_apply_Gt(GridDev g, const double *__res
     else q[e] = s;
 }
 
-// one thread per node: f[n, y] += -g sum over the elements around n of WF[a][e]  (free dofs only)
+struct Dir3 {
+    double d[3];
+};
+
+// one thread per node: f[n, c] += g dir[c] sum over the elements around n of WF[a][e]  (free dofs only; dir is
+// -1 on the vertical axis, or the gravity direction gHat)
 template <int DIM>
-__global__ void k_force(GridDev g, const double *__restrict__ WF, double gravity,
+__global__ void k_force(GridDev g, const double *__restrict__ WF, double gravity, Dir3 dir,
                         const unsigned char *__restrict__ mask, double *__restrict__ f) {
     constexpr int NV = 1 << DIM;
     const size_t nnodes = (size_t)g.nnx * g.nny * g.nnz;
     const size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (n >= nnodes) return;
-    const size_t dof = n * DIM + 1;
-    if (mask[dof]) return;
     const int gx = (int)(n % g.nnx), gy = (int)((n / g.nnx) % g.nny), gz = (int)(n / ((size_t)g.nnx * g.nny));
     double acc = 0.0;
 #pragma unroll
@@ -208,7 +216,11 @@ __global__ void k_force(GridDev g, const double *__restrict__ WF, double gravity
         if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
         acc += WF[(size_t)a * g.nel + (size_t)ex + (size_t)g.nx * ((size_t)ey + (size_t)g.ny * ez)];
     }
-    f[dof] += -gravity * acc;
+#pragma unroll
+    for (int c = 0; c < DIM; c++) {
+        const size_t dof = n * DIM + c;
+        if (dir.d[c] != 0.0 && !mask[dof]) f[dof] += gravity * dir.d[c] * acc;
+    }
 }
 
 template <int DIM>
@@ -560,11 +572,25 @@ int ust_k_apply_G(const ust_grid *gr, const double *WG, const unsigned char *mas
     hipStream_t st = ust_stream(stream);
     const size_t nnodes = ndof / g.dim;
     if (g.dim == 3)
-        hipLaunchKernelGGL(k_apply_G<3>, dim3(ust_grid_size(nnodes, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WG, mask,
+        hipLaunchKernelGGL((k_apply_G<3, false>), dim3(ust_grid_size(nnodes, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WG, mask,
                            p, y);
     else
-        hipLaunchKernelGGL(k_apply_G<2>, dim3(ust_grid_size(nnodes, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WG, mask,
+        hipLaunchKernelGGL((k_apply_G<2, false>), dim3(ust_grid_size(nnodes, UST_BLOCK)), dim3(UST_BLOCK), 0, st, g, WG, mask,
                            p, y);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+int ust_k_abs_G_rowsum(const ust_grid *gr, const double *WG, const unsigned char *mask, double *y, void *stream) {
+    GridDev g = grid_dev(gr);
+    if (!WG || !mask || !y) return UST_EINVAL;
+    const unsigned grid = ust_grid_size((size_t)g.nnx * g.nny * g.nnz, UST_BLOCK);
+    if (g.dim == 3)
+        hipLaunchKernelGGL((k_apply_G<3, true>), dim3(grid), dim3(UST_BLOCK), 0, ust_stream(stream), g, WG, mask,
+                           (const double *)nullptr, y);
+    else
+        hipLaunchKernelGGL((k_apply_G<2, true>), dim3(grid), dim3(UST_BLOCK), 0, ust_stream(stream), g, WG, mask,
+                           (const double *)nullptr, y);
     UST_LAUNCH_CHECK();
     return UST_OK;
 }
@@ -578,27 +604,36 @@ int ust_k_apply_Gt(const ust_grid *gr, const double *WG, const unsigned char *ma
     if (g.dim == 3) {
         if (bc_only == 1) hipLaunchKernelGGL((k_apply_Gt<3, 1>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
         else if (bc_only == 2) hipLaunchKernelGGL((k_apply_Gt<3, 2>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
+        else if (bc_only == 3) hipLaunchKernelGGL((k_apply_Gt<3, 3>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
         else hipLaunchKernelGGL((k_apply_Gt<3, 0>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
     } else {
         if (bc_only == 1) hipLaunchKernelGGL((k_apply_Gt<2, 1>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
         else if (bc_only == 2) hipLaunchKernelGGL((k_apply_Gt<2, 2>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
+        else if (bc_only == 3) hipLaunchKernelGGL((k_apply_Gt<2, 3>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
         else hipLaunchKernelGGL((k_apply_Gt<2, 0>), dim3(grid), dim3(UST_BLOCK), 0, st, g, WG, mask, u, q);
     }
     UST_LAUNCH_CHECK();
     return UST_OK;
 }
 
-int ust_k_force(const ust_grid *gr, const double *WF, double gravity, const unsigned char *mask, double *f,
-                void *stream) {
+int ust_k_force_dir(const ust_grid *gr, const double *WF, double gravity, const double *dir,
+                    const unsigned char *mask, double *f, void *stream) {
     GridDev g = grid_dev(gr);
-    if (!WF || !mask || !f) return UST_EINVAL;
+    if (!WF || !mask || !f || !dir) return UST_EINVAL;
+    Dir3 d = {{dir[0], dir[1], g.dim == 3 ? dir[2] : 0.0}};
     const unsigned grid = ust_grid_size((size_t)g.nnx * g.nny * g.nnz, UST_BLOCK);
     if (g.dim == 3)
-        hipLaunchKernelGGL(k_force<3>, dim3(grid), dim3(UST_BLOCK), 0, ust_stream(stream), g, WF, gravity, mask, f);
+        hipLaunchKernelGGL(k_force<3>, dim3(grid), dim3(UST_BLOCK), 0, ust_stream(stream), g, WF, gravity, d, mask, f);
     else
-        hipLaunchKernelGGL(k_force<2>, dim3(grid), dim3(UST_BLOCK), 0, ust_stream(stream), g, WF, gravity, mask, f);
+        hipLaunchKernelGGL(k_force<2>, dim3(grid), dim3(UST_BLOCK), 0, ust_stream(stream), g, WF, gravity, d, mask, f);
     UST_LAUNCH_CHECK();
     return UST_OK;
+}
+
+int ust_k_force(const ust_grid *gr, const double *WF, double gravity, const unsigned char *mask, double *f,
+                void *stream) {
+    const double down[3] = {0.0, -1.0, 0.0};
+    return ust_k_force_dir(gr, WF, gravity, down, mask, f, stream);
 }
 
 int ust_k_inv_diag(const ust_grid *gr, const double *WK, const unsigned char *mask, double *invd64, float *invd32,

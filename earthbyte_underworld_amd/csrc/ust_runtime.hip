@@ -10,6 +10,8 @@ void ust_set_hip_error(hipError_t e, const char *file, int line) {
     snprintf(g_err, sizeof(g_err), "%s (%s:%d)", hipGetErrorString(e), file, line);
 }
 
+void ust_set_error_text(const char *text) { snprintf(g_err, sizeof(g_err), "%s", text); }
+
 #define HIPCALL(x)                                         \
     do {                                                   \
         hipError_t _e = (x);                               \

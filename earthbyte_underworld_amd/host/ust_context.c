@@ -48,6 +48,7 @@ int ust_create(ust_context **out, int device) {
     ust_context *ctx = (ust_context *)calloc(1, sizeof(ust_context));
     if (!ctx) return UST_ENOMEM;
     ctx->device = device;
+    ctx->buoy.vertical_axis = 1;
     /* UST_NO_GRAPH=1 launches the inner Krylov iteration kernel by kernel instead of as a captured graph
      * (same kernels and arguments; for tools that do not follow graph launches) */
     const char *ng = getenv("UST_NO_GRAPH");
@@ -354,6 +355,31 @@ int ust_swarm_upload(ust_context *ctx, int np, const int *cell_off, const double
     return rc;
 }
 
+void ust_buoyancy_direction(const ust_context *ctx, double dir[3]) {
+    for (int d = 0; d < 3; d++) dir[d] = 0.0;
+    if (ctx->buoy.use_direction) {
+        for (int d = 0; d < 3; d++) dir[d] = ctx->buoy.direction[d];
+    } else {
+        dir[ctx->buoy.vertical_axis] = -1.0;
+    }
+}
+
+int ust_set_buoyancy(ust_context *ctx, const ust_buoyancy *b) {
+    if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "no mesh");
+    if (!b) {
+        memset(&ctx->buoy, 0, sizeof(ctx->buoy));
+        ctx->buoy.vertical_axis = 1;
+    } else {
+        if (b->vertical_axis < 0 || b->vertical_axis >= ctx->grid.dim)
+            UST_FAIL(ctx, UST_EINVAL, "VerticalAxis %d outside 0..%d", b->vertical_axis, ctx->grid.dim - 1);
+        if (b->hydrostatic && b->hs_v != 0 && !(b->hs_lower_density != 0.0 && b->hs_width + b->hs_v * b->hs_time != 0.0))
+            UST_FAIL(ctx, UST_EINVAL, "HydrostaticTerm: a moving boundary needs a lower density and width + v t != 0");
+        ctx->buoy = *b;
+    }
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
 int ust_set_temperature(ust_context *ctx, const double *T) {
     if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "no mesh");
     CKD(ctx, ust_memcpy_h2d(ctx->T, T, (size_t)ctx->nnodes * sizeof(double), ctx->stream));
@@ -401,26 +427,34 @@ int ust_assemble(ust_context *ctx, double gravity) {
     }
     size_t nd = (size_t)ctx->ndof, ne = (size_t)ctx->nel;
     /* the temperature and strain-rate/pressure lookups are compiled into the particle kernel only when a
-     * material's rheology chain reads them */
-    int needT = 0, needV = 0;
-    for (int m = 0; m < ctx->nmat; m++)
+     * material's rheology chain or density reads them; the density law runs only when a material has a
+     * thermal or depth coefficient or a hydrostatic background is set (else rho = the material density) */
+    int needT = 0, needV = 0, fullDensity = ctx->buoy.hydrostatic != 0;
+    for (int m = 0; m < ctx->nmat; m++) {
         for (int r = 0; r < ctx->mats_h[m].nrh; r++) {
             int t = ctx->mats_h[m].rh[r].type;
             if (t == UST_RH_ARRHENIUS) needT = 1;
             if (t == UST_RH_VONMISES || t == UST_RH_DRUCKERPRAGER) needV = 1;
         }
+        if (ctx->mats_h[m].alpha != 0.0) needT = fullDensity = 1;
+        if (ctx->mats_h[m].beta != 0.0) fullDensity = 1;
+    }
     const int ppc_mean = (ctx->np + ctx->nel - 1) / ctx->nel;
     /* the integration points: the material points themselves, or the points a weights calculator moved */
     const int sep = ctx->ixi0 != NULL;
     const double *q0 = sep ? ctx->ixi0 : ctx->xi0, *q1 = sep ? ctx->ixi1 : ctx->xi1, *q2 = sep ? ctx->ixi2 : ctx->xi2;
     CKD(ctx, ust_k_particle_weights(&ctx->grid, ctx->cell_off, ppc_mean, q0, q1, q2, ctx->w,
                                     ctx->mat, ctx->coh, ctx->mats_d, ctx->nmat, needT ? ctx->T : NULL,
-                                    needV ? ctx->u : NULL, needV ? ctx->p : NULL, ctx->prev_solution, gravity,
-                                    ctx->eta, ctx->yielded, ctx->WK, ctx->WG, ctx->WF, ctx->stream));
+                                    needV ? ctx->u : NULL, needV ? ctx->p : NULL, ctx->prev_solution,
+                                    fullDensity ? &ctx->buoy : NULL, ctx->eta, ctx->yielded, ctx->WK, ctx->WG,
+                                    ctx->WF, ctx->stream));
+    if (ctx->buoy.adjust) CKD(ctx, ust_k_adjust_force_weights(&ctx->grid, ctx->WG, ctx->WF, ctx->stream));
     CKD(ctx, ust_k_uzawa_diag(&ctx->grid, ctx->WK, ctx->WG, ctx->approxS, ctx->stream));
     /* f = buoyancy - K_{free,bc} u_bc ; h = -G_bc^T u_bc */
     CKD(ctx, ust_k_fill_f64(ctx->f, 0.0, nd, ctx->stream));
-    CKD(ctx, ust_k_force(&ctx->grid, ctx->WF, gravity, ctx->mask_d, ctx->f, ctx->stream));
+    double dir[3];
+    ust_buoyancy_direction(ctx, dir);
+    CKD(ctx, ust_k_force_dir(&ctx->grid, ctx->WF, gravity, dir, ctx->mask_d, ctx->f, ctx->stream));
     CKD(ctx, ust_k_fill_f64(ctx->h, 0.0, ne, ctx->stream));
     if (ctx->bc_nonzero) { /* homogeneous Dirichlet values lift nothing */
         CKD(ctx, ust_k_lift_K(&ctx->grid, ctx->WK, ctx->mask_d, ctx->bcval_d, ctx->f, ctx->stream));
@@ -494,7 +528,11 @@ static int get_elem_vec(ust_context *ctx, int first, int count, double *out, int
     double *buf = NULL;
     CKD(ctx, ust_dev_malloc((void **)&buf, (size_t)count * nd * sizeof(double)));
     if (which == 0) rc = ust_k_element_G(&ctx->grid, ctx->WG, first, count, buf, ctx->stream);
-    else rc = ust_k_element_f(&ctx->grid, ctx->WF, ctx->gravity, first, count, buf, ctx->stream);
+    else {
+        double dir[3];
+        ust_buoyancy_direction(ctx, dir);
+        rc = ust_k_element_f_dir(&ctx->grid, ctx->WF, ctx->gravity, dir, first, count, buf, ctx->stream);
+    }
     if (!rc) rc = ust_memcpy_d2h(out, buf, (size_t)count * nd * sizeof(double), ctx->stream);
     ust_dev_free(buf);
     if (rc) UST_FAIL(ctx, rc, "element vectors: %s", ust_dev_error_string());

@@ -85,6 +85,7 @@ struct ust_context {
     /* assembly */
     int assembled;
     double gravity;
+    ust_buoyancy buoy;   /* BuoyancyForceTerm options (ust_set_buoyancy; default: VerticalAxis 1, nothing else) */
     double *WK, *WG, *WF;
     double *approxS;
     double *f; /* nodal RHS after lifting, zero on BC dofs */
@@ -141,6 +142,8 @@ int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32);
 void ust_levels_free(ust_context *ctx);
 void ust_graphs_reset(ust_context *ctx);
 int ust_read_scalar(ust_context *ctx, int slot, double *v);
+/* the unit direction of the body force: gHat when given, else -1 on the vertical axis */
+void ust_buoyancy_direction(const ust_context *ctx, double dir[3]);
 int ust_mlp_schur_pc(ust_context *ctx, double *pc_dev);
 void ust_mlp_free(ust_context *ctx);
 int ust_dev_alloc_zero(ust_context *ctx, void **p, size_t bytes);

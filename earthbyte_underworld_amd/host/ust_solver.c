@@ -653,6 +653,38 @@ static int back_solve(ust_context *ctx, const ust_solve_opts *o, int ns, double 
     return stokes_residual(ctx, st);
 }
 
+/* the diagonal Schur preconditioner of opts->schur_pc: approxS ("uw"), approxS scaled by the network or the
+ * user's factors, or diag(G^T diag(K)^-1 G) ("gkgdiag", from the assembled diagonal of the finest level) */
+static int schur_pc_vector(ust_context *ctx, const ust_solve_opts *o, const double **pc) {
+    int rc;
+    *pc = ctx->approxS;
+    if (o->schur_pc == UST_SCHUR_UW) return UST_OK;
+    if (o->schur_pc == UST_SCHUR_LEARNED) {
+        if ((rc = ust_mlp_schur_pc(ctx, ctx->ppc))) return rc;
+    } else if (o->schur_pc == UST_SCHUR_SCALED) {
+        if (!ctx->user_scale) UST_FAIL(ctx, UST_ESTATE, "UST_SCHUR_SCALED needs ust_set_schur_scaling");
+        CKD(ctx, ust_k_apply_scaling(ctx->approxS, ctx->user_scale, ctx->ppc, ctx->nel, S(ctx)));
+    } else if (o->schur_pc == UST_SCHUR_GKGDIAG) {
+        CKD(ctx, ust_k_apply_Gt(&ctx->grid, ctx->WG, ctx->mask_d, ctx->lev[0].invd64, ctx->ppc, 3, S(ctx)));
+    } else {
+        UST_FAIL(ctx, UST_EINVAL, "unknown Schur preconditioner %d", o->schur_pc);
+    }
+    *pc = ctx->ppc;
+    return UST_OK;
+}
+
+int ust_get_schur_preconditioner(ust_context *ctx, const ust_solve_opts *opts, double *pc_host) {
+    ust_solve_opts o;
+    if (opts) o = *opts;
+    else ust_solve_opts_default(&o);
+    int rc;
+    if ((rc = ust_solver_prepare(ctx, &o))) return rc;
+    const double *pc;
+    if ((rc = schur_pc_vector(ctx, &o, &pc))) return rc;
+    CKD(ctx, ust_memcpy_d2h(pc_host, pc, (size_t)ctx->nel * sizeof(double), S(ctx)));
+    return UST_OK;
+}
+
 int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st) {
     ust_solve_opts o;
     if (opts) o = *opts;
@@ -678,19 +710,33 @@ int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st)
     double hh;
     CKD(ctx, dotv(ctx, ctx->ph, ctx->ph, ne, SC_T0, &hh));
     double hn = sqrt(hh) > 0 ? sqrt(hh) : 1.0;
+    /* h_hat tiny against ||G||_inf: the Schur solve runs on h_hat / hnorm and the pressure is scaled back
+     * (auglag-driver-DGTGD.c:333-340, 356-358) */
+    double hscale = 1.0;
+    {
+        double gnorm;
+        CKD(ctx, ust_k_abs_G_rowsum(&ctx->grid, ctx->WG, ctx->mask_d, ctx->tmpv, S(ctx)));
+        CKD(ctx, ust_k_absmax_f64(ctx->tmpv, n, ctx->partial, ctx->scal + SC_T0, S(ctx)));
+        CKD(ctx, ust_read_scalar(ctx, SC_T0, &gnorm));
+        const double hnorm = gnorm > 0.0 ? sqrt(hh) / gnorm : 0.0;
+        if (hnorm < 1e-6 && hnorm > 1e-20) {
+            hscale = hnorm;
+            CKD(ctx, ust_k_axpby_f64(0.0, ctx->ph, 1.0 / hscale, ctx->ph, ne, S(ctx)));
+            hn /= hscale;
+            st->hhat_rescaled = 1;
+        }
+    }
     CKD(ctx, ust_k_fill_f64(ctx->p, 0.0, ne, S(ctx)));
     CKD(ctx, ust_memcpy_d2d(ctx->pr, ctx->ph, ne * sizeof(double), S(ctx)));
     int stalled = 0, conv = 0;
     const double *pc = ctx->approxS;
     const double *pc_last = ctx->approxS; /* the preconditioner the Schur CG finished with */
-    if (o.schur_pc == UST_SCHUR_LEARNED || o.schur_pc == UST_SCHUR_SCALED) {
-        if (o.schur_pc == UST_SCHUR_LEARNED) {
-            if ((rc = ust_mlp_schur_pc(ctx, ctx->ppc))) return rc;
-        } else {
-            if (!ctx->user_scale) UST_FAIL(ctx, UST_ESTATE, "UST_SCHUR_SCALED needs ust_set_schur_scaling");
-            CKD(ctx, ust_k_apply_scaling(ctx->approxS, ctx->user_scale, ctx->ppc, ctx->nel, S(ctx)));
-        }
-        pc = ctx->ppc;
+    if (o.schur_pc == UST_SCHUR_GKGDIAG) {
+        if ((rc = schur_pc_vector(ctx, &o, &pc))) return rc;
+        pc_last = pc;
+        if ((rc = schur_cg(ctx, &o, ns, pc, 0, hn, st, &stalled, &conv))) return rc;
+    } else if (o.schur_pc == UST_SCHUR_LEARNED || o.schur_pc == UST_SCHUR_SCALED) {
+        if ((rc = schur_pc_vector(ctx, &o, &pc))) return rc;
         pc_last = pc;
         if ((rc = schur_cg(ctx, &o, ns, pc, o.stall_window, hn, st, &stalled, &conv))) return rc;
         if (stalled) {
@@ -703,6 +749,11 @@ int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *st)
         }
     } else {
         if ((rc = schur_cg(ctx, &o, ns, pc, 0, hn, st, &stalled, &conv))) return rc;
+    }
+    if (hscale != 1.0) {
+        CKD(ctx, ust_k_axpby_f64(0.0, ctx->ph, hscale, ctx->ph, ne, S(ctx)));
+        CKD(ctx, ust_k_axpby_f64(0.0, ctx->p, hscale, ctx->p, ne, S(ctx)));
+        hn *= hscale;
     }
     double bs_scale = 0.3;
     if ((rc = back_solve(ctx, &o, ns, bs_scale, st))) return rc;

@@ -57,7 +57,9 @@ def DruckerPrager(**keys):
     return _rh("DruckerPrager", **keys)
 
 
-def RheologyMaterial(rheologies, density=0.0):
+def RheologyMaterial(rheologies, density=0.0, alpha=0.0, referenceTemperature=0.0, beta=0.0, referenceDepth=0.0):
+    """A RheologyMaterial with its BuoyancyForceTerm material keys (density, alpha, referenceTemperature, beta,
+    referenceDepth)."""
     m = L.Material()
     if not 1 <= len(rheologies) <= 4:
         raise UstError("a RheologyMaterial holds 1..4 rheologies")
@@ -65,7 +67,27 @@ def RheologyMaterial(rheologies, density=0.0):
     for i, r in enumerate(rheologies):
         m.rh[i] = r
     m.density = float(density)
+    m.alpha, m.ref_temperature = float(alpha), float(referenceTemperature)
+    m.beta, m.ref_depth = float(beta), float(referenceDepth)
     return m
+
+
+def BuoyancyForceTerm(VerticalAxis=1, gravityDirection=None, adjust=False, HydrostaticTerm=None):
+    """BuoyancyForceTerm component keys; HydrostaticTerm: a dict of its keys (upper_density, upper_alpha,
+    lower_density, lower_alpha, height, material_boundary, T0, linear_coefficient, exponential_coefficient1,
+    exponential_coefficient2, v, width) plus time."""
+    b = L.Buoyancy()
+    b.vertical_axis = int(VerticalAxis)
+    if gravityDirection is not None:
+        b.use_direction = 1
+        for d, v in enumerate(gravityDirection):
+            b.direction[d] = float(v)
+    b.adjust = int(bool(adjust))
+    if HydrostaticTerm:
+        b.hydrostatic = 1
+        for k, v in HydrostaticTerm.items():
+            setattr(b, "hs_" + k, float(v))
+    return b
 
 
 def shape(kind, material, *params):
@@ -215,6 +237,10 @@ class StokesSystem:
         self._check(self.lib.ust_swarm_download_integration_points(self.ctx, _ptr(xi), _ptr(w)))
         return xi, w
 
+    def set_buoyancy(self, term=None):
+        """BuoyancyForceTerm options (None: the defaults)."""
+        self._check(self.lib.ust_set_buoyancy(self.ctx, None if term is None else C.byref(term)))
+
     def set_temperature(self, T):
         T = np.ascontiguousarray(T, dtype=np.float64)
         assert T.shape[0] == self.nnodes
@@ -320,6 +346,13 @@ class StokesSystem:
         st = L.SolveStats()
         self._check(self.lib.ust_solve(self.ctx, C.byref(o), C.byref(st)))
         return st.as_dict()
+
+    def schur_preconditioner(self, opts=None, **kw):
+        """The diagonal Schur preconditioner ust_solve would use with these options (one value per element)."""
+        o = opts if opts is not None else default_opts(**kw)
+        pc = np.zeros(self.nel)
+        self._check(self.lib.ust_get_schur_preconditioner(self.ctx, C.byref(o), _ptr(pc)))
+        return pc
 
     def solve_uzawa(self, tolerance=1e-5, maxIterations=1000, minIterations=1, useAbsoluteTolerance=False,
                     preconditioner=True, opts=None, **kw):

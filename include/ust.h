@@ -24,6 +24,8 @@ typedef struct ust_context ust_context;
 #define UST_SCHUR_UW 0       /* approxS = g^T diag(K)^-1 g per element ("-Q22_pc_type uw") */
 #define UST_SCHUR_LEARNED 1  /* approxS scaled by the MLP; plain "uw" remains the fallback */
 #define UST_SCHUR_SCALED 2   /* approxS scaled by 10^s, s set by ust_set_schur_scaling; plain "uw" the fallback */
+#define UST_SCHUR_GKGDIAG 3  /* diag(G^T diag(K)^-1 G) with the assembled diagonal of K ("-Q22_pc_type gkgdiag",
+                                BSSCR_FormSchurApproximationDiag, preconditioner.c:268-310) */
 
 typedef struct {
     int precision;           /* UST_PREC_* */
@@ -69,6 +71,8 @@ typedef struct {
     double nullspace_ratio;  /* ||G 1|| / ||1|| / ||D||_inf of the assembly (removed when < 1e-7) */
     int converged;           /* 1: the stopping test held (stokes_rtol when set, else outer_rtol); 0: max_outer
                                 ended the Schur CG first (the solve still returns UST_OK, as KSP at its limit) */
+    int hhat_rescaled;       /* ||h_hat|| / ||G||_inf was in (1e-20, 1e-6): the Schur solve ran on h_hat scaled by
+                                its inverse and the pressure was scaled back (auglag-driver-DGTGD.c:333-358) */
 } ust_solve_stats;
 
 void ust_solve_opts_default(ust_solve_opts *o);
@@ -138,6 +142,11 @@ int ust_swarm_download_integration_points(ust_context *ctx, double *xi, double *
 /* per-particle cohesion factors (np floats; 1 where none were set) */
 int ust_get_cohesion_scale(ust_context *ctx, float *coh);
 
+/* ---- BuoyancyForceTerm options (BuoyancyForceTerm.c:139-160; material keys density, alpha,
+ * referenceTemperature, beta, referenceDepth are ust_material fields): VerticalAxis, gravityDirection,
+ * adjust, HydrostaticTerm.  NULL restores the defaults (VerticalAxis 1, force along -y). ---- */
+int ust_set_buoyancy(ust_context *ctx, const ust_buoyancy *b);
+
 /* ---- fields read by the rheologies (PpcManager lookups, PpcManager.c:503-606) ---- */
 int ust_set_temperature(ust_context *ctx, const double *T_nodal);
 /* velocity (nodal, n_nodes*dim, BC values included) and pressure (per element) used by yield */
@@ -177,6 +186,9 @@ int ust_apply_Gt(ust_context *ctx, const double *u, double *q);
 /* ---- BSSCR solve (KSPSolve_BSSCR BSSCR.c:151-229 / BSSCR_DRIVER_auglag auglag-driver-DGTGD.c:53-441) ---- */
 int ust_solve(ust_context *ctx, const ust_solve_opts *opts, ust_solve_stats *stats);
 int ust_get_solution(ust_context *ctx, double *u_nodal, double *p);
+/* the diagonal Schur preconditioner of kind schur_pc (UST_SCHUR_*) that ust_solve with opts would use (nel
+ * values, host; the learned and scaled kinds need their network / scaling set) */
+int ust_get_schur_preconditioner(ust_context *ctx, const ust_solve_opts *opts, double *pc);
 int ust_get_solution_dev(ust_context *ctx, const double **u_nodal_dev, const double **p_dev);
 /* ---- PETSc binary dumps (-matdumpdir / writeMatVec.c role, auglag-driver-DGTGD.c:101-113) ----
  * Big-endian PETSc binary AIJ matrices and vectors, the format of the reference's .matrix fixtures.  K and G

@@ -48,8 +48,25 @@ typedef struct {
 typedef struct {
     int nrh;
     ust_rheology rh[UST_MAX_RHEOLOGIES];
-    double density;
+    /* BuoyancyForceTerm material extension (BuoyancyForceTerm.c:118-122): "density", "alpha",
+     * "referenceTemperature", "beta", "referenceDepth" (all 0 by default) */
+    double density, alpha, ref_temperature, beta, ref_depth;
 } ust_material;
+
+/* BuoyancyForceTerm component keys (BuoyancyForceTerm.c:139-160): "VerticalAxis" (default 1),
+ * "gravityDirection" (gHat, used when use_direction), "adjust", and the optional "HydrostaticTerm" whose
+ * background density (HydrostaticTerm.c:267-330) is subtracted from every particle's density (its keys, and
+ * hs_time for the context's current time, which moves the layer boundaries when hs_v != 0). */
+typedef struct {
+    int vertical_axis;
+    int use_direction;
+    double direction[3];
+    int adjust;
+    int hydrostatic;
+    double hs_upper_density, hs_upper_alpha, hs_lower_density, hs_lower_alpha, hs_height, hs_material_boundary;
+    double hs_T0, hs_linear_coefficient, hs_exponential_coefficient1, hs_exponential_coefficient2;
+    double hs_v, hs_width, hs_time;
+} ust_buoyancy;
 
 /* Geometry of one (multigrid) level of the Cartesian mesh. */
 typedef struct {
@@ -135,8 +152,11 @@ int ust_k_particle_weights(const ust_grid *g, const int *cell_off, int ppc_hint,
                            const unsigned char *mat, const float *coh_scale,
                            const ust_material *materials_dev, int nmat, const double *temperature,
                            const double *velocity, const double *pressure, int prev_solution,
-                           double gravity, double *eta_out, unsigned char *yield_out, double *WK,
+                           const ust_buoyancy *buoyancy, double *eta_out, unsigned char *yield_out, double *WK,
                            double *WG, double *WF, void *stream);
+/* BuoyancyForceTerm "adjust" (BuoyancyForceTerm.c:362-375): WF of each element times 2^dim / (sum of its particle
+ * weights), the sum recovered from WG (sum_b WG[b] = detJ sum_p w_p) */
+int ust_k_adjust_force_weights(const ust_grid *g, const double *WG, double *WF, void *stream);
 
 /* K_e for elements [first, first+count): Ke (count * nd * nd) row-major, FP64 MFMA. */
 int ust_k_element_matrices(const ust_grid *g, const double *WK, int first, int count, double *Ke,
@@ -145,6 +165,10 @@ int ust_k_element_matrices(const ust_grid *g, const double *WK, int first, int c
 int ust_k_element_G(const ust_grid *g, const double *WG, int first, int count, double *Ge, void *stream);
 int ust_k_element_f(const ust_grid *g, const double *WF, double gravity, int first, int count,
                     double *fe, void *stream);
+/* the same along a direction: fe[a*dim + d] = gravity * dir[d] * WF[a] (dir = gHat, or -1 on the vertical
+ * axis; ust_k_element_f is dir = (0, -1, 0)) */
+int ust_k_element_f_dir(const ust_grid *g, const double *WF, double gravity, const double *dir, int first,
+                        int count, double *fe, void *stream);
 /* approxS_e = sum_v G_e[v]^2 / K_e[v][v] (UzawaPreconditionerTerm.c:200-212) */
 int ust_k_uzawa_diag(const ust_grid *g, const double *WK, const double *WG, double *approxS, void *stream);
 
@@ -233,14 +257,20 @@ int ust_k_lift_K(const ust_grid *g, const double *WK, const unsigned char *mask,
                  double *y, void *stream);
 /* y = G p (velocity dofs, free only); ust_k_apply_Gt mode (bc_only): 0 q = G^T u on the free dofs,
  * 1 q -= G^T u_bc over the BC dofs (continuity lifting), 2 q_e = sum over free dofs of |G_e| (row
- * sums of |D| for the infinity norm) */
+ * sums of |D| for the infinity norm), 3 q_e = sum over free dofs of G_e^2 u (u = diag(K)^-1: the diagonal
+ * of G^T diag(K)^-1 G, BSSCR's "gkgdiag" Schur approximation, preconditioner.c:268-310) */
 int ust_k_apply_G(const ust_grid *g, const double *WG, const unsigned char *mask, const double *p,
                   double *y, void *stream);
 int ust_k_apply_Gt(const ust_grid *g, const double *WG, const unsigned char *mask, const double *u,
                    double *q, int bc_only, void *stream);
+/* y = sum over the elements of |G| per velocity dof (free dofs; the row sums whose maximum is ||G||_inf) */
+int ust_k_abs_G_rowsum(const ust_grid *g, const double *WG, const unsigned char *mask, double *y, void *stream);
 /* nodal force vector f (free dofs) from WF; accumulated into f */
 int ust_k_force(const ust_grid *g, const double *WF, double gravity, const unsigned char *mask, double *f,
                 void *stream);
+/* the same along dir (3 doubles): f[n, d] += gravity * dir[d] * sum of WF around n, free dofs */
+int ust_k_force_dir(const ust_grid *g, const double *WF, double gravity, const double *dir,
+                    const unsigned char *mask, double *f, void *stream);
 /* inverse diagonal of K on free dofs (1 on BC dofs) */
 int ust_k_inv_diag(const ust_grid *g, const double *WK, const unsigned char *mask, double *invd64,
                    float *invd32, void *stream);

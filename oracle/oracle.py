@@ -28,7 +28,32 @@ class Rheology(C.Structure):
 
 
 class Material(C.Structure):
-    _fields_ = [("nrh", C.c_int), ("rh", Rheology * 4), ("density", C.c_double)]
+    _fields_ = [("nrh", C.c_int), ("rh", Rheology * 4), ("density", C.c_double), ("alpha", C.c_double),
+                ("ref_temperature", C.c_double), ("beta", C.c_double), ("ref_depth", C.c_double)]
+
+
+class Buoyancy(C.Structure):
+    _fields_ = [("vertical_axis", C.c_int), ("use_direction", C.c_int), ("direction", C.c_double * 3),
+                ("adjust", C.c_int), ("hydrostatic", C.c_int)] + \
+        [(k, C.c_double) for k in ("hs_upper_density", "hs_upper_alpha", "hs_lower_density", "hs_lower_alpha",
+                                   "hs_height", "hs_material_boundary", "hs_T0", "hs_linear_coefficient",
+                                   "hs_exponential_coefficient1", "hs_exponential_coefficient2", "hs_v",
+                                   "hs_width", "hs_time")]
+
+
+def make_buoyancy(vertical_axis=1, direction=None, adjust=False, hydrostatic=None):
+    b = Buoyancy()
+    b.vertical_axis = vertical_axis
+    if direction is not None:
+        b.use_direction = 1
+        for d, v in enumerate(direction):
+            b.direction[d] = v
+    b.adjust = int(bool(adjust))
+    if hydrostatic:
+        b.hydrostatic = 1
+        for k, v in hydrostatic.items():
+            setattr(b, "hs_" + k, float(v))
+    return b
 
 
 class Fields(C.Structure):
@@ -102,8 +127,9 @@ def make_mesh(dim, nel, xmin=(0, 0, 0), xmax=(1, 1, 1)):
     return m
 
 
-def make_material(rheologies, density=0.0):
+def make_material(rheologies, density=0.0, alpha=0.0, ref_temperature=0.0, beta=0.0, ref_depth=0.0):
     mat = Material()
+    mat.alpha, mat.ref_temperature, mat.beta, mat.ref_depth = alpha, ref_temperature, beta, ref_depth
     mat.nrh = len(rheologies)
     for i, (t, params) in enumerate(rheologies):
         mat.rh[i].type = t
@@ -190,6 +216,34 @@ def eval_viscosity(mesh, materials, cell_off, xi, mat_of, temperature=None, velo
     lib().orc_eval_viscosity(C.byref(mesh), mats, np_, _p(cell_off.astype(np.int32)), _p(np.ascontiguousarray(xi)),
                              _p(mat_of.astype(np.int32)), C.byref(f), _p(eta), _p(yielded))
     return eta, yielded
+
+
+def eval_density(mesh, materials, cell_off, xi, mat_of, temperature=None, buoyancy=None):
+    np_ = xi.shape[0]
+    mats = (Material * len(materials))(*materials)
+    rho = np.zeros(np_)
+    T = None if temperature is None else np.ascontiguousarray(temperature, dtype=np.float64)
+    lib().orc_eval_density(C.byref(mesh), mats, np_, _p(cell_off.astype(np.int32)), _p(np.ascontiguousarray(xi)),
+                           _p(mat_of.astype(np.int32)), _p(T), None if buoyancy is None else C.byref(buoyancy),
+                           _p(rho))
+    return rho
+
+
+def element_forces(mesh, cell_off, xi, w, rho, gravity=1.0, buoyancy=None):
+    """f_e of every element (nel, 2^dim * dim) from per-particle densities."""
+    nel = mesh_nel(mesh)
+    nd = (1 << mesh.dim) * mesh.dim
+    fe = np.zeros((nel, nd))
+    off = cell_off.astype(np.int32)
+    xi = np.ascontiguousarray(xi)
+    for e in range(nel):
+        p0, n = int(off[e]), int(off[e + 1] - off[e])
+        out = np.zeros(nd)
+        lib().orc_element_force(C.byref(mesh), e, n, _p(xi[p0:p0 + n]), _p(np.ascontiguousarray(w[p0:p0 + n])),
+                                _p(np.ascontiguousarray(rho[p0:p0 + n])), C.c_double(gravity),
+                                None if buoyancy is None else C.byref(buoyancy), _p(out))
+        fe[e] = out
+    return fe
 
 
 def assemble_elements(mesh, cell_off, xi, w, eta, rho=None, gravity=1.0):

@@ -534,6 +534,95 @@ void orc_element_f(const ORC_Mesh *m, int e, int n, const double *xi, const doub
     }
 }
 
+/* HydrostaticTerm_Current_Heights and HydrostaticTerm_Density (HydrostaticTerm.c:267-330): the background
+ * density at height y = coord[1] of the Gale-compatible hydrostatic term. */
+static double orc_hydrostatic_density(const ORC_Buoyancy *b, double y) {
+    double thick, boundary, height;
+    if (b->hs_v != 0) {
+        thick = (b->hs_height - b->hs_material_boundary) * b->hs_width / (b->hs_width + b->hs_v * b->hs_time);
+        boundary = b->hs_material_boundary +
+                   (b->hs_upper_density / b->hs_lower_density) * (b->hs_height - b->hs_material_boundary - thick);
+    } else {
+        thick = b->hs_height - b->hs_material_boundary;
+        boundary = b->hs_material_boundary;
+    }
+    height = thick + boundary;
+    double h = b->hs_height - y, T, density, alpha;
+    if (h < 0) T = b->hs_T0;
+    else T = b->hs_T0 + b->hs_linear_coefficient * h +
+             b->hs_exponential_coefficient1 * (1 - exp(-b->hs_exponential_coefficient2 * h));
+    if (y > height) density = alpha = 0;
+    else if (y > boundary) {
+        density = b->hs_upper_density;
+        alpha = b->hs_upper_alpha;
+    } else {
+        density = b->hs_lower_density;
+        alpha = b->hs_lower_alpha;
+    }
+    return density * (1 - alpha * T);
+}
+
+/* BuoyancyForceTerm.c:391-425: depth below the top of the box along VerticalAxis, temperature at xi,
+ * rho = rho0 (1 - alpha (T - Tref)), times (1 + beta (depth - depthref)) when beta != 0, less the
+ * hydrostatic background density. */
+double orc_particle_density(const ORC_Mesh *m, const ORC_Material *mat, int e, const double *xi, const double *T,
+                            const ORC_Buoyancy *b) {
+    int dim = m->dim, nen = 1 << dim, nodes[8];
+    double N[8], coord[3] = {0, 0, 0}, vx[3], temperature = 0.0;
+    orc_element_nodes(m, e, nodes);
+    orc_shape_functions(dim, xi, N);
+    for (int a = 0; a < nen; a++) {
+        orc_vertex(m, nodes[a], vx);
+        for (int d = 0; d < dim; d++) coord[d] += N[a] * vx[d];
+    }
+    const int va = b ? b->vertical_axis : 1;
+    const double depth = fabs(m->xmax[va] - coord[va]);
+    if (T)
+        for (int a = 0; a < nen; a++) temperature += N[a] * T[nodes[a]];
+    double background = 0.0;
+    if (b && b->hydrostatic) background = orc_hydrostatic_density(b, coord[1]);
+    double density = mat->density * (1.0 - mat->alpha * (temperature - mat->ref_temperature));
+    if (mat->beta != 0) density *= (1.0 + mat->beta * (depth - mat->ref_depth));
+    return density - background;
+}
+
+void orc_eval_density(const ORC_Mesh *m, const ORC_Material *mats, int np, const int *cell_off, const double *xi,
+                      const int *mat_of, const double *T, const ORC_Buoyancy *b, double *rho) {
+    int nel = orc_nelements(m);
+    (void)np;
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < nel; e++)
+        for (int p = cell_off[e]; p < cell_off[e + 1]; p++)
+            rho[p] = orc_particle_density(m, &mats[mat_of ? mat_of[p] : 0], e, &xi[3 * (size_t)p], T, b);
+}
+
+/* BuoyancyForceTerm.c:362-375 (adjust: cellLocalVolume 2^dim over the sum of the weights) and :435-448
+ * (factor = detJ w adjust rho g; along gHat, or -1 along VerticalAxis). */
+void orc_element_force(const ORC_Mesh *m, int e, int n, const double *xi, const double *w, const double *rho,
+                       double gravity, const ORC_Buoyancy *b, double *fe) {
+    int dim = m->dim, nen = 1 << dim;
+    double adjust = 1.0;
+    if (b && b->adjust) {
+        double tw = 0.0;
+        for (int p = 0; p < n; p++) tw += w[p];
+        adjust = (double)(1 << dim) / tw;
+    }
+    const int va = b ? b->vertical_axis : 1;
+    for (int p = 0; p < n; p++) {
+        double GNx[3][8], detJ, N[8];
+        orc_shape_global_derivs(m, e, &xi[3 * p], &detJ, GNx);
+        orc_shape_functions(dim, &xi[3 * p], N);
+        const double factor = detJ * w[p] * adjust * (rho[p] * gravity);
+        for (int a = 0; a < nen; a++) {
+            if (b && b->use_direction) {
+                for (int d = 0; d < dim; d++) fe[a * dim + d] += b->direction[d] * factor * N[a];
+            } else {
+                fe[a * dim + va] += -1.0 * factor * N[a];
+            }
+        }
+    }
+}
+
 /* _UzawaPreconditionerTerm_AssembleElement (UzawaPreconditionerTerm.c:195-212), P0: 1x1. */
 double orc_element_uzawa(int dim, const double *Ke, const double *Ge) {
     int nd = (1 << dim) * dim;

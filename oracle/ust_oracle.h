@@ -47,8 +47,23 @@ typedef struct {
 typedef struct {
     int nrh;
     ORC_Rheology rh[ORC_MAX_RHEOLOGIES];
-    double density;  /* BuoyancyForceTerm material density (alpha = beta = 0) */
+    /* BuoyancyForceTerm material extension (BuoyancyForceTerm.c:118-122): "density", "alpha",
+     * "referenceTemperature", "beta", "referenceDepth" */
+    double density, alpha, ref_temperature, beta, ref_depth;
 } ORC_Material;
+
+/* BuoyancyForceTerm component keys (BuoyancyForceTerm.c:139-160) and its optional HydrostaticTerm
+ * (HydrostaticTerm.c AssignFromXML keys; the background density of HydrostaticTerm_Density, :290-330) */
+typedef struct {
+    int vertical_axis;        /* "VerticalAxis" (default 1) */
+    int use_direction;        /* "gravityDirection" given */
+    double direction[3];
+    int adjust;               /* "adjust" */
+    int hydrostatic;          /* "HydrostaticTerm" given */
+    double hs_upper_density, hs_upper_alpha, hs_lower_density, hs_lower_alpha, hs_height, hs_material_boundary;
+    double hs_T0, hs_linear_coefficient, hs_exponential_coefficient1, hs_exponential_coefficient2;
+    double hs_v, hs_width, hs_time;  /* the moving-boundary terms and the context's current time */
+} ORC_Buoyancy;
 
 /* Per-assembly field context for the rheologies (PpcManager lookups). */
 typedef struct {
@@ -89,6 +104,17 @@ double orc_particle_viscosity(const ORC_Mesh *m, const ORC_Material *mat, int e,
 void   orc_eval_viscosity(const ORC_Mesh *m, const ORC_Material *mats, int np, const int *cell_off,
                           const double *xi, const int *mat_of, const ORC_Fields *f,
                           double *eta, unsigned char *yielded);
+
+/* ---- buoyancy (BuoyancyForceTerm_AssembleElement, BuoyancyForceTerm.c:377-425) ---- */
+/* density of the integration point at local coordinates xi of element e (T nodal or NULL) */
+double orc_particle_density(const ORC_Mesh *m, const ORC_Material *mat, int e, const double *xi, const double *T,
+                            const ORC_Buoyancy *b);
+void orc_eval_density(const ORC_Mesh *m, const ORC_Material *mats, int np, const int *cell_off, const double *xi,
+                      const int *mat_of, const double *T, const ORC_Buoyancy *b, double *rho);
+/* f_e of one element from per-particle densities: the force along -VerticalAxis or along gravityDirection,
+ * with the "adjust" factor cellLocalVolume / sum of weights (BuoyancyForceTerm.c:362-375, 435-448) */
+void orc_element_force(const ORC_Mesh *m, int e, int n, const double *xi, const double *w, const double *rho,
+                       double gravity, const ORC_Buoyancy *b, double *fe);
 
 /* ---- element terms ---- */
 void orc_element_K(const ORC_Mesh *m, int e, int n, const double *xi, const double *w,

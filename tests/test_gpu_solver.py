@@ -196,3 +196,48 @@ def test_converged_flag_and_long_stall_window():
     s.set_schur_scaling(np.zeros(s.nel, np.float32))  # a scaling of 1: approxS itself, which never stalls
     sc = s.solve(schur_pc=ust._lib.SCHUR_SCALED, outer_rtol=1e-8, inner_rtol=1e-10, stall_window=5000)
     assert sc["converged"] == 1 and sc["fallback_used"] == 0 and sc["outer_its"] == st["outer_its"]
+
+
+def test_gkgdiag_schur_preconditioner(oracle):
+    """-Q22_pc_type gkgdiag (BSSCR_FormSchurApproximationDiag, preconditioner.c:268-310): diag(G^T diag(K)^-1 G)
+    with the assembled diagonal of K, against the same product of the oracle's CSR operators; the BSSCR solve
+    with it reaches the oracle's solution, in as many outer iterations as the oracle's CG with that diagonal."""
+    import earthbyte_underworld_amd as ust
+    s = _spheres_3d(nel=(12, 12, 12), ppc=8, nsph=6)
+    mask, _ = ust.free_slip_walls(s)
+    s.set_bcs(mask)
+    s.assemble(gravity=1.0)
+    o = OracleProblem(s)
+    gs = o.system(mask)
+    K, G = gs.scipy()
+    ref = np.asarray(G.multiply(G).T @ (1.0 / K.diagonal())).ravel()
+    pc = s.schur_preconditioner(schur_pc=ust._lib.SCHUR_GKGDIAG)
+    assert rel_err(pc, ref) < 1e-12
+    assert rel_err(pc, s.approxS()) > 1e-3  # another diagonal than "uw"
+    st = s.solve(schur_pc=ust._lib.SCHUR_GKGDIAG, outer_rtol=1e-9, inner_rtol=1e-11)
+    uo, po, sto = gs.solve(ref, inner_rtol=1e-12, outer_rtol=1e-9, smooth_its=2)
+    u, p = s.solution()
+    free = gs.eq >= 0
+    print(f"gkgdiag: {st['outer_its']} outer its (oracle {sto.outer_its}); uw: "
+          f"{s.solve(outer_rtol=1e-9, inner_rtol=1e-11)['outer_its']}")
+    assert st["stokes_residual"] < 1e-7 and abs(st["outer_its"] - sto.outer_its) <= 2
+    assert rel_err(u[free], uo) < 1e-6 and rel_err(p, po) < 1e-6
+
+
+def test_hhat_rescale():
+    """h_hat small against ||G||_inf (a body force of 1e-9): BSSCR scales h_hat up by the inverse ratio for the
+    Schur solve and the pressure back (auglag-driver-DGTGD.c:333-358); by linearity the fields are 1e-9 times
+    those of the unit body force."""
+    import earthbyte_underworld_amd as ust
+    s = _spheres_3d(nel=(8, 8, 8), ppc=8, nsph=4)
+    s.set_bcs(*ust.free_slip_walls(s))
+    s.assemble(gravity=1.0)
+    st1 = s.solve(outer_rtol=1e-10, inner_rtol=1e-12)
+    u1, p1 = s.solution()
+    s.assemble(gravity=1e-9)
+    st2 = s.solve(outer_rtol=1e-10, inner_rtol=1e-12)
+    u2, p2 = s.solution()
+    assert st1["hhat_rescaled"] == 0 and st2["hhat_rescaled"] == 1
+    assert st2["stokes_residual"] < 1e-7
+    assert rel_err(u2, 1e-9 * u1) < 1e-7 and rel_err(p2, 1e-9 * p1) < 1e-7
+    assert abs(st2["outer_its"] - st1["outer_its"]) <= 1

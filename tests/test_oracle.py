@@ -270,3 +270,71 @@ def test_bsscr_nullspace_gauss_quadrature(oracle):
     u, p, st = gs.solve(aS, inner_rtol=1e-12, outer_rtol=1e-10)
     assert abs(p.sum()) < 1e-12
     assert st.stokes_rel_residual < 1e-8
+
+
+def _buoyancy_setup(O, dim=3, nel=(4, 3, 2), ppc=9, seed=3, box=(2.0, 1.0, 1.5)):
+    m = O.make_mesh(dim, nel[:dim], (0,) * dim, box[:dim])
+    ne = int(np.prod(nel[:dim]))
+    coords = O.layout_random(m, ppc, seed)
+    off = np.arange(0, ne * ppc + 1, ppc).astype(np.int32)
+    xi, w = O.map_particles(m, coords, off)
+    nn = int(np.prod([k + 1 for k in nel[:dim]]))
+    return m, coords, off, xi, w, nn
+
+
+def test_buoyancy_density_law_closed_forms(oracle):
+    """BuoyancyForceTerm.c:391-425 on the oracle, pinned by closed forms: rho = rho0 at alpha = beta = 0; a
+    uniform temperature T scales the force by (1 - alpha (T - Tref)); with beta the total vertical force is
+    -g rho0 sum_p w detJ (1 + beta (depth_p - dref)), depth from the particles' own global coordinates (the
+    layout's, not the oracle's local-to-global map); VerticalAxis 2 measures depth and pushes along z."""
+    O = oracle
+    m, coords, off, xi, w, nn = _buoyancy_setup(O)
+    mat0 = np.zeros(len(w), np.int32)
+    detJ = (2.0 / 4) * (1.0 / 3) * (1.5 / 2) / 8
+    base = [O.make_material([(O.RH_MATERIALVISCOSITY, (1.0,))], 2.5)]
+    rho = O.eval_density(m, base, off, xi, mat0)
+    assert np.all(rho == 2.5)
+    f0 = O.element_forces(m, off, xi, w, rho, gravity=1.3)
+    T = np.full(nn, 0.7)
+    hot = [O.make_material([(O.RH_MATERIALVISCOSITY, (1.0,))], 2.5, alpha=0.2, ref_temperature=0.3)]
+    fa = O.element_forces(m, off, xi, w, O.eval_density(m, hot, off, xi, mat0, temperature=T), gravity=1.3)
+    assert np.allclose(fa, (1.0 - 0.2 * (0.7 - 0.3)) * f0, rtol=1e-13, atol=1e-15)
+    deep = [O.make_material([(O.RH_MATERIALVISCOSITY, (1.0,))], 2.5, beta=0.4, ref_depth=0.25)]
+    fb = O.element_forces(m, off, xi, w, O.eval_density(m, deep, off, xi, mat0), gravity=1.3)
+    depth = 1.0 - coords[:, 1]
+    total = -1.3 * 2.5 * np.sum(w * detJ * (1.0 + 0.4 * (depth - 0.25)))
+    assert abs(fb.reshape(len(fb), 8, 3)[:, :, 1].sum() - total) < 1e-12 * abs(total)
+    assert np.all(fb.reshape(len(fb), 8, 3)[:, :, [0, 2]] == 0.0)
+    bz = O.make_buoyancy(vertical_axis=2)
+    fz = O.element_forces(m, off, xi, w, O.eval_density(m, deep, off, xi, mat0, buoyancy=bz), gravity=1.3, buoyancy=bz)
+    totz = -1.3 * 2.5 * np.sum(w * detJ * (1.0 + 0.4 * ((1.5 - coords[:, 2]) - 0.25)))
+    assert abs(fz.reshape(len(fz), 8, 3)[:, :, 2].sum() - totz) < 1e-12 * abs(totz)
+    assert np.all(fz.reshape(len(fz), 8, 3)[:, :, :2] == 0.0)
+
+
+def test_buoyancy_direction_adjust_hydrostatic(oracle):
+    """gravityDirection turns the force: f[a, d] = -gHat[d] f_default[a, vertical]; adjust rescales each
+    element's weights to sum to 2^dim (halved weights give the unhalved force); a HydrostaticTerm whose two
+    layers carry the material density leaves no force, and a two-layer background subtracts the layer
+    density picked by each particle's own height."""
+    O = oracle
+    m, coords, off, xi, w, nn = _buoyancy_setup(O)
+    mat0 = np.zeros(len(w), np.int32)
+    base = [O.make_material([(O.RH_MATERIALVISCOSITY, (1.0,))], 2.5)]
+    rho = O.eval_density(m, base, off, xi, mat0)
+    f0 = O.element_forces(m, off, xi, w, rho, gravity=1.0).reshape(-1, 8, 3)
+    gh = (0.6, -0.8, 0.0)
+    fd = O.element_forces(m, off, xi, w, rho, gravity=1.0, buoyancy=O.make_buoyancy(direction=gh)).reshape(-1, 8, 3)
+    for d in range(3):
+        assert np.allclose(fd[:, :, d], -gh[d] * f0[:, :, 1], rtol=1e-14, atol=1e-16)
+    fh = O.element_forces(m, off, xi, 0.5 * w, rho, gravity=1.0, buoyancy=O.make_buoyancy(adjust=True))
+    assert np.allclose(fh.reshape(-1, 8, 3), f0, rtol=1e-13, atol=1e-16)
+    flat = dict(upper_density=2.5, lower_density=2.5, height=5.0, material_boundary=0.5, T0=1.0)
+    bh = O.make_buoyancy(hydrostatic=flat)
+    assert np.all(O.eval_density(m, base, off, xi, mat0, buoyancy=bh) == 0.0)
+    two = dict(upper_density=1.0, lower_density=2.0, upper_alpha=0.1, lower_alpha=0.0, height=0.9,
+               material_boundary=0.4, T0=0.5, linear_coefficient=0.0)
+    rh = O.eval_density(m, base, off, xi, mat0, buoyancy=O.make_buoyancy(hydrostatic=two))
+    y = coords[:, 1]
+    bg = np.where(y > 0.9, 0.0, np.where(y > 0.4, 1.0 * (1 - 0.1 * 0.5), 2.0))
+    assert np.allclose(rh, 2.5 - bg, rtol=1e-13, atol=1e-13)
