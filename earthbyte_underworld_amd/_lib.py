@@ -152,6 +152,8 @@ _SIGS = {
     # thin kernel ABI used directly by tests and the benchmark timers
     "ust_k_apply_work_len": (C.c_size_t, [C.POINTER(Grid)]),
     "ust_k_apply_K_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp]),
+    "ust_k_apply_K_f64_noalloc": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp]),
+    "ust_k_apply_noalloc_max_work": (C.c_size_t, []),
     "ust_k_apply_K_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp]),
     "ust_k_apply_K_residual_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp]),
     "ust_k_apply_K_cheby_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, D, D, I, vp, vp]),

@@ -15,6 +15,9 @@
 // Grids with fewer than 1024 column layers (the coarse multigrid levels) use 16x4x4 tiles instead, one per
 // workgroup, which gives small grids more workgroups.
 // 2D: one workgroup per 16x16 tile, node block reduced in LDS, the same workspace scheme on the tile faces.
+#include <mutex>
+#include <stdio.h>
+
 #include "ust_common.h"
 
 // axes orthogonal to k, in increasing order
@@ -978,4 +981,63 @@ extern "C" size_t ust_k_weights_len(const ust_grid *gr) {
 extern "C" size_t ust_k_weights_index(const ust_grid *gr, int e, int q) {
     GridDev g = grid_dev(gr);
     return widx(g, e, q);
+}
+
+// ------------------------------------------------------------------------------------------
+// K u without a caller workspace, for callers that cannot allocate (code under HIP graph capture, serving
+// loops): the per-tile node partials go to a file-scope device array.  Each GPU the library runs on holds its
+// own copy until the process ends; its device address is looked up once per device and kept.  Every slot the
+// gather pass reads is written by the same call's tile pass first, so nothing needs clearing between calls;
+// calls that overlap on one GPU share the array and race.
+// ------------------------------------------------------------------------------------------
+#define UST_NOALLOC_WORK (size_t(1) << 22)  // doubles: 32 MiB
+#define UST_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[UST_NOALLOC_WORK];
+static double *g_scratch_dev[UST_NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int scratch_address(double **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        ust_set_hip_error(e, __FILE__, __LINE__);
+        return UST_EDEVICE;
+    }
+    if (dev < 0 || dev >= UST_NOALLOC_MAX_DEVICES) {
+        ust_set_error_text("ust_k_apply_K_f64_noalloc: device index beyond the library's table");
+        return UST_EDEVICE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            ust_set_hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol, __FILE__, __LINE__);
+            return UST_EDEVICE;
+        }
+        g_scratch_dev[dev] = (double *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return UST_OK;
+}
+
+extern "C" size_t ust_k_apply_noalloc_max_work(void) { return UST_NOALLOC_WORK; }
+
+extern "C" int ust_k_apply_K_f64_noalloc(const ust_grid *g, const double *WK, const unsigned char *mask,
+                                         const double *u, double *y, void *stream) {
+    if (!g) return UST_EINVAL;
+    const size_t need = work_len(grid_dev(g));
+    if (need > UST_NOALLOC_WORK) {
+        char msg[256];
+        snprintf(msg, sizeof(msg),
+                 "ust_k_apply_K_f64_noalloc: the grid needs %zu workspace doubles, more than the %zu of the "
+                 "library's static workspace (use ust_k_apply_K_f64 with a workspace)",
+                 need, (size_t)UST_NOALLOC_WORK);
+        ust_set_error_text(msg);
+        return UST_EINVAL;
+    }
+    double *ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc != UST_OK) return rc;
+    return apply_plain<double, double>(g, WK, mask, u, y, ws, stream);
 }

@@ -247,6 +247,14 @@ int ust_k_apply_K_cheby_f32_bf16w(const ust_grid *g, const unsigned short *WK, c
 int ust_k_apply_K_cheby_prolong_f32_bf16w(const ust_grid *g, const unsigned short *WK, const unsigned char *mask,
                                           float *x, const float *b, const float *invd, float *d, float c2,
                                           const float *xc, float *work, void *stream);
+/* y = K u as ust_k_apply_K_f64, without a caller workspace.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops).  The tile partials go to a
+ * static device array of ust_k_apply_noalloc_max_work() doubles (32 MiB) that each GPU holds until the process
+ * ends; a grid whose ust_k_apply_work_len exceeds it is rejected (UST_EINVAL, text in ust_dev_error_string).
+ * Calls on one GPU share the array: calls that overlap (other streams, other threads) race.  Allocates nothing. */
+size_t ust_k_apply_noalloc_max_work(void);
+int ust_k_apply_K_f64_noalloc(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
+                              double *y, void *stream);
 /* same operator, one element per thread with per-dof atomic adds (reference variant for checks) */
 int ust_k_apply_K_ref_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
                           double *y, void *stream);
