@@ -224,6 +224,15 @@ def kernel_only(args):
     lib, ctx = s.lib, s.ctx
     stream = lib.ust_get_stream(ctx)
     r, kern = fine_level_kernels(lib, ctx, s, stream)
+    # element stiffness matrices for the per-element callbacks (FP64 MFMA), 65536 elements per launch
+    ne = min(65536, s.nel)
+    ke = C.c_void_p()
+    assert lib.ust_dev_malloc(C.byref(ke), ne * s.nd * s.nd * 8) == 0
+    for first in (0, ne, 2 * ne):
+        if first + ne <= s.nel:
+            assert lib.ust_get_element_matrices_dev(ctx, first, ne, ke) == 0
+    lib.ust_stream_sync(stream)
+    lib.ust_dev_free(ke)
     nb = 256 << 20
     a, b = C.c_void_p(), C.c_void_p()
     lib.ust_dev_malloc(C.byref(a), nb)
