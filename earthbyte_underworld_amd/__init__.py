@@ -4,6 +4,6 @@ The compute lives in libust.so (gfx950 HIP kernels + C host layer); this package
 """
 from ._lib import LIB_PATH, load  # noqa: F401
 from .stokes import (Arrhenius, BelowCosinePlane, Box, BuoyancyForceTerm, DruckerPrager, Everywhere,  # noqa: F401
-                     MaterialViscosity,
+                     MaterialViscosity, StrainWeakening,
                      RheologyMaterial, Sphere, StokesSystem, UstError, ViscosityLimiter, VonMises, default_opts,
                      free_slip_walls, freeslip_sides_noslip_topbottom, wall_mask)

@@ -21,7 +21,7 @@ MAX_MATERIALS = 64
 
 
 class Rheology(C.Structure):
-    _fields_ = [("type", C.c_int), ("p", C.c_double * 6)]
+    _fields_ = [("type", C.c_int), ("p", C.c_double * 8)]
 
 
 class Material(C.Structure):
@@ -40,6 +40,14 @@ class Buoyancy(C.Structure):
 
 class Shape(C.Structure):
     _fields_ = [("type", C.c_int), ("material", C.c_int), ("p", C.c_double * 6)]
+
+
+class StrainWeakening(C.Structure):
+    _fields_ = [("softening_strain", C.c_double), ("healing_rate", C.c_double),
+                ("initial_damage_fraction", C.c_double), ("initial_damage_wavenumber", C.c_double),
+                ("wavenumber_sin", C.c_double * 3), ("wavenumber_cos", C.c_double * 3),
+                ("initial_damage_factor", C.c_double), ("random_seed", C.c_long), ("has_shape", C.c_int),
+                ("initial_strain_shape", Shape)]
 
 
 class Grid(C.Structure):
@@ -102,6 +110,13 @@ _SIGS = {
     "ust_write_petsc_vector": (I, [vp, I, I, C.c_char_p]),
     "ust_set_temperature": (I, [vp, vp]),
     "ust_set_buoyancy": (I, [vp, C.POINTER(Buoyancy)]),
+    "ust_strain_weakening_default": (None, [C.POINTER(StrainWeakening)]),
+    "ust_set_strain_weakening": (I, [vp, C.POINTER(StrainWeakening)]),
+    "ust_strain_weakening_initialise": (I, [vp]),
+    "ust_strain_weakening_step": (I, [vp, C.c_double]),
+    "ust_get_strain_weakening": (I, [vp, vp, vp]),
+    "ust_set_strain_weakening_state": (I, [vp, vp, vp]),
+    "ust_get_tensile_failure": (I, [vp, vp]),
     "ust_set_solution": (I, [vp, vp, vp]),
     "ust_set_previous_solution_exists": (I, [vp, I]),
     "ust_set_bcs": (I, [vp, vp, vp]),
@@ -174,7 +189,9 @@ _SIGS = {
     "ust_k_prolong_add_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp]),
     "ust_k_prolong_add_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp]),
     "ust_k_particle_weights": (I, [C.POINTER(Grid), vp, I, vp, vp, vp, vp, vp, vp, vp, I, vp, vp, vp, I, vp, vp,
-                                   vp, vp, vp, vp, vp]),
+                                   vp, vp, vp, vp, vp, vp]),
+    "ust_k_strain_weakening_step": (I, [vp, vp, I, C.c_double, vp]),
+    "ust_k_permute_f64": (I, [vp, vp, vp, I, vp]),
     "ust_event_create": (I, [C.POINTER(vp)]),
     "ust_event_destroy": (I, [vp]),
     "ust_event_record": (I, [vp, vp]),

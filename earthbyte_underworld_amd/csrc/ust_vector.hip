@@ -38,6 +38,15 @@ __global__ void kv_d2f(const double *x, float *y, size_t n) {
 __global__ void kv_d2bf(const double *x, unsigned short *y, size_t n) {
     GRID_STRIDE(i, n) y[i] = f32_to_bf16((float)x[i]);
 }
+__global__ void kv_sw_step(double *pfw, const double *inc, double dt, size_t n) {
+    GRID_STRIDE(i, n) {
+        const double v = pfw[i] + dt * inc[i];
+        pfw[i] = v < 0.0 ? 0.0 : v;
+    }
+}
+__global__ void kv_permute(const unsigned *perm, const double *src, double *dst, size_t n) {
+    GRID_STRIDE(i, n) dst[i] = src[perm[i]];
+}
 __global__ void kv_f2d(const float *x, double *y, size_t n) {
     GRID_STRIDE(i, n) y[i] = (double)x[i];
 }
@@ -301,6 +310,14 @@ int ust_k_fill_sign_f64(double *x, size_t n, unsigned long long seed, void *stre
 int ust_k_copy_f64_to_f32(const double *x, float *y, size_t n, void *stream) { LV(kv_d2f, n, x, y, n); }
 int ust_k_copy_f32_to_f64(const float *x, double *y, size_t n, void *stream) { LV(kv_f2d, n, x, y, n); }
 int ust_k_copy_f64_to_bf16(const double *x, unsigned short *y, size_t n, void *stream) { LV(kv_d2bf, n, x, y, n); }
+int ust_k_strain_weakening_step(double *pfw, const double *pfw_inc, int np, double dt, void *stream) {
+    if (np < 0 || (np > 0 && (!pfw || !pfw_inc))) return UST_EINVAL;
+    LV(kv_sw_step, (size_t)np, pfw, pfw_inc, dt, (size_t)np);
+}
+int ust_k_permute_f64(const unsigned *perm, const double *src, double *dst, int n, void *stream) {
+    if (n < 0 || (n > 0 && (!perm || !src || !dst))) return UST_EINVAL;
+    LV(kv_permute, (size_t)n, perm, src, dst, (size_t)n);
+}
 int ust_k_axpy_f64(double a, const double *x, double *y, size_t n, void *stream) { LV(kv_axpy, n, a, x, y, n); }
 int ust_k_axpby_f64(double a, const double *x, double b, double *y, size_t n, void *stream) {
     LV(kv_axpby, n, a, x, b, y, n);

@@ -120,6 +120,7 @@ static void free_swarm(ust_context *ctx) {
     DFREE(ctx->xi0); DFREE(ctx->xi1); DFREE(ctx->xi2); DFREE(ctx->w);
     free_integration_points(ctx);
     DFREE(ctx->mat); DFREE(ctx->coh); DFREE(ctx->eta); DFREE(ctx->yielded);
+    DFREE(ctx->tensile); DFREE(ctx->pfw); DFREE(ctx->pfw_inc);
     ctx->np = 0;
 }
 
@@ -253,6 +254,8 @@ int ust_rheology_from_xml(const char *type_name, int nkeys, const char *const *k
             if (streq(key, "cohesion")) slot = 0;
             if (streq(key, "minimumViscosity")) slot = 1;
             if (streq(key, "strainRateSoftening")) slot = 2;
+            if (streq(key, "cohesionAfterSoftening")) slot = 3;
+            if (streq(key, "StrainWeakening")) slot = 4;
         }
         if (t == UST_RH_DRUCKERPRAGER) {
             if (streq(key, "cohesion")) slot = 0;
@@ -260,6 +263,9 @@ int ust_rheology_from_xml(const char *type_name, int nkeys, const char *const *k
             if (streq(key, "minimumYieldStress")) slot = 2;
             if (streq(key, "minimumViscosity")) slot = 3;
             if (streq(key, "strainRateSoftening")) slot = 4;
+            if (streq(key, "cohesionAfterSoftening")) slot = 5;
+            if (streq(key, "frictionCoefficientAfterSoftening")) slot = 6;
+            if (streq(key, "StrainWeakening")) slot = 7;
         }
         if (slot < 0) return UST_EINVAL;
         out->p[slot] = v;
@@ -284,6 +290,7 @@ int ust_swarm_upload_dev(ust_context *ctx, int np, const int *cell_off_dev, cons
     DALLOC(ctx, ctx->w, n * sizeof(double));
     DALLOC(ctx, ctx->eta, n * sizeof(double));
     DALLOC(ctx, ctx->yielded, n);
+    DALLOC(ctx, ctx->tensile, n);
     CKD(ctx, ust_memcpy_d2d(ctx->cell_off, cell_off_dev, (size_t)(ctx->nel + 1) * sizeof(int), ctx->stream));
     CKD(ctx, ust_memcpy_d2d(ctx->xi0, xi0_dev, n * sizeof(double), ctx->stream));
     CKD(ctx, ust_memcpy_d2d(ctx->xi1, xi1_dev, n * sizeof(double), ctx->stream));
@@ -380,6 +387,149 @@ int ust_set_buoyancy(ust_context *ctx, const ust_buoyancy *b) {
     return UST_OK;
 }
 
+/* ---- StrainWeakening ---- */
+void ust_strain_weakening_default(ust_strain_weakening *sw) {
+    memset(sw, 0, sizeof(*sw));
+    sw->softening_strain = HUGE_VAL;
+    sw->initial_damage_wavenumber = -1.0;
+    for (int d = 0; d < 3; d++) sw->wavenumber_sin[d] = sw->wavenumber_cos[d] = -1.0;
+    sw->initial_damage_factor = 1.0;
+}
+
+/* the per-particle state arrays, zero when first allocated for the current swarm */
+static int sw_state_alloc(ust_context *ctx) {
+    const size_t n = (size_t)ctx->np;
+    if (!ctx->pfw) DALLOC(ctx, ctx->pfw, n * sizeof(double) + 8);
+    if (!ctx->pfw_inc) DALLOC(ctx, ctx->pfw_inc, n * sizeof(double) + 8);
+    return UST_OK;
+}
+
+int ust_set_strain_weakening(ust_context *ctx, const ust_strain_weakening *sw) {
+    if (!sw) {
+        ctx->sw_on = 0;
+        DFREE(ctx->pfw);
+        DFREE(ctx->pfw_inc);
+        ctx->assembled = 0;
+        return UST_OK;
+    }
+    if (!(sw->softening_strain > 0.0)) UST_FAIL(ctx, UST_EINVAL, "softeningStrain must be > 0");
+    ctx->sw = *sw;
+    ctx->sw_on = 1;
+    if (ctx->np) {
+        int rc = sw_state_alloc(ctx);
+        if (rc) return rc;
+    }
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
+static int sw_shape_inside(const ust_shape *sh, const double *x, int dim) {
+    if (sh->type == UST_SHAPE_EVERYWHERE) return 1;
+    if (sh->type == UST_SHAPE_BELOWCOSINEPLANE)
+        return x[1] <= sh->p[0] + sh->p[1] * cos((2 * M_PI * x[0] / sh->p[2]) + sh->p[3]);
+    if (sh->type == UST_SHAPE_SPHERE) {
+        double r2 = 0.0;
+        for (int d = 0; d < dim; d++) r2 += (x[d] - sh->p[d]) * (x[d] - sh->p[d]);
+        return r2 <= sh->p[3] * sh->p[3];
+    }
+    if (sh->type == UST_SHAPE_BOX) {
+        for (int d = 0; d < dim; d++)
+            if (!(x[d] >= sh->p[d] && x[d] <= sh->p[3 + d])) return 0;
+        return 1;
+    }
+    return 0;
+}
+
+int ust_strain_weakening_initialise(ust_context *ctx) {
+    if (!ctx->sw_on) UST_FAIL(ctx, UST_ESTATE, "strain weakening is off (ust_set_strain_weakening)");
+    if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
+    const int dim = ctx->grid.dim, nel = ctx->nel, np = ctx->np;
+    const ust_strain_weakening *sw = &ctx->sw;
+    int *off = (int *)malloc((size_t)(nel + 1) * sizeof(int));
+    double *xi = (double *)malloc((size_t)np * 3 * sizeof(double));
+    double *pfw = (double *)malloc((size_t)np * sizeof(double));
+    if (!off || !xi || !pfw) {
+        free(off); free(xi); free(pfw);
+        UST_FAIL(ctx, UST_ENOMEM, "host allocation failed");
+    }
+    int rc = ust_swarm_download(ctx, off, xi, NULL, NULL);
+    if (rc) {
+        free(off); free(xi); free(pfw);
+        return rc;
+    }
+    /* the material points in swarm order; srand(randomSeed) either way (StrainWeakening.c:338-345) */
+    srand((unsigned)sw->random_seed);
+    for (int e = 0; e < nel; e++) {
+        int idx[3] = {e % ctx->grid.nel[0], (e / ctx->grid.nel[0]) % ctx->grid.nel[1],
+                      dim == 3 ? e / (ctx->grid.nel[0] * ctx->grid.nel[1]) : 0};
+        for (int p = off[e]; p < off[e + 1]; p++) {
+            double v = 0.0;
+            if (rand() < (RAND_MAX * sw->initial_damage_fraction)) {
+                double x[3] = {0.0, 0.0, 0.0};
+                for (int d = 0; d < dim; d++) {
+                    const double lo = ctx->grid.xmin[d] + idx[d] * ctx->grid.h[d];
+                    x[d] = lo + 0.5 * (xi[3 * (size_t)p + d] + 1.0) * ctx->grid.h[d];
+                }
+                if (sw->has_shape && !sw_shape_inside(&sw->initial_strain_shape, x, dim)) {
+                    pfw[p] = 0.0;
+                    continue;
+                }
+                v = sw->initial_damage_factor * rand() * sw->softening_strain / RAND_MAX;
+                if (sw->initial_damage_wavenumber > 0.0 && sw->wavenumber_cos[0] == -1.0)
+                    v *= pow(cos(M_PI * x[0] * sw->initial_damage_wavenumber), 2.0);
+                /* order of the reference: CosI, SinI, CosJ, SinJ, CosK, SinK */
+                for (int d = 0; d < 3; d++) {
+                    if (sw->wavenumber_cos[d] > 0.0) v *= pow(cos(M_PI * x[d] * sw->wavenumber_cos[d]), 2.0);
+                    if (sw->wavenumber_sin[d] > 0.0) v *= pow(sin(M_PI * x[d] * sw->wavenumber_sin[d]), 2.0);
+                }
+            }
+            pfw[p] = v;
+        }
+    }
+    rc = sw_state_alloc(ctx);
+    if (!rc) rc = ust_memcpy_h2d(ctx->pfw, pfw, (size_t)np * sizeof(double), ctx->stream);
+    if (!rc) rc = ust_k_fill_f64(ctx->pfw_inc, 0.0, (size_t)np, ctx->stream);
+    if (!rc) rc = ust_stream_sync(ctx->stream);
+    free(off); free(xi); free(pfw);
+    if (rc) UST_FAIL(ctx, rc, "strain weakening set-up failed: %s", ust_dev_error_string());
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
+int ust_strain_weakening_step(ust_context *ctx, double dt) {
+    if (!ctx->sw_on || !ctx->pfw) UST_FAIL(ctx, UST_ESTATE, "strain weakening is off");
+    CKD(ctx, ust_k_strain_weakening_step(ctx->pfw, ctx->pfw_inc, ctx->np, dt, ctx->stream));
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
+int ust_get_strain_weakening(ust_context *ctx, double *pfw, double *pfw_inc) {
+    if (!ctx->sw_on || !ctx->pfw) UST_FAIL(ctx, UST_ESTATE, "strain weakening is off");
+    const size_t n = (size_t)ctx->np * sizeof(double);
+    if (pfw) CKD(ctx, ust_memcpy_d2h(pfw, ctx->pfw, n, ctx->stream));
+    if (pfw_inc) CKD(ctx, ust_memcpy_d2h(pfw_inc, ctx->pfw_inc, n, ctx->stream));
+    return UST_OK;
+}
+
+int ust_set_strain_weakening_state(ust_context *ctx, const double *pfw, const double *pfw_inc) {
+    if (!ctx->sw_on) UST_FAIL(ctx, UST_ESTATE, "strain weakening is off");
+    if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
+    int rc = sw_state_alloc(ctx);
+    if (rc) return rc;
+    const size_t n = (size_t)ctx->np * sizeof(double);
+    if (pfw) CKD(ctx, ust_memcpy_h2d(ctx->pfw, pfw, n, ctx->stream));
+    if (pfw_inc) CKD(ctx, ust_memcpy_h2d(ctx->pfw_inc, pfw_inc, n, ctx->stream));
+    CKD(ctx, ust_stream_sync(ctx->stream));
+    ctx->assembled = 0;
+    return UST_OK;
+}
+
+int ust_get_tensile_failure(ust_context *ctx, unsigned char *tensile) {
+    if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
+    CKD(ctx, ust_memcpy_d2h(tensile, ctx->tensile, (size_t)ctx->np, ctx->stream));
+    return UST_OK;
+}
+
 int ust_set_temperature(ust_context *ctx, const double *T) {
     if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "no mesh");
     CKD(ctx, ust_memcpy_h2d(ctx->T, T, (size_t)ctx->nnodes * sizeof(double), ctx->stream));
@@ -440,12 +590,21 @@ int ust_assemble(ust_context *ctx, double gravity) {
         if (ctx->mats_h[m].beta != 0.0) fullDensity = 1;
     }
     const int ppc_mean = (ctx->np + ctx->nel - 1) / ctx->nel;
+    ust_yield_state ys = {NULL, NULL, ctx->tensile, 1.0, 0.0};
+    if (ctx->sw_on) {
+        int rc = sw_state_alloc(ctx);
+        if (rc) return rc;
+        ys.pfw = ctx->pfw;
+        ys.pfw_inc = ctx->pfw_inc;
+        ys.softening_strain = ctx->sw.softening_strain;
+        ys.healing_rate = ctx->sw.healing_rate;
+    }
     /* the integration points: the material points themselves, or the points a weights calculator moved */
     const int sep = ctx->ixi0 != NULL;
     const double *q0 = sep ? ctx->ixi0 : ctx->xi0, *q1 = sep ? ctx->ixi1 : ctx->xi1, *q2 = sep ? ctx->ixi2 : ctx->xi2;
     CKD(ctx, ust_k_particle_weights(&ctx->grid, ctx->cell_off, ppc_mean, q0, q1, q2, ctx->w,
                                     ctx->mat, ctx->coh, ctx->mats_d, ctx->nmat, needT ? ctx->T : NULL,
-                                    needV ? ctx->u : NULL, needV ? ctx->p : NULL, ctx->prev_solution,
+                                    needV ? ctx->u : NULL, needV ? ctx->p : NULL, ctx->prev_solution, &ys,
                                     fullDensity ? &ctx->buoy : NULL, ctx->eta, ctx->yielded, ctx->WK, ctx->WG,
                                     ctx->WF, ctx->stream));
     if (ctx->buoy.adjust) CKD(ctx, ust_k_adjust_force_weights(&ctx->grid, ctx->WG, ctx->WF, ctx->stream));
@@ -722,6 +881,7 @@ int ust_swarm_random(ust_context *ctx, const int *cell_off_host, int ppc, long s
     DALLOC(ctx, ctx->w, n * sizeof(double));
     DALLOC(ctx, ctx->eta, n * sizeof(double));
     DALLOC(ctx, ctx->yielded, n);
+    DALLOC(ctx, ctx->tensile, n);
     DALLOC(ctx, ctx->mat, n);
     CKD(ctx, ust_k_random_layout(&ctx->grid, ctx->cell_off, np, seed, ctx->xi0, ctx->xi1, ctx->xi2, ctx->w,
                                  ctx->stream));
@@ -881,11 +1041,27 @@ int ust_swarm_advect(ust_context *ctx, double dt, int order, int allow_fallback,
         rc = ust_k_swarm_gather(dim, kept, (const unsigned *)perm, (const unsigned *)key_s, (const int *)off, n0, n1,
                                 n2, ctx->mat, ctx->coh, o0, o1, o2, (double *)ow, (unsigned char *)omat,
                                 (float *)ocoh, ctx->stream);
-    if (!rc)
+    /* the strain-weakening state travels with its particles */
+    void *opfw = NULL, *oinc = NULL;
+    if (!rc && ctx->pfw) {
+        rc = ust_dev_malloc(&opfw, k * sizeof(double) + 8);
+        if (!rc) rc = ust_dev_malloc(&oinc, k * sizeof(double) + 8);
+        if (!rc) rc = ust_k_permute_f64((const unsigned *)perm, ctx->pfw, (double *)opfw, kept, ctx->stream);
+        if (!rc) rc = ust_k_permute_f64((const unsigned *)perm, ctx->pfw_inc, (double *)oinc, kept, ctx->stream);
+    }
+    if (!rc) {
         rc = ust_swarm_upload_dev(ctx, kept, (const int *)off, o0, o1, dim == 3 ? o2 : NULL, (const double *)ow,
                                   (const unsigned char *)omat, (const float *)ocoh, max_ppc);
-    else if (!ctx->err[0] || rc != UST_ENOTCONV)
+        if (!rc && opfw) { /* upload_dev released the old state: the permuted arrays become the state */
+            ctx->pfw = (double *)opfw;
+            ctx->pfw_inc = (double *)oinc;
+            opfw = oinc = NULL;
+        }
+    } else if (!ctx->err[0] || rc != UST_ENOTCONV) {
         snprintf(ctx->err, sizeof(ctx->err), "swarm advection failed (%d): %s", rc, ust_dev_error_string());
+    }
+    ust_dev_free(opfw);
+    ust_dev_free(oinc);
     void *tmp[] = {key, key_s, perm, perm_t, nxi, flag, work, off, oxi, ow, omat, ocoh};
     for (size_t i = 0; i < sizeof(tmp) / sizeof(tmp[0]); i++) ust_dev_free(tmp[i]);
     if (rc) return rc;

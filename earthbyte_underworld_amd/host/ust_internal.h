@@ -67,6 +67,10 @@ struct ust_context {
     float *coh;
     double *eta;
     unsigned char *yielded;
+    unsigned char *tensile;  /* DruckerPrager tensileFailure per particle */
+    double *pfw, *pfw_inc;   /* StrainWeakening postFailureWeakening and its increment per particle (when on) */
+    int sw_on;
+    ust_strain_weakening sw;
     /* fields */
     double *T;  /* nodal */
     double *u;  /* nodal solution incl. BC values */

@@ -90,6 +90,27 @@ def BuoyancyForceTerm(VerticalAxis=1, gravityDirection=None, adjust=False, Hydro
     return b
 
 
+def StrainWeakening(softeningStrain=float("inf"), healingRate=0.0, initialDamageFraction=0.0,
+                    initialDamageWavenumber=-1.0, initialDamageWavenumberSinI=-1.0, initialDamageWavenumberCosI=-1.0,
+                    initialDamageWavenumberSinJ=-1.0, initialDamageWavenumberCosJ=-1.0,
+                    initialDamageWavenumberSinK=-1.0, initialDamageWavenumberCosK=-1.0, initialDamageFactor=1.0,
+                    randomSeed=0, initialStrainShape=None):
+    """StrainWeakening component keys (defaults as StrainWeakening.c:232-245); initialStrainShape: a shape."""
+    sw = L.StrainWeakening()
+    sw.softening_strain, sw.healing_rate = float(softeningStrain), float(healingRate)
+    sw.initial_damage_fraction = float(initialDamageFraction)
+    sw.initial_damage_wavenumber = float(initialDamageWavenumber)
+    sw.wavenumber_sin[:] = [float(initialDamageWavenumberSinI), float(initialDamageWavenumberSinJ),
+                            float(initialDamageWavenumberSinK)]
+    sw.wavenumber_cos[:] = [float(initialDamageWavenumberCosI), float(initialDamageWavenumberCosJ),
+                            float(initialDamageWavenumberCosK)]
+    sw.initial_damage_factor, sw.random_seed = float(initialDamageFactor), int(randomSeed)
+    if initialStrainShape is not None:
+        sw.has_shape = 1
+        sw.initial_strain_shape = initialStrainShape
+    return sw
+
+
 def shape(kind, material, *params):
     s = L.Shape()
     s.type = kind
@@ -236,6 +257,34 @@ class StokesSystem:
         w = np.zeros(n)
         self._check(self.lib.ust_swarm_download_integration_points(self.ctx, _ptr(xi), _ptr(w)))
         return xi, w
+
+    def set_strain_weakening(self, sw=None):
+        """StrainWeakening component (StrainWeakening(...)) or None: off."""
+        self._check(self.lib.ust_set_strain_weakening(self.ctx, None if sw is None else C.byref(sw)))
+
+    def strain_weakening_initialise(self):
+        self._check(self.lib.ust_strain_weakening_initialise(self.ctx))
+
+    def strain_weakening_step(self, dt):
+        self._check(self.lib.ust_strain_weakening_step(self.ctx, float(dt)))
+
+    def strain_weakening(self):
+        """(postFailureWeakening, postFailureWeakeningIncrement) per particle, swarm order."""
+        n = self.lib.ust_swarm_count(self.ctx)
+        pfw, inc = np.zeros(n), np.zeros(n)
+        self._check(self.lib.ust_get_strain_weakening(self.ctx, _ptr(pfw), _ptr(inc)))
+        return pfw, inc
+
+    def set_strain_weakening_state(self, pfw=None, inc=None):
+        pfw = None if pfw is None else np.ascontiguousarray(pfw, dtype=np.float64)
+        inc = None if inc is None else np.ascontiguousarray(inc, dtype=np.float64)
+        self._check(self.lib.ust_set_strain_weakening_state(self.ctx, _ptr(pfw), _ptr(inc)))
+
+    def tensile_failure(self):
+        n = self.lib.ust_swarm_count(self.ctx)
+        t = np.zeros(n, np.uint8)
+        self._check(self.lib.ust_get_tensile_failure(self.ctx, _ptr(t)))
+        return t
 
     def set_buoyancy(self, term=None):
         """BuoyancyForceTerm options (None: the defaults)."""

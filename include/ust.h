@@ -89,7 +89,8 @@ int ust_set_mesh(ust_context *ctx, int dim, const int nel[3], const double xmin[
 /* ---- materials: RheologyMaterial "Rheology" list, in registration order (RheologyMaterial.c:240-270) ---- */
 int ust_set_materials(ust_context *ctx, int nmat, const ust_material *mats);
 /* XML component Type + keys -> rheology (Arrhenius.c:110-133, MaterialViscosity.c:115,
- * ViscosityLimiter.c:101-103, VonMises.c:157-159, YieldRheology.c:145, DruckerPrager.c:173-189) */
+ * ViscosityLimiter.c:101-103, VonMises.c:157-159, YieldRheology.c:140-145, DruckerPrager.c:173-189).  The
+ * "StrainWeakening" key takes 1 when the rheology names the context's StrainWeakening component. */
 int ust_rheology_from_xml(const char *type_name, int nkeys, const char *const *keys, const double *values,
                           ust_rheology *out);
 
@@ -146,6 +147,38 @@ int ust_get_cohesion_scale(ust_context *ctx, float *coh);
  * referenceTemperature, beta, referenceDepth are ust_material fields): VerticalAxis, gravityDirection,
  * adjust, HydrostaticTerm.  NULL restores the defaults (VerticalAxis 1, force along -y). ---- */
 int ust_set_buoyancy(ust_context *ctx, const ust_buoyancy *b);
+
+/* ---- StrainWeakening component (StrainWeakening.c:217-280 keys, defaults as there) ---- */
+typedef struct {
+    double softening_strain;          /* "softeningStrain" (HUGE_VAL) */
+    double healing_rate;              /* "healingRate" (0) */
+    double initial_damage_fraction;   /* "initialDamageFraction" (0) */
+    double initial_damage_wavenumber; /* "initialDamageWavenumber" (-1: off) */
+    double wavenumber_sin[3];         /* "initialDamageWavenumberSinI/J/K" (-1: off) */
+    double wavenumber_cos[3];         /* "initialDamageWavenumberCosI/J/K" (-1: off) */
+    double initial_damage_factor;     /* "initialDamageFactor" (1) */
+    long random_seed;                 /* "randomSeed" (0) */
+    int has_shape;                    /* "initialStrainShape" given */
+    ust_shape initial_strain_shape;
+} ust_strain_weakening;
+void ust_strain_weakening_default(ust_strain_weakening *sw);
+/* Turns the per-particle weakening state on (sw) or off (NULL).  Yield rheologies whose StrainWeakening slot
+ * is set read it (VonMises.c:194-199, DruckerPrager.c:283-291) and every assembly with a previous solution
+ * writes the increment (YieldRheology.c:224-226).  The state starts at 0 for every particle. */
+int ust_set_strain_weakening(ust_context *ctx, const ust_strain_weakening *sw);
+/* _StrainWeakening_Initialise (StrainWeakening.c:318-399): the initial damage of every particle, in swarm order,
+ * from the C library's srand(randomSeed)/rand() sequence as the reference draws it, modulated by the
+ * wavenumber factors and restricted to initialStrainShape; increments set to 0. */
+int ust_strain_weakening_initialise(ust_context *ctx);
+/* one TimeIntegrand step of postFailureWeakening: += dt * increment, then negatives to 0
+ * (StrainWeakening.c:401-426) */
+int ust_strain_weakening_step(ust_context *ctx, double dt);
+/* per-particle state in swarm order (np each; either pointer may be NULL) */
+int ust_get_strain_weakening(ust_context *ctx, double *pfw, double *pfw_inc);
+int ust_set_strain_weakening_state(ust_context *ctx, const double *pfw, const double *pfw_inc);
+/* DruckerPrager tensileFailure flag of every particle from the last assembly with a previous solution
+ * (DruckerPrager.c:312) */
+int ust_get_tensile_failure(ust_context *ctx, unsigned char *tensile);
 
 /* ---- fields read by the rheologies (PpcManager lookups, PpcManager.c:503-606) ---- */
 int ust_set_temperature(ust_context *ctx, const double *T_nodal);

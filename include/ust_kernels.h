@@ -30,7 +30,15 @@ extern "C" {
 #define UST_ESTATE (-4)
 #define UST_ENOTCONV (-5)
 
-/* Rheology kinds (Underworld/Rheology component Type names); parameters as in ust.h. */
+/* Rheology kinds (Underworld/Rheology component Type names).  Parameter slots p[]:
+ *   MaterialViscosity  p0 eta0
+ *   Arrhenius          p0 eta0, p1 activationEnergy, p2 activationVolume, p3 referenceTemperature
+ *   ViscosityLimiter   p0 maxViscosity, p1 minViscosity
+ *   VonMises           p0 cohesion, p1 minimumViscosity, p2 strainRateSoftening (0/1),
+ *                      p3 cohesionAfterSoftening, p4 StrainWeakening (1: softened by the strain-weakening state)
+ *   DruckerPrager      p0 cohesion, p1 frictionCoefficient, p2 minimumYieldStress, p3 minimumViscosity,
+ *                      p4 strainRateSoftening (0/1), p5 cohesionAfterSoftening,
+ *                      p6 frictionCoefficientAfterSoftening, p7 StrainWeakening (0/1) */
 #define UST_RH_NONE 0
 #define UST_RH_MATERIALVISCOSITY 1
 #define UST_RH_ARRHENIUS 2
@@ -42,7 +50,7 @@ extern "C" {
 
 typedef struct {
     int type;
-    double p[6];
+    double p[8];
 } ust_rheology;
 
 typedef struct {
@@ -52,6 +60,19 @@ typedef struct {
      * "referenceTemperature", "beta", "referenceDepth" (all 0 by default) */
     double density, alpha, ref_temperature, beta, ref_depth;
 } ust_material;
+
+/* Per-particle state of the yield rheologies.  pfw: StrainWeakening's postFailureWeakening (StrainWeakening.h
+ * particle extension; NULL: no strain weakening); the yield criterion of a rheology with the StrainWeakening
+ * flag interpolates its parameters by ratio = min(max(pfw, 0) / softening_strain, 1) (StrainWeakening.c:
+ * 484-503), and the assembly writes postFailureWeakeningIncrement to pfw_inc (StrainWeakening.c:448-482,
+ * YieldRheology.c:224-226).  tensile: DruckerPrager's tensileFailure flag (DruckerPrager.c:312), or NULL. */
+typedef struct {
+    double *pfw;
+    double *pfw_inc;
+    unsigned char *tensile;
+    double softening_strain; /* "softeningStrain" */
+    double healing_rate;     /* "healingRate" */
+} ust_yield_state;
 
 /* BuoyancyForceTerm component keys (BuoyancyForceTerm.c:139-160): "VerticalAxis" (default 1),
  * "gravityDirection" (gHat, used when use_direction), "adjust", and the optional "HydrostaticTerm" whose
@@ -146,14 +167,21 @@ const char *ust_dev_error_string(void);
 /* eta_out/yield_out (np) may be NULL.  temperature (nodal), velocity (nodal, interleaved dim),
  * pressure (per element), coh_scale (per particle) may be NULL when no rheology needs them.
  * WK (nq per element, tile-blocked), WG (nv*nel), WF (nv*nel): nq = 3^dim, nv = 2^dim.  ppc_hint: the
- * mean particle count per cell (sizes the lane segment of an element; any count is handled). */
+ * mean particle count per cell (sizes the lane segment of an element; any count is handled).  yield_state
+ * may be NULL (no strain weakening, no tensile flags). */
 int ust_k_particle_weights(const ust_grid *g, const int *cell_off, int ppc_hint, const double *xi0,
                            const double *xi1, const double *xi2, const double *w,
                            const unsigned char *mat, const float *coh_scale,
                            const ust_material *materials_dev, int nmat, const double *temperature,
                            const double *velocity, const double *pressure, int prev_solution,
-                           const ust_buoyancy *buoyancy, double *eta_out, unsigned char *yield_out, double *WK,
-                           double *WG, double *WF, void *stream);
+                           const ust_yield_state *yield_state, const ust_buoyancy *buoyancy, double *eta_out,
+                           unsigned char *yield_out, double *WK, double *WG, double *WF, void *stream);
+/* StrainWeakening time step: pfw += dt * pfw_inc, then negative values to 0 (the TimeIntegrand update of
+ * postFailureWeakening, whose time derivative is the stored increment, StrainWeakening.c:401-407, and
+ * _StrainWeakening_MakeValuesPositive, :411-426) */
+int ust_k_strain_weakening_step(double *pfw, const double *pfw_inc, int np, double dt, void *stream);
+/* dst[s] = src[perm[s]] for s < n (per-particle state carried through a swarm re-sort) */
+int ust_k_permute_f64(const unsigned *perm, const double *src, double *dst, int n, void *stream);
 /* BuoyancyForceTerm "adjust" (BuoyancyForceTerm.c:362-375): WF of each element times 2^dim / (sum of its particle
  * weights), the sum recovered from WG (sum_b WG[b] = detJ sum_p w_p) */
 int ust_k_adjust_force_weights(const ust_grid *g, const double *WG, double *WF, void *stream);

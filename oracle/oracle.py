@@ -24,7 +24,7 @@ class Mesh(C.Structure):
 
 
 class Rheology(C.Structure):
-    _fields_ = [("type", C.c_int), ("p", C.c_double * 6)]
+    _fields_ = [("type", C.c_int), ("p", C.c_double * 8)]
 
 
 class Material(C.Structure):
@@ -58,7 +58,9 @@ def make_buoyancy(vertical_axis=1, direction=None, adjust=False, hydrostatic=Non
 
 class Fields(C.Structure):
     _fields_ = [("temperature", C.c_void_p), ("velocity", C.c_void_p), ("pressure", C.c_void_p),
-                ("coh_scale", C.c_void_p), ("previous_solution_exists", C.c_int)]
+                ("coh_scale", C.c_void_p), ("previous_solution_exists", C.c_int), ("pfw", C.c_void_p),
+                ("pfw_inc", C.c_void_p), ("tensile", C.c_void_p), ("softening_strain", C.c_double),
+                ("healing_rate", C.c_double)]
 
 
 class CSR(C.Structure):
@@ -109,7 +111,8 @@ def lib():
         _LIB.orc_stokes_residual.restype = C.c_double
         _LIB.orc_particle_viscosity.restype = C.c_double
         _LIB.orc_particle_viscosity.argtypes = [C.POINTER(Mesh), C.POINTER(Material), C.c_int, C.c_void_p,
-                                                C.POINTER(Fields), C.c_float, C.c_void_p]
+                                                C.POINTER(Fields), C.c_float, C.c_long, C.c_void_p]
+        _LIB.orc_strain_weakening_step.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_double]
     return _LIB
 
 
@@ -199,7 +202,11 @@ def global_to_local(mesh, e, x):
 
 
 def eval_viscosity(mesh, materials, cell_off, xi, mat_of, temperature=None, velocity=None, pressure=None,
-                   coh_scale=None, previous_solution_exists=False):
+                   coh_scale=None, previous_solution_exists=False, strain_weakening=None, tensile=False):
+    """Per-particle viscosity and yield flag.  strain_weakening: None, or a dict with pfw (per particle,
+    clamped in place to >= 0 where read), softening_strain and healing_rate; the increment is then returned in
+    its "pfw_inc" entry (kept from the dict's "pfw_inc" where no increment is assigned).  tensile=True also
+    returns the DruckerPrager tensileFailure flags."""
     np_ = xi.shape[0]
     mats = (Material * len(materials))(*materials)
     f = Fields()
@@ -211,11 +218,29 @@ def eval_viscosity(mesh, materials, cell_off, xi, mat_of, temperature=None, velo
             keep.append(a)
             setattr(f, name, a.ctypes.data)
     f.previous_solution_exists = int(previous_solution_exists)
+    if strain_weakening is not None:
+        sw = strain_weakening
+        sw["pfw"] = np.ascontiguousarray(sw["pfw"], dtype=np.float64)
+        sw["pfw_inc"] = np.ascontiguousarray(sw.get("pfw_inc", np.zeros(np_)), dtype=np.float64).copy()
+        f.pfw, f.pfw_inc = sw["pfw"].ctypes.data, sw["pfw_inc"].ctypes.data
+        f.softening_strain, f.healing_rate = float(sw["softening_strain"]), float(sw.get("healing_rate", 0.0))
+    tens = np.zeros(np_, dtype=np.uint8)
+    if tensile:
+        f.tensile = tens.ctypes.data
     eta = np.zeros(np_)
     yielded = np.zeros(np_, dtype=np.uint8)
     lib().orc_eval_viscosity(C.byref(mesh), mats, np_, _p(cell_off.astype(np.int32)), _p(np.ascontiguousarray(xi)),
                              _p(mat_of.astype(np.int32)), C.byref(f), _p(eta), _p(yielded))
+    if tensile:
+        return eta, yielded, tens
     return eta, yielded
+
+
+def strain_weakening_step(pfw, inc, dt):
+    pfw = np.ascontiguousarray(pfw, dtype=np.float64).copy()
+    lib().orc_strain_weakening_step(len(pfw), _p(pfw), _p(np.ascontiguousarray(inc, dtype=np.float64)),
+                                    C.c_double(dt))
+    return pfw
 
 
 def eval_density(mesh, materials, cell_off, xi, mat_of, temperature=None, buoyancy=None):

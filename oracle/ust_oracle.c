@@ -354,7 +354,7 @@ static void particle_strain_rate(const ORC_Mesh *m, int e, const double GNx[3][8
 }
 
 double orc_particle_viscosity(const ORC_Mesh *m, const ORC_Material *mat, int e, const double *xi,
-                              const ORC_Fields *f, float coh_scale, unsigned char *yielded) {
+                              const ORC_Fields *f, float coh_scale, long p, unsigned char *yielded) {
     int dim = m->dim, nen = 1 << dim, nodes[8];
     double eta = 0.0;
     orc_element_nodes(m, e, nodes);
@@ -398,11 +398,24 @@ double orc_particle_viscosity(const ORC_Mesh *m, const ORC_Material *mat, int e,
             int nsym = (dim == 2) ? 3 : 6;
             for (int i = 0; i < nsym; i++) st[i] = (i < dim) ? 2.0 * eta * sr[i] : eta * 2.0 * sr[i];
             double indicator = second_invariant(dim, st); /* VonMises.c:206-230 */
-            double cohesion = rh->p[0] * (double)coh_scale, criterion;
+            /* StrainWeakening_CalcRatio (StrainWeakening.c:484-503): the stored value is clamped to >= 0 */
+            int sw = f->pfw && rh->p[dp ? 7 : 4] != 0.0;
+            double ratio = 0.0, pfw = 0.0;
+            if (sw) {
+                if (f->pfw[p] < 0.0) f->pfw[p] = 0.0;
+                pfw = f->pfw[p];
+                ratio = pfw / f->softening_strain;
+                if (ratio > 1.0) ratio = 1.0;
+            }
+            /* effective cohesion, VonMises.c:194-199 / DruckerPrager.c:283-291 (the cohesion factor of the
+             * swarm's noise scales the softened cohesion) */
+            double cohesion = (rh->p[0] * (1.0 - ratio) + rh->p[dp ? 5 : 3] * ratio) * (double)coh_scale;
+            double criterion;
             if (!dp) {
                 criterion = cohesion; /* VonMises.c:182-204 */
             } else {                  /* DruckerPrager.c:236-321 */
-                double pressure = f->pressure[e], phi = atan(rh->p[1]), k, c;
+                double friction = rh->p[1] * (1.0 - ratio) + rh->p[6] * ratio;
+                double pressure = f->pressure[e], phi = atan(friction), k, c;
                 if (dim == 3) {
                     double sinphi = sin(phi), oneOver = 1.0 / (sqrt(3.0) * (3.0 - sinphi));
                     k = 6.0 * sinphi * oneOver;
@@ -412,7 +425,16 @@ double orc_particle_viscosity(const ORC_Mesh *m, const ORC_Material *mat, int e,
                     k = sin(phi);
                 }
                 criterion = k * pressure + c;
+                if (f->tensile) f->tensile[p] = (unsigned char)(criterion <= 0.0); /* :312 */
                 if (criterion < rh->p[2]) criterion = rh->p[2];
+            }
+            if (sw) { /* StrainWeakening_AssignIncrement before yielding (YieldRheology.c:224-226,
+                         _StrainWeakening_CalcIncrementIsotropic, StrainWeakening.c:448-482) */
+                double beta = (criterion == 0.0 || indicator == 0.0) ? 0.0 : 1.0 - criterion / indicator;
+                if (beta <= 0.0)
+                    f->pfw_inc[p] = -f->healing_rate * pfw;
+                else
+                    f->pfw_inc[p] = (criterion / eta) * (beta / (1.0 - beta) - f->healing_rate * pfw);
             }
             if (yielded) *yielded = (unsigned char)(criterion < indicator);
             if (criterion < indicator) { /* VonMises.c:232-269 */
@@ -434,6 +456,13 @@ double orc_particle_viscosity(const ORC_Mesh *m, const ORC_Material *mat, int e,
     return eta;
 }
 
+void orc_strain_weakening_step(int np, double *pfw, const double *inc, double dt) {
+    for (int i = 0; i < np; i++) {
+        pfw[i] += dt * inc[i];
+        if (pfw[i] < 0.0) pfw[i] = 0.0;
+    }
+}
+
 void orc_eval_viscosity(const ORC_Mesh *m, const ORC_Material *mats, int np, const int *cell_off,
                         const double *xi, const int *mat_of, const ORC_Fields *f, double *eta,
                         unsigned char *yielded) {
@@ -443,7 +472,7 @@ void orc_eval_viscosity(const ORC_Mesh *m, const ORC_Material *mats, int np, con
     for (int e = 0; e < nel; e++)
         for (int p = cell_off[e]; p < cell_off[e + 1]; p++) {
             float cs = f->coh_scale ? f->coh_scale[p] : 1.0f;
-            eta[p] = orc_particle_viscosity(m, &mats[mat_of[p]], e, &xi[3 * (size_t)p], f, cs,
+            eta[p] = orc_particle_viscosity(m, &mats[mat_of[p]], e, &xi[3 * (size_t)p], f, cs, p,
                                             yielded ? &yielded[p] : NULL);
         }
 }

@@ -33,15 +33,18 @@ enum {
     ORC_RH_MATERIALVISCOSITY = 1, /* p0 = eta0 */
     ORC_RH_ARRHENIUS = 2,         /* p0 = eta0, p1 = activationEnergy, p2 = activationVolume, p3 = referenceTemperature */
     ORC_RH_VISCOSITYLIMITER = 3,  /* p0 = maxViscosity, p1 = minViscosity */
-    ORC_RH_VONMISES = 4,          /* p0 = cohesion, p1 = minimumViscosity, p2 = strainRateSoftening(0/1) */
+    ORC_RH_VONMISES = 4,          /* p0 = cohesion, p1 = minimumViscosity, p2 = strainRateSoftening(0/1),
+                                     p3 = cohesionAfterSoftening, p4 = StrainWeakening(0/1) */
     ORC_RH_DRUCKERPRAGER = 5      /* p0 = cohesion, p1 = frictionCoefficient, p2 = minimumYieldStress,
-                                     p3 = minimumViscosity, p4 = strainRateSoftening(0/1) */
+                                     p3 = minimumViscosity, p4 = strainRateSoftening(0/1),
+                                     p5 = cohesionAfterSoftening, p6 = frictionCoefficientAfterSoftening,
+                                     p7 = StrainWeakening(0/1) */
 };
 
 #define ORC_MAX_RHEOLOGIES 4
 typedef struct {
     int type;
-    double p[6];
+    double p[8];
 } ORC_Rheology;
 
 typedef struct {
@@ -72,6 +75,13 @@ typedef struct {
     const double *pressure;      /* per element (P0) or NULL */
     const float  *coh_scale;     /* per particle cohesion factor (1+noise) or NULL */
     int previous_solution_exists;
+    /* StrainWeakening (StrainWeakening.c): postFailureWeakening per particle (NULL: none; negative values are
+     * clamped to 0 in place when read, :491-492), the increment written per particle, and the component's
+     * softeningStrain / healingRate; DruckerPrager tensileFailure per particle (NULL: not kept) */
+    double *pfw;
+    double *pfw_inc;
+    unsigned char *tensile;
+    double softening_strain, healing_rate;
 } ORC_Fields;
 
 /* ---- random particle layout ---- */
@@ -99,8 +109,11 @@ void orc_map_particles(const ORC_Mesh *m, int np, const double *coords, const in
                        double *xi, double *w);
 
 /* ---- rheology chain (one particle) ---- */
+/* p: the particle's index in the per-particle arrays of f */
 double orc_particle_viscosity(const ORC_Mesh *m, const ORC_Material *mat, int e, const double *xi,
-                              const ORC_Fields *f, float coh_scale, unsigned char *yielded);
+                              const ORC_Fields *f, float coh_scale, long p, unsigned char *yielded);
+/* StrainWeakening time step (StrainWeakening.c:401-426): pfw += dt * inc, then negatives to 0 */
+void orc_strain_weakening_step(int np, double *pfw, const double *inc, double dt);
 void   orc_eval_viscosity(const ORC_Mesh *m, const ORC_Material *mats, int np, const int *cell_off,
                           const double *xi, const int *mat_of, const ORC_Fields *f,
                           double *eta, unsigned char *yielded);

@@ -338,3 +338,91 @@ def test_buoyancy_direction_adjust_hydrostatic(oracle):
     y = coords[:, 1]
     bg = np.where(y > 0.9, 0.0, np.where(y > 0.4, 1.0 * (1 - 0.1 * 0.5), 2.0))
     assert np.allclose(rh, 2.5 - bg, rtol=1e-13, atol=1e-13)
+
+
+def _pure_shear_particle(O):
+    """One particle in a unit 2D element under u = (y, 0): strain rate xy = 1/2, so tau_II = eta."""
+    m = O.make_mesh(2, (1, 1))
+    vel = np.zeros((4, 2))
+    vel[:, 0] = [0, 0, 1, 1]
+    return m, np.array([[0.1, -0.3, 0.0]]), np.array([0, 1], np.int32), vel.reshape(-1)
+
+
+def test_strain_weakening_von_mises_closed_forms(oracle):
+    """StrainWeakening (StrainWeakening.c:448-503) with VonMises (VonMises.c:182-199) on one particle with
+    tau_II = eta0 = 3: the cohesion is interpolated by ratio = min(max(pfw, 0) / softeningStrain, 1), the
+    yielded viscosity equals the softened cohesion, and the increment obeys
+    inc = tau_y (1/eta_y - 1/eta0) - healingRate pfw tau_y / eta0 (the plastic strain rate of the yielded
+    particle less healing); below yield inc = -healingRate pfw; negative pfw reads (and is stored) as 0."""
+    O = oracle
+    m, xi, off, vel = _pure_shear_particle(O)
+    mat = O.make_material([(O.RH_MATERIALVISCOSITY, (3.0,)), (O.RH_VONMISES, (0.4, 0.01, 0, 0.1, 1))])
+    for pfw, heal in [(0.0, 0.0), (0.5, 0.0), (1.0, 0.2), (2.0, 0.2), (3.0, 0.0), (-1.0, 0.3)]:
+        sw = {"pfw": np.array([pfw]), "softening_strain": 2.0, "healing_rate": heal}
+        eta, y = O.eval_viscosity(m, [mat], off, xi, np.zeros(1, np.int32), velocity=vel,
+                                  previous_solution_exists=True, strain_weakening=sw)
+        pc = max(pfw, 0.0)
+        r = min(pc / 2.0, 1.0)
+        crit = 0.4 * (1 - r) + 0.1 * r
+        assert y[0] == 1 and abs(eta[0] - crit) < 1e-14
+        inc = crit * (1 / eta[0] - 1 / 3.0) - heal * pc * crit / 3.0
+        assert abs(sw["pfw_inc"][0] - inc) < 1e-14
+        assert sw["pfw"][0] == pc
+    # below yield (cohesion 5 > tau_II = 3): healing only; no strain-weakening flag: no increment
+    strong = O.make_material([(O.RH_MATERIALVISCOSITY, (3.0,)), (O.RH_VONMISES, (5.0, 0.01, 0, 5.0, 1))])
+    sw = {"pfw": np.array([0.7]), "softening_strain": 2.0, "healing_rate": 0.25}
+    eta, y = O.eval_viscosity(m, [strong], off, xi, np.zeros(1, np.int32), velocity=vel,
+                              previous_solution_exists=True, strain_weakening=sw)
+    assert y[0] == 0 and eta[0] == 3.0 and abs(sw["pfw_inc"][0] + 0.25 * 0.7) < 1e-15
+    plain = O.make_material([(O.RH_MATERIALVISCOSITY, (3.0,)), (O.RH_VONMISES, (0.4, 0.01, 0, 0.1, 0))])
+    sw = {"pfw": np.array([1.0]), "pfw_inc": np.array([7.0]), "softening_strain": 2.0}
+    eta, _ = O.eval_viscosity(m, [plain], off, xi, np.zeros(1, np.int32), velocity=vel,
+                              previous_solution_exists=True, strain_weakening=sw)
+    assert abs(eta[0] - 0.4) < 1e-14 and sw["pfw_inc"][0] == 7.0
+    # before the first solve nothing is assigned
+    sw = {"pfw": np.array([1.0]), "pfw_inc": np.array([7.0]), "softening_strain": 2.0}
+    O.eval_viscosity(m, [mat], off, xi, np.zeros(1, np.int32), velocity=vel, previous_solution_exists=False,
+                     strain_weakening=sw)
+    assert sw["pfw_inc"][0] == 7.0
+    # the time step: pfw + dt inc, negatives to 0
+    assert np.array_equal(O.strain_weakening_step(np.array([1.0, 0.2, 0.0]), np.array([0.5, -1.0, 2.0]), 0.4),
+                          np.array([1.2, 0.0, 0.8]))
+
+
+def test_strain_weakening_drucker_prager_and_tensile_flag(oracle):
+    """DruckerPrager.c:283-315: a softened particle (ratio r) behaves as the unsoftened rheology with cohesion and
+    friction interpolated by r, in 2D and 3D; tensileFailure is set where the frictional strength k p + c <= 0,
+    before the minimumYieldStress clamp (DruckerPrager.c:312-315)."""
+    O = oracle
+    for dim in (2, 3):
+        m = O.make_mesh(dim, (1,) * dim)
+        vel = np.zeros((2 ** dim, dim))
+        vel[:, 0] = [(a >> 1) & 1 for a in range(2 ** dim)]
+        off = np.array([0, 1], np.int32)
+        xi = np.array([[0.1, -0.3, 0.2]]) if dim == 3 else np.array([[0.1, -0.3, 0.0]])
+        for pfw in (0.0, 0.6, 1.5, 4.0):
+            r = min(pfw / 1.5, 1.0)
+            sw = {"pfw": np.array([pfw]), "softening_strain": 1.5}
+            soft = O.make_material([(O.RH_MATERIALVISCOSITY, (3.0,)),
+                                    (O.RH_DRUCKERPRAGER, (0.9, 0.6, 0.0, 0.01, 0, 0.2, 0.1, 1))])
+            same = O.make_material([(O.RH_MATERIALVISCOSITY, (3.0,)),
+                                    (O.RH_DRUCKERPRAGER, (0.9 * (1 - r) + 0.2 * r, 0.6 * (1 - r) + 0.1 * r, 0.0, 0.01,
+                                                          0))])
+            args = dict(velocity=vel.reshape(-1), pressure=np.array([1.3]), previous_solution_exists=True)
+            e1, y1 = O.eval_viscosity(m, [soft], off, xi, np.zeros(1, np.int32), strain_weakening=sw, **args)
+            e2, y2 = O.eval_viscosity(m, [same], off, xi, np.zeros(1, np.int32), **args)
+            assert abs(e1[0] - e2[0]) < 1e-14 * e2[0] and y1[0] == y2[0]
+        # tensile failure: 2D k = sin(phi), c = C cos(phi); 3D the triaxial-compression constants
+        phi = np.arctan(0.6)
+        if dim == 2:
+            k, c = np.sin(phi), 0.3 * np.cos(phi)
+        else:
+            d = np.sqrt(3) * (3 - np.sin(phi))
+            k, c = 6 * np.sin(phi) / d, 6 * 0.3 * np.cos(phi) / d
+        mat = O.make_material([(O.RH_MATERIALVISCOSITY, (3.0,)), (O.RH_DRUCKERPRAGER, (0.3, 0.6, 0.05, 0.01, 0))])
+        for p in (-1.0, -0.9 * c / k, -1.1 * c / k, 1.0):
+            eta, y, t = O.eval_viscosity(m, [mat], off, xi, np.zeros(1, np.int32), velocity=vel.reshape(-1),
+                                         pressure=np.array([p]), previous_solution_exists=True, tensile=True)
+            fs = k * p + c
+            assert t[0] == (1 if fs <= 0 else 0)
+            assert abs(eta[0] - max(fs, 0.05)) < 1e-14  # yielded: eta_y = criterion (tau_II = eta)
