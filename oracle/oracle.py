@@ -271,6 +271,20 @@ def element_forces(mesh, cell_off, xi, w, rho, gravity=1.0, buoyancy=None):
     return fe
 
 
+def element_matrices(mesh, elements, cell_off, xi, w, eta):
+    """K_e (len(elements), nd, nd) of the listed elements from their particles (cell_off indexes xi, w, eta)."""
+    nd = (1 << mesh.dim) * mesh.dim
+    out = np.zeros((len(elements), nd, nd))
+    xi, w, eta = np.ascontiguousarray(xi), np.ascontiguousarray(w), np.ascontiguousarray(eta)
+    for k, e in enumerate(elements):
+        p0, n = int(cell_off[e]), int(cell_off[e + 1] - cell_off[e])
+        ke = np.zeros(nd * nd)
+        lib().orc_element_K(C.byref(mesh), int(e), n, _p(np.ascontiguousarray(xi[p0:p0 + n])),
+                            _p(np.ascontiguousarray(w[p0:p0 + n])), _p(np.ascontiguousarray(eta[p0:p0 + n])), _p(ke))
+        out[k] = ke.reshape(nd, nd)
+    return out
+
+
 def assemble_elements(mesh, cell_off, xi, w, eta, rho=None, gravity=1.0):
     nel = mesh_nel(mesh)
     nd = (1 << mesh.dim) * mesh.dim

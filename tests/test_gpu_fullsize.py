@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from tests.helpers import rel_err
+from tests.helpers import max_rel_err, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -52,4 +52,60 @@ def test_random_field_128x128x64_full_size():
     from earthbyte_underworld_amd.configs import random_field_3d
     s = random_field_3d(nel=(128, 128, 64), ppc=20, seed=4242)
     _pair(s)
+    s.close()
+
+
+def test_slab_256x256x128_full_size(oracle):
+    """The full-pipeline configuration: 8.4M Q1/P0 elements (256 x 128 x 256, y vertical, box 2 x 1 x 2), 20
+    particles per cell (168M particles), layered slab + inclusions with a 1e6 viscosity contrast, free-slip.
+    Checked on samples, as the oracle cannot assemble or solve it in test time: K_e of 1000 random elements within
+    1e-12 of the oracle's from the same particles; K u on 300 random free rows within 1e-12 of the oracle's element
+    sum; and the mixed-precision solve with the gkgdiag Schur preconditioner at the FP64 Stokes residual bar."""
+    import earthbyte_underworld_amd as ust
+    from earthbyte_underworld_amd import _lib as L
+    from earthbyte_underworld_amd.configs import slab_3d
+    from tests.helpers import oracle_mesh
+    s = slab_3d()
+    assert s.lib.ust_swarm_count(s.ctx) == 20 * s.nel == 167772160
+    s.assemble(gravity=1.0)
+    off, xi, w, mat = s.swarm()
+    eta = np.array([m.rh[0].p[0] for m in s.materials])[mat]  # MaterialViscosity: eta = eta0 (every material here)
+    eta_gpu, _ = s.particle_viscosity()
+    assert np.array_equal(eta_gpu, eta)
+    mesh = oracle_mesh(s)
+    rng = np.random.default_rng(8)
+    sample = np.sort(rng.choice(s.nel, 1000, replace=False))
+    Ke_o = oracle.element_matrices(mesh, sample, off, xi, w, eta)
+    Ke_g = np.array([s.element_matrices(int(e), 1)[0] for e in sample])
+    assert max_rel_err(Ke_g, Ke_o) < 1e-12
+    # K u on sampled rows: the sum over the (up to 8) elements of the row's node, Dirichlet columns masked
+    u = rng.normal(size=s.ndof)
+    y = s.apply_K(u)
+    um = np.where(s.bc_mask == 0, u, 0.0)
+    nx, ny, nz = s.nel_axis
+    nnx, nny = nx + 1, ny + 1
+    free = np.flatnonzero(s.bc_mask == 0)
+    rows = rng.choice(free, 300, replace=False)
+    ref = np.zeros(len(rows))
+    for q, r in enumerate(rows):
+        node, c = divmod(int(r), 3)
+        i, j, k = node % nnx, (node // nnx) % nny, node // (nnx * nny)
+        for ez in (k - 1, k):
+            for ey in (j - 1, j):
+                for ex in (i - 1, i):
+                    if 0 <= ex < nx and 0 <= ey < ny and 0 <= ez < nz:
+                        e = ex + nx * (ey + ny * ez)
+                        ke = oracle.element_matrices(mesh, [e], off, xi, w, eta)[0]
+                        a = (i - ex) + 2 * (j - ey) + 4 * (k - ez)
+                        n0 = ex + nnx * (ey + nny * ez)
+                        dofs = [(n0 + (b & 1) + nnx * ((b >> 1) & 1) + nnx * nny * ((b >> 2) & 1)) * 3 + d
+                                for b in range(8) for d in range(3)]
+                        ref[q] += ke[a * 3 + c] @ um[dofs]
+    assert max_rel_err(y[rows], ref) < 1e-12
+    st = s.solve(ust.default_opts(precision=L.PREC_MIXED, schur_pc=L.SCHUR_GKGDIAG, inner_relax=1, smoother_degree=1,
+                                  coarse_degree=2, cheby_ratio=30.0, outer_rtol=1e-7, inner_rtol=1e-8, max_outer=3000,
+                                  stokes_rtol=1e-6))
+    print(f"slab 256x128x256 gkgdiag mixed: {st['outer_its']} outer / {st['inner_its']} inner, "
+          f"residual {st['stokes_residual']:.2e}")
+    assert st["converged"] and st["stokes_residual"] < 1e-6
     s.close()
