@@ -50,6 +50,19 @@ class StrainWeakening(C.Structure):
                 ("initial_strain_shape", Shape)]
 
 
+class Pcdvc(C.Structure):
+    _fields_ = [("res", C.c_int * 3), ("upper_t", C.c_double), ("lower_t", C.c_double),
+                ("max_deletions", C.c_int), ("max_splits", C.c_int), ("split_in_interface_cells", C.c_int),
+                ("delete_in_interface_cells", C.c_int), ("allow_splitting_in_boundary_interface_cells", C.c_int),
+                ("inflow", C.c_int)]
+
+
+class PcdvcOpts(C.Structure):
+    _fields_ = [("upper_t", C.c_double), ("lower_t", C.c_double), ("max_deletions", C.c_int),
+                ("max_splits", C.c_int), ("split_in_interface_cells", C.c_int),
+                ("delete_in_interface_cells", C.c_int)]
+
+
 class Grid(C.Structure):
     _fields_ = [("dim", C.c_int), ("nel", C.c_int * 3), ("h", C.c_double * 3), ("xmin", C.c_double * 3),
                 ("ymax", C.c_double), ("xmax", C.c_double * 3)]
@@ -111,6 +124,8 @@ _SIGS = {
     "ust_set_temperature": (I, [vp, vp]),
     "ust_set_buoyancy": (I, [vp, C.POINTER(Buoyancy)]),
     "ust_strain_weakening_default": (None, [C.POINTER(StrainWeakening)]),
+    "ust_pcdvc_default": (None, [C.POINTER(Pcdvc)]),
+    "ust_swarm_pcdvc": (I, [vp, C.POINTER(Pcdvc), C.POINTER(I), C.POINTER(I)]),
     "ust_set_strain_weakening": (I, [vp, C.POINTER(StrainWeakening)]),
     "ust_strain_weakening_initialise": (I, [vp]),
     "ust_strain_weakening_step": (I, [vp, C.c_double]),
@@ -192,6 +207,9 @@ _SIGS = {
                                    vp, vp, vp, vp, vp, vp]),
     "ust_k_strain_weakening_step": (I, [vp, vp, I, C.c_double, vp]),
     "ust_k_permute_f64": (I, [vp, vp, vp, I, vp]),
+    "ust_k_pcdvc_plan": (I, [C.POINTER(Grid), vp, vp, vp, C.POINTER(PcdvcOpts), vp, vp, vp, vp]),
+    "ust_k_pcdvc_build": (I, [I, I, I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                              vp, vp]),
     "ust_event_create": (I, [C.POINTER(vp)]),
     "ust_event_destroy": (I, [vp]),
     "ust_event_record": (I, [vp, vp]),

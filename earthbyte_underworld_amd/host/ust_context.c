@@ -1070,6 +1070,117 @@ int ust_swarm_advect(ust_context *ctx, double dt, int order, int allow_fallback,
     return UST_OK;
 }
 
+void ust_pcdvc_default(ust_pcdvc *o) {
+    memset(o, 0, sizeof(*o));
+    o->res[0] = o->res[1] = o->res[2] = 10;
+    o->upper_t = 25.0;
+    o->lower_t = 0.6;
+    o->max_deletions = 2;
+    o->max_splits = 3;
+}
+
+int ust_swarm_pcdvc(ust_context *ctx, const ust_pcdvc *opts, int *n_split, int *n_deleted) {
+    if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");
+    if (!opts) UST_FAIL(ctx, UST_EINVAL, "PCDVC options missing");
+    if (opts->inflow || opts->allow_splitting_in_boundary_interface_cells)
+        UST_FAIL(ctx, UST_EINVAL, "PCDVC: the Inflow branch (random re-seeding of elements, PCDVC.c:1216-1312) is "
+                                  "not supported");
+    if (opts->max_splits < 0 || opts->max_deletions < 0 || opts->max_splits > 4096)
+        UST_FAIL(ctx, UST_EINVAL, "PCDVC: maxSplits %d / maxDeletions %d out of range", opts->max_splits,
+                 opts->max_deletions);
+    ust_pcdvc_opts ko;
+    ko.upper_t = opts->upper_t;
+    ko.lower_t = opts->lower_t;
+    if (ko.upper_t < ko.lower_t) { /* the reference resets both (PCDVC.c:270-274) */
+        ko.lower_t = 0.6;
+        ko.upper_t = 25.0;
+    }
+    ko.max_deletions = opts->max_deletions;
+    ko.max_splits = opts->max_splits;
+    ko.split_in_interface_cells = opts->split_in_interface_cells;
+    ko.delete_in_interface_cells = opts->delete_in_interface_cells;
+    const int dim = ctx->grid.dim, nel = ctx->nel, np = ctx->np;
+    /* 1. the regions of the current particles (weights and centroids in the integration points) */
+    int rc = ust_swarm_dvc_weights(ctx, opts->res, 1);
+    if (rc) return rc;
+    /* 2. the per-element plans */
+    void *cnt = NULL, *list = NULL, *flag = NULL, *noff = NULL, *o = NULL, *ow = NULL, *omat = NULL, *ocoh = NULL;
+    void *os0 = NULL, *os1 = NULL;
+    int hflag[4] = {0, 0, 0, 0};
+    int *cnt_h = NULL, *off_h = NULL;
+    const size_t lcap = (size_t)np + (size_t)nel * (size_t)ko.max_splits;
+    if (!rc) rc = ust_dev_malloc(&cnt, (size_t)nel * sizeof(int) + 8);
+    if (!rc) rc = ust_dev_malloc(&list, lcap * sizeof(int) + 8);
+    if (!rc) rc = ust_dev_malloc(&flag, 2 * sizeof(double));
+    if (!rc) rc = ust_k_fill_f64((double *)flag, 0.0, 2, ctx->stream); /* all-zero bits: the ints read 0 */
+    if (!rc) rc = ust_k_pcdvc_plan(&ctx->grid, ctx->cell_off, ctx->w, ctx->mat, &ko, (int *)cnt, (int *)list,
+                                   (int *)flag, ctx->stream);
+    if (!rc) rc = ust_memcpy_d2h(hflag, flag, 3 * sizeof(int), ctx->stream);
+    cnt_h = (int *)malloc((size_t)nel * sizeof(int));
+    off_h = (int *)malloc((size_t)(nel + 1) * sizeof(int));
+    if (!rc && (!cnt_h || !off_h)) rc = UST_ENOMEM;
+    if (!rc) rc = ust_memcpy_d2h(cnt_h, cnt, (size_t)nel * sizeof(int), ctx->stream);
+    long long total = 0;
+    int max_ppc = 0;
+    if (!rc) {
+        off_h[0] = 0;
+        for (int e = 0; e < nel; e++) {
+            total += cnt_h[e];
+            if (cnt_h[e] > max_ppc) max_ppc = cnt_h[e];
+            off_h[e + 1] = total > 2147483647LL ? 0 : (int)total;
+        }
+        if (total > 2147483647LL) rc = UST_EINVAL;
+    }
+    if (!rc && (hflag[0] & 1)) {
+        free(cnt_h);
+        free(off_h);
+        ust_dev_free(cnt);
+        ust_dev_free(list);
+        ust_dev_free(flag);
+        UST_FAIL(ctx, UST_EINVAL, "under resolved cell after population control: a changed element keeps fewer than 3 "
+                                  "particles (PCDVC.c:1418-1421)");
+    }
+    /* 3. the new swarm */
+    const size_t k = (size_t)total;
+    if (!rc) rc = ust_dev_malloc(&noff, (size_t)(nel + 1) * sizeof(int));
+    if (!rc) rc = ust_memcpy_h2d(noff, off_h, (size_t)(nel + 1) * sizeof(int), ctx->stream);
+    if (!rc) rc = ust_dev_malloc(&o, 3 * k * sizeof(double) + 8);
+    if (!rc) rc = ust_dev_malloc(&ow, k * sizeof(double) + 8);
+    if (!rc && ctx->mat) rc = ust_dev_malloc(&omat, k + 8);
+    if (!rc && ctx->coh) rc = ust_dev_malloc(&ocoh, k * sizeof(float) + 8);
+    if (!rc && ctx->pfw) rc = ust_dev_malloc(&os0, k * sizeof(double) + 8);
+    if (!rc && ctx->pfw) rc = ust_dev_malloc(&os1, k * sizeof(double) + 8);
+    double *o0 = (double *)o, *o1 = o0 + k, *o2 = o0 + 2 * k;
+    if (!rc)
+        rc = ust_k_pcdvc_build(dim, nel, ko.max_splits, ctx->cell_off, (const int *)noff, (const int *)list, ctx->xi0,
+                               ctx->xi1, ctx->xi2, ctx->ixi0, ctx->ixi1, ctx->ixi2, ctx->mat, ctx->coh, ctx->pfw,
+                               ctx->pfw_inc, o0, o1, dim == 3 ? o2 : NULL, (unsigned char *)omat, (float *)ocoh,
+                               (double *)os0, (double *)os1, ctx->stream);
+    if (!rc) rc = ust_k_fill_f64((double *)ow, 0.0, k, ctx->stream);
+    if (!rc) {
+        rc = ust_swarm_upload_dev(ctx, (int)k, (const int *)noff, o0, o1, dim == 3 ? o2 : NULL, (const double *)ow,
+                                  (const unsigned char *)omat, (const float *)ocoh, max_ppc);
+        if (!rc && os0) { /* upload_dev released the old state: the rebuilt arrays become the state */
+            ctx->pfw = (double *)os0;
+            ctx->pfw_inc = (double *)os1;
+            os0 = os1 = NULL;
+        }
+    } else if (!ctx->err[0]) {
+        snprintf(ctx->err, sizeof(ctx->err), "PCDVC population control failed (%d): %s", rc, ust_dev_error_string());
+    }
+    free(cnt_h);
+    free(off_h);
+    void *tmp[] = {cnt, list, flag, noff, o, ow, omat, ocoh, os0, os1};
+    for (size_t i = 0; i < sizeof(tmp) / sizeof(tmp[0]); i++) ust_dev_free(tmp[i]);
+    if (rc) return rc;
+    /* 4. the regions of the new particles: integration points at the centroids, weights the volumes */
+    rc = ust_swarm_dvc_weights(ctx, opts->res, 1);
+    if (rc) return rc;
+    if (n_split) *n_split = hflag[1];
+    if (n_deleted) *n_deleted = hflag[2];
+    return UST_OK;
+}
+
 int ust_swarm_count(ust_context *ctx) { return ctx->np; }
 
 int ust_swarm_download_integration_points(ust_context *ctx, double *xi, double *w) {

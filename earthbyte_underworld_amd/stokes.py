@@ -445,6 +445,24 @@ class StokesSystem:
         arr = (C.c_int * 3)(*res[:3])
         self._check(self.lib.ust_swarm_dvc_weights(self.ctx, arr, int(bool(set_centroids))))
 
+    def pcdvc(self, resolution=10, upperT=25.0, lowerT=0.6, maxDeletions=2, maxSplits=3,
+              splitInInterfaceCells=False, deleteInInterfaceCells=False, allowSplittingInBoundaryInterfaceCells=False,
+              Inflow=False):
+        """PCDVC weights with population control (PCDVC component keys); returns (splits, deletions)."""
+        o = L.Pcdvc()
+        self.lib.ust_pcdvc_default(C.byref(o))
+        res = [resolution] * 3 if np.isscalar(resolution) else list(resolution) + [1] * (3 - len(resolution))
+        o.res[:] = [int(r) for r in res[:3]]
+        o.upper_t, o.lower_t = float(upperT), float(lowerT)
+        o.max_deletions, o.max_splits = int(maxDeletions), int(maxSplits)
+        o.split_in_interface_cells = int(bool(splitInInterfaceCells))
+        o.delete_in_interface_cells = int(bool(deleteInInterfaceCells))
+        o.allow_splitting_in_boundary_interface_cells = int(bool(allowSplittingInBoundaryInterfaceCells))
+        o.inflow = int(bool(Inflow))
+        ns, nd = C.c_int(), C.c_int()
+        self._check(self.lib.ust_swarm_pcdvc(self.ctx, C.byref(o), C.byref(ns), C.byref(nd)))
+        return ns.value, nd.value
+
     def solution(self):
         u = np.zeros(self.ndof)
         p = np.zeros(self.nel)

@@ -148,6 +148,26 @@ int ust_get_cohesion_scale(ust_context *ctx, float *coh);
  * adjust, HydrostaticTerm.  NULL restores the defaults (VerticalAxis 1, force along -y). ---- */
 int ust_set_buoyancy(ust_context *ctx, const ust_buoyancy *b);
 
+/* ---- PCDVC population control (PICellerator/Utils/src/PCDVC.c) ---- */
+typedef struct {
+    int res[3];                 /* DVCWeights "resolutionX/Y/Z" (default 10) */
+    double upper_t, lower_t;    /* "upperT" (25), "lowerT" (0.6), percent of the element volume */
+    int max_deletions;          /* "maxDeletions" (2) */
+    int max_splits;             /* "maxSplits" (3) */
+    int split_in_interface_cells;  /* "splitInInterfaceCells" (default Inflow, 0) */
+    int delete_in_interface_cells; /* "deleteInInterfaceCells" (0) */
+    int allow_splitting_in_boundary_interface_cells; /* (0); 1 turns boundary elements into Inflow elements */
+    int inflow;                 /* "Inflow" (0); the Inflow branch re-seeds elements with rand() and is not built */
+} ust_pcdvc;
+void ust_pcdvc_default(ust_pcdvc *o);
+/* _PCDVC_Calculate over every element (PCDVC.c:1493-1523): DVC weights, then splits of particles whose region
+ * exceeds upperT % of the element (a copy at the region centroid, with the parent's material and state) and
+ * deletions of particles whose region is below lowerT %, then DVC again; the integration points end at the
+ * region centroids with the region volumes as weights.  Deletion candidates are taken in decreasing swarm
+ * position within the element.  Returns the numbers of splits and deletions; UST_EINVAL for the Inflow
+ * options, for an element without particles, and for a changed element left with fewer than 3 particles. */
+int ust_swarm_pcdvc(ust_context *ctx, const ust_pcdvc *opts, int *n_split, int *n_deleted);
+
 /* ---- StrainWeakening component (StrainWeakening.c:217-280 keys, defaults as there) ---- */
 typedef struct {
     double softening_strain;          /* "softeningStrain" (HUGE_VAL) */

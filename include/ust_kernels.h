@@ -74,6 +74,15 @@ typedef struct {
     double healing_rate;     /* "healingRate" */
 } ust_yield_state;
 
+/* Per-element population-control thresholds of PCDVC (PCDVC.c:239-285): upper_t / lower_t in percent of the
+ * element volume, at most max_splits splits and max_deletions deletions per element, and whether interface
+ * elements (particles of more than one material) may split or delete. */
+typedef struct {
+    double upper_t, lower_t;
+    int max_deletions, max_splits;
+    int split_in_interface_cells, delete_in_interface_cells;
+} ust_pcdvc_opts;
+
 /* BuoyancyForceTerm component keys (BuoyancyForceTerm.c:139-160): "VerticalAxis" (default 1),
  * "gravityDirection" (gHat, used when use_direction), "adjust", and the optional "HydrostaticTerm" whose
  * background density (HydrostaticTerm.c:267-330) is subtracted from every particle's density (its keys, and
@@ -176,6 +185,20 @@ int ust_k_particle_weights(const ust_grid *g, const int *cell_off, int ppc_hint,
                            const double *velocity, const double *pressure, int prev_solution,
                            const ust_yield_state *yield_state, const ust_buoyancy *buoyancy, double *eta_out,
                            unsigned char *yield_out, double *WK, double *WG, double *WF, void *stream);
+/* PCDVC population control (PCDVC.c:1338-1432 2D, :880-990 3D, non-Inflow branch), in two passes around a
+ * host-side scan of the counts.  plan: one thread per element reads the DVC region volumes w of its particles
+ * and writes its new particle count and its new particle list (entries s: original particle s of the element;
+ * s | 2^30: a split copy of s) at list[cell_off[e] + e * max_splits], capacity n_e + max_splits; flag[0] gets 1
+ * where a changed element keeps fewer than 3 particles, flag[1] and flag[2] count the splits and deletions.
+ * build: writes the new swarm from the lists (split copies at the parent's region centroid c0/c1/c2, the
+ * parent's material, cohesion factor and two optional per-particle states). */
+int ust_k_pcdvc_plan(const ust_grid *g, const int *cell_off, const double *w, const unsigned char *mat,
+                     const ust_pcdvc_opts *opts, int *new_count, int *list, int *flag, void *stream);
+int ust_k_pcdvc_build(int dim, int nel, int max_splits, const int *cell_off, const int *new_off, const int *list,
+                      const double *xi0, const double *xi1, const double *xi2, const double *c0, const double *c1,
+                      const double *c2, const unsigned char *mat, const float *coh, const double *state0,
+                      const double *state1, double *o0, double *o1, double *o2, unsigned char *omat, float *ocoh,
+                      double *ostate0, double *ostate1, void *stream);
 /* StrainWeakening time step: pfw += dt * pfw_inc, then negative values to 0 (the TimeIntegrand update of
  * postFailureWeakening, whose time derivative is the stored increment, StrainWeakening.c:401-407, and
  * _StrainWeakening_MakeValuesPositive, :411-426) */

@@ -186,3 +186,79 @@ def test_dvc_moves_integration_points_not_material_points():
     assert np.array_equal(offc1, offc) and np.max(np.abs(xc1 - xc0)) < 1e-14
     for s in (a, b, c):
         s.close()
+
+
+@pytest.mark.parametrize("dim,nel,res", [(2, (9, 7), (8, 8)), (3, (5, 4, 3), (8, 6, 5))])
+def test_pcdvc_matches_oracle(dim, nel, res):
+    """PCDVC population control (PCDVC.c, non-Inflow branch) against the oracle on a ragged swarm with interface
+    elements: element offsets, material points, materials, integration points and weights bit-equal; the
+    strain-weakening state follows its particles (split copies take their parent's)."""
+    import earthbyte_underworld_amd as ust
+    from oracle import pcdvc_oracle as P
+    s = _system(dim, nel)
+    ne = int(np.prod(nel))
+    rng = np.random.default_rng(7)
+    counts = rng.integers(3, 45, ne)
+    counts[::4] = 3
+    s.random_particle_layout(seed=29, cell_counts=counts)
+    s.assign_materials([ust.Everywhere(0), ust.Sphere(1, (0.5,) * dim, 0.3)])
+    s.set_strain_weakening(ust.StrainWeakening(softeningStrain=1.0))
+    off, xi, _, mat = s.swarm()
+    state = np.arange(len(xi), dtype=np.float64) * 0.5
+    s.set_strain_weakening_state(state, -state)
+    opts = dict(upper_t=20.0, lower_t=1.0, max_deletions=3, max_splits=2, split_in_interface_cells=False,
+                delete_in_interface_cells=True)
+    ref = P.pcdvc(dim, nel, res, off, xi, mat, **opts)
+    new_off, src, split, mxi, nmat, ixi, w = ref
+    ns, nd = s.pcdvc(resolution=res, upperT=20.0, lowerT=1.0, maxDeletions=3, maxSplits=2,
+                     deleteInInterfaceCells=True)
+    assert ns == int(split.sum()) > 0 and nd == len(xi) + ns - len(src) > 0
+    off2, xi2, w2, mat2 = s.swarm()
+    ixi2, iw2 = s.integration_points()
+    assert np.array_equal(off2, new_off)
+    assert np.array_equal(xi2[:, :dim], mxi[:, :dim]) and np.array_equal(mat2, nmat)
+    assert np.array_equal(ixi2[:, :dim], ixi[:, :dim]) and np.array_equal(iw2, w) and np.array_equal(w2, w)
+    pfw, inc = s.strain_weakening()
+    assert np.array_equal(pfw, state[src]) and np.array_equal(inc, -state[src])
+    s.close()
+
+
+def test_pcdvc_suite_statistics():
+    """PCDVCSuite.c:140-353 on the GPU path (testPCDVC.xml: 2x2 elements on [0,4]^2, 20 particles per cell, 10x10
+    sub-grid, maxDeletions = maxSplits = 0, 1000 seeds): the suite's expected error statistics of three element-0
+    integrals over the integration points."""
+    import earthbyte_underworld_amd as ust
+    from tests import weights_suite as W
+    expected = {"CircleInterface": (0.104, 0.05, 0.06, 0.01), "ExponentialInterface": (0.055, 0.04, 0.042, 0.01),
+                "PolynomialFunction": (0.0092, 0.001, 0.0022, 0.001)}
+    s = ust.StokesSystem(2, W.NEL, W.BOX[0], W.BOX[1])
+    s.set_materials([ust.RheologyMaterial([ust.MaterialViscosity(eta0=1.0)])])
+    xs, ws = [], []
+    for seed in range(W.SEEDS):
+        s.random_particle_layout(particlesPerCell=W.PPC, seed=seed)
+        assert s.pcdvc(resolution=10, maxDeletions=0, maxSplits=0) == (0, 0)
+        ixi, w = s.integration_points()
+        xs.append(ixi[:W.PPC])
+        ws.append(w[:W.PPC])
+    stats = W.element_integrals(np.array(xs), np.array(ws), np.ones((W.SEEDS, W.PPC)))
+    for name, (mean, mtol, std, stol) in expected.items():
+        mm, ss = stats[name]
+        assert abs(mm - mean) < mtol and abs(ss - std) < stol, (name, mm, ss)
+    s.close()
+
+
+def test_pcdvc_refusals():
+    """The Inflow branch is not built (refused); a changed element left with fewer than 3 particles is the
+    reference's error (PCDVC.c:1418-1421)."""
+    import earthbyte_underworld_amd as ust
+    s = _system(2, (2, 2))
+    s.random_particle_layout(seed=3, cell_counts=[5, 5, 5, 5])
+    with pytest.raises(ust.UstError):
+        s.pcdvc(Inflow=True)
+    with pytest.raises(ust.UstError):
+        s.pcdvc(allowSplittingInBoundaryInterfaceCells=True)
+    # every region below lowerT 90 %: maxDeletions 100 > n - 4 becomes n // 2, so 5 -> 3 particles per element
+    assert s.pcdvc(lowerT=90.0, upperT=95.0, maxDeletions=100) == (0, 8)
+    with pytest.raises(ust.UstError):  # then 3 -> 2: a changed element below 3 particles
+        s.pcdvc(lowerT=90.0, upperT=95.0, maxDeletions=100)
+    s.close()
