@@ -262,3 +262,58 @@ def test_pcdvc_refusals():
     with pytest.raises(ust.UstError):  # then 3 -> 2: a changed element below 3 particles
         s.pcdvc(lowerT=90.0, upperT=95.0, maxDeletions=100)
     s.close()
+
+
+def test_pcdvc_refills_sparse_elements():
+    """Elements left with 3 or 4 particles (each owning more than upperT = 25 % of the element) gain split copies,
+    at most maxSplits = 3 per pass, until no region exceeds the threshold; every pass equals the oracle's."""
+    from oracle import pcdvc_oracle as P
+    s = _system(2, (8, 6))
+    counts = np.full(48, 20)
+    counts[[0, 9, 17, 30, 47]] = [3, 3, 4, 3, 4]
+    s.random_particle_layout(seed=41, cell_counts=counts)
+    for _ in range(4):
+        off, xi, _, mat = s.swarm()
+        ref_off, _, _, mxi, _, _, w = P.pcdvc(2, (8, 6), (10, 10), off, xi, mat)
+        s.pcdvc(resolution=10)
+        off2, xi2, w2, _ = s.swarm()
+        assert np.array_equal(off2, ref_off) and np.array_equal(xi2[:, :2], mxi[:, :2]) and np.array_equal(w2, w)
+    n = np.diff(s.swarm()[0])
+    assert np.all(n[[0, 9, 17, 30, 47]] >= 5)
+    s.close()
+
+
+def test_pcdvc_advected_96cube_keeps_elements_populated():
+    """96^3 elements, 30 particles per element (26.5M particles) advected through a divergence-free cellular flow
+    at CFL 1 with population control after every step: no element empties, the particle count changes exactly by
+    the splits, deletions and removals, and every element's weights sum to its volume."""
+    import time
+    import earthbyte_underworld_amd as ust
+    from earthbyte_underworld_amd.configs import spheres_3d
+    s = spheres_3d(nel=(96, 96, 96), ppc=30)
+    xyz = s.node_coords()
+    u = np.zeros(s.ndof)
+    u[0::3] = np.sin(np.pi * xyz[:, 0]) * np.cos(np.pi * xyz[:, 1])
+    u[1::3] = -np.cos(np.pi * xyz[:, 0]) * np.sin(np.pi * xyz[:, 1])
+    s.set_solution(u, None)
+    np_ = s.lib.ust_swarm_count(s.ctx)
+    for step in range(4):
+        t = time.perf_counter()
+        ns, nd = s.pcdvc(resolution=10)
+        t = time.perf_counter() - t
+        np1 = s.lib.ust_swarm_count(s.ctx)
+        assert np1 == np_ + ns - nd
+        nr, ne = s.advect(1.0 / 96.0, order=2, allowFallbackToFirstOrder=True)
+        off, _, _, _ = s.swarm()
+        c = np.diff(off)
+        print(f"step {step}: pcdvc {t:.3f} s, {ns} splits, {nd} deletions; advect removed {nr}, empty {ne}, "
+              f"min {c.min()} particles per element")
+        assert ne == 0 and c.min() >= 3
+        np_ = np1 - nr
+        assert s.lib.ust_swarm_count(s.ctx) == np_
+    ns, nd = s.pcdvc(resolution=10)
+    off, _, _, _ = s.swarm()
+    w = s.integration_points()[1]
+    sums = np.add.reduceat(w, off[:-1])
+    assert np.max(np.abs(sums - 8.0)) < 1e-9
+    s.close()
