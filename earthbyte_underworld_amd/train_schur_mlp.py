@@ -9,7 +9,15 @@ iteration count against the plain approxS preconditioner.
 
     python -m earthbyte_underworld_amd.train_schur_mlp --size 32 --train 24 --heldout 4
 
-writes earthbyte_underworld_amd/data/schur_mlp_3d.npz (w, b, and a JSON training record).
+writes earthbyte_underworld_amd/data/schur_mlp_3d.npz (w, b, and a JSON training record).  The protocol of the
+random-field benchmark (200 training fields and 50 held-out fields at 128 x 128 x 64, 20 particles per cell,
+log10 eta sampled per particle):
+
+    python -m earthbyte_underworld_amd.train_schur_mlp --kinds grf --size 128 --ppc 20 --train 200 \
+        --heldout 50 --sample-cells 65536 --eval-path headline
+
+With --eval-path headline the held-out fields are solved on the benchmark's path (mixed precision, relaxed inner
+tolerance, Chebyshev degree 1/2, ratio 30, Stokes bar 1e-6) with and without the learned scaling.
 """
 import argparse
 import json
@@ -38,16 +46,23 @@ def field(kind, n, seed, ppc, grf_box=(2.0, 2.0, 1.0)):
     return configs.random_field_3d(nel=(n, n, n // 2), ppc=ppc, seed=seed, box=tuple(grf_box))
 
 
-def probe(s, inner_rtol):
+def probe(s, inner_rtol, precision=L.PREC_FP64):
     s.assemble()
-    o = default_opts(precision=L.PREC_FP64, inner_rtol=inner_rtol)
+    o = default_opts(precision=precision, inner_rtol=inner_rtol)
     return s.mlp_features(), s.mlp_probe_target(o)
 
 
-def outer_its(s, learned, rtol=1e-7):
-    o = default_opts(precision=L.PREC_FP64, schur_pc=L.SCHUR_LEARNED if learned else L.SCHUR_UW,
-                     outer_rtol=rtol, inner_rtol=1e-8, max_outer=3000, stall_window=0)
+def outer_its(s, learned, rtol=1e-7, path="fp64"):
+    if path == "headline":
+        o = default_opts(precision=L.PREC_MIXED, schur_pc=L.SCHUR_LEARNED if learned else L.SCHUR_UW,
+                         outer_rtol=rtol, inner_rtol=1e-8, max_outer=2000, inner_relax=1, smoother_degree=1,
+                         coarse_degree=2, cheby_ratio=30.0, stokes_rtol=1e-6)
+    else:
+        o = default_opts(precision=L.PREC_FP64, schur_pc=L.SCHUR_LEARNED if learned else L.SCHUR_UW,
+                         outer_rtol=rtol, inner_rtol=1e-8, max_outer=3000, stall_window=0)
+    t = time.time()
     st = s.solve(o)
+    st["seconds"] = time.time() - t
     return st
 
 
@@ -64,17 +79,26 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--inner-rtol", type=float, default=1e-7)
+    ap.add_argument("--kinds", default="spheres,grf", help="training field kinds, alternating")
+    ap.add_argument("--sample-cells", type=int, default=0, help="cells kept per training field (0: all)")
+    ap.add_argument("--eval-path", choices=["fp64", "headline"], default="fp64")
+    ap.add_argument("--probe-precision", choices=["fp64", "mixed"], default="fp64",
+                    help="A11 solves of the Schur-diagonal probe (mixed: FP32 V-cycle inside FP64 PCG)")
     ap.add_argument("--out", default=DEFAULT_OUT)
     a = ap.parse_args(argv)
 
     t0 = time.time()
     X, Y = [], []
-    kinds = ["spheres", "grf"]
+    kinds = a.kinds.split(",")
+    rng = np.random.default_rng(a.seed)
     for k in range(a.train):
-        kind = kinds[k % 2]
+        kind = kinds[k % len(kinds)]
         s = field(kind, (a.grf_size or a.size) if kind == "grf" else a.size, 1000 + k, a.ppc, a.grf_box)
-        f, t = probe(s, a.inner_rtol)
+        f, t = probe(s, a.inner_rtol, L.PREC_MIXED if a.probe_precision == "mixed" else L.PREC_FP64)
         s.close()
+        if a.sample_cells and a.sample_cells < len(t):
+            keep = rng.choice(len(t), a.sample_cells, replace=False)
+            f, t = f[keep], t[keep]
         X.append(f)
         Y.append(t)
         print(f"[train] field {k} ({kind}): target mean {t.mean():+.3f} std {t.std():.3f} "
@@ -83,7 +107,7 @@ def main(argv=None):
     Y = np.concatenate(Y)
     t_probe = time.time() - t0
 
-    s = field("spheres", a.size, 1, a.ppc)
+    s = field(kinds[0], 16, 1, 4)
     s.mlp_init(seed=a.seed)
     t1 = time.time()
     losses = []
@@ -100,20 +124,25 @@ def main(argv=None):
     # held-out evaluation: the outer iteration count of plain vs learned BSSCR
     evals = []
     for k in range(a.heldout):
-        kind = kinds[k % 2]
+        kind = kinds[k % len(kinds)]
         s = field(kind, (a.grf_size or a.size) if kind == "grf" else a.size, 5000 + k, a.ppc, a.grf_box)
         s.assemble()
         s.mlp_set_params(w, b)
-        plain = outer_its(s, False)
-        learned = outer_its(s, True)
+        plain = outer_its(s, False, path=a.eval_path)
+        s.set_solution(np.zeros(s.ndof), np.zeros(s.nel))
+        learned = outer_its(s, True, path=a.eval_path)
         evals.append({"kind": kind, "seed": 5000 + k, "plain_outer": plain["outer_its"],
                       "learned_outer": learned["outer_its"], "plain_inner": plain["inner_its"],
                       "learned_inner": learned["inner_its"], "plain_residual": plain["stokes_residual"],
-                      "learned_residual": learned["stokes_residual"]})
+                      "learned_residual": learned["stokes_residual"], "plain_s": round(plain["seconds"], 3),
+                      "learned_s": round(learned["seconds"], 3), "fallback": learned.get("fallback_used", 0)})
         print(f"[eval] {kind} seed {5000 + k}: outer its plain {plain['outer_its']} learned "
-              f"{learned['outer_its']}", flush=True)
+              f"{learned['outer_its']} ({plain['seconds']:.2f} s / {learned['seconds']:.2f} s)", flush=True)
         s.close()
-    record = {"size": a.size, "grf_size": a.grf_size or a.size, "grf_box": list(a.grf_box), "ppc": a.ppc, "train_fields": a.train, "samples": int(len(Y)),
+    record = {"size": a.size, "grf_size": a.grf_size or a.size, "grf_box": list(a.grf_box), "ppc": a.ppc,
+              "kinds": kinds, "train_fields": a.train, "sample_cells": a.sample_cells, "eval_path": a.eval_path,
+              "probe_precision": a.probe_precision, "inner_rtol": a.inner_rtol,
+              "samples": int(len(Y)),
               "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "final_loss": float(losses[-1]),
               "target_variance": float(Y.var()), "probe_seconds": round(t_probe, 1),
               "train_seconds": round(t_train, 1), "heldout": evals}

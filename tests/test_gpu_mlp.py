@@ -152,6 +152,33 @@ def test_learned_scaling_gain_on_heldout_seed():
     s.close()
 
 
+def test_learned_scaling_gain_on_heldout_random_field_128x128x64():
+    """The shipped network on a held-out field of the benchmark configuration (128x128x64, 20 particles per cell,
+    field seed 5001, never used in training), on the benchmark's path (mixed precision, relaxed inner tolerance,
+    Stokes bar): fewer outer iterations than approxS (109 against 119 when measured), no fallback, both at the
+    bar, fields within 1e-6."""
+    import earthbyte_underworld_amd as ust
+    from earthbyte_underworld_amd import _lib as L
+    from earthbyte_underworld_amd.configs import random_field_3d
+    d = np.load(WEIGHTS)
+    s = random_field_3d(nel=(128, 128, 64), ppc=20, seed=5001)
+    s.assemble()
+    s.mlp_set_params(d["w"], d["b"])
+    kw = dict(precision=L.PREC_MIXED, outer_rtol=1e-7, inner_rtol=1e-8, max_outer=2000, inner_relax=1,
+              smoother_degree=1, coarse_degree=2, cheby_ratio=30.0, stokes_rtol=1e-6)
+    plain = s.solve(ust.default_opts(schur_pc=L.SCHUR_UW, **kw))
+    u0, p0 = s.solution()
+    s.set_solution(np.zeros(s.ndof), np.zeros(s.nel))
+    learned = s.solve(ust.default_opts(schur_pc=L.SCHUR_LEARNED, **kw))
+    u1, p1 = s.solution()
+    print(f"outer its: approxS {plain['outer_its']}, learned {learned['outer_its']}")
+    assert learned["fallback_used"] == 0
+    assert learned["outer_its"] <= 0.95 * plain["outer_its"]
+    assert plain["stokes_residual"] < 1e-6 and learned["stokes_residual"] < 1e-6
+    assert rel_err(u1, u0) < 1e-6 and rel_err(p1, p0) < 1e-6
+    s.close()
+
+
 def test_fallback_to_plain_bsscr_on_stall():
     """A network that scrambles the scaling stalls the learned run; plain BSSCR takes over from the current
     iterate and reaches the same tolerance and solution."""
