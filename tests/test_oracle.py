@@ -426,3 +426,31 @@ def test_strain_weakening_drucker_prager_and_tensile_flag(oracle):
             fs = k * p + c
             assert t[0] == (1 if fs <= 0 else 0)
             assert abs(eta[0] - max(fs, 0.05)) < 1e-14  # yielded: eta_y = criterion (tau_II = eta)
+
+
+def test_solcx_reference_library_properties():
+    """The reference's analytic solCx solution as built into oracle/_ref: free slip on the unit box (no normal
+    velocity on the walls), divergence-free (central differences), the isoviscous case antisymmetric in x about
+    the centre line (cos(pi x) forcing), pressure finite."""
+    from oracle import solcx as S
+    if not S.available():
+        pytest.skip("oracle/_ref/libsolcx.so not built (the reference tree is absent)")
+    t = np.linspace(0.05, 0.95, 7)
+    for eta_b in (1.0, 1e6):
+        v, _ = S.solcx(np.zeros(7), t, 1.0, eta_b, 0.5, 1)
+        assert np.max(np.abs(v[:, 0])) < 1e-12
+        v, _ = S.solcx(t, np.ones(7), 1.0, eta_b, 0.5, 1)
+        assert np.max(np.abs(v[:, 1])) < 1e-12
+        X, Z = np.meshgrid(t, t)
+        d = 1e-6
+        vxp, _ = S.solcx(X + d, Z, 1.0, eta_b, 0.5, 1)
+        vxm, _ = S.solcx(X - d, Z, 1.0, eta_b, 0.5, 1)
+        vzp, _ = S.solcx(X, Z + d, 1.0, eta_b, 0.5, 1)
+        vzm, _ = S.solcx(X, Z - d, 1.0, eta_b, 0.5, 1)
+        div = (vxp[:, 0] - vxm[:, 0] + vzp[:, 1] - vzm[:, 1]) / (2 * d)
+        scale = np.max(np.abs(vxp)) / 0.5
+        ok = np.abs(X.ravel() - 0.5) > 2 * d  # away from the viscosity jump
+        assert np.max(np.abs(div[ok])) < 1e-6 * scale
+    v1, p1 = S.solcx(t, 0.3 * np.ones(7), 1.0, 1.0, 0.5, 1)
+    v2, _ = S.solcx(1 - t, 0.3 * np.ones(7), 1.0, 1.0, 0.5, 1)
+    assert np.allclose(v1[:, 1], -v2[:, 1], atol=1e-14) and np.all(np.isfinite(p1))
