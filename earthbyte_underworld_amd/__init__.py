@@ -6,4 +6,5 @@ from ._lib import LIB_PATH, load  # noqa: F401
 from .stokes import (Arrhenius, BelowCosinePlane, Box, BuoyancyForceTerm, DruckerPrager, Everywhere,  # noqa: F401
                      MaterialViscosity, StrainWeakening,
                      RheologyMaterial, Sphere, StokesSystem, UstError, ViscosityLimiter, VonMises, default_opts,
-                     free_slip_walls, freeslip_sides_noslip_topbottom, wall_mask)
+                     free_slip_walls, freeslip_sides_noslip_topbottom, read_checkpoint, wall_mask,
+                     write_checkpoint)

@@ -14,6 +14,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 SCHUR_UW, SCHUR_LEARNED, SCHUR_SCALED, SCHUR_GKGDIAG = 0, 1, 2, 3
 MAT_K, MAT_G = 0, 1
 VEC_F, VEC_H, VEC_U, VEC_P = 0, 1, 2, 3
+FIELD_VELOCITY, FIELD_PRESSURE, FIELD_TEMPERATURE = 0, 1, 2
 RH_MATERIALVISCOSITY, RH_ARRHENIUS, RH_VISCOSITYLIMITER, RH_VONMISES, RH_DRUCKERPRAGER = 1, 2, 3, 4, 5
 SHAPE_EVERYWHERE, SHAPE_BELOWCOSINEPLANE, SHAPE_SPHERE, SHAPE_BOX = 0, 1, 2, 3
 TERM_CONSTITUTIVE, TERM_GRADIENT, TERM_UZAWA_PC, TERM_BUOYANCY = 1, 2, 3, 4
@@ -118,9 +119,17 @@ _SIGS = {
     "ust_swarm_count": (I, [vp]),
     "ust_swarm_download_integration_points": (I, [vp, vp, vp]),
     "ust_swarm_advect": (I, [vp, D, I, I, C.POINTER(I), C.POINTER(I)]),
+    "ust_swarm_spacefiller": (I, [vp, D]),
     "ust_swarm_dvc_weights": (I, [vp, C.POINTER(I), I]),
     "ust_write_petsc_matrix": (I, [vp, I, I, C.c_char_p]),
     "ust_write_petsc_vector": (I, [vp, I, I, C.c_char_p]),
+    "ust_save_field_hdf5": (I, [vp, I, I, C.c_char_p]),
+    "ust_load_field_hdf5": (I, [vp, I, C.c_char_p]),
+    "ust_hdf5_field_info": (I, [C.c_char_p, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(C.c_long),
+                                C.POINTER(I)]),
+    "ust_hdf5_field_read": (I, [C.c_char_p, vp, C.c_long, I]),
+    "ust_hdf5_field_write": (I, [C.c_char_p, I, C.POINTER(I), C.c_long, I, vp]),
+    "ust_hdf5_error": (C.c_char_p, []),
     "ust_set_temperature": (I, [vp, vp]),
     "ust_set_buoyancy": (I, [vp, C.POINTER(Buoyancy)]),
     "ust_strain_weakening_default": (None, [C.POINTER(StrainWeakening)]),

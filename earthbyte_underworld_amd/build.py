@@ -49,7 +49,7 @@ def build(verbose=False):
                 print(r.stdout + r.stderr)
     objs = [j[-1] for j in jobs]
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-ldl"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -292,8 +292,18 @@ constexpr int CPX = UST_TX3 + 1, CPY = UST_TY3 + 1, CNP = CPX * CPY, CNPD = CNP 
 constexpr int CNR = (CNPD + UST_TILE - 1) / UST_TILE;
 constexpr int EBS = UST_TILE + 1;  // row stride of the element-result buffer (rows shift across LDS banks)
 
+// Build-time tuning of the FP32 column passes: workgroups per CU the register budget must allow, and whether
+// the next layer's operands are prefetched in registers.  Measured on the 128x128x64 solve (MI355X): 2 with
+// prefetch is fastest overall; 4 without prefetch speeds the residual pass (58 -> 50 us) but spills in the
+// prolongation pass (76 -> 112 us), 3 spills in both.
+#ifndef UST_COL_MINB_F32
+#define UST_COL_MINB_F32 2
+#endif
+#ifndef UST_COL_PREFETCH
+#define UST_COL_PREFETCH 1
+#endif
 template <typename T, typename WT, int MODE>
-__global__ __launch_bounds__(256, 2) void k_applyK3_col(GridDev g, const WT *__restrict__ W,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? UST_COL_MINB_F32 : 2) void k_applyK3_col(GridDev g, const WT *__restrict__ W,
                                                      const unsigned char *__restrict__ mask, const T *u, Epi<T> ep,
                                                      T *__restrict__ work) {
     __shared__ T su[2][CNPD];     // staged node planes k and k+1 (free dofs; Dirichlet dofs read 0)
@@ -386,7 +396,7 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_col(GridDev g, const WT *__r
     };
     // FP32 (and bfloat16 weights): the next layer's node plane and weights are prefetched in registers during
     // this layer; FP64 has no registers to spare for them and loads them at the layer
-    constexpr bool wpre = sizeof(T) == 4;
+    constexpr bool wpre = sizeof(T) == 4 && UST_COL_PREFETCH;
     // ---- prologue: the coarse ring (planes z0/2 and z0/2 + 1), plane z0, and the loads for layer z0 ----
     if (MODE == EPI_PCHEBY0) {
         const int c0 = z0 >> 1;

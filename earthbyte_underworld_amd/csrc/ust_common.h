@@ -56,6 +56,11 @@ __host__ __device__ constexpr int sgn(int a) { return 2 * a - 1; }
 #define UST_TILE 256
 #define UST_TZ3_MAX 16
 #define UST_COL_MIN 1024
+// workgroups a column launch aims at (the z-chunk length follows): two per CU; 1024 (four per CU, shorter
+// chunks) measured slower on the 128x128x64 solve (FP64 product 93 -> 100 us)
+#ifndef UST_COL_WG_TARGET
+#define UST_COL_WG_TARGET 512
+#endif
 
 struct GridDev {
     int dim;
@@ -111,7 +116,7 @@ static inline GridDev grid_dev(const ust_grid *g) {
             d.tny = cny;
             // z-chunk: long enough that few node planes sit on chunk boundaries, short enough that the launch
             // has about 512 workgroups (two resident per CU on 256 CUs) or more
-            long long zc = layers / 512;
+            long long zc = layers / UST_COL_WG_TARGET;
             if (zc < 2) zc = 2;
             if (zc > UST_TZ3_MAX) zc = UST_TZ3_MAX;
             if (zc > d.nz) zc = d.nz;

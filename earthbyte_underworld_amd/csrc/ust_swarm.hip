@@ -67,6 +67,58 @@ __global__ void k_random_layout(GridDev g, BoxDev box, const int *__restrict__ c
     w[p] = (double)(1 << DIM) / (double)(cell_off[e + 1] - cell_off[e]);
 }
 
+// SpaceFillerParticleLayout (StgDomain/Swarm/src/SpaceFillerParticleLayout.c:218-240 with GlobalParticleLayout.c:
+// 275-318): global particle p sits at min + (max - min) s_d per axis, s_d its (p+1)-th Sobol number over 2^30 --
+// the XOR of the axis' direction numbers m_k << (29 - k) over the set bits k of the Gray code of p + 1, which
+// is what SobolGenerator_GetNextNumber's recurrence yields (SobolGenerator.c:295-338).  Its element follows
+// Mesh_RegularAlgorithms_SearchElements (Mesh_RegularAlgorithms.c:194-226): per axis the integer part of
+// (x - min) / sep, one lower when the fraction is within 1e-12 of zero (a point on an interior face belongs to
+// the lower element), the last element within 1e-12 of max.  The coordinate products are not contracted into
+// fused multiply-adds, so that x and the element decision round as the reference's host code does.
+struct SobolDirs {
+    unsigned m[3][32];
+};
+
+template <int DIM>
+__global__ void k_spacefiller(GridDev g, BoxDev box, int np, SobolDirs dirs, unsigned *__restrict__ key,
+                              double *__restrict__ xi0, double *__restrict__ xi1, double *__restrict__ xi2) {
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const unsigned n = (unsigned)p + 1u, gray = n ^ (n >> 1);
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    int ind[3] = {0, 0, 0};
+    double xi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        unsigned s = 0;
+        for (int k = 0; k < 30; k++)
+            if ((gray >> k) & 1u) s ^= dirs.m[d][k] << (29 - k);
+        const double lo = box.xmin[d], hi = box.xmax[d];
+        const double x = (hi - lo) * ((double)s / 1073741824.0) + lo;
+        int i;
+        if (x - hi >= -1e-12 && x - hi <= 1e-12) {
+            i = nn[d] - 1;
+        } else {
+            const double out = (x - lo) / ((hi - lo) / (double)nn[d]);
+            const double ip = floor(out), frac = out - ip;
+            i = (int)ip;
+            if (i > 0 && frac >= -1e-12 && frac <= 1e-12) i--;
+            i = i < 0 ? 0 : (i >= nn[d] ? nn[d] - 1 : i);
+        }
+        ind[d] = i;
+        const double mn = vcoord(lo, hi, i, nn[d]), mx = vcoord(lo, hi, i + 1, nn[d]);
+        const double c = 0.5 * (mn + mx), hh = 0.5 * (mx - mn);
+        double t = (x - c) / hh;
+        t += (x - (c + hh * t)) / hh;  // one Newton correction of the affine map, as the random layout
+        xi[d] = t;
+    }
+    key[p] = (unsigned)(ind[0] + g.nx * (ind[1] + (DIM == 3 ? g.ny * ind[2] : 0)));
+    xi0[p] = xi[0];
+    xi1[p] = xi[1];
+    if (DIM == 3) xi2[p] = xi[2];
+}
+
 template <int DIM>
 __global__ void k_assign_materials(GridDev g, BoxDev box, const int *__restrict__ cell_off, int np,
                                    const double *__restrict__ xi0, const double *__restrict__ xi1,
@@ -168,6 +220,24 @@ extern "C" int ust_k_random_layout(const ust_grid *gr, const int *cell_off, int 
     else
         hipLaunchKernelGGL(k_random_layout<2>, dim3(ust_grid_size(np, 256)), dim3(256), 0, ust_stream(stream), g,
                            box_of(gr), cell_off, np, s0, xi0, xi1, xi2, w);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+
+extern "C" int ust_k_spacefiller_layout(const ust_grid *gr, int np, const unsigned *dirnum, unsigned *key,
+                                        double *xi0, double *xi1, double *xi2, void *stream) {
+    GridDev g = grid_dev(gr);
+    if (!dirnum || !key || !xi0 || !xi1 || (g.dim == 3 && !xi2) || np < 0) return UST_EINVAL;
+    if (np == 0) return UST_OK;
+    SobolDirs dirs;
+    for (int d = 0; d < 3; d++)
+        for (int k = 0; k < 32; k++) dirs.m[d][k] = k < 30 ? dirnum[d * 32 + k] : 0u;
+    if (g.dim == 3)
+        hipLaunchKernelGGL(k_spacefiller<3>, dim3(ust_grid_size(np, 256)), dim3(256), 0, ust_stream(stream), g,
+                           box_of(gr), np, dirs, key, xi0, xi1, xi2);
+    else
+        hipLaunchKernelGGL(k_spacefiller<2>, dim3(ust_grid_size(np, 256)), dim3(256), 0, ust_stream(stream), g,
+                           box_of(gr), np, dirs, key, xi0, xi1, xi2);
     UST_LAUNCH_CHECK();
     return UST_OK;
 }

@@ -890,6 +890,78 @@ int ust_swarm_random(ust_context *ctx, const int *cell_off_host, int ppc, long s
     return UST_OK;
 }
 
+/* Direction numbers of the first three generators of SobolGenerator's table (SobolGenerator.c:340-470: primitive
+ * polynomials x + 1, x^2 + x + 1, x^3 + x + 1 with initial numbers {1}, {1, 1}, {1, 3, 7}, Joe & Kuo's values),
+ * continued by the recurrence m_i = XOR_k a_k 2^k m_{i-k} ^ 2^s m_{i-s} ^ m_{i-s} (SobolGenerator.c:245-290). */
+static void sobol_direction_numbers(unsigned m[3][32]) {
+    static const unsigned degree[3] = {1, 2, 3}, coefficient[3] = {0, 1, 1};
+    static const unsigned init[3][3] = {{1, 0, 0}, {1, 1, 0}, {1, 3, 7}};
+    for (int d = 0; d < 3; d++) {
+        const unsigned s = degree[d], a = coefficient[d];
+        memset(m[d], 0, sizeof(m[d]));
+        for (unsigned i = 0; i < s; i++) m[d][i] = init[d][i];
+        for (unsigned i = s; i < 30; i++) {
+            unsigned v = 0;
+            for (unsigned k = 1; k < s; k++)
+                if ((a >> (s - 1 - k)) & 1u) v ^= (1u << k) * m[d][i - k];
+            v ^= (1u << s) * m[d][i - s];
+            v ^= m[d][i - s];
+            m[d][i] = v;
+        }
+    }
+}
+
+int ust_swarm_spacefiller(ust_context *ctx, double avg_ppc) {
+    if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "ust_set_mesh must come before the swarm");
+    const int nel = ctx->nel, dim = ctx->grid.dim;
+    /* GlobalParticleLayout.c:183: totalInitialParticles = cells x averageInitialParticlesPerCell, truncated */
+    const double total_d = (double)(unsigned)nel * avg_ppc;
+    if (!(avg_ppc > 0.0) || !(total_d >= 1.0) || total_d >= 1073741824.0)
+        UST_FAIL(ctx, UST_EINVAL, "averageInitialParticlesPerCell %g gives no particles or too many", avg_ppc);
+    const int np = (int)(unsigned)total_d;
+    unsigned dirs[3][32];
+    sobol_direction_numbers(dirs);
+    const size_t n = (size_t)np, wb = ust_k_sort_by_cell_work_bytes(np, nel);
+    void *key = NULL, *key_s = NULL, *perm = NULL, *perm_t = NULL, *nxi = NULL, *work = NULL, *off = NULL;
+    void *oxi = NULL, *ow = NULL;
+    int rc = UST_OK, max_ppc = 0;
+    if (!rc) rc = ust_dev_malloc(&key, n * sizeof(unsigned) + 8);
+    if (!rc) rc = ust_dev_malloc(&key_s, n * sizeof(unsigned) + 8);
+    if (!rc) rc = ust_dev_malloc(&perm, n * sizeof(unsigned) + 8);
+    if (!rc) rc = ust_dev_malloc(&perm_t, n * sizeof(unsigned) + 8);
+    if (!rc) rc = ust_dev_malloc(&nxi, 3 * n * sizeof(double) + 8);
+    if (!rc) rc = ust_dev_malloc(&work, wb + 8);
+    if (!rc) rc = ust_dev_malloc(&off, (size_t)(nel + 1) * sizeof(int));
+    if (!rc) rc = ust_dev_malloc(&oxi, 3 * n * sizeof(double) + 8);
+    if (!rc) rc = ust_dev_malloc(&ow, n * sizeof(double) + 8);
+    double *n0 = (double *)nxi, *n1 = n0 + n, *n2 = n0 + 2 * n;
+    double *o0 = (double *)oxi, *o1 = o0 + n, *o2 = o0 + 2 * n;
+    if (!rc) rc = ust_k_spacefiller_layout(&ctx->grid, np, &dirs[0][0], (unsigned *)key, n0, n1, n2, ctx->stream);
+    if (!rc)
+        rc = ust_k_sort_by_cell((const unsigned *)key, (unsigned *)key_s, (unsigned *)perm, (unsigned *)perm_t, np,
+                                nel, work, wb, ctx->stream);
+    if (!rc) rc = ust_k_cell_offsets((const unsigned *)key_s, np, nel, (int *)off, ctx->stream);
+    int *off_h = (int *)malloc((size_t)(nel + 1) * sizeof(int));
+    if (!off_h && !rc) rc = UST_ENOMEM;
+    if (!rc) rc = ust_memcpy_d2h(off_h, off, (size_t)(nel + 1) * sizeof(int), ctx->stream);
+    if (!rc)
+        for (int e = 0; e < nel; e++)
+            if (off_h[e + 1] - off_h[e] > max_ppc) max_ppc = off_h[e + 1] - off_h[e];
+    free(off_h);
+    if (!rc)
+        rc = ust_k_swarm_gather(dim, np, (const unsigned *)perm, (const unsigned *)key_s, (const int *)off, n0, n1, n2,
+                                NULL, NULL, o0, o1, o2, (double *)ow, NULL, NULL, ctx->stream);
+    if (!rc) {
+        rc = ust_swarm_upload_dev(ctx, np, (const int *)off, o0, o1, dim == 3 ? o2 : NULL, (const double *)ow, NULL,
+                                  NULL, max_ppc);
+    } else {
+        snprintf(ctx->err, sizeof(ctx->err), "space-filler layout failed (%d): %s", rc, ust_dev_error_string());
+    }
+    void *tmp[] = {key, key_s, perm, perm_t, nxi, work, off, oxi, ow};
+    for (size_t i = 0; i < sizeof(tmp) / sizeof(tmp[0]); i++) ust_dev_free(tmp[i]);
+    return rc;
+}
+
 int ust_swarm_assign_materials(ust_context *ctx, int nshape, const ust_shape *shapes,
                                const unsigned char *cell_material) {
     if (!ctx->np) UST_FAIL(ctx, UST_ESTATE, "no swarm");

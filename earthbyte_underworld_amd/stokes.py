@@ -190,7 +190,6 @@ class StokesSystem:
 
     # ---- geometry helpers ----
     def node_coords(self):
-        axes = [np.linspace(self.min[d], self.max[d], self.nel_axis[d] + 1) for d in range(self.dim)]
         axes = [self.min[d] + np.arange(self.nel_axis[d] + 1) * (self.max[d] - self.min[d]) / self.nel_axis[d]
                 for d in range(self.dim)]
         grids = np.meshgrid(*axes, indexing="ij")
@@ -223,6 +222,11 @@ class StokesSystem:
             self._check(self.lib.ust_swarm_random(self.ctx, _ptr(off), 0, seed))
         else:
             self._check(self.lib.ust_swarm_random(self.ctx, None, int(particlesPerCell), seed))
+
+    def spacefiller_particle_layout(self, averageInitialParticlesPerCell=20):
+        """SpaceFillerParticleLayout (the reference's default material swarm layout): Sobol points over the box,
+        coincident integration points, ConstantWeights."""
+        self._check(self.lib.ust_swarm_spacefiller(self.ctx, float(averageInitialParticlesPerCell)))
 
     def assign_materials(self, shapes=(), cell_material=None):
         arr = (L.Shape * max(1, len(shapes)))(*shapes)
@@ -429,6 +433,21 @@ class StokesSystem:
         code = {"f": L.VEC_F, "h": L.VEC_H, "u": L.VEC_U, "p": L.VEC_P}[which]
         self._check(self.lib.ust_write_petsc_vector(self.ctx, code, int(bool(free_only)), str(path).encode()))
 
+    _FIELDS = {"VelocityField": L.FIELD_VELOCITY, "PressureField": L.FIELD_PRESSURE,
+               "TemperatureField": L.FIELD_TEMPERATURE}
+
+    def save_field(self, name, path, saveCoords=False):
+        """FeVariable_SaveToFile: "VelocityField", "PressureField" or "TemperatureField" as an HDF5 checkpoint
+        file (the reference's attributes and /data layout)."""
+        self._check(self.lib.ust_save_field_hdf5(self.ctx, self._FIELDS[name], int(bool(saveCoords)),
+                                                 str(path).encode()))
+
+    def load_field(self, name, path):
+        """FeVariable_ReadFromFile: a checkpoint file into the velocity or pressure solution (the next solve's
+        initial guess) or the temperature field, with the reference's version, dimension, resolution and
+        node-count checks."""
+        self._check(self.lib.ust_load_field_hdf5(self.ctx, self._FIELDS[name], str(path).encode()))
+
     def advect(self, dt, order=2, allowFallbackToFirstOrder=False):
         """SwarmAdvector step through the current velocity solution, then the re-sort by element; returns
         (particles removed, elements left empty)."""
@@ -552,6 +571,34 @@ class StokesSystem:
 
 
 # ---- boundary-condition helpers (VariableCondition XMLs) ----
+def read_checkpoint(path):
+    """An FeVariable checkpoint file (HDF5): {"version", "dimensions", "mesh resolution" (None when absent),
+    "data" (rows x columns)}.  Needs no GPU."""
+    lib = L.load()
+    ver, dim, rows, cols = C.c_int(), C.c_int(), C.c_long(), C.c_int()
+    res = (C.c_int * 3)()
+    p = str(path).encode()
+    if lib.ust_hdf5_field_info(p, C.byref(ver), C.byref(dim), res, C.byref(rows), C.byref(cols)) != L.UST_OK:
+        raise UstError(lib.ust_hdf5_error().decode())
+    data = np.empty((rows.value, cols.value), np.float64)
+    if lib.ust_hdf5_field_read(p, _ptr(data), rows.value, cols.value) != L.UST_OK:
+        raise UstError(lib.ust_hdf5_error().decode())
+    r = [res[d] for d in range(max(dim.value, 0))]
+    return {"version": ver.value, "dimensions": dim.value, "mesh resolution": r if any(r) else None, "data": data}
+
+
+def write_checkpoint(path, data, dimensions, meshResolution=None):
+    """Writes rows x columns doubles as an FeVariable checkpoint file (version 2; "mesh resolution" when given).
+    Needs no GPU."""
+    lib = L.load()
+    a = np.ascontiguousarray(data, np.float64)
+    if a.ndim != 2:
+        raise ValueError("data must be rows x columns")
+    res = None if meshResolution is None else (C.c_int * 3)(*(list(meshResolution) + [0] * 3)[:3])
+    if lib.ust_hdf5_field_write(str(path).encode(), int(dimensions), res, a.shape[0], a.shape[1], _ptr(a)) != L.UST_OK:
+        raise UstError(lib.ust_hdf5_error().decode())
+
+
 def wall_mask(sys, walls):
     """walls: dict {(axis, side): components}, side 0 = min wall, 1 = max wall."""
     dim = sys.dim

@@ -107,6 +107,13 @@ int ust_swarm_upload_dev(ust_context *ctx, int np, const int *cell_off_dev, cons
 /* Swarm generated on the device: RandomParticleLayout + coincident mapping + ConstantWeights.
  * cell_off (host, nel+1) gives the particle count of every cell (NULL: ppc in every cell). */
 int ust_swarm_random(ust_context *ctx, const int *cell_off, int ppc, long seed);
+/* SpaceFillerParticleLayout, the material swarm layout of the reference's PIC components
+ * (PICellerator/Apps/PIC_Components/MaterialPointSwarm.xml; SpaceFillerParticleLayout.c:218-240,
+ * GlobalParticleLayout.c:160-318): (unsigned)(nel x avg_ppc) particles at the points of the first dim Sobol
+ * sequences of SobolGenerator's table scaled to the box, each in the element Mesh_RegularAlgorithms_SearchElements
+ * gives it (faces to the lower element), in generation order within an element; coincident mapping and
+ * ConstantWeights.  Elements may be left empty (population control or DVC then decide). */
+int ust_swarm_spacefiller(ust_context *ctx, double avg_ppc);
 /* material index per particle from cell_material (host, nel; may be NULL) and shapes in order */
 int ust_swarm_assign_materials(ust_context *ctx, int nshape, const ust_shape *shapes,
                                const unsigned char *cell_material);
@@ -255,6 +262,30 @@ int ust_get_solution_dev(ust_context *ctx, const double **u_nodal_dev, const dou
 #define UST_VEC_P 3 /* element pressure */
 int ust_write_petsc_matrix(ust_context *ctx, int which, int free_only, const char *path);
 int ust_write_petsc_vector(ust_context *ctx, int which, int free_only, const char *path);
+
+/* ---- FeVariable checkpoint files, HDF5 (FeVariable_SaveToFile, StgFEM/Discretisation/src/FeVariable.c:2136-2389;
+ * FeVariable_ReadFromFile, :2452-2730) ----
+ * Root attributes "checkpoint file version" (2), "dimensions" and, for a field on the Cartesian linear mesh,
+ * "mesh resolution" (elements per axis), 32-bit big-endian integers; dataset "/data" of doubles, one row per
+ * mesh node in global order: the node's coordinates first when saved, then the components.  The HDF5 C library
+ * (1.10 or later) is loaded at the first call: UST_HDF5_LIB names it, else libhdf5.so.103 / libhdf5.so on the
+ * loader's path or under /opt/conda/lib; UST_EINVAL when none loads. */
+#define UST_FIELD_VELOCITY 0    /* VelocityField: linear mesh, dim components (the solution, BC values included) */
+#define UST_FIELD_PRESSURE 1    /* PressureField: constant mesh (one node per element, at its centre), 1 component */
+#define UST_FIELD_TEMPERATURE 2 /* TemperatureField: linear mesh, 1 component */
+int ust_save_field_hdf5(ust_context *ctx, int field, int save_coords, const char *path);
+/* reads a field back with FeVariable_ReadFromFile's checks (known version, dimensions, the mesh resolution of
+ * a linear-mesh field, node count; coordinates detected from the row width; version-1 rows skip one leading
+ * column); the velocity and pressure become the context's solution (the initial guess of the next solve), the
+ * temperature its temperature field */
+int ust_load_field_hdf5(ust_context *ctx, int field, const char *path);
+/* any checkpoint file, without a context: header (version 1 when the attribute is absent; res[3] zero where
+ * "mesh resolution" is absent), rows x cols data (row-major), writing one; the last failure's text in
+ * ust_hdf5_error() */
+int ust_hdf5_field_info(const char *path, int *version, int *dim, int *res, long *rows, int *cols);
+int ust_hdf5_field_read(const char *path, double *data, long rows, int cols);
+int ust_hdf5_field_write(const char *path, int dim, const int *res, long rows, int cols, const double *data);
+const char *ust_hdf5_error(void);
 
 /* ---- Uzawa solve (Stokes_SLE_UzawaSolver, Stokes_SLE_UzawaSolver.c:407-983) ----
  * Preconditioned CG on the pressure, Schur complement G^T K^-1 G applied through one velocity solve per

@@ -125,6 +125,12 @@ typedef struct {
  * particle is independent), xi by the Q1 inverse map, w = 2^dim / n_cell (ConstantWeights.c:180-189). */
 int ust_k_random_layout(const ust_grid *g, const int *cell_off, int np, long seed, double *xi0, double *xi1,
                         double *xi2, double *w, void *stream);
+/* SpaceFillerParticleLayout (SpaceFillerParticleLayout.c:218-240, GlobalParticleLayout.c:275-318): global
+ * particle p (0..np-1) at min + (max - min) s_d(p+1) per axis, s_d the Sobol numbers of direction numbers
+ * dirnum[d*32 + k] (k < 30); key[p] its element by Mesh_RegularAlgorithms_SearchElements (a point on an interior
+ * face goes to the lower element), xi its local coordinates there.  The caller sorts by key. */
+int ust_k_spacefiller_layout(const ust_grid *g, int np, const unsigned *dirnum, unsigned *key, double *xi0,
+                             double *xi1, double *xi2, void *stream);
 /* material of each particle: cell_material[e] (or 0) overridden by each containing shape in order */
 int ust_k_assign_materials(const ust_grid *g, const int *cell_off, int np, const double *xi0, const double *xi1,
                            const double *xi2, const ust_shape *shapes_dev, int nshape,

@@ -1,0 +1,134 @@
+"""FeVariable checkpoint files (HDF5) without a GPU: the reader against the reference's own regression fields,
+the writer against their structure (FeVariable_SaveToFile / FeVariable_ReadFromFile,
+StgFEM/Discretisation/src/FeVariable.c:2136-2389 and :2452-2730).
+
+Fixtures in tests/golden/h5/ are the reference's files, renamed with their model:
+  RayleighTaylorBenchmark_lowres_{Velocity,Pressure}Field.00005.h5
+      Underworld/SysTest/RegressionTests/lowres_expected/RayleighTaylorBenchmark-referenceTest/
+  CylinderRiseThermal_lowres_VelocityField.00005.h5
+      Underworld/SysTest/RegressionTests/lowres_expected/CylinderRiseThermal-referenceTest/
+  PeriodicBenchmark_{Velocity,Pressure}Field.00010.h5 (version 1: a node-index column and the coordinates)
+      Underworld/SysTest/RegressionTests/expected/PeriodicBenchmark-referenceTest/
+h5dump (the HDF5 command-line tool), where installed, is the independent reader the tests compare with."""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "h5")
+H5DUMP = shutil.which("h5dump") or ("/opt/conda/bin/h5dump" if os.path.exists("/opt/conda/bin/h5dump") else None)
+
+
+def _ust():
+    import earthbyte_underworld_amd as ust
+    try:
+        ust.read_checkpoint(_gold(CASES[0][0]))
+    except ust.UstError as e:
+        if "no HDF5" in str(e) or "HDF5 1." in str(e):
+            pytest.skip(f"no usable HDF5 C library: {e}")
+        raise
+    return ust
+
+
+def _gold(name):
+    return os.path.join(GOLD, name)
+
+
+def _h5dump_values(path):
+    """the /data values as h5dump prints them with 17 significant digits"""
+    out = subprocess.run([H5DUMP, "-m", "%.17g", "-d", "/data", "-y", path], capture_output=True, text=True,
+                         check=True).stdout
+    body = out[out.index("DATA {") + 6:out.rindex("}")]
+    return np.array([float(v) for v in re.findall(r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?", body)])
+
+
+def _h5dump_header(path):
+    out = subprocess.run([H5DUMP, "-A", path], capture_output=True, text=True, check=True).stdout
+    return out.split("\n", 1)[1]  # drop the line that names the file
+
+
+CASES = [  # file, version, dimensions, mesh resolution, rows, columns
+    ("RayleighTaylorBenchmark_lowres_VelocityField.00005.h5", 2, 2, [10, 10], 121, 2),
+    ("RayleighTaylorBenchmark_lowres_PressureField.00005.h5", 2, 2, None, 100, 1),
+    ("CylinderRiseThermal_lowres_VelocityField.00005.h5", 2, 3, [10, 10, 10], 1331, 3),
+    ("PeriodicBenchmark_VelocityField.00010.h5", 1, 0, None, 1089, 5),
+    ("PeriodicBenchmark_PressureField.00010.h5", 1, 0, None, 1024, 4),
+]
+
+
+@pytest.mark.parametrize("name,version,dim,res,rows,cols", CASES)
+def test_reader_on_reference_fields(name, version, dim, res, rows, cols):
+    ust = _ust()
+    r = ust.read_checkpoint(_gold(name))
+    assert r["version"] == version and r["dimensions"] == dim and r["mesh resolution"] == res
+    assert r["data"].shape == (rows, cols) and np.all(np.isfinite(r["data"]))
+    if res is not None:  # a linear-mesh field: one row per vertex
+        assert rows == int(np.prod([n + 1 for n in res]))
+    if H5DUMP:
+        assert np.array_equal(_h5dump_values(_gold(name)), r["data"].ravel())
+
+
+def test_reference_layouts():
+    """What the rows hold, read from the reference's files: version-1 rows are [node index, coordinates,
+    values], the pressure's nodes are the element centres (32 x 32 elements on the unit square), the velocity's
+    the vertices (x fastest); the free-slip box walls of the lowres Rayleigh-Taylor run carry zero normal
+    velocity."""
+    ust = _ust()
+    v = ust.read_checkpoint(_gold("PeriodicBenchmark_VelocityField.00010.h5"))["data"]
+    p = ust.read_checkpoint(_gold("PeriodicBenchmark_PressureField.00010.h5"))["data"]
+    assert np.array_equal(v[:, 0], np.arange(1089)) and np.array_equal(p[:, 0], np.arange(1024))
+    i = np.arange(1089)
+    assert np.allclose(v[:, 1], (i % 33) / 32, atol=1e-15) and np.allclose(v[:, 2], (i // 33) / 32, atol=1e-15)
+    e = np.arange(1024)
+    assert np.allclose(p[:, 1], ((e % 32) + 0.5) / 32, atol=1e-15)
+    assert np.allclose(p[:, 2], ((e // 32) + 0.5) / 32, atol=1e-15)
+    rt = ust.read_checkpoint(_gold("RayleighTaylorBenchmark_lowres_VelocityField.00005.h5"))["data"]
+    n = np.arange(121)
+    ix, iy = n % 11, n // 11
+    assert np.all(rt[(ix == 0) | (ix == 10), 0] == 0.0) and np.all(rt[(iy == 0) | (iy == 10), 1] == 0.0)
+    assert np.abs(rt).max() > 1e-4  # the field itself is not zero
+
+
+@pytest.mark.parametrize("dim,res,cols", [(2, [10, 10], 2), (2, None, 1), (3, [4, 3, 2], 3), (3, None, 4)])
+def test_write_read_round_trip(tmp_path, dim, res, cols):
+    ust = _ust()
+    rows = int(np.prod([n + 1 for n in res])) if res else 37
+    a = np.random.default_rng(dim * 10 + cols).normal(size=(rows, cols)) * 10.0 ** np.arange(-3, 3)[:cols]
+    path = str(tmp_path / "f.h5")
+    ust.write_checkpoint(path, a, dim, res)
+    r = ust.read_checkpoint(path)
+    assert r["version"] == 2 and r["dimensions"] == dim and r["mesh resolution"] == res
+    assert np.array_equal(r["data"], a)
+    if H5DUMP:
+        assert np.array_equal(_h5dump_values(path), a.ravel())
+
+
+@pytest.mark.skipif(H5DUMP is None, reason="h5dump not installed")
+@pytest.mark.parametrize("name", [c[0] for c in CASES[:3]])
+def test_writer_reproduces_reference_files(tmp_path, name):
+    """A reference field rewritten through the writer dumps (attributes, types, dataspaces, values) exactly as
+    the reference's file does."""
+    ust = _ust()
+    r = ust.read_checkpoint(_gold(name))
+    path = str(tmp_path / name)
+    ust.write_checkpoint(path, r["data"], r["dimensions"], r["mesh resolution"])
+    assert _h5dump_header(path) == _h5dump_header(_gold(name))
+    full = [subprocess.run([H5DUMP, "-m", "%.17g", f], capture_output=True, text=True, check=True).stdout
+            .split("\n", 1)[1] for f in (path, _gold(name))]
+    assert full[0] == full[1]
+
+
+def test_errors(tmp_path):
+    ust = _ust()
+    with pytest.raises(ust.UstError, match="cannot open"):
+        ust.read_checkpoint(str(tmp_path / "missing.h5"))
+    with pytest.raises(ValueError):
+        ust.write_checkpoint(str(tmp_path / "x.h5"), np.zeros(4), 2)
+    p = tmp_path / "not_hdf5.h5"
+    p.write_bytes(b"\x00" * 64)
+    with pytest.raises(ust.UstError):
+        ust.read_checkpoint(str(p))
