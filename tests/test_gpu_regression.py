@@ -1,23 +1,28 @@
-"""The reference's own regression run, reproduced on the GPU: RayleighTaylorBenchmark.xml at the lowres settings
-of Underworld/SysTest/RegressionTests/testAll_lowres.py (elementResI/J = 10, runSteps = 5), compared with its
-expected fields (lowres_expected/RayleighTaylorBenchmark-referenceTest/{Velocity,Pressure}Field.00005.h5) by
-credo's ReferenceTest measure and tolerance: per component, the global normalised L2 error
+"""The reference's own regression runs, reproduced on the GPU and compared with their expected fields by credo's
+ReferenceTest measure and tolerance: per component, the global normalised L2 error
 sqrt(sum_el int (f - f_ref)^2) / sqrt(sum_el int f_ref^2) (FieldTest_GenerateErrFields,
 StgFEM/Discretisation/src/FieldTest.c:955-1080; 2x2 Gauss points per element here) below defFieldTol = 1e-2
 (credo/credo/systest/referenceTest.py:82).
 
-The model, from the XML chain (RayleighTaylorBenchmark.xml -> RayleighTaylor.xml -> LidDrivenPIC.xml):
-  * 10 x 10 Q1/P0 on [0, 0.9142] x [0, 1]; material swarm by SpaceFillerParticleLayout, 20 per cell on average
-    (PIC_Components/MaterialPointSwarm.xml), coincident integration points weighted by PCDVC (resolution 10,
-    lowerT 0.6, upperT 25, maxDeletions 3, maxSplits 3);
-  * background density 1, eta 1; the light layer below the cosine plane (offset 0.2, amplitude 0.02,
-    wavelength 1.8284) density 0, eta 1; gravity 1 (BuoyancyForceTerm);
-  * velocityBCs.freeslipSides.xml: no slip at top and bottom, vx = 0 on the sides;
-  * per step (AbstractContext.c:806-905): solve, then dt = 0.5 min(h) / max |u| (Stokes_SLE_UpdateDt,
-    UpdateDt.c:52-72; timestepFactor 1), advection by the second-order TimeIntegrator with
-    allowFallbackToFirstOrder, PCDVC again on the moved swarm; the fields of timestep 5 are those of the sixth
-    solve, after five advections.
-The reference solved with Stokes_SLE_UzawaSolver to tolerance 1e-5; the solve here is BSSCR to 1e-9, whose
+Cases (expected files in tests/golden/h5/, copied from the reference):
+  * RayleighTaylorBenchmark.xml at the lowres settings of Underworld/SysTest/RegressionTests/testAll_lowres.py
+    (elementResI/J = 10, runSteps = 5; lowres_expected/RayleighTaylorBenchmark-referenceTest/*.00005.h5) and at
+    its own resolution (64 x 64, runSteps = 10; expected/RayleighTaylorBenchmark-referenceTest/*.00010.h5).
+    From the XML chain (RayleighTaylorBenchmark.xml -> RayleighTaylor.xml -> LidDrivenPIC.xml): [0, 0.9142] x
+    [0, 1]; material swarm by SpaceFillerParticleLayout, 20 per cell on average (PIC_Components/
+    MaterialPointSwarm.xml), coincident integration points weighted by PCDVC (resolution 10, lowerT 0.6,
+    upperT 25, maxDeletions 3, maxSplits 3); background density 1, eta 1, the light layer below the cosine
+    plane (offset 0.2, amplitude 0.02, wavelength 1.8284) density 0, eta 1; gravity 1; velocityBCs.
+    freeslipSides.xml: no slip at top and bottom, vx = 0 on the sides.  Per step (AbstractContext.c:806-905):
+    solve, then dt = 0.5 min(h) / max |u| (Stokes_SLE_UpdateDt, UpdateDt.c:52-72; timestepFactor 1), advection
+    by the second-order TimeIntegrator with allowFallbackToFirstOrder, PCDVC again on the moved swarm; the fields
+    of timestep N are those of the solve after N advections (the first solve is timestep 0).
+  * PICellerator/SysTest/RegressionTests/LidDrivenPIC2D.xml (20 x 20, 36 particles per cell, isoviscous, partial
+    lid; expected/LidDrivenPIC2D.xml/*.00010.h5).  That run's context numbered its first solve timestep 1
+    (expected/LidDrivenPIC2D.xml/FrequentOutput.dat: timestep 1 at time 0, timestep 2 at time 0.025), so its
+    timestep 10 is the solve after nine advections; the other step counts miss the expected fields by 10-20x
+    more (measured: 4e-6 in velocity after nine, 4e-5 to 1e-4 after eight or ten).
+The reference solved with Stokes_SLE_UzawaSolver to tolerance 1e-5; the solves here are BSSCR to 1e-9, whose
 difference is far below the tolerance."""
 import os
 
@@ -70,23 +75,49 @@ def _gauss_error(nel, lo, hi, num, ref, kind):
     return np.sqrt(err2 / nrm2)
 
 
-def rayleigh_taylor_lowres(steps=5, rtol=1e-9):
+def run_model(model, nel, steps, rtol=1e-9, on_step=None):
+    """The reference model on the GPU, `steps` time steps after the first solve; returns the system and a log
+    row per advection step.  on_step(s, step) is called after every solve."""
     import earthbyte_underworld_amd as ust
-    lo, hi, nel = (0.0, 0.0), (0.9142, 1.0), (10, 10)
-    s = ust.StokesSystem(2, nel, lo, hi)
-    s.spacefiller_particle_layout(20)
-    s.set_materials([ust.RheologyMaterial([ust.MaterialViscosity(eta0=1.0)], density=1.0),
-                     ust.RheologyMaterial([ust.MaterialViscosity(eta0=1.0)], density=0.0)])
-    s.assign_materials([ust.Everywhere(0), ust.BelowCosinePlane(1, 0.2, 0.02, 1.8284, 0.0)])
-    s.set_bcs(*ust.freeslip_sides_noslip_topbottom(s))
     pc = dict(resolution=10, upperT=25, lowerT=0.6, maxDeletions=3, maxSplits=3)
+    if model == "RayleighTaylorBenchmark":
+        lo, hi = (0.0, 0.0), (0.9142, 1.0)
+        s = ust.StokesSystem(2, nel, lo, hi)
+        s.spacefiller_particle_layout(20)
+        s.set_materials([ust.RheologyMaterial([ust.MaterialViscosity(eta0=1.0)], density=1.0),
+                         ust.RheologyMaterial([ust.MaterialViscosity(eta0=1.0)], density=0.0)])
+        s.assign_materials([ust.Everywhere(0), ust.BelowCosinePlane(1, 0.2, 0.02, 1.8284, 0.0)])
+        s.set_bcs(*ust.freeslip_sides_noslip_topbottom(s))
+        gravity = 1.0
+    elif model == "LidDrivenPIC2D":
+        # PICellerator/SysTest/RegressionTests/LidDrivenPIC2D.xml: isoviscous (viscosity 1 on the PIC points),
+        # 36 particles per cell, lidDrivenVelocityBCs.xml: vx = 0 on the sides, vy = 0 at the bottom, at the top
+        # vy = 0 and vx = Velocity_PartialLid_TopLayer (1 where min + 1.1 h < x < max - 1.1 h, else 0;
+        # StandardConditionFunctions.c:949-971)
+        lo, hi = (0.0, 0.0), (1.0, 1.0)
+        s = ust.StokesSystem(2, nel, lo, hi)
+        s.spacefiller_particle_layout(36)
+        s.set_materials([ust.RheologyMaterial([ust.MaterialViscosity(eta0=1.0)], density=0.0)])
+        s.assign_materials([ust.Everywhere(0)])
+        xy = s.node_coords()
+        mask = ust.wall_mask(s, {(0, 0): [0], (0, 1): [0], (1, 0): [1], (1, 1): [0, 1]})
+        vals = np.zeros(s.ndof)
+        margin = 1.1 * min((hi[0] - lo[0]) / nel[0], (hi[1] - lo[1]) / nel[1])
+        top = xy[:, 1] == hi[1]
+        vals[0::2][top & (xy[:, 0] < hi[0] - margin) & (xy[:, 0] > lo[0] + margin)] = 1.0
+        s.set_bcs(mask, vals)
+        gravity = 0.0
+    else:
+        raise ValueError(model)
     hmin = min((hi[0] - lo[0]) / nel[0], (hi[1] - lo[1]) / nel[1])
     s.pcdvc(**pc)
     log = []
     for step in range(steps + 1):
-        s.assemble(gravity=1.0)
+        s.assemble(gravity=gravity)
         st = s.solve(outer_rtol=rtol, inner_rtol=rtol * 1e-1)
         assert st["stokes_residual"] < 1e-7, st
+        if on_step:
+            on_step(s, step)
         if step == steps:
             break
         u, _ = s.solution()
@@ -98,19 +129,29 @@ def rayleigh_taylor_lowres(steps=5, rtol=1e-9):
     return s, log
 
 
-def test_rayleigh_taylor_lowres_matches_reference_fields():
+CASES = [  # model, elements, steps, expected files (velocity, pressure)
+    ("RayleighTaylorBenchmark", (10, 10), 5, "RayleighTaylorBenchmark_lowres_{}Field.00005.h5"),
+    ("RayleighTaylorBenchmark", (64, 64), 10, "RayleighTaylorBenchmark_{}Field.00010.h5"),
+    ("LidDrivenPIC2D", (20, 20), 9, "LidDrivenPIC2D_{}Field.00010.h5"),
+]
+
+
+@pytest.mark.parametrize("model,nel,steps,files", CASES)
+def test_regression_matches_reference_fields(model, nel, steps, files):
     import earthbyte_underworld_amd as ust
     try:
-        ref_u = ust.read_checkpoint(os.path.join(GOLD, "RayleighTaylorBenchmark_lowres_VelocityField.00005.h5"))
-        ref_p = ust.read_checkpoint(os.path.join(GOLD, "RayleighTaylorBenchmark_lowres_PressureField.00005.h5"))
+        ref_u = ust.read_checkpoint(os.path.join(GOLD, files.format("Velocity")))
+        ref_p = ust.read_checkpoint(os.path.join(GOLD, files.format("Pressure")))
     except ust.UstError as e:
         pytest.skip(f"no usable HDF5 C library: {e}")
-    s, log = rayleigh_taylor_lowres()
+    assert ref_u["mesh resolution"] == list(nel)
+    s, log = run_model(model, nel, steps)
     for row in log:
         print("step %d dt %.5g vmax %.5g removed %d empty %d splits %d deletions %d particles %d" % row)
     u, p = s.solution()
-    eu = _gauss_error((10, 10), (0.0, 0.0), (0.9142, 1.0), u.reshape(-1, 2), ref_u["data"], "q1")
-    ep = _gauss_error((10, 10), (0.0, 0.0), (0.9142, 1.0), p.reshape(-1, 1), ref_p["data"], "p0")
-    print("normalised errors: vx %.3e vy %.3e p %.3e" % (eu[0], eu[1], ep[0]))
+    lo, hi = (0.0, 0.0), tuple(s.max)
+    eu = _gauss_error(nel, lo, hi, u.reshape(-1, 2), ref_u["data"], "q1")
+    ep = _gauss_error(nel, lo, hi, p.reshape(-1, 1), ref_p["data"], "p0")
+    print("%s %s: normalised errors vx %.3e vy %.3e p %.3e" % (model, nel, eu[0], eu[1], ep[0]))
     assert np.all(eu < 1e-2) and ep[0] < 1e-2
     s.close()

@@ -9,6 +9,10 @@ Fixtures in tests/golden/h5/ are the reference's files, renamed with their model
       Underworld/SysTest/RegressionTests/lowres_expected/CylinderRiseThermal-referenceTest/
   PeriodicBenchmark_{Velocity,Pressure}Field.00010.h5 (version 1: a node-index column and the coordinates)
       Underworld/SysTest/RegressionTests/expected/PeriodicBenchmark-referenceTest/
+  RayleighTaylorBenchmark_{Velocity,Pressure}Field.00010.h5
+      Underworld/SysTest/RegressionTests/expected/RayleighTaylorBenchmark-referenceTest/
+  LidDrivenPIC2D_{Velocity,Pressure}Field.00010.h5
+      PICellerator/SysTest/RegressionTests/expected/LidDrivenPIC2D.xml/
 h5dump (the HDF5 command-line tool), where installed, is the independent reader the tests compare with."""
 import os
 import re
@@ -57,6 +61,10 @@ CASES = [  # file, version, dimensions, mesh resolution, rows, columns
     ("CylinderRiseThermal_lowres_VelocityField.00005.h5", 2, 3, [10, 10, 10], 1331, 3),
     ("PeriodicBenchmark_VelocityField.00010.h5", 1, 0, None, 1089, 5),
     ("PeriodicBenchmark_PressureField.00010.h5", 1, 0, None, 1024, 4),
+    ("RayleighTaylorBenchmark_VelocityField.00010.h5", 2, 2, [64, 64], 4225, 2),
+    ("RayleighTaylorBenchmark_PressureField.00010.h5", 2, 2, None, 4096, 1),
+    ("LidDrivenPIC2D_VelocityField.00010.h5", 2, 2, [20, 20], 441, 2),
+    ("LidDrivenPIC2D_PressureField.00010.h5", 2, 2, None, 400, 1),
 ]
 
 
