@@ -292,15 +292,22 @@ constexpr int CPX = UST_TX3 + 1, CPY = UST_TY3 + 1, CNP = CPX * CPY, CNPD = CNP 
 constexpr int CNR = (CNPD + UST_TILE - 1) / UST_TILE;
 constexpr int EBS = UST_TILE + 1;  // row stride of the element-result buffer (rows shift across LDS banks)
 
-// Build-time tuning of the FP32 column passes: workgroups per CU the register budget must allow, and whether
-// the next layer's operands are prefetched in registers.  Measured on the 128x128x64 solve (MI355X): 2 with
-// prefetch is fastest overall; 4 without prefetch speeds the residual pass (58 -> 50 us) but spills in the
-// prolongation pass (76 -> 112 us), 3 spills in both.
+// Build-time tuning of the column passes.  UST_COL_MINB_F32: workgroups per CU the register budget of the FP32
+// passes must allow.  UST_COL_PF_*: when the next layer's node plane and weights are loaded -- 1: at the start of
+// the layer, held in registers across the element apply; 2: right after the element apply, so that they are live
+// only across the reduction and the staging; 0: when they are used.  Measured on the 128x128x64 passes (MI355X,
+// 50 launches each): loads after the element apply take the FP32 residual pass from 58.2 to 55.0 us (168
+// instead of 246 VGPRs), the prolongation pass from 75.3 to 72.1 us and the FP64 product from 92.0 to 90.7 us
+// (its loads otherwise wait at the layer); 4 FP32 workgroups per CU without prefetch speed the residual pass
+// (50 us) but spill in the prolongation pass (112 us), 3 spill in both.
 #ifndef UST_COL_MINB_F32
 #define UST_COL_MINB_F32 2
 #endif
-#ifndef UST_COL_PREFETCH
-#define UST_COL_PREFETCH 1
+#ifndef UST_COL_PF_F32
+#define UST_COL_PF_F32 2
+#endif
+#ifndef UST_COL_PF_F64
+#define UST_COL_PF_F64 2
 #endif
 template <typename T, typename WT, int MODE>
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? UST_COL_MINB_F32 : 2) void k_applyK3_col(GridDev g, const WT *__restrict__ W,
@@ -394,9 +401,10 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? UST_COL_MINB_F32 : 2) void k_
 #pragma unroll
         for (int q = 0; q < 27; q++) w[q] = W[wbase + ((size_t)k * 27 + q) * UST_TILE];
     };
-    // FP32 (and bfloat16 weights): the next layer's node plane and weights are prefetched in registers during
-    // this layer; FP64 has no registers to spare for them and loads them at the layer
-    constexpr bool wpre = sizeof(T) == 4 && UST_COL_PREFETCH;
+    // when the next layer's node plane and weights are loaded (UST_COL_PF_*): wpre at the start of this layer,
+    // wlate after its element apply; held: in registers (xr, mr, wr) until the next layer uses them
+    constexpr int pf = sizeof(T) == 4 ? UST_COL_PF_F32 : UST_COL_PF_F64;
+    constexpr bool wpre = pf == 1, wlate = pf == 2, held = pf != 0;
     // ---- prologue: the coarse ring (planes z0/2 and z0/2 + 1), plane z0, and the loads for layer z0 ----
     if (MODE == EPI_PCHEBY0) {
         const int c0 = z0 >> 1;
@@ -420,9 +428,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? UST_COL_MINB_F32 : 2) void k_
             carry[r] = (T)0;  // plane z0 receives nothing from below within this chunk
         }
     }
-    if (wpre) load_raw(z0 + 1, xr, mr);
+    if (held) load_raw(z0 + 1, xr, mr);
     WT wr[27];
-    if (wpre) load_w(z0, wr);
+    if (held) load_w(z0, wr);
     // EPI_PCHEBY0: staging an even plane p writes coarse plane p/2 + 1 into the ring (the odd plane p + 1 needs
     // it); its value is loaded one layer ahead
     T cv = (T)0;
@@ -434,7 +442,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? UST_COL_MINB_F32 : 2) void k_
         // this layer's operands, loaded during the previous one
         T x1[CNR];
         unsigned char m1[CNR];
-        if (wpre) {
+        if (held) {
 #pragma unroll
             for (int r = 0; r < CNR; r++) {
                 x1[r] = xr[r];
@@ -444,7 +452,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? UST_COL_MINB_F32 : 2) void k_
             load_raw(k + 1, x1, m1);
         }
         T wq[27];
-        if (wpre) {
+        if (held) {
 #pragma unroll
             for (int q = 0; q < 27; q++) wq[q] = wload<T, WT>(wr, q);
         }
@@ -469,7 +477,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? UST_COL_MINB_F32 : 2) void k_
         }
         __syncthreads();
         {
-            if (!wpre) {
+            if (!held) {
 #pragma unroll
                 for (int q = 0; q < 27; q++) wq[q] = wload<T, WT>(W, wbase + ((size_t)k * 27 + q) * UST_TILE);
             }
@@ -488,6 +496,8 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? UST_COL_MINB_F32 : 2) void k_
 #pragma unroll
                 for (int i = 0; i < 3; i++) eb[(a * 3 + i) * EBS + tid] = active ? ye[a][i] : (T)0;
         }
+        if (wlate && k + 2 <= z1) load_raw(k + 2, xr, mr);
+        if (wlate && k + 1 < z1) load_w(k + 1, wr);
         if (!wpre) load_ops(k, zsh, ops);
         __syncthreads();
         // plane k: the carried contributions of layer k-1 plus those of layer k below the plane; the

@@ -17,6 +17,8 @@ Cases (expected files in tests/golden/h5/, copied from the reference):
     solve, then dt = 0.5 min(h) / max |u| (Stokes_SLE_UpdateDt, UpdateDt.c:52-72; timestepFactor 1), advection
     by the second-order TimeIntegrator with allowFallbackToFirstOrder, PCDVC again on the moved swarm; the fields
     of timestep N are those of the solve after N advections (the first solve is timestep 0).
+  * StgFEM/SysTest/RegressionTests/Multigrid.xml (32 x 16 on [0, 2] x [0, 1], Gauss-point integration, isoviscous
+    lid-driven cavity; expected/Multigrid.xml/*.00010.h5): steady, the first solve.
   * PICellerator/SysTest/RegressionTests/LidDrivenPIC2D.xml (20 x 20, 36 particles per cell, isoviscous, partial
     lid; expected/LidDrivenPIC2D.xml/*.00010.h5).  That run's context numbered its first solve timestep 1
     (expected/LidDrivenPIC2D.xml/FrequentOutput.dat: timestep 1 at time 0, timestep 2 at time 0.025), so its
@@ -107,10 +109,32 @@ def run_model(model, nel, steps, rtol=1e-9, on_step=None):
         vals[0::2][top & (xy[:, 0] < hi[0] - margin) & (xy[:, 0] > lo[0] + margin)] = 1.0
         s.set_bcs(mask, vals)
         gravity = 0.0
+    elif model == "Multigrid":
+        # StgFEM/SysTest/RegressionTests/Multigrid.xml: finite elements only -- the GaussSwarm (2x2 points of
+        # weight 1 per element) integrates the isoviscous term (viscosity 1, StgFEM_Components/StokesFlowUzawa.xml);
+        # lid-driven: vx = 0 on the sides, vy = 0 at the bottom, at the top vx = 1 (the top wall's condition,
+        # applied last, holds at the corners) and vy = 0; steady, so every timestep holds the first solve
+        lo, hi = (0.0, 0.0), (2.0, 1.0)
+        s = ust.StokesSystem(2, nel, lo, hi)
+        g = 1.0 / np.sqrt(3.0)
+        ne = s.nel
+        xi = np.zeros((4 * ne, 3))
+        for k, (a, b) in enumerate([(-g, -g), (g, -g), (-g, g), (g, g)]):
+            xi[k::4, 0], xi[k::4, 1] = a, b
+        s.set_materials([ust.RheologyMaterial([ust.MaterialViscosity(eta0=1.0)], density=0.0)])
+        s.upload_swarm(np.arange(0, 4 * ne + 1, 4).astype(np.int32), xi, np.ones(4 * ne), np.zeros(4 * ne, np.uint8))
+        xy = s.node_coords()
+        mask = ust.wall_mask(s, {(0, 0): [0], (0, 1): [0], (1, 0): [1], (1, 1): [0, 1]})
+        vals = np.zeros(s.ndof)
+        vals[0::2][xy[:, 1] == hi[1]] = 1.0
+        s.set_bcs(mask, vals)
+        gravity = 0.0
+        pc = None
     else:
         raise ValueError(model)
     hmin = min((hi[0] - lo[0]) / nel[0], (hi[1] - lo[1]) / nel[1])
-    s.pcdvc(**pc)
+    if pc:
+        s.pcdvc(**pc)
     log = []
     for step in range(steps + 1):
         s.assemble(gravity=gravity)
@@ -133,6 +157,7 @@ CASES = [  # model, elements, steps, expected files (velocity, pressure)
     ("RayleighTaylorBenchmark", (10, 10), 5, "RayleighTaylorBenchmark_lowres_{}Field.00005.h5"),
     ("RayleighTaylorBenchmark", (64, 64), 10, "RayleighTaylorBenchmark_{}Field.00010.h5"),
     ("LidDrivenPIC2D", (20, 20), 9, "LidDrivenPIC2D_{}Field.00010.h5"),
+    ("Multigrid", (32, 16), 0, "Multigrid_{}Field.00010.h5"),
 ]
 
 

@@ -13,6 +13,8 @@ Fixtures in tests/golden/h5/ are the reference's files, renamed with their model
       Underworld/SysTest/RegressionTests/expected/RayleighTaylorBenchmark-referenceTest/
   LidDrivenPIC2D_{Velocity,Pressure}Field.00010.h5
       PICellerator/SysTest/RegressionTests/expected/LidDrivenPIC2D.xml/
+  Multigrid_{Velocity,Pressure}Field.00010.h5
+      StgFEM/SysTest/RegressionTests/expected/Multigrid.xml/
 h5dump (the HDF5 command-line tool), where installed, is the independent reader the tests compare with."""
 import os
 import re
@@ -65,6 +67,8 @@ CASES = [  # file, version, dimensions, mesh resolution, rows, columns
     ("RayleighTaylorBenchmark_PressureField.00010.h5", 2, 2, None, 4096, 1),
     ("LidDrivenPIC2D_VelocityField.00010.h5", 2, 2, [20, 20], 441, 2),
     ("LidDrivenPIC2D_PressureField.00010.h5", 2, 2, None, 400, 1),
+    ("Multigrid_VelocityField.00010.h5", 2, 2, [32, 16], 561, 2),
+    ("Multigrid_PressureField.00010.h5", 2, 2, None, 512, 1),
 ]
 
 
