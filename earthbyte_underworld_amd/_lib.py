@@ -75,7 +75,8 @@ class SolveOpts(C.Structure):
                 ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("max_inner", C.c_int),
                 ("remove_nullspace", C.c_int), ("stall_window", C.c_int), ("inner_relax", C.c_int),
                 ("inner_relax_cap", C.c_double), ("stokes_rtol", C.c_double), ("verbose", C.c_int),
-                ("coarse_degree", C.c_int), ("coarsest_degree", C.c_int), ("vcycle_bf16", C.c_int)]
+                ("coarse_degree", C.c_int), ("coarsest_degree", C.c_int), ("vcycle_bf16", C.c_int),
+                ("inner_fp32", C.c_int)]
 
 
 class UzawaOpts(C.Structure):

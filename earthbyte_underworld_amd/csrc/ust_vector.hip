@@ -246,6 +246,44 @@ __global__ __launch_bounds__(256) void kv_cg_xr_rr_guarded(double *x, double *r,
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// The FP32 velocity PCG inside FP64 iterative refinement: p = z + beta p, and x += a p, r -= a q with the
+// block partials of r . r accumulated in FP64
+__global__ void kv_cg_p_guarded_f32(const float *z, float *p, const double *c, size_t n) {
+    if (c[1] != 0.0) return;
+    const float beta = (float)c[3];
+    GRID_STRIDE(i, n) p[i] = z[i] + beta * p[i];
+}
+
+__global__ __launch_bounds__(256) void kv_cg_xr_rr_guarded_f32(float *x, float *r, const float *p, const float *q,
+                                                               const double *rz, const double *pq, const double *c,
+                                                               size_t n, double *partial) {
+    __shared__ double sh[256 / 64];
+    double s = 0.0;
+    if (c[1] == 0.0) {
+        const float a = (float)(rz[0] / pq[0]);
+        GRID_STRIDE(i, n) {
+            x[i] += a * p[i];
+            const float ri = r[i] - a * q[i];
+            r[i] = ri;
+            s += (double)ri * (double)ri;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// y = (float)(s x): the FP64 residual scaled to unit norm for the FP32 correction solve
+__global__ void kv_d2f_scaled(const double *x, float *y, double s, size_t n) {
+    GRID_STRIDE(i, n) y[i] = (float)(s * x[i]);
+}
+
+// y += s x with x in FP32: the correction of a refinement step
+__global__ void kv_axpy_f32_to_f64(double s, const float *x, double *y, size_t n) {
+    GRID_STRIDE(i, n) y[i] += s * (double)x[i];
+}
+
 __global__ void kv_cg_check(const double *rr, double *c) {
     if (c[1] != 0.0) return;
     c[2] += 1.0;
@@ -442,6 +480,32 @@ int ust_k_cg_xr_rr_guarded_f64(double *x, double *r, const double *p, const doub
 int ust_k_cheby_first_from_f64(const double *b, float *b32, const float *invd, float *d, float *x, float c2,
                                size_t n, void *stream) {
     LV(kv_cheby_first_from64, n, b, b32, invd, d, x, c2, n);
+}
+int ust_k_cg_p_guarded_f32(const float *z, float *p, const double *ctl, size_t n, void *stream) {
+    LV(kv_cg_p_guarded_f32, n, z, p, ctl, n);
+}
+int ust_k_cg_xr_rr_guarded_f32(float *x, float *r, const float *p, const float *q, const double *rz, const double *pq,
+                               double *ctl, size_t n, double *partial, double *rr, void *stream) {
+    const unsigned g = rgrid(n);
+    hipLaunchKernelGGL(kv_cg_xr_rr_guarded_f32, dim3(g), dim3(256), 0, ust_stream(stream), x, r, p, q, rz, pq, ctl, n,
+                       partial);
+    hipLaunchKernelGGL(kv_reduce_check, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, rr, ctl);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+int ust_k_dot_beta_f32(const float *x, const float *y, size_t n, double *partial, double *rz, double *rz_old,
+                       double *ctl, void *stream) {
+    const unsigned g = rgrid(n);
+    hipLaunchKernelGGL((kv_dot_stage1<float, true>), dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
+    hipLaunchKernelGGL(kv_reduce_beta, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, rz, rz_old, ctl);
+    UST_LAUNCH_CHECK();
+    return UST_OK;
+}
+int ust_k_copy_f64_to_f32_scaled(const double *x, float *y, double s, size_t n, void *stream) {
+    LV(kv_d2f_scaled, n, x, y, s, n);
+}
+int ust_k_axpy_f32_to_f64(double s, const float *x, double *y, size_t n, void *stream) {
+    LV(kv_axpy_f32_to_f64, n, s, x, y, n);
 }
 int ust_k_cg_check(const double *rr, double *ctl, void *stream) {
     hipLaunchKernelGGL(kv_cg_check, dim3(1), dim3(1), 0, ust_stream(stream), rr, ctl);

@@ -107,9 +107,10 @@ struct ust_context {
     double *partial, *scal, *scal_h;
     double *vr, *vz, *vp, *vq, *vt;   /* FP64 velocity work (inner PCG) */
     double *vx;                       /* inner PCG iterate */
-    void *pcg_exec[2];                /* captured inner PCG iteration (FP64 / FP32 V-cycle), NULL if stale */
+    void *pcg_exec[3];                /* captured inner PCG iteration (FP64 / FP32 V-cycle / FP32 PCG), NULL if stale */
     int use_graphs;                   /* 0: launch the inner iteration kernel by kernel (UST_NO_GRAPH=1) */
     int pcg_hint;                     /* iterations of the last inner solve: first convergence check */
+    int pcg32_hint[16];               /* the same per refinement step of the FP32 velocity PCG */
     const double *b64_src;            /* set while a level-0 FP32 V-cycle starts from an FP64 right-hand side */
     float *wr, *wz, *wp, *wq, *wx;    /* FP32 velocity work (mixed inner PCG) */
     double *va, *vb;                  /* FP64 velocity work (Schur / IR) */

@@ -47,7 +47,7 @@ static int grid_nel(const ust_grid *g) {
 }
 
 void ust_graphs_reset(ust_context *ctx) {
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < 3; k++) {
         if (ctx->pcg_exec[k]) ust_graph_destroy(ctx->pcg_exec[k]);
         ctx->pcg_exec[k] = NULL;
     }
@@ -342,6 +342,11 @@ int ust_levels_setup(ust_context *ctx, const ust_solve_opts *o, int need32) {
         ctx->levels_req != o->mg_levels) {
         if ((rc = levels_structure(ctx, o, need32 || ctx->levels_prec32))) return rc;
     }
+    /* the FP32 velocity PCG applies K with FP32 weights on the finest level, beside the V-cycle's bfloat16 ones */
+    if (need32 && o->inner_fp32 && !ctx->lev[0].WK32) {
+        DALLOC(ctx, ctx->lev[0].WK32, ust_k_weights_len(&ctx->lev[0].g) * sizeof(float));
+        ctx->levels_ready = 0;
+    }
     if (ctx->levels_ready && ctx->levels_ratio == o->cheby_ratio) return UST_OK;
     ust_graphs_reset(ctx); /* captured smoother coefficients change with the operator */
     int nlev = ctx->nlev;
@@ -495,6 +500,106 @@ static int pcg(ust_context *ctx, int pc32, const double *b, double *x, double rt
     return UST_OK;
 }
 
+/* One iteration of the FP32 velocity PCG (iterative refinement's inner solve): the V-cycle on the FP32 residual
+ * wr, p = z + beta p, q = K p with the FP32 weights of the finest level, and the x (wx), r update; dots and r . r
+ * accumulate in FP64 and the stopping test runs on the device, as in the FP64 PCG. */
+static int pcg32_iteration(ust_context *ctx) {
+    size_t n = (size_t)ctx->ndof;
+    double *sc = ctx->scal;
+    ust_level *L0 = &ctx->lev[0];
+    int rc = vcycle(ctx, 0, 1, ctx->wr, ctx->wz, 0);
+    if (rc) return rc;
+    CKD(ctx, ust_k_dot_beta_f32(ctx->wr, ctx->wz, n, ctx->partial, sc + SC_RZ, sc + SC_RZOLD, sc + SC_CG, S(ctx)));
+    CKD(ctx, ust_k_cg_p_guarded_f32(ctx->wz, ctx->wp, sc + SC_CG, n, S(ctx)));
+    CKD(ctx, ust_k_apply_K_f32(&L0->g, L0->WK32, L0->mask, ctx->wp, ctx->wq, (float *)ctx->kwork, S(ctx)));
+    CKD(ctx, ust_k_dot_f32(ctx->wp, ctx->wq, n, ctx->partial, sc + SC_PQ, S(ctx)));
+    CKD(ctx, ust_k_cg_xr_rr_guarded_f32(ctx->wx, ctx->wr, ctx->wp, ctx->wq, sc + SC_RZ, sc + SC_PQ, sc + SC_CG, n,
+                                        ctx->partial, sc + SC_RR, S(ctx)));
+    return UST_OK;
+}
+
+/* FP32 PCG on K e = wr (wr at unit norm) to ||r|| <= rtol from e = 0; iterations as in pcg(), the launches
+ * before the first check taken from the same refinement step of the previous solve (steps differ in length, so
+ * one shared count would launch iterations past convergence) */
+static int pcg32(ust_context *ctx, int step, double rtol, int maxit, int *its) {
+    size_t n = (size_t)ctx->ndof;
+    double *sc = ctx->scal;
+    int rc;
+    CKD(ctx, ust_k_fill_f32(ctx->wx, 0.0f, n, S(ctx)));
+    CKD(ctx, ust_k_fill_f32(ctx->wp, 0.0f, n, S(ctx)));
+    CKD(ctx, ust_k_dot_f32(ctx->wr, ctx->wr, n, ctx->partial, sc + SC_BB, S(ctx)));
+    CKD(ctx, ust_k_cg_start(sc + SC_BB, rtol * rtol, sc + SC_CG, sc + SC_RZOLD, S(ctx)));
+    if (ctx->use_graphs && !ctx->pcg_exec[2]) {
+        CKD(ctx, ust_graph_begin(S(ctx)));
+        if ((rc = pcg32_iteration(ctx))) {
+            ust_graph_abort(S(ctx));
+            return rc;
+        }
+        CKD(ctx, ust_graph_end(S(ctx), &ctx->pcg_exec[2]));
+    }
+    int *hint = &ctx->pcg32_hint[step < 16 ? step : 15];
+    int done = 0, first = *hint > 2 ? *hint - 1 : 1;
+    double state = 0.0, count = 0.0;
+    while (done < maxit) {
+        int k = done == 0 ? first : 1;
+        if (k > maxit - done) k = maxit - done;
+        for (int j = 0; j < k; j++) {
+            if (ctx->use_graphs) CKD(ctx, ust_graph_launch(ctx->pcg_exec[2], S(ctx)));
+            else if ((rc = pcg32_iteration(ctx))) return rc;
+        }
+        done += k;
+        CKD(ctx, ust_memcpy_d2h(ctx->scal_h + SC_CG + 1, sc + SC_CG + 1, 2 * sizeof(double), S(ctx)));
+        state = ctx->scal_h[SC_CG + 1];
+        count = ctx->scal_h[SC_CG + 2];
+        if (state != 0.0) break;
+    }
+    *its = (int)count;
+    if (state == 2.0) UST_FAIL(ctx, UST_ENOTCONV, "A11 CG produced NaN (StokesBlockKSPInterface.c:32 check)");
+    if (*its > 0) *hint = *its;
+    return UST_OK;
+}
+
+/* FP32 correction solves to at most this relative reduction each (FP32 recurrences drift from the true residual
+ * below it), and the FP64 residual is recomputed when the requested tolerance is tighter than IR_TRUST */
+#define IR_TAU_MIN 1e-5
+#define IR_TRUST 1e-4
+#define IR_MAX_STEPS 12
+
+/* x = K^-1 b by iterative refinement: x accumulates in FP64 (vt), r = b - K x in FP64 (vr); each step solves
+ * K e = r / ||r|| with the FP32 PCG and adds ||r|| e */
+static int pcg_refined(ust_context *ctx, const double *b, double *x, double rtol, int maxit, int *its,
+                       int *steps) {
+    size_t n = (size_t)ctx->ndof;
+    double *sc = ctx->scal, bb, rr;
+    int total = 0, k, rc;
+    *steps = 0;
+    CKD(ctx, ust_k_fill_f64(ctx->vt, 0.0, n, S(ctx)));
+    CKD(ctx, ust_memcpy_d2d(ctx->vr, b, n * sizeof(double), S(ctx)));
+    CKD(ctx, ust_k_dot_f64(b, b, n, ctx->partial, sc + SC_T0, S(ctx)));
+    CKD(ctx, ust_read_scalar(ctx, SC_T0, &bb));
+    rr = bb;
+    for (k = 0; k < IR_MAX_STEPS && total < maxit; k++) {
+        if (!(rr > rtol * rtol * bb)) break;  /* converged (or b = 0) */
+        const double rn = sqrt(rr), want = rtol * sqrt(bb) / rn;
+        const double tau = want > IR_TAU_MIN ? want : IR_TAU_MIN;
+        int it = 0;
+        CKD(ctx, ust_k_copy_f64_to_f32_scaled(ctx->vr, ctx->wr, 1.0 / rn, n, S(ctx)));
+        if ((rc = pcg32(ctx, k, tau, maxit - total, &it))) return rc;
+        total += it;
+        CKD(ctx, ust_k_axpy_f32_to_f64(rn, ctx->wx, ctx->vt, n, S(ctx)));
+        (*steps)++;
+        if (want >= IR_TRUST) break;  /* a loose request: the FP32 recurrence is trusted */
+        if ((rc = lev_apply(ctx, 0, 0, ctx->vt, ctx->vq))) return rc;
+        CKD(ctx, ust_k_residual_f64(b, ctx->vq, ctx->vr, n, S(ctx)));
+        CKD(ctx, ust_k_dot_f64(ctx->vr, ctx->vr, n, ctx->partial, sc + SC_T0, S(ctx)));
+        CKD(ctx, ust_read_scalar(ctx, SC_T0, &rr));
+        if (!(rr == rr)) UST_FAIL(ctx, UST_ENOTCONV, "A11 refinement produced NaN");
+    }
+    *its = total;
+    CKD(ctx, ust_memcpy_d2d(x, ctx->vt, n * sizeof(double), S(ctx)));
+    return UST_OK;
+}
+
 int ust_set_schur_scaling(ust_context *ctx, const float *log10_scale) {
     if (!ctx->has_mesh) UST_FAIL(ctx, UST_ESTATE, "set the mesh first");
     if (!log10_scale) UST_FAIL(ctx, UST_EINVAL, "no scaling given");
@@ -514,6 +619,13 @@ int ust_solver_prepare(ust_context *ctx, const ust_solve_opts *o) {
 static int a11_solve(ust_context *ctx, const ust_solve_opts *o, const double *b, double *x, ust_solve_stats *st) {
     int its = 0, rc;
     st->inner_solves++;
+    if (o->precision == UST_PREC_MIXED && o->inner_fp32) {
+        int steps = 0;
+        rc = pcg_refined(ctx, b, x, o->inner_rtol, o->max_inner, &its, &steps);
+        st->inner_its += its;
+        st->refinements += steps;
+        return rc;
+    }
     rc = pcg(ctx, o->precision == UST_PREC_MIXED, b, x, o->inner_rtol, o->max_inner, &its);
     st->inner_its += its;
     if (o->precision == UST_PREC_MIXED) st->refinements += its;

@@ -54,6 +54,12 @@ typedef struct {
                                 solve (0: 24) */
     int vcycle_bf16;         /* mixed precision: 1 (default) the FP32 V-cycle reads bfloat16 K weights (half the
                                 weight stream of every multigrid pass), 0 FP32 weights */
+    int inner_fp32;          /* mixed precision: 1 runs the velocity PCG itself in FP32 (FP32 K weights and Krylov
+                                vectors, dots accumulated in FP64) inside FP64 iterative refinement: each refinement
+                                solves for the correction of the FP64 residual, rounded to FP32 at unit norm, to the
+                                remaining tolerance (at most 1e-5 per step), and the FP64 residual is recomputed only
+                                when the requested tolerance is tighter than 1e-4; 0 (default) FP64 PCG around the
+                                FP32 V-cycle */
 } ust_solve_opts;
 
 typedef struct {

@@ -434,6 +434,15 @@ int ust_k_cg_p_guarded_zf(const float *z, double *p, const double *ctl, size_t n
 int ust_k_cg_xr_rr_guarded_f64(double *x, double *r, const double *p, const double *q, const double *rz,
                                const double *pq, double *ctl, size_t n, double *partial, double *rr,
                                void *stream);
+/* The FP32 velocity PCG of iterative refinement: the same steps on FP32 vectors (dots and r . r accumulated in
+ * FP64 against the same control block), the scaled rounding of the FP64 residual and the FP64 correction. */
+int ust_k_cg_p_guarded_f32(const float *z, float *p, const double *ctl, size_t n, void *stream);
+int ust_k_cg_xr_rr_guarded_f32(float *x, float *r, const float *p, const float *q, const double *rz, const double *pq,
+                               double *ctl, size_t n, double *partial, double *rr, void *stream);
+int ust_k_dot_beta_f32(const float *x, const float *y, size_t n, double *partial, double *rz, double *rz_old,
+                       double *ctl, void *stream);
+int ust_k_copy_f64_to_f32_scaled(const double *x, float *y, double s, size_t n, void *stream);
+int ust_k_axpy_f32_to_f64(double s, const float *x, double *y, size_t n, void *stream);
 
 /* Chebyshev smoother step (Jacobi-preconditioned).  r = b - y (y = A x given), then
  * d = c1*d + c2*invd.*r ; x += d  (c1 = 0 on the first step) */

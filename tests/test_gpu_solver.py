@@ -74,6 +74,14 @@ def test_bsscr_spheres_3d_fp64_and_mixed(oracle):
     assert st32["refinements"] > 0
     assert rel_err(u32, u64) < 1e-6
     assert rel_err(p32, p64) < 1e-6
+    # the velocity PCG itself in FP32 inside FP64 iterative refinement: a tight inner tolerance takes several
+    # refinement steps and still reaches the FP64 solution
+    stir = s.solve(precision=ust._lib.PREC_MIXED, inner_fp32=1, outer_rtol=1e-9, inner_rtol=1e-11)
+    uir, pir = s.solution()
+    assert stir["stokes_residual"] < 1e-6, stir
+    assert stir["refinements"] > stir["inner_solves"], stir
+    assert rel_err(uir, u64) < 1e-6
+    assert rel_err(pir, p64) < 1e-6
 
 
 def test_velocity_solve_matches_scipy(oracle):
@@ -156,11 +164,15 @@ def test_graph_replays_match_eager_launches(monkeypatch):
         s.assemble(gravity=1.0)
         st = s.solve(outer_rtol=1e-8, inner_rtol=1e-10)
         st32 = s.solve(precision=ust._lib.PREC_MIXED, outer_rtol=1e-8, inner_rtol=1e-10)
-        out.append((s.solution(), st, st32))
+        sol32 = s.solution()
+        stir = s.solve(precision=ust._lib.PREC_MIXED, inner_fp32=1, outer_rtol=1e-8, inner_rtol=1e-10)
+        out.append((s.solution(), st, st32, sol32, stir))
         s.close()
-    (ua, pa), sta, sta32 = out[0]
-    (ub, pb), stb, stb32 = out[1]
+    (ua, pa), sta, sta32, (ua32, pa32), stair = out[0]
+    (ub, pb), stb, stb32, (ub32, pb32), stbir = out[1]
     assert sta["inner_its"] == stb["inner_its"] and sta32["inner_its"] == stb32["inner_its"]
+    assert stair["inner_its"] == stbir["inner_its"] and stair["refinements"] == stbir["refinements"]
+    assert np.array_equal(ua32, ub32) and np.array_equal(pa32, pb32)
     assert np.array_equal(ua, ub) and np.array_equal(pa, pb)
 
 
