@@ -561,8 +561,12 @@ static int pcg32(ust_context *ctx, int step, double rtol, int maxit, int *its) {
 
 /* FP32 correction solves to at most this relative reduction each (FP32 recurrences drift from the true residual
  * below it), and the FP64 residual is recomputed when the requested tolerance is tighter than IR_TRUST */
+#ifndef IR_TAU_MIN
 #define IR_TAU_MIN 1e-5
+#endif
+#ifndef IR_TRUST
 #define IR_TRUST 1e-4
+#endif
 #define IR_MAX_STEPS 12
 
 /* x = K^-1 b by iterative refinement: x accumulates in FP64 (vt), r = b - K x in FP64 (vr); each step solves
