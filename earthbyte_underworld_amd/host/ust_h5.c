@@ -66,6 +66,7 @@ static struct {
 } H5;
 
 static char h5_err[512];
+static char h5_load_err[256]; /* why the library did not load: reported again by every later call */
 
 static int h5_fail(int code, const char *fmt, ...) {
     va_list ap;
@@ -83,9 +84,17 @@ static void *sym(const char *name) {
     return p;
 }
 
+static int h5_open_library(void);
+
 static int h5_load(void) {
-    if (H5.tried) return H5.ok ? UST_OK : h5_fail(UST_EINVAL, "%s", h5_err);
+    if (H5.tried) return H5.ok ? UST_OK : h5_fail(UST_EINVAL, "%s", h5_load_err);
     H5.tried = 1;
+    int rc = h5_open_library();
+    if (rc) snprintf(h5_load_err, sizeof(h5_load_err), "%s", h5_err);
+    return rc;
+}
+
+static int h5_open_library(void) {
     const char *env = getenv("UST_HDF5_LIB");
     const char *names[] = {env, "libhdf5.so.103", "libhdf5.so", "/opt/conda/lib/libhdf5.so.103",
                            "/opt/conda/lib/libhdf5.so"};
