@@ -211,7 +211,7 @@ __device__ __forceinline__ bool on_tile_face(int gc, int T, int tn) {
 
 // What an apply writes for dof i whose product (K u)_i is v (v = 0 on Dirichlet dofs):
 //   EPI_Y       y_i = v
-//   EPI_RES     y_i = b_i - v                                  (multigrid residual)
+//   EPI_RES     y_i = b_i - v, 0 on Dirichlet dofs             (multigrid residual on the free dofs)
 //   EPI_CHEBY   r = b_i - v; d_i = c1 d_i + c2 invd_i r; x_i += d_i with x the input u: one Jacobi-
 //               Chebyshev smoothing step in place.  In place is safe: a node interior to a tile is
 //               staged only by that tile, before its update, and tile-face nodes are updated in
@@ -277,7 +277,7 @@ __device__ __forceinline__ void epilogue_store(const Epi<T> &ep, const EpiOps<T>
     if (MODE == EPI_Y) {
         ep.y[i] = v;
     } else if (MODE == EPI_RES) {
-        ep.y[i] = o.b - v;
+        ep.y[i] = o.m ? (T)0 : o.b - v;
     } else {
         const T dn = (MODE == EPI_CHEBY ? ep.c1 * o.d : (T)0) + ep.c2 * o.invd * (o.b - v);
         ep.d[i] = dn;
@@ -309,8 +309,28 @@ constexpr int EBS = UST_TILE + 1;  // row stride of the element-result buffer (r
 #ifndef UST_COL_PF_F64
 #define UST_COL_PF_F64 2
 #endif
+// The FP32 product and residual passes (EPI_Y, EPI_RES) take their own values: four workgroups per CU, loads
+// when used, on a grid of ~1024 workgroups (UST_COL_WG_TARGET_F32_YR).  Measured on the 128x128x64 passes:
+// 53.5 -> 49.5 us (product) and 55.2 -> 51.8 us (residual) against the settings of the other passes; the same
+// register budget on the 512-workgroup grid was slower (61 us), and the 1024-workgroup grid slows the
+// prolongation pass (71.5 -> 77.9 us) and the FP64 product (88.8 -> 96.1 us), which keep 512.
+#ifndef UST_COL_MINB_F32_YR
+#define UST_COL_MINB_F32_YR 4
+#endif
+#ifndef UST_COL_PF_F32_YR
+#define UST_COL_PF_F32_YR 0
+#endif
+__host__ __device__ constexpr int col_minb(int tsize, int mode) {
+    return tsize == 4 ? (mode == EPI_Y || mode == EPI_RES ? UST_COL_MINB_F32_YR : UST_COL_MINB_F32) : 2;
+}
+__host__ __device__ constexpr int col_pf(int tsize, int mode) {
+    return tsize == 4 ? (mode == EPI_Y || mode == EPI_RES ? UST_COL_PF_F32_YR : UST_COL_PF_F32) : UST_COL_PF_F64;
+}
+constexpr long long col_wg_target(int tsize, int mode) {
+    return tsize == 4 && (mode == EPI_Y || mode == EPI_RES) ? UST_COL_WG_TARGET_F32_YR : UST_COL_WG_TARGET;
+}
 template <typename T, typename WT, int MODE>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? UST_COL_MINB_F32 : 2) void k_applyK3_col(GridDev g, const WT *__restrict__ W,
+__global__ __launch_bounds__(256, col_minb(sizeof(T), MODE)) void k_applyK3_col(GridDev g, const WT *__restrict__ W,
                                                      const unsigned char *__restrict__ mask, const T *u, Epi<T> ep,
                                                      T *__restrict__ work) {
     __shared__ T su[2][CNPD];     // staged node planes k and k+1 (free dofs; Dirichlet dofs read 0)
@@ -403,7 +423,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? UST_COL_MINB_F32 : 2) void k_
     };
     // when the next layer's node plane and weights are loaded (UST_COL_PF_*): wpre at the start of this layer,
     // wlate after its element apply; held: in registers (xr, mr, wr) until the next layer uses them
-    constexpr int pf = sizeof(T) == 4 ? UST_COL_PF_F32 : UST_COL_PF_F64;
+    constexpr int pf = col_pf(sizeof(T), MODE);
     constexpr bool wpre = pf == 1, wlate = pf == 2, held = pf != 0;
     // ---- prologue: the coarse ring (planes z0/2 and z0/2 + 1), plane z0, and the loads for layer z0 ----
     if (MODE == EPI_PCHEBY0) {
@@ -863,7 +883,7 @@ static inline size_t work_len(const GridDev &g) {
 template <typename T, typename WT, int MODE>
 static int apply_tiled(const ust_grid *gr, const WT *W, const unsigned char *mask, const T *u, Epi<T> ep, T *work,
                        void *stream) {
-    GridDev g = grid_dev(gr);
+    GridDev g = grid_dev(gr, col_wg_target(sizeof(T), MODE));
     if (!W || !mask || !u || !ep.y || !work) return UST_EINVAL;
     if (MODE != EPI_Y && !ep.b) return UST_EINVAL;
     if (cheby_mode(MODE) && (!ep.invd || !ep.d || ep.y != u)) return UST_EINVAL;
@@ -920,7 +940,11 @@ static int apply_cheby_prolong(const ust_grid *g, const WT *W, const unsigned ch
     return apply_tiled<T, WT, EPI_PCHEBY0>(g, W, mask, x, ep, work, stream);
 }
 
-extern "C" size_t ust_k_apply_work_len(const ust_grid *gr) { return work_len(grid_dev(gr)); }
+// the largest workspace any pass needs (the passes' column grids differ in z-chunk length)
+extern "C" size_t ust_k_apply_work_len(const ust_grid *gr) {
+    const size_t a = work_len(grid_dev(gr, UST_COL_WG_TARGET)), b = work_len(grid_dev(gr, UST_COL_WG_TARGET_F32_YR));
+    return a > b ? a : b;
+}
 
 extern "C" int ust_k_apply_K_cheby_prolong_f64(const ust_grid *g, const double *WK, const unsigned char *mask,
                                                double *x, const double *b, const double *invd, double *d, double c2,
@@ -1046,7 +1070,7 @@ extern "C" size_t ust_k_apply_noalloc_max_work(void) { return UST_NOALLOC_WORK; 
 extern "C" int ust_k_apply_K_f64_noalloc(const ust_grid *g, const double *WK, const unsigned char *mask,
                                          const double *u, double *y, void *stream) {
     if (!g) return UST_EINVAL;
-    const size_t need = work_len(grid_dev(g));
+    const size_t need = work_len(grid_dev(g, col_wg_target(sizeof(double), EPI_Y)));
     if (need > UST_NOALLOC_WORK) {
         char msg[256];
         snprintf(msg, sizeof(msg),

@@ -57,9 +57,13 @@ __host__ __device__ constexpr int sgn(int a) { return 2 * a - 1; }
 #define UST_TZ3_MAX 16
 #define UST_COL_MIN 1024
 // workgroups a column launch aims at (the z-chunk length follows): two per CU; 1024 (four per CU, shorter
-// chunks) measured slower on the 128x128x64 solve (FP64 product 93 -> 100 us)
+// chunks) measured slower on the 128x128x64 solve (FP64 product 93 -> 100 us).  The FP32 product and residual
+// passes, which fit four workgroups per CU, aim at 1024 (UST_COL_WG_TARGET_F32_YR).
 #ifndef UST_COL_WG_TARGET
 #define UST_COL_WG_TARGET 512
+#endif
+#ifndef UST_COL_WG_TARGET_F32_YR
+#define UST_COL_WG_TARGET_F32_YR 1024
 #endif
 
 struct GridDev {
@@ -94,7 +98,8 @@ __host__ __device__ __forceinline__ size_t widx(const GridDev &g, int e, int q) 
     return ((size_t)t * 9 + q) * UST_TILE + l;
 }
 
-static inline GridDev grid_dev(const ust_grid *g) {
+// wg_target: the workgroups a column launch aims at (the z-chunk length follows)
+static inline GridDev grid_dev(const ust_grid *g, long long wg_target = UST_COL_WG_TARGET) {
     GridDev d;
     d.dim = g->dim;
     d.nx = g->nel[0];
@@ -115,8 +120,8 @@ static inline GridDev grid_dev(const ust_grid *g) {
             d.tnx = cnx;
             d.tny = cny;
             // z-chunk: long enough that few node planes sit on chunk boundaries, short enough that the launch
-            // has about 512 workgroups (two resident per CU on 256 CUs) or more
-            long long zc = layers / UST_COL_WG_TARGET;
+            // has about wg_target workgroups or more
+            long long zc = layers / wg_target;
             if (zc < 2) zc = 2;
             if (zc > UST_TZ3_MAX) zc = UST_TZ3_MAX;
             if (zc > d.nz) zc = d.nz;

@@ -88,7 +88,7 @@ static int lev_apply(ust_context *ctx, int l, int f32, const void *x, void *y) {
     return UST_OK;
 }
 
-/* r = b - A x on level l (fused product and residual) */
+/* r = b - A x on level l (fused product and residual), r = 0 on the Dirichlet dofs */
 static int lev_residual(ust_context *ctx, int l, int f32, const void *x, const void *b, void *r) {
     ust_level *L = &ctx->lev[l];
     if (f32 && L->WK16)
@@ -170,22 +170,23 @@ static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x, int 
      * separate prolongation: the fused FP64 pass needs more registers than two waves per SIMD allow) */
     int pfuse = L->g.dim == 3 && L->cheb_deg > 0 && f32;
     if (f32) {
+        /* the residual is zero on the Dirichlet dofs, so the restriction need not read the fine mask */
         if ((rc = lev_residual(ctx, l, 1, x, b, L->ax32))) return rc;
         if (fuse)
-            CKD(ctx, ust_k_restrict_cheby_first_f32(&L->g, L->mask, C->mask, L->ax32, C->b32, C->invd32, C->d32,
+            CKD(ctx, ust_k_restrict_cheby_first_f32(&L->g, NULL, C->mask, L->ax32, C->b32, C->invd32, C->d32,
                                                     C->x32, (float)C->c2[0], S(ctx)));
         else
-            CKD(ctx, ust_k_restrict_f32(&L->g, L->mask, C->mask, L->ax32, C->b32, S(ctx)));
+            CKD(ctx, ust_k_restrict_f32(&L->g, NULL, C->mask, L->ax32, C->b32, S(ctx)));
         if ((rc = vcycle(ctx, l + 1, 1, C->b32, C->x32, fuse))) return rc;
         if (pfuse) return lev_smooth(ctx, l, 1, b, x, 0, C->x32);
         CKD(ctx, ust_k_prolong_add_f32(&L->g, L->mask, C->x32, (float *)x, S(ctx)));
     } else {
         if ((rc = lev_residual(ctx, l, 0, x, b, L->ax64))) return rc;
         if (fuse)
-            CKD(ctx, ust_k_restrict_cheby_first_f64(&L->g, L->mask, C->mask, L->ax64, C->b64, C->invd64, C->d64,
+            CKD(ctx, ust_k_restrict_cheby_first_f64(&L->g, NULL, C->mask, L->ax64, C->b64, C->invd64, C->d64,
                                                     C->x64, C->c2[0], S(ctx)));
         else
-            CKD(ctx, ust_k_restrict_f64(&L->g, L->mask, C->mask, L->ax64, C->b64, S(ctx)));
+            CKD(ctx, ust_k_restrict_f64(&L->g, NULL, C->mask, L->ax64, C->b64, S(ctx)));
         if ((rc = vcycle(ctx, l + 1, 0, C->b64, C->x64, fuse))) return rc;
         if (pfuse) return lev_smooth(ctx, l, 0, b, x, 0, C->x64);
         CKD(ctx, ust_k_prolong_add_f64(&L->g, L->mask, C->x64, (double *)x, S(ctx)));

@@ -271,7 +271,8 @@ int ust_k_apply_K_f64(const ust_grid *g, const double *WK, const unsigned char *
 int ust_k_apply_K_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
                       float *y, float *work, void *stream);
 /* fused epilogues of the same product (one pass, no intermediate K u vector):
- * residual: r = b - K u (the multigrid residual before restriction, PETScMGSolver.c:641-716);
+ * residual: r = b - K u on the free dofs, r = 0 on the Dirichlet dofs (the multigrid residual before
+ * restriction, PETScMGSolver.c:641-716);
  * cheby: one Jacobi-Chebyshev smoothing step in place, r = b - K x, d = c1 d + c2 invd.*r, x += d
  * (first: c1 = 0 and d is not read). */
 int ust_k_apply_K_residual_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
@@ -342,6 +343,8 @@ int ust_k_inv_diag(const ust_grid *g, const double *WK, const unsigned char *mas
 
 /* --- multigrid transfers (fine grid gf, coarse has nel/2) --- */
 int ust_k_coarsen_weights(const ust_grid *gf, const double *WKf, double *WKc, void *stream);
+/* restriction bc = P^T rf over the free fine dofs, bc = 0 on the Dirichlet coarse dofs (maskc); maskf may be
+ * NULL when rf is already zero on the Dirichlet fine dofs (as the residual pass leaves it) */
 int ust_k_restrict_f64(const ust_grid *gf, const unsigned char *maskf, const unsigned char *maskc,
                        const double *rf, double *bc, void *stream);
 int ust_k_prolong_add_f64(const ust_grid *gf, const unsigned char *maskf, const double *xc, double *xf,

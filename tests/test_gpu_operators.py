@@ -138,6 +138,13 @@ def test_fused_residual_and_chebyshev_step(dim, nel):
     du, db, dr = _Dev(lib, u), _Dev(lib, b), _Dev(lib, np.zeros(n))
     assert lib.ust_k_apply_K_residual_f64(C.byref(g), WK, mask, du.p, db.p, dr.p, ws.p, None) == 0
     assert np.array_equal(dr.get(), b - y)
+    # the residual is zero on the Dirichlet dofs whatever b holds there
+    fixed = s.bc_mask.reshape(-1) != 0
+    bd = np.where(fixed, rng.normal(size=n), b)
+    dbd = _Dev(lib, bd)
+    assert lib.ust_k_apply_K_residual_f64(C.byref(g), WK, mask, du.p, dbd.p, dr.p, ws.p, None) == 0
+    assert np.array_equal(dr.get(), np.where(fixed, 0.0, bd - y))
+    dbd.free()
     c1, c2 = 0.37, 1.9
     for first in (0, 1):
         dx, dd, di = _Dev(lib, u), _Dev(lib, d0), _Dev(lib, invd)
@@ -295,6 +302,12 @@ def test_restriction_is_adjoint_of_prolongation(dim, nel):
         assert getattr(lib, f"ust_k_restrict_{prec}")(C.byref(g), mask, dcm.p, dr.p, db.p, None) == 0
         b = db.get().astype(np.float64)
         assert np.linalg.norm(b - ref) <= tol * np.linalg.norm(ref)
+        # no fine mask: the same sums from a residual that is already zero on the Dirichlet fine dofs
+        dr0, db0 = _Dev(lib, np.where(fm, 0.0, r).astype(dt)), _Dev(lib, np.zeros(cm.size, dt))
+        assert getattr(lib, f"ust_k_restrict_{prec}")(C.byref(g), None, dcm.p, dr0.p, db0.p, None) == 0
+        assert np.array_equal(db0.get(), db.get())
+        dr0.free()
+        db0.free()
         dxc, dxf = _Dev(lib, xc.astype(dt)), _Dev(lib, np.zeros(fm.size, dt))
         assert getattr(lib, f"ust_k_prolong_add_{prec}")(C.byref(g), mask, dxc.p, dxf.p, None) == 0
         Px = dxf.get().astype(np.float64)
