@@ -220,13 +220,16 @@ __device__ __forceinline__ bool on_tile_face(int gc, int T, int tn) {
 //   EPI_PCHEBY0 (3D) the multigrid prolongation x += P xc of the coarse-grid correction xc, then the
 //               EPI_CHEBY0 step on the corrected x, in one pass: the staged velocities are x + P xc,
 //               interpolated from the tile's coarse node block held in LDS.
-enum { EPI_Y = 0, EPI_RES = 1, EPI_CHEBY = 2, EPI_CHEBY0 = 3, EPI_PCHEBY0 = 4 };
+//   EPI_RES0    the first Jacobi-Chebyshev step from a zero guess, x = c2 invd b, then the residual
+//               r = b - K x (0 on Dirichlet dofs), in one pass: the staged velocities are c2 invd b (u is not
+//               read); y = r, d = x (the previous d is neither read nor written).
+enum { EPI_Y = 0, EPI_RES = 1, EPI_CHEBY = 2, EPI_CHEBY0 = 3, EPI_PCHEBY0 = 4, EPI_RES0 = 5 };
 template <typename T>
 struct Epi {
-    T *y;  // EPI_Y / EPI_RES: output; EPI_CHEBY*: x, the input u
+    T *y;  // EPI_Y / EPI_RES / EPI_RES0: output; EPI_CHEBY*: x, the input u
     const T *b;
     const T *invd;
-    T *d;
+    T *d;  // EPI_CHEBY*: the Chebyshev direction; EPI_RES0: x after the first step
     T c1, c2;
     const T *xc;  // EPI_PCHEBY0: coarse-grid correction (nodes of the grid with half the elements per axis)
 };
@@ -249,6 +252,13 @@ __device__ __forceinline__ T prolong_value(int i, int j, int k, F cval) {
     return wz * (wy * a0 + oy * a1) + oz * (wy * a2 + oy * a3);
 }
 
+// the velocity a pass stages for dof i: u, or for EPI_RES0 the first Chebyshev step from zero, c2 invd b
+template <int MODE, typename T>
+__device__ __forceinline__ T stage_load(const Epi<T> &ep, const T *u, unsigned i) {
+    if (MODE == EPI_RES0) return ep.c2 * ep.invd[i] * ep.b[i];
+    return u[i];
+}
+
 // operands the epilogue reads, loaded ahead of the stores so that a thread's loads overlap
 template <typename T>
 struct EpiOps {
@@ -262,6 +272,7 @@ __device__ __forceinline__ EpiOps<T> epilogue_load(const Epi<T> &ep, const unsig
     o.m = mask[i];
     o.b = o.invd = o.d = o.x = (T)0;
     if (MODE != EPI_Y) o.b = ep.b[i];
+    if (MODE == EPI_RES0) o.invd = ep.invd[i];
     if (cheby_mode(MODE)) {
         o.invd = ep.invd[i];
         o.x = ep.y[i];
@@ -276,8 +287,9 @@ __device__ __forceinline__ void epilogue_store(const Epi<T> &ep, const EpiOps<T>
     v = o.m ? (T)0 : v;
     if (MODE == EPI_Y) {
         ep.y[i] = v;
-    } else if (MODE == EPI_RES) {
+    } else if (MODE == EPI_RES || MODE == EPI_RES0) {
         ep.y[i] = o.m ? (T)0 : o.b - v;
+        if (MODE == EPI_RES0) ep.d[i] = ep.c2 * o.invd * o.b;  // as staged (kv_cheby_first's rounding)
     } else {
         const T dn = (MODE == EPI_CHEBY ? ep.c1 * o.d : (T)0) + ep.c2 * o.invd * (o.b - v);
         ep.d[i] = dn;
@@ -321,13 +333,16 @@ constexpr int EBS = UST_TILE + 1;  // row stride of the element-result buffer (r
 #define UST_COL_PF_F32_YR 0
 #endif
 __host__ __device__ constexpr int col_minb(int tsize, int mode) {
-    return tsize == 4 ? (mode == EPI_Y || mode == EPI_RES ? UST_COL_MINB_F32_YR : UST_COL_MINB_F32) : 2;
+    return tsize == 4 ? (mode == EPI_Y || mode == EPI_RES || mode == EPI_RES0 ? UST_COL_MINB_F32_YR : UST_COL_MINB_F32)
+                      : 2;
 }
 __host__ __device__ constexpr int col_pf(int tsize, int mode) {
-    return tsize == 4 ? (mode == EPI_Y || mode == EPI_RES ? UST_COL_PF_F32_YR : UST_COL_PF_F32) : UST_COL_PF_F64;
+    return tsize == 4 ? (mode == EPI_Y || mode == EPI_RES || mode == EPI_RES0 ? UST_COL_PF_F32_YR : UST_COL_PF_F32)
+                      : UST_COL_PF_F64;
 }
 constexpr long long col_wg_target(int tsize, int mode) {
-    return tsize == 4 && (mode == EPI_Y || mode == EPI_RES) ? UST_COL_WG_TARGET_F32_YR : UST_COL_WG_TARGET;
+    return tsize == 4 && (mode == EPI_Y || mode == EPI_RES || mode == EPI_RES0) ? UST_COL_WG_TARGET_F32_YR
+                                                                              : UST_COL_WG_TARGET;
 }
 template <typename T, typename WT, int MODE>
 __global__ __launch_bounds__(256, col_minb(sizeof(T), MODE)) void k_applyK3_col(GridDev g, const WT *__restrict__ W,
@@ -367,7 +382,7 @@ __global__ __launch_bounds__(256, col_minb(sizeof(T), MODE)) void k_applyK3_col(
         for (int r = 0; r < CNR; r++) {
             const unsigned dof = pofs[r] >= 0 ? (unsigned)pofs[r] + (unsigned)gz * pstride : 0u;
             m[r] = pofs[r] >= 0 ? mask[dof] : (unsigned char)1;
-            x[r] = u[dof];
+            x[r] = stage_load<MODE, T>(ep, u, dof);
         }
     };
     // coarse plane c under the column: thread t < NCP loads one value (0 outside the coarse grid)
@@ -600,7 +615,7 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_tiled(GridDev g, const WT *_
         stok[r] = t < NN * 3 && gx < g.nnx && gy < g.nny && gz < g.nnz;
         const unsigned dof = stok[r] ? ((unsigned)gx + (unsigned)g.nnx * ((unsigned)gy + (unsigned)g.nny * gz)) * 3u + c : 0u;
         stm[r] = mask[dof];
-        stv[r] = u[dof];
+        stv[r] = stage_load<MODE, T>(ep, u, dof);
     }
     // FP32: the weights are loaded here, ahead of the barrier; FP64 loads them after it, which keeps the
     // register count within two waves per SIMD
@@ -718,7 +733,7 @@ __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const WT *__re
         stok[r] = t < NN * 2 && gx < g.nnx && gy < g.nny;
         const unsigned dof = stok[r] ? ((unsigned)gx + (unsigned)g.nnx * gy) * 2u + c : 0u;
         stm[r] = mask[dof];
-        stv[r] = u[dof];
+        stv[r] = stage_load<MODE, T>(ep, u, dof);
     }
     T wq[9];
 #pragma unroll
@@ -887,6 +902,7 @@ static int apply_tiled(const ust_grid *gr, const WT *W, const unsigned char *mas
     if (!W || !mask || !u || !ep.y || !work) return UST_EINVAL;
     if (MODE != EPI_Y && !ep.b) return UST_EINVAL;
     if (cheby_mode(MODE) && (!ep.invd || !ep.d || ep.y != u)) return UST_EINVAL;
+    if (MODE == EPI_RES0 && (!ep.invd || !ep.d)) return UST_EINVAL;
     if (MODE == EPI_PCHEBY0 && (g.dim != 3 || !ep.xc || g.nx % 2 || g.ny % 2 || g.nz % 2)) return UST_EINVAL;
     if ((size_t)g.nnx * g.nny * g.nnz * g.dim >= (1ull << 32)) return UST_EINVAL;  // 32-bit dof indices
     hipStream_t st = ust_stream(stream);
@@ -941,6 +957,19 @@ static int apply_cheby_prolong(const ust_grid *g, const WT *W, const unsigned ch
 }
 
 // the largest workspace any pass needs (the passes' column grids differ in z-chunk length)
+template <typename T, typename WT>
+static int apply_residual_first(const ust_grid *g, const WT *W, const unsigned char *mask, const T *b, const T *invd,
+                                T *x, T *r, T c2, T *work, void *stream) {
+    Epi<T> ep = {r, b, invd, x, (T)0, c2, nullptr};
+    return apply_tiled<T, WT, EPI_RES0>(g, W, mask, b, ep, work, stream);  // u unused: b stands in
+}
+
+extern "C" int ust_k_apply_K_residual_first_f32(const ust_grid *g, const float *WK, const unsigned char *mask,
+                                                const float *b, const float *invd, float *x, float *r, float c2,
+                                                float *work, void *stream) {
+    return apply_residual_first<float, float>(g, WK, mask, b, invd, x, r, c2, work, stream);
+}
+
 extern "C" size_t ust_k_apply_work_len(const ust_grid *gr) {
     const size_t a = work_len(grid_dev(gr, UST_COL_WG_TARGET)), b = work_len(grid_dev(gr, UST_COL_WG_TARGET_F32_YR));
     return a > b ? a : b;

@@ -162,7 +162,9 @@ static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x, int 
         return lev_smooth(ctx, l, f32, b, x, first_done ? 2 : 1, NULL);
     }
     ust_level *C = &ctx->lev[l + 1];
-    if ((rc = lev_smooth(ctx, l, f32, b, x, first_done ? 2 : 1, NULL))) return rc;
+    /* FP32, one pre-smoothing step from zero: the step rides on the residual pass */
+    const int rfuse = f32 && !first_done && L->cheb_deg == 1 && L->WK32 && !L->WK16 && !ctx->b64_src;
+    if (!rfuse && (rc = lev_smooth(ctx, l, f32, b, x, first_done ? 2 : 1, NULL))) return rc;
     ctx->b64_src = NULL;
     /* the coarse level starts with a smoothing step from zero unless it is the dense coarse solve */
     int fuse = C->cheb_deg > 0 && !(l + 1 == ctx->nlev - 1 && ctx->coarse_dense);
@@ -171,7 +173,11 @@ static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x, int 
     int pfuse = L->g.dim == 3 && L->cheb_deg > 0 && f32;
     if (f32) {
         /* the residual is zero on the Dirichlet dofs, so the restriction need not read the fine mask */
-        if ((rc = lev_residual(ctx, l, 1, x, b, L->ax32))) return rc;
+        if (rfuse)
+            CKD(ctx, ust_k_apply_K_residual_first_f32(&L->g, L->WK32, L->mask, (const float *)b, L->invd32, (float *)x,
+                                                      L->ax32, (float)L->c2[0], (float *)ctx->kwork, S(ctx)));
+        else if ((rc = lev_residual(ctx, l, 1, x, b, L->ax32)))
+            return rc;
         if (fuse)
             CKD(ctx, ust_k_restrict_cheby_first_f32(&L->g, NULL, C->mask, L->ax32, C->b32, C->invd32, C->d32,
                                                     C->x32, (float)C->c2[0], S(ctx)));

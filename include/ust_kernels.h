@@ -279,6 +279,11 @@ int ust_k_apply_K_residual_f64(const ust_grid *g, const double *WK, const unsign
                                const double *b, double *r, double *work, void *stream);
 int ust_k_apply_K_residual_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
                                const float *b, float *r, float *work, void *stream);
+/* the first Jacobi-Chebyshev step from a zero guess, x = c2 invd.*b, and the residual r = b - K x (0 on the
+ * Dirichlet dofs) in one pass: a V-cycle level's pre-smoothing of degree 1 and its residual
+ * (PETScMGSolver.c:641-716); the same x and r as ust_k_cheby_first_f32 followed by the residual pass */
+int ust_k_apply_K_residual_first_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *b,
+                                     const float *invd, float *x, float *r, float c2, float *work, void *stream);
 int ust_k_apply_K_cheby_f64(const ust_grid *g, const double *WK, const unsigned char *mask, double *x,
                             const double *b, const double *invd, double *d, double c1, double c2, int first,
                             double *work, void *stream);

@@ -255,6 +255,39 @@ def test_prolongation_fused_chebyshev_step(prec):
     s.close()
 
 
+@pytest.mark.parametrize("nel", [(128, 128, 64), (64, 64, 64), (34, 10, 10), (40, 24)])
+def test_first_chebyshev_step_fused_into_residual(nel):
+    """x = c2 invd b (the first Chebyshev step from zero) and r = b - K x in one pass, against the step kernel
+    followed by the residual pass: x and r bit for bit (column grid, 16x4x4 tiles with partial tiles, 2D)."""
+    import ctypes as C
+    from earthbyte_underworld_amd import _lib as L
+    lib = L.load()
+    dim = len(nel)
+    g = _grid(dim, nel)
+    n = int(np.prod([k + 1 for k in nel])) * dim
+    wl = lib.ust_k_weights_len(C.byref(g))
+    rng = np.random.default_rng(23)
+    W64 = _Dev(lib, np.zeros(wl))
+    assert lib.ust_k_fill_hash_f64(W64.p, wl, 23, None) == 0
+    W = _Dev(lib, np.zeros(wl, np.float32))
+    assert lib.ust_k_copy_f64_to_f32(W64.p, W.p, wl, None) == 0
+    dm = _Dev(lib, (rng.random(n) < 0.05).astype(np.uint8))
+    ws = _Dev(lib, np.zeros(lib.ust_k_apply_work_len(C.byref(g)), np.float32))
+    b = rng.normal(size=n).astype(np.float32)  # nonzero on Dirichlet dofs too: x follows the step kernel there
+    invd = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    db, di = _Dev(lib, b), _Dev(lib, invd)
+    c2 = 1.3
+    xa, da, ra = (_Dev(lib, np.zeros(n, np.float32)) for _ in range(3))
+    assert lib.ust_k_cheby_first_f32(db.p, di.p, da.p, xa.p, c2, n, None) == 0
+    assert lib.ust_k_apply_K_residual_f32(C.byref(g), W.p, dm.p, xa.p, db.p, ra.p, ws.p, None) == 0
+    xb, rb = _Dev(lib, np.full(n, np.nan, np.float32)), _Dev(lib, np.full(n, np.nan, np.float32))
+    assert lib.ust_k_apply_K_residual_first_f32(C.byref(g), W.p, dm.p, db.p, di.p, xb.p, rb.p, c2, ws.p, None) == 0
+    assert np.array_equal(xa.get(), xb.get()) and np.array_equal(ra.get(), rb.get())
+    assert np.array_equal(xb.get(), (np.float32(c2) * invd) * b)
+    for a in (W64, W, dm, ws, db, di, xa, da, ra, xb, rb):
+        a.free()
+
+
 def _restrict_numpy(r, fmask, cmask, nnf, dim):
     """b_c = P^T r over the free fine dofs: fine nodes 2I-1..2I+1 per axis, weights 1/2 off the centre"""
     nnc = tuple((n - 1) // 2 + 1 for n in nnf)
