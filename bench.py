@@ -150,8 +150,9 @@ def cpu_baseline(nel_full, ppc, seed, outer_full, threads, sizes=(8, 12, 16, 24)
 
 def fine_level_kernels(lib, ctx, s, stream, reps=50):
     """The three operator passes every inner iteration of the headline solve makes on the finest grid, all with
-    FP32 K weights: the FP32 product q = K p of the velocity PCG (EPI_Y), the multigrid residual r = b - K x
-    (EPI_RES) and the prolongation + first post-smoothing step (EPI_PCHEBY0); and the FP64 product of the
+    FP32 K weights: the FP32 product q = K p of the velocity PCG (EPI_Y), the first pre-smoothing step from zero
+    with the multigrid residual, x = c2 D^-1 b and r = b - K x (EPI_RES0), and the prolongation + first
+    post-smoothing step (EPI_PCHEBY0); and the FP64 product of the
     refinement residuals (a few per inner solve).  Each is launched reps times through the kernel ABI on the
     library stream and timed with HIP events there; algorithmic HBM bytes per launch are the weights (27 per
     element at the operator's weight precision) plus the vectors each pass must read and write once.  Returns
@@ -182,10 +183,10 @@ def fine_level_kernels(lib, ctx, s, stream, reps=50):
         "k_applyK3_col<float,float,EPI_Y> + k_gather_tiles (FP32 PCG product)": (
             lambda: lib.ust_k_apply_K_f32(C.byref(g), B["W32"], mask, B["x"], B["r"], B["work"], stream),
             s.nel * 27 * 4 + n * (2 * 4 + 1), True),
-        "k_applyK3_col<float,float,EPI_RES> + k_gather_tiles (FP32 multigrid residual)": (
-            lambda: lib.ust_k_apply_K_residual_f32(C.byref(g), B["W32"], mask, B["x"], B["b"], B["r"], B["work"],
-                                                   stream),
-            s.nel * 27 * 4 + n * (3 * 4 + 1), True),
+        "k_applyK3_col<float,float,EPI_RES0> + k_gather_tiles (FP32 first Chebyshev step + multigrid residual)": (
+            lambda: lib.ust_k_apply_K_residual_first_f32(C.byref(g), B["W32"], mask, B["b"], B["invd"], B["x"],
+                                                         B["r"], C.c_float(0.5), B["work"], stream),
+            s.nel * 27 * 4 + n * (4 * 4 + 1), True),
         "k_applyK3_col<float,float,EPI_PCHEBY0> + k_gather_tiles (FP32 prolongation + Chebyshev step)": (
             lambda: lib.ust_k_apply_K_cheby_prolong_f32(C.byref(g), B["W32"], mask, B["x"], B["b"], B["invd"],
                                                         B["d"], C.c_float(0.5), B["xc"], B["work"], stream),
