@@ -202,6 +202,9 @@ _SIGS = {
     "ust_k_apply_K_cheby_prolong_f64": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, D, vp, vp, vp]),
     "ust_k_apply_K_cheby_prolong_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, C.c_float, vp, vp, vp]),
     "ust_k_apply_K_residual_first_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, C.c_float, vp, vp]),
+    "ust_k_apply_K_cheby_oop_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_int,
+                                        vp, vp]),
+    "ust_k_apply_K_cheby_prolong_oop_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp, C.c_float, vp, vp, vp]),
     "ust_k_cheby_first_f32": (I, [vp, vp, vp, vp, C.c_float, C.c_size_t, vp]),
     "ust_k_apply_K_f32_bf16w": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp]),
     "ust_k_apply_K_residual_f32_bf16w": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp]),

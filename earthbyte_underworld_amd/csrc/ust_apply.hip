@@ -226,12 +226,13 @@ __device__ __forceinline__ bool on_tile_face(int gc, int T, int tn) {
 enum { EPI_Y = 0, EPI_RES = 1, EPI_CHEBY = 2, EPI_CHEBY0 = 3, EPI_PCHEBY0 = 4, EPI_RES0 = 5 };
 template <typename T>
 struct Epi {
-    T *y;  // EPI_Y / EPI_RES / EPI_RES0: output; EPI_CHEBY*: x, the input u
+    T *y;  // EPI_Y / EPI_RES / EPI_RES0: output; EPI_CHEBY*: x after the step
     const T *b;
     const T *invd;
     T *d;  // EPI_CHEBY*: the Chebyshev direction; EPI_RES0: x after the first step
     T c1, c2;
     const T *xc;  // EPI_PCHEBY0: coarse-grid correction (nodes of the grid with half the elements per axis)
+    const T *xs;  // EPI_CHEBY*: x before the step, the input u (y itself when the step runs in place)
 };
 
 __host__ __device__ constexpr bool cheby_mode(int m) { return m == EPI_CHEBY || m == EPI_CHEBY0 || m == EPI_PCHEBY0; }
@@ -275,7 +276,7 @@ __device__ __forceinline__ EpiOps<T> epilogue_load(const Epi<T> &ep, const unsig
     if (MODE == EPI_RES0) o.invd = ep.invd[i];
     if (cheby_mode(MODE)) {
         o.invd = ep.invd[i];
-        o.x = ep.y[i];
+        o.x = ep.xs[i];
     }
     if (MODE == EPI_CHEBY) o.d = ep.d[i];
     return o;
@@ -711,6 +712,132 @@ __global__ __launch_bounds__(256, 2) void k_applyK3_tiled(GridDev g, const WT *_
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// 3D halo-tile kernel (FP32, the small coarse multigrid levels): a 256-thread workgroup owns an 8x4x4 block of
+// NODES and computes every element touching them -- its 8x4x4 elements and a one-element halo on the low side
+// of each axis, 9x5x5 slots, one per thread -- so every owned node's sum is complete inside the workgroup: no
+// workspace and no gather launch.  The halo elements are computed twice (by two workgroups), which costs less
+// on the latency-bound small grids than the gather pass it replaces.  Neighbouring workgroups stage the nodes
+// a workgroup writes, so the Chebyshev modes must run out of place (ep.xs != ep.y).  Each node sums its 8
+// element contributions in a fixed order: deterministic.
+// ------------------------------------------------------------------------------------------
+constexpr int HOX = 8, HOY = 4, HOZ = 4;                    // owned nodes per axis
+constexpr int HEX = HOX + 1, HEY = HOY + 1, HEZ = HOZ + 1;  // element slots per axis
+constexpr int HNX = HOX + 2, HNY = HOY + 2, HNZ = HOZ + 2;  // staged nodes per axis
+constexpr int HNE = HEX * HEY * HEZ, HNN = HNX * HNY * HNZ;  // 225 element slots, 360 staged nodes
+constexpr int HND = HOX * HOY * HOZ * 3;                    // 384 owned dofs
+static_assert(HNE <= UST_TILE && HND <= 2 * UST_TILE, "one element and at most two owned dofs per thread");
+
+// Grids up to this many elements take the halo-tile kernel; larger tile grids keep tiles + gather.  Measured on
+// the 128x128x64 solve (same box, graph-replayed): the halo passes on the 32x32x16 and 16x16x8 levels take it
+// from 0.431-0.434 s to 0.403-0.407 s; on the 64x64x32 level as well, 0.415-0.417 s (the duplicated halo
+// elements cost more there than the gather).  A 16x4x4-node block (two element slots per thread) was slower.
+#ifndef UST_HALO_MAX_NEL
+#define UST_HALO_MAX_NEL 65536
+#endif
+
+__host__ __device__ inline int halo_tiles(const GridDev &g) {
+    return ((g.nnx + HOX - 1) / HOX) * ((g.nny + HOY - 1) / HOY) * ((g.nnz + HOZ - 1) / HOZ);
+}
+
+template <typename T, typename WT, int MODE>
+__global__ __launch_bounds__(256) void k_applyK3_halo(GridDev g, const WT *__restrict__ W,
+                                                      const unsigned char *__restrict__ mask, const T *u, Epi<T> ep) {
+    __shared__ T su[HNN * 3];   // staged node block, origin (nx0 - 1, ny0 - 1, nz0 - 1)
+    __shared__ T eb[24 * HNE];  // element results: eb[(a*3 + i) * HNE + slot] (HNE odd: rows shift banks)
+    const int hnx = (g.nnx + HOX - 1) / HOX, hny = (g.nny + HOY - 1) / HOY;
+    const int tile = blockIdx.x, tid = threadIdx.x;
+    const int nx0 = (tile % hnx) * HOX, ny0 = ((tile / hnx) % hny) * HOY, nz0 = (tile / (hnx * hny)) * HOZ;
+    const int cnx = g.nx / 2 + 1, cny = g.ny / 2 + 1;  // EPI_PCHEBY0: the coarse grid
+    // staging: value t of the block is node n = t / 3, component c = t % 3; nodes outside the grid stage 0
+    constexpr int NST = (HNN * 3 + UST_TILE - 1) / UST_TILE;
+    T stv[NST];
+    unsigned char stm[NST];
+    bool stok[NST];
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = tid + UST_TILE * r, c = t % 3, n = t / 3;
+        const int gx = nx0 - 1 + n % HNX, gy = ny0 - 1 + (n / HNX) % HNY, gz = nz0 - 1 + n / (HNX * HNY);
+        stok[r] = t < HNN * 3 && gx >= 0 && gy >= 0 && gz >= 0 && gx < g.nnx && gy < g.nny && gz < g.nnz;
+        const unsigned dof = stok[r] ? ((unsigned)gx + (unsigned)g.nnx * ((unsigned)gy + (unsigned)g.nny * gz)) * 3u + c
+                                     : 0u;
+        stm[r] = mask[dof];
+        stv[r] = stage_load<MODE, T>(ep, u, dof);
+    }
+    // the thread's element slot and its weights, loaded while the block is staged
+    const int lx = tid % HEX, ly = (tid / HEX) % HEY, lz = tid / (HEX * HEY);
+    const int ex = nx0 - 1 + lx, ey = ny0 - 1 + ly, ez = nz0 - 1 + lz;
+    const bool valid = tid < HNE && ex >= 0 && ey >= 0 && ez >= 0 && ex < g.nx && ey < g.ny && ez < g.nz;
+    const int e = valid ? ex + g.nx * (ey + g.ny * ez) : 0;
+    T wq[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) wq[q] = wload<T, WT>(W, widx(g, e, q));
+    if (MODE == EPI_PCHEBY0) {  // x + P xc on the free dofs, the coarse values read from global memory
+#pragma unroll
+        for (int r = 0; r < NST; r++) {
+            const int t = tid + UST_TILE * r, c = t % 3, n = t / 3;
+            const int gx = nx0 - 1 + n % HNX, gy = ny0 - 1 + (n / HNX) % HNY, gz = nz0 - 1 + n / (HNX * HNY);
+            if (stok[r] && !stm[r])
+                stv[r] += prolong_value<T>(gx, gy, gz, [&](int I, int J, int K) {
+                    return ep.xc[((unsigned)I + (unsigned)cnx * ((unsigned)J + (unsigned)cny * K)) * 3u + c];
+                });
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NST; r++) {
+        const int t = tid + UST_TILE * r;
+        if (t < HNN * 3) su[t] = (stok[r] && !stm[r]) ? stv[r] : (T)0;
+    }
+    __syncthreads();
+    if (tid < HNE) {
+        T ye[8][3];
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) ye[a][i] = (T)0;
+        const T s[3] = {(T)(2.0 / g.hx), (T)(2.0 / g.hy), (T)(2.0 / g.hz)};
+        const T *sb = &su[(lx + HNX * (ly + HNY * lz)) * 3];
+        element_apply3<T>([&](int a, int i) -> T {
+            return sb[((a & 1) + HNX * (((a >> 1) & 1) + HNY * (a >> 2))) * 3 + i];
+        }, wq, s, ye);
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) eb[(a * 3 + i) * HNE + tid] = valid ? ye[a][i] : (T)0;
+    }
+    __syncthreads();
+    // the owned dofs d = tid, tid + 256 (< 384): node d / 3 of the block, component d % 3; each sums its 8
+    // elements in a fixed order, then the epilogue
+    EpiOps<T> ops[2];
+    unsigned dofs[2];
+    bool own[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int d = tid + UST_TILE * r, n = d / 3, c = d % 3;
+        const int gx = nx0 + n % HOX, gy = ny0 + (n / HOX) % HOY, gz = nz0 + n / (HOX * HOY);
+        own[r] = d < HND && gx < g.nnx && gy < g.nny && gz < g.nnz;
+        dofs[r] = own[r] ? ((unsigned)gx + (unsigned)g.nnx * ((unsigned)gy + (unsigned)g.nny * gz)) * 3u + c : 0u;
+        ops[r] = epilogue_load<MODE, T>(ep, mask, dofs[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        if (!own[r]) continue;
+        const int d = tid + UST_TILE * r, n = d / 3, c = d % 3;
+        const int ox = n % HOX, oy = (n / HOX) % HOY, oz = n / (HOX * HOY);
+        T v = (T)0;
+#pragma unroll
+        for (int dz = 1; dz >= 0; dz--)
+#pragma unroll
+            for (int dy = 1; dy >= 0; dy--)
+#pragma unroll
+                for (int dx = 1; dx >= 0; dx--)  // element slot (ox + 1 - dx, ...) has this node as vertex (dx, dy, dz)
+                    v += eb[((dx + 2 * dy + 4 * dz) * 3 + c) * HNE + (ox + 1 - dx) + HEX * ((oy + 1 - dy) + HEY * (oz + 1 - dz))];
+        EpiOps<T> o = ops[r];
+        if (MODE == EPI_PCHEBY0 && !o.m) o.x = su[((ox + 1) + HNX * ((oy + 1) + HNY * (oz + 1))) * 3 + c];  // as staged
+        epilogue_store<MODE, T>(ep, o, dofs[r], v);
+    }
+}
+
 template <typename T, typename WT, int MODE>
 __global__ __launch_bounds__(256) void k_applyK2_tiled(GridDev g, const WT *__restrict__ W,
                                                        const unsigned char *__restrict__ mask, const T *u,
@@ -901,12 +1028,19 @@ static int apply_tiled(const ust_grid *gr, const WT *W, const unsigned char *mas
     GridDev g = grid_dev(gr, col_wg_target(sizeof(T), MODE));
     if (!W || !mask || !u || !ep.y || !work) return UST_EINVAL;
     if (MODE != EPI_Y && !ep.b) return UST_EINVAL;
-    if (cheby_mode(MODE) && (!ep.invd || !ep.d || ep.y != u)) return UST_EINVAL;
+    if (cheby_mode(MODE) && (!ep.invd || !ep.d || ep.xs != u)) return UST_EINVAL;
     if (MODE == EPI_RES0 && (!ep.invd || !ep.d)) return UST_EINVAL;
     if (MODE == EPI_PCHEBY0 && (g.dim != 3 || !ep.xc || g.nx % 2 || g.ny % 2 || g.nz % 2)) return UST_EINVAL;
     if ((size_t)g.nnx * g.nny * g.nnz * g.dim >= (1ull << 32)) return UST_EINVAL;  // 32-bit dof indices
     hipStream_t st = ust_stream(stream);
-    if (g.dim == 3) {
+    // FP32 on a small tile grid: the halo-tile kernel, unless a Chebyshev step runs in place
+    const bool halo = sizeof(T) == 4 && g.dim == 3 && !g.col && g.nel <= UST_HALO_MAX_NEL &&
+                      (!cheby_mode(MODE) || ep.y != ep.xs);
+    if (halo) {
+        if constexpr (sizeof(T) == 4)
+            hipLaunchKernelGGL((k_applyK3_halo<T, WT, MODE>), dim3(halo_tiles(g)), dim3(UST_TILE), 0, st, g, W, mask,
+                               u, ep);
+    } else if (g.dim == 3) {
         if (g.col)
             hipLaunchKernelGGL((k_applyK3_col<T, WT, MODE>), dim3(g.ntiles), dim3(UST_TILE), 0, st, g, W, mask, u, ep,
                                work);
@@ -930,37 +1064,37 @@ static int apply_tiled(const ust_grid *gr, const WT *W, const unsigned char *mas
 template <typename T, typename WT>
 static int apply_plain(const ust_grid *g, const WT *W, const unsigned char *mask, const T *u, T *y, T *work,
                        void *stream) {
-    Epi<T> ep = {y, nullptr, nullptr, nullptr, (T)0, (T)0, nullptr};
+    Epi<T> ep = {y, nullptr, nullptr, nullptr, (T)0, (T)0, nullptr, nullptr};
     return apply_tiled<T, WT, EPI_Y>(g, W, mask, u, ep, work, stream);
 }
 
 template <typename T, typename WT>
 static int apply_residual(const ust_grid *g, const WT *W, const unsigned char *mask, const T *u, const T *b, T *r,
                           T *work, void *stream) {
-    Epi<T> ep = {r, b, nullptr, nullptr, (T)0, (T)0, nullptr};
+    Epi<T> ep = {r, b, nullptr, nullptr, (T)0, (T)0, nullptr, nullptr};
     return apply_tiled<T, WT, EPI_RES>(g, W, mask, u, ep, work, stream);
 }
 
+// xi: x before the step; x: after it (the same vector when the step runs in place)
 template <typename T, typename WT>
-static int apply_cheby(const ust_grid *g, const WT *W, const unsigned char *mask, T *x, const T *b, const T *invd,
-                       T *d, T c1, T c2, int first, T *work, void *stream) {
-    Epi<T> ep = {x, b, invd, d, c1, c2, nullptr};
-    return first ? apply_tiled<T, WT, EPI_CHEBY0>(g, W, mask, x, ep, work, stream)
-                 : apply_tiled<T, WT, EPI_CHEBY>(g, W, mask, x, ep, work, stream);
+static int apply_cheby(const ust_grid *g, const WT *W, const unsigned char *mask, const T *xi, T *x, const T *b,
+                       const T *invd, T *d, T c1, T c2, int first, T *work, void *stream) {
+    Epi<T> ep = {x, b, invd, d, c1, c2, nullptr, xi};
+    return first ? apply_tiled<T, WT, EPI_CHEBY0>(g, W, mask, xi, ep, work, stream)
+                 : apply_tiled<T, WT, EPI_CHEBY>(g, W, mask, xi, ep, work, stream);
 }
 
 template <typename T, typename WT>
-static int apply_cheby_prolong(const ust_grid *g, const WT *W, const unsigned char *mask, T *x, const T *b,
-                               const T *invd, T *d, T c2, const T *xc, T *work, void *stream) {
-    Epi<T> ep = {x, b, invd, d, (T)0, c2, xc};
-    return apply_tiled<T, WT, EPI_PCHEBY0>(g, W, mask, x, ep, work, stream);
+static int apply_cheby_prolong(const ust_grid *g, const WT *W, const unsigned char *mask, const T *xi, T *x,
+                               const T *b, const T *invd, T *d, T c2, const T *xc, T *work, void *stream) {
+    Epi<T> ep = {x, b, invd, d, (T)0, c2, xc, xi};
+    return apply_tiled<T, WT, EPI_PCHEBY0>(g, W, mask, xi, ep, work, stream);
 }
 
-// the largest workspace any pass needs (the passes' column grids differ in z-chunk length)
 template <typename T, typename WT>
 static int apply_residual_first(const ust_grid *g, const WT *W, const unsigned char *mask, const T *b, const T *invd,
                                 T *x, T *r, T c2, T *work, void *stream) {
-    Epi<T> ep = {r, b, invd, x, (T)0, c2, nullptr};
+    Epi<T> ep = {r, b, invd, x, (T)0, c2, nullptr, nullptr};
     return apply_tiled<T, WT, EPI_RES0>(g, W, mask, b, ep, work, stream);  // u unused: b stands in
 }
 
@@ -970,6 +1104,7 @@ extern "C" int ust_k_apply_K_residual_first_f32(const ust_grid *g, const float *
     return apply_residual_first<float, float>(g, WK, mask, b, invd, x, r, c2, work, stream);
 }
 
+// the largest workspace any pass needs (the passes' column grids differ in z-chunk length)
 extern "C" size_t ust_k_apply_work_len(const ust_grid *gr) {
     const size_t a = work_len(grid_dev(gr, UST_COL_WG_TARGET)), b = work_len(grid_dev(gr, UST_COL_WG_TARGET_F32_YR));
     return a > b ? a : b;
@@ -978,13 +1113,29 @@ extern "C" size_t ust_k_apply_work_len(const ust_grid *gr) {
 extern "C" int ust_k_apply_K_cheby_prolong_f64(const ust_grid *g, const double *WK, const unsigned char *mask,
                                                double *x, const double *b, const double *invd, double *d, double c2,
                                                const double *xc, double *work, void *stream) {
-    return apply_cheby_prolong<double, double>(g, WK, mask, x, b, invd, d, c2, xc, work, stream);
+    return apply_cheby_prolong<double, double>(g, WK, mask, x, x, b, invd, d, c2, xc, work, stream);
 }
 
 extern "C" int ust_k_apply_K_cheby_prolong_f32(const ust_grid *g, const float *WK, const unsigned char *mask,
                                                float *x, const float *b, const float *invd, float *d, float c2,
                                                const float *xc, float *work, void *stream) {
-    return apply_cheby_prolong<float, float>(g, WK, mask, x, b, invd, d, c2, xc, work, stream);
+    return apply_cheby_prolong<float, float>(g, WK, mask, x, x, b, invd, d, c2, xc, work, stream);
+}
+
+// out of place: x before the step, xo after it (distinct vectors); on FP32 tile grids this runs the halo-tile
+// kernel (one pass, no gather)
+extern "C" int ust_k_apply_K_cheby_oop_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *x,
+                                           float *xo, const float *b, const float *invd, float *d, float c1, float c2,
+                                           int first, float *work, void *stream) {
+    if (!x || x == xo) return UST_EINVAL;
+    return apply_cheby<float, float>(g, WK, mask, x, xo, b, invd, d, c1, c2, first, work, stream);
+}
+
+extern "C" int ust_k_apply_K_cheby_prolong_oop_f32(const ust_grid *g, const float *WK, const unsigned char *mask,
+                                                   const float *x, float *xo, const float *b, const float *invd,
+                                                   float *d, float c2, const float *xc, float *work, void *stream) {
+    if (!x || x == xo) return UST_EINVAL;
+    return apply_cheby_prolong<float, float>(g, WK, mask, x, xo, b, invd, d, c2, xc, work, stream);
 }
 
 extern "C" int ust_k_apply_K_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
@@ -1010,13 +1161,13 @@ extern "C" int ust_k_apply_K_residual_f32(const ust_grid *g, const float *WK, co
 extern "C" int ust_k_apply_K_cheby_f64(const ust_grid *g, const double *WK, const unsigned char *mask, double *x,
                                        const double *b, const double *invd, double *d, double c1, double c2,
                                        int first, double *work, void *stream) {
-    return apply_cheby<double, double>(g, WK, mask, x, b, invd, d, c1, c2, first, work, stream);
+    return apply_cheby<double, double>(g, WK, mask, x, x, b, invd, d, c1, c2, first, work, stream);
 }
 
 extern "C" int ust_k_apply_K_cheby_f32(const ust_grid *g, const float *WK, const unsigned char *mask, float *x,
                                        const float *b, const float *invd, float *d, float c1, float c2, int first,
                                        float *work, void *stream) {
-    return apply_cheby<float, float>(g, WK, mask, x, b, invd, d, c1, c2, first, work, stream);
+    return apply_cheby<float, float>(g, WK, mask, x, x, b, invd, d, c1, c2, first, work, stream);
 }
 
 // FP32 operator with bfloat16 K weights (the multigrid V-cycle's operator; half the weight stream)
@@ -1034,14 +1185,14 @@ extern "C" int ust_k_apply_K_residual_f32_bf16w(const ust_grid *g, const unsigne
 extern "C" int ust_k_apply_K_cheby_f32_bf16w(const ust_grid *g, const unsigned short *WK, const unsigned char *mask,
                                              float *x, const float *b, const float *invd, float *d, float c1, float c2,
                                              int first, float *work, void *stream) {
-    return apply_cheby<float, unsigned short>(g, WK, mask, x, b, invd, d, c1, c2, first, work, stream);
+    return apply_cheby<float, unsigned short>(g, WK, mask, x, x, b, invd, d, c1, c2, first, work, stream);
 }
 
 extern "C" int ust_k_apply_K_cheby_prolong_f32_bf16w(const ust_grid *g, const unsigned short *WK,
                                                      const unsigned char *mask, float *x, const float *b,
                                                      const float *invd, float *d, float c2, const float *xc,
                                                      float *work, void *stream) {
-    return apply_cheby_prolong<float, unsigned short>(g, WK, mask, x, b, invd, d, c2, xc, work, stream);
+    return apply_cheby_prolong<float, unsigned short>(g, WK, mask, x, x, b, invd, d, c2, xc, work, stream);
 }
 
 extern "C" size_t ust_k_weights_len(const ust_grid *gr) {

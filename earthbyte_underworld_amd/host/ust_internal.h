@@ -30,6 +30,7 @@ typedef struct {
     /* work vectors */
     double *x64, *b64, *ax64, *d64;
     float *x32, *b32, *ax32, *d32;
+    float *x32b;           /* coarse 3D levels: the second x of the out-of-place FP32 smoothing passes */
 } ust_level;
 
 typedef struct {

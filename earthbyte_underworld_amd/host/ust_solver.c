@@ -67,7 +67,7 @@ void ust_levels_free(ust_context *ctx) {
         DFREE(L->invd64);
         DFREE(L->invd32);
         DFREE(L->x64); DFREE(L->b64); DFREE(L->ax64); DFREE(L->d64);
-        DFREE(L->x32); DFREE(L->b32); DFREE(L->ax32); DFREE(L->d32);
+        DFREE(L->x32); DFREE(L->x32b); DFREE(L->b32); DFREE(L->ax32); DFREE(L->d32);
         memset(L, 0, sizeof(*L));
     }
     DFREE(ctx->Ainv);
@@ -105,13 +105,22 @@ static int lev_residual(ust_context *ctx, int l, int f32, const void *x, const v
 
 /* Jacobi-Chebyshev smoothing of A x = b (PETScMGSolver.c:674-709 smoother role).  x_zero 1: x is not read
  * (taken as zero); the first step is then d = c2 D^-1 b, x = d, without a product.  x_zero 2: that first
- * step was already applied (by the restriction).  Every other step is one fused product + update pass. */
-static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void *x, int x_zero, const void *xc) {
+ * step was already applied (by the restriction).  Every other step is one fused product + update pass.
+ * *xp holds x; with alt (the level's second FP32 vector) the product passes run out of place, each writing
+ * the other vector, and *xp / *alt are swapped after each (the FP32 tile-grid kernel needs it). */
+static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void **xp, void **alt, int x_zero,
+                      const void *xc) {
     ust_level *L = &ctx->lev[l];
     size_t n = (size_t)L->ndof;
+    const int oop = f32 && alt && *alt && L->WK32 && !L->WK16;
     for (int k = x_zero == 2 ? 1 : 0; k < L->cheb_deg; k++) {
+        void *x = *xp;
         if (k == 0 && xc) { /* prolongation of the coarse correction fused into the first step */
-            if (f32 && L->WK16)
+            if (oop)
+                CKD(ctx, ust_k_apply_K_cheby_prolong_oop_f32(&L->g, L->WK32, L->mask, (const float *)x, (float *)*alt,
+                                                             (const float *)b, L->invd32, L->d32, (float)L->c2[0],
+                                                             (const float *)xc, (float *)ctx->kwork, S(ctx)));
+            else if (f32 && L->WK16)
                 CKD(ctx, ust_k_apply_K_cheby_prolong_f32_bf16w(&L->g, L->WK16, L->mask, (float *)x, (const float *)b,
                                                                L->invd32, L->d32, (float)L->c2[0], (const float *)xc,
                                                                (float *)ctx->kwork, S(ctx)));
@@ -133,6 +142,11 @@ static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void *x, 
             else
                 CKD(ctx, ust_k_cheby_first_f64((const double *)b, L->invd64, L->d64, (double *)x, L->c2[0], n,
                                                S(ctx)));
+            continue; /* elementwise: in place whatever the level */
+        } else if (oop) {
+            CKD(ctx, ust_k_apply_K_cheby_oop_f32(&L->g, L->WK32, L->mask, (const float *)x, (float *)*alt,
+                                                 (const float *)b, L->invd32, L->d32, (float)L->c1[k], (float)L->c2[k],
+                                                 k == 0, (float *)ctx->kwork, S(ctx)));
         } else if (f32 && L->WK16) {
             CKD(ctx, ust_k_apply_K_cheby_f32_bf16w(&L->g, L->WK16, L->mask, (float *)x, (const float *)b, L->invd32,
                                                    L->d32, (float)L->c1[k], (float)L->c2[k], k == 0,
@@ -144,60 +158,85 @@ static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void *x, 
             CKD(ctx, ust_k_apply_K_cheby_f64(&L->g, L->WK64, L->mask, (double *)x, (const double *)b, L->invd64,
                                              L->d64, L->c1[k], L->c2[k], k == 0, (double *)ctx->kwork, S(ctx)));
         }
+        if (oop) { /* the step wrote the other vector */
+            *xp = *alt;
+            *alt = x;
+        }
     }
     return UST_OK;
 }
 
 /* one V-cycle: x = M^-1 b on level l (x is overwritten, not read).  first_done: the first smoothing step
- * from the zero guess was already applied to x (fused into the restriction that produced b). */
-static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x, int first_done) {
+ * from the zero guess was already applied to x (fused into the restriction that produced b).  *res: the
+ * vector that holds the result -- x, or on an FP32 coarse level whose passes run out of place, possibly the
+ * level's second vector x32b. */
+static int vcycle_r(ust_context *ctx, int l, int f32, const void *b, void *x, int first_done, void **res) {
     ust_level *L = &ctx->lev[l];
+    void *cur = x, *alt = (l > 0 && f32) ? (void *)L->x32b : NULL;
     int rc;
+    *res = x;
     if (l == ctx->nlev - 1) {
         if (ctx->coarse_dense) {
             if (f32) CKD(ctx, ust_k_dense_matvec_f32(ctx->Ainv, ctx->nc, (const float *)b, (float *)x, S(ctx)));
             else CKD(ctx, ust_k_dense_matvec_f64(ctx->Ainv, ctx->nc, (const double *)b, (double *)x, S(ctx)));
             return UST_OK;
         }
-        return lev_smooth(ctx, l, f32, b, x, first_done ? 2 : 1, NULL);
+        rc = lev_smooth(ctx, l, f32, b, &cur, &alt, first_done ? 2 : 1, NULL);
+        *res = cur;
+        return rc;
     }
     ust_level *C = &ctx->lev[l + 1];
     /* FP32, one pre-smoothing step from zero: the step rides on the residual pass */
     const int rfuse = f32 && !first_done && L->cheb_deg == 1 && L->WK32 && !L->WK16 && !ctx->b64_src;
-    if (!rfuse && (rc = lev_smooth(ctx, l, f32, b, x, first_done ? 2 : 1, NULL))) return rc;
+    if (!rfuse && (rc = lev_smooth(ctx, l, f32, b, &cur, &alt, first_done ? 2 : 1, NULL))) return rc;
     ctx->b64_src = NULL;
     /* the coarse level starts with a smoothing step from zero unless it is the dense coarse solve */
     int fuse = C->cheb_deg > 0 && !(l + 1 == ctx->nlev - 1 && ctx->coarse_dense);
     /* 3D FP32: the prolongation of the correction rides on the first post-smoothing pass (FP64 keeps the
      * separate prolongation: the fused FP64 pass needs more registers than two waves per SIMD allow) */
     int pfuse = L->g.dim == 3 && L->cheb_deg > 0 && f32;
+    void *xcr = NULL; /* the coarse level's result */
     if (f32) {
         /* the residual is zero on the Dirichlet dofs, so the restriction need not read the fine mask */
         if (rfuse)
-            CKD(ctx, ust_k_apply_K_residual_first_f32(&L->g, L->WK32, L->mask, (const float *)b, L->invd32, (float *)x,
-                                                      L->ax32, (float)L->c2[0], (float *)ctx->kwork, S(ctx)));
-        else if ((rc = lev_residual(ctx, l, 1, x, b, L->ax32)))
+            CKD(ctx, ust_k_apply_K_residual_first_f32(&L->g, L->WK32, L->mask, (const float *)b, L->invd32,
+                                                      (float *)cur, L->ax32, (float)L->c2[0], (float *)ctx->kwork,
+                                                      S(ctx)));
+        else if ((rc = lev_residual(ctx, l, 1, cur, b, L->ax32)))
             return rc;
         if (fuse)
             CKD(ctx, ust_k_restrict_cheby_first_f32(&L->g, NULL, C->mask, L->ax32, C->b32, C->invd32, C->d32,
                                                     C->x32, (float)C->c2[0], S(ctx)));
         else
             CKD(ctx, ust_k_restrict_f32(&L->g, NULL, C->mask, L->ax32, C->b32, S(ctx)));
-        if ((rc = vcycle(ctx, l + 1, 1, C->b32, C->x32, fuse))) return rc;
-        if (pfuse) return lev_smooth(ctx, l, 1, b, x, 0, C->x32);
-        CKD(ctx, ust_k_prolong_add_f32(&L->g, L->mask, C->x32, (float *)x, S(ctx)));
+        if ((rc = vcycle_r(ctx, l + 1, 1, C->b32, C->x32, fuse, &xcr))) return rc;
+        if (pfuse) {
+            rc = lev_smooth(ctx, l, 1, b, &cur, &alt, 0, xcr);
+            *res = cur;
+            return rc;
+        }
+        CKD(ctx, ust_k_prolong_add_f32(&L->g, L->mask, (const float *)xcr, (float *)cur, S(ctx)));
     } else {
-        if ((rc = lev_residual(ctx, l, 0, x, b, L->ax64))) return rc;
+        if ((rc = lev_residual(ctx, l, 0, cur, b, L->ax64))) return rc;
         if (fuse)
             CKD(ctx, ust_k_restrict_cheby_first_f64(&L->g, NULL, C->mask, L->ax64, C->b64, C->invd64, C->d64,
                                                     C->x64, C->c2[0], S(ctx)));
         else
             CKD(ctx, ust_k_restrict_f64(&L->g, NULL, C->mask, L->ax64, C->b64, S(ctx)));
-        if ((rc = vcycle(ctx, l + 1, 0, C->b64, C->x64, fuse))) return rc;
-        if (pfuse) return lev_smooth(ctx, l, 0, b, x, 0, C->x64);
-        CKD(ctx, ust_k_prolong_add_f64(&L->g, L->mask, C->x64, (double *)x, S(ctx)));
+        if ((rc = vcycle_r(ctx, l + 1, 0, C->b64, C->x64, fuse, &xcr))) return rc;
+        CKD(ctx, ust_k_prolong_add_f64(&L->g, L->mask, (const double *)xcr, (double *)cur, S(ctx)));
     }
-    return lev_smooth(ctx, l, f32, b, x, 0, NULL);
+    rc = lev_smooth(ctx, l, f32, b, &cur, &alt, 0, NULL);
+    *res = cur;
+    return rc;
+}
+
+/* the V-cycle on the finest level, whose passes run in place: the result is in x */
+static int vcycle(ust_context *ctx, int l, int f32, const void *b, void *x, int first_done) {
+    void *res = NULL;
+    int rc = vcycle_r(ctx, l, f32, b, x, first_done, &res);
+    if (rc == UST_OK && res != x) UST_FAIL(ctx, UST_EINVAL, "V-cycle result left in the level's second vector");
+    return rc;
 }
 
 /* lambda_max of D^-1 A by power iteration (FP64) */
@@ -306,6 +345,8 @@ static int levels_structure(ust_context *ctx, const ust_solve_opts *o, int need3
             if (o->vcycle_bf16) DALLOC(ctx, L->WK16, ust_k_weights_len(&L->g) * sizeof(unsigned short));
             else DALLOC(ctx, L->WK32, ust_k_weights_len(&L->g) * sizeof(float));
             DALLOC(ctx, L->x32, n * sizeof(float));
+            /* coarse 3D levels smooth out of place with FP32 weights (the tile-grid kernel needs it) */
+            if (l > 0 && dim == 3 && !o->vcycle_bf16) DALLOC(ctx, L->x32b, n * sizeof(float));
             DALLOC(ctx, L->b32, n * sizeof(float));
             DALLOC(ctx, L->ax32, n * sizeof(float));
             DALLOC(ctx, L->d32, n * sizeof(float));

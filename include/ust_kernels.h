@@ -279,6 +279,15 @@ int ust_k_apply_K_residual_f64(const ust_grid *g, const double *WK, const unsign
                                const double *b, double *r, double *work, void *stream);
 int ust_k_apply_K_residual_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
                                const float *b, float *r, float *work, void *stream);
+/* the same Chebyshev passes out of place: the step reads x and writes xo (distinct vectors; d is read and
+ * written per dof).  On small 3D grids (at most 65,536 elements: the coarser multigrid levels) the FP32 pass is
+ * one halo-tile launch without a workspace gather (work is then not used). */
+int ust_k_apply_K_cheby_oop_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *x,
+                                float *xo, const float *b, const float *invd, float *d, float c1, float c2, int first,
+                                float *work, void *stream);
+int ust_k_apply_K_cheby_prolong_oop_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *x,
+                                        float *xo, const float *b, const float *invd, float *d, float c2,
+                                        const float *xc, float *work, void *stream);
 /* the first Jacobi-Chebyshev step from a zero guess, x = c2 invd.*b, and the residual r = b - K x (0 on the
  * Dirichlet dofs) in one pass: a V-cycle level's pre-smoothing of degree 1 and its residual
  * (PETScMGSolver.c:641-716); the same x and r as ust_k_cheby_first_f32 followed by the residual pass */

@@ -288,6 +288,65 @@ def test_first_chebyshev_step_fused_into_residual(nel):
         a.free()
 
 
+@pytest.mark.parametrize("nel", [(64, 64, 32), (32, 32, 16), (16, 16, 8), (34, 10, 10), (64, 64, 64)])
+def test_out_of_place_chebyshev_passes(nel):
+    """The out-of-place Chebyshev passes (on tile grids the FP32 halo-tile kernel: every node summed in the
+    workgroup that owns it, no gather) against the in-place passes (tiles + gather, or the column kernel on the
+    64^3 grid): the same step to FP32 rounding, the input vector untouched, and bit for bit the same result on
+    every repeat."""
+    import ctypes as C
+    from earthbyte_underworld_amd import _lib as L
+    lib = L.load()
+    g = _grid(3, nel)
+    n = int(np.prod([k + 1 for k in nel])) * 3
+    cn = int(np.prod([k // 2 + 1 for k in nel])) * 3
+    wl = lib.ust_k_weights_len(C.byref(g))
+    rng = np.random.default_rng(31)
+    W64 = _Dev(lib, np.zeros(wl))
+    assert lib.ust_k_fill_hash_f64(W64.p, wl, 31, None) == 0
+    W = _Dev(lib, np.zeros(wl, np.float32))
+    assert lib.ust_k_copy_f64_to_f32(W64.p, W.p, wl, None) == 0
+    dm = _Dev(lib, (rng.random(n) < 0.05).astype(np.uint8))
+    ws = _Dev(lib, np.zeros(lib.ust_k_apply_work_len(C.byref(g)), np.float32))
+    x0, b = rng.normal(size=n).astype(np.float32), rng.normal(size=n).astype(np.float32)
+    invd, d0 = rng.uniform(0.5, 2.0, n).astype(np.float32), rng.normal(size=n).astype(np.float32)
+    xc = rng.normal(size=cn).astype(np.float32)
+    db, di, dxc = _Dev(lib, b), _Dev(lib, invd), _Dev(lib, xc)
+    c1, c2 = 0.4, 1.3
+    for kind in ("cheby", "cheby0", "prolong"):
+        xa, da = _Dev(lib, x0), _Dev(lib, d0)
+        if kind == "prolong":
+            assert lib.ust_k_apply_K_cheby_prolong_f32(C.byref(g), W.p, dm.p, xa.p, db.p, di.p, da.p, c2, dxc.p,
+                                                       ws.p, None) == 0
+        else:
+            assert lib.ust_k_apply_K_cheby_f32(C.byref(g), W.p, dm.p, xa.p, db.p, di.p, da.p, c1, c2,
+                                               int(kind == "cheby0"), ws.p, None) == 0
+        outs = []
+        for rep in range(2):
+            xi, xo, dd = _Dev(lib, x0), _Dev(lib, np.full(n, np.nan, np.float32)), _Dev(lib, d0)
+            if kind == "prolong":
+                assert lib.ust_k_apply_K_cheby_prolong_oop_f32(C.byref(g), W.p, dm.p, xi.p, xo.p, db.p, di.p, dd.p,
+                                                               c2, dxc.p, ws.p, None) == 0
+            else:
+                assert lib.ust_k_apply_K_cheby_oop_f32(C.byref(g), W.p, dm.p, xi.p, xo.p, db.p, di.p, dd.p, c1, c2,
+                                                       int(kind == "cheby0"), ws.p, None) == 0
+            assert np.array_equal(xi.get(), x0)
+            outs.append((xo.get(), dd.get()))
+            for a in (xi, xo, dd):
+                a.free()
+        assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+        xr, dr = xa.get().astype(np.float64), da.get().astype(np.float64)
+        assert rel_err(outs[0][0].astype(np.float64), xr) < 1e-5, kind
+        assert rel_err(outs[0][1].astype(np.float64), dr) < 1e-5, kind
+        xa.free()
+        da.free()
+    # the same pointer for x and xo is refused
+    assert lib.ust_k_apply_K_cheby_oop_f32(C.byref(g), W.p, dm.p, db.p, db.p, db.p, di.p, di.p, c1, c2, 0, ws.p,
+                                           None) == -1
+    for a in (W64, W, dm, ws, db, di, dxc):
+        a.free()
+
+
 def _restrict_numpy(r, fmask, cmask, nnf, dim):
     """b_c = P^T r over the free fine dofs: fine nodes 2I-1..2I+1 per axis, weights 1/2 off the centre"""
     nnc = tuple((n - 1) // 2 + 1 for n in nnf)
