@@ -469,34 +469,109 @@ __global__ void k_dense_bc_identity(const unsigned char *__restrict__ mask, doub
     if (mask[i]) A[(size_t)i * n + i] = 1.0;
 }
 
-// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one launch per pivot k over all n^2 entries.
-// Pivot k reads the snapshot (row k, column k) of the matrix before the step -- r = A[k][:], c = A[:][k],
-// p = A[k][k] -- and sets
-//   A[i][j] -= c_i r_j / p  (i, j != k),   A[k][j] = r_j / p,   A[i][k] = -c_i / p,   A[k][k] = 1 / p;
-// after the last pivot A holds A^-1.  The threads that write row and column k+1 also write the snapshot the
-// next launch reads (two snapshot pairs, alternating), so no entry is read after another thread wrote it.
-__global__ void k_gj_snapshot0(const double *__restrict__ A, int n, double *__restrict__ row,
-                               double *__restrict__ col) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    row[i] = A[i];
-    col[i] = A[(size_t)i * n];
+// In-place blocked Gauss-Jordan inverse of an SPD matrix (no pivoting: every pivot block is a Schur complement of
+// an SPD matrix, so it is SPD).  Pivot blocks of GJB rows; per block K = [k0, k0 + b), with Pi = A_KK^-1:
+//   A_Kj <- Pi A_Kj (j outside K),  A_ij <- A_ij - A_iK A_Kj (i, j outside K, the new A_Kj),
+//   A_iK <- -A_iK Pi (i outside K),  A_KK <- Pi;
+// after the last block A holds A^-1.  Three launches per block: the pivot block's inverse (one workgroup, the
+// scalar Gauss-Jordan steps in LDS), the row panel together with a copy of the column panel (so the update
+// reads the column panel as it was), and the update of every row outside K.  Each launch writes only entries
+// the launch itself does not read from other threads; the sums run in a fixed order (deterministic).
+constexpr int GJB = 32;
+
+__global__ __launch_bounds__(256) void k_gjb_pivot(double *__restrict__ A, int n, int k0, int b,
+                                                   double *__restrict__ Pi) {
+    __shared__ double M[GJB][GJB + 1];
+    const int t = threadIdx.x;
+    for (int e = t; e < b * b; e += 256) M[e / b][e % b] = A[(size_t)(k0 + e / b) * n + k0 + e % b];
+    __syncthreads();
+    for (int p = 0; p < b; p++) {
+        // pivot p of the block: new values from the old ones, written after every thread has read
+        double v[GJB * GJB / 256];
+        const double rp = 1.0 / M[p][p];
+#pragma unroll
+        for (int q = 0; q < GJB * GJB / 256; q++) {
+            const int e = t + 256 * q, r = e / b, c = e % b;
+            if (e >= b * b) continue;
+            if (r == p) v[q] = c == p ? rp : M[p][c] * rp;
+            else if (c == p) v[q] = -M[r][p] * rp;
+            else v[q] = M[r][c] - M[r][p] * (M[p][c] * rp);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < GJB * GJB / 256; q++) {
+            const int e = t + 256 * q;
+            if (e < b * b) M[e / b][e % b] = v[q];
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < b * b; e += 256) {
+        const int r = e / b, c = e % b;
+        A[(size_t)(k0 + r) * n + k0 + c] = M[r][c];
+        Pi[r * GJB + c] = M[r][c];
+    }
 }
 
-__global__ __launch_bounds__(256) void k_gj_step(double *__restrict__ A, int n, int k,
-                                                 const double *__restrict__ row, const double *__restrict__ col,
-                                                 double *__restrict__ row1, double *__restrict__ col1) {
-    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= n || j >= n) return;
-    const double rp = 1.0 / row[k];
-    double v;
-    if (i == k) v = j == k ? rp : row[j] * rp;
-    else if (j == k) v = -col[i] * rp;
-    else v = A[(size_t)i * n + j] - col[i] * (row[j] * rp);
-    A[(size_t)i * n + j] = v;
-    if (i == k + 1) row1[j] = v;
-    if (j == k + 1) col1[i] = v;
+// workgroups [0, nrp): the row panel, 64 columns each (A_Kj <- Pi A_Kj, columns outside K); the rest: the copy
+// Cw[i][m] = A[i][k0 + m] of the column panel
+__global__ __launch_bounds__(256) void k_gjb_panels(double *__restrict__ A, int n, int k0, int b,
+                                                    const double *__restrict__ Pi, double *__restrict__ Cw, int nrp) {
+    const int t = threadIdx.x;
+    if ((int)blockIdx.x < nrp) {
+        __shared__ double R[GJB][64];
+        __shared__ double Ps[GJB][GJB + 1];
+        const int j0 = blockIdx.x * 64;
+        for (int e = t; e < b * b; e += 256) Ps[e / b][e % b] = Pi[(e / b) * GJB + e % b];
+        for (int e = t; e < b * 64; e += 256) {
+            const int r = e / 64, j = j0 + e % 64;
+            R[r][e % 64] = j < n ? A[(size_t)(k0 + r) * n + j] : 0.0;
+        }
+        __syncthreads();
+        for (int e = t; e < b * 64; e += 256) {
+            const int r = e / 64, c = e % 64, j = j0 + c;
+            if (j >= n || (j >= k0 && j < k0 + b)) continue;
+            double s = 0.0;
+            for (int m = 0; m < b; m++) s += Ps[r][m] * R[m][c];
+            A[(size_t)(k0 + r) * n + j] = s;
+        }
+    } else {
+        const size_t e = (size_t)(blockIdx.x - nrp) * 256 + t;
+        if (e < (size_t)n * b) {
+            const size_t i = e / b;
+            const int m = (int)(e % b);
+            Cw[i * GJB + m] = A[i * n + k0 + m];
+        }
+    }
+}
+
+// rows outside K: A_ij <- A_ij - Cw_i. A_Kj (j outside K), A_iK <- -Cw_i. Pi; a workgroup holds 16 rows x 64
+// columns (4 rows per thread) and stages the 64 columns' rows of K (or of Pi) and its rows' column panel
+constexpr int GJR = 16;
+__global__ __launch_bounds__(256) void k_gjb_update(double *__restrict__ A, int n, int k0, int b,
+                                                    const double *__restrict__ Pi, const double *__restrict__ Cw) {
+    __shared__ double R[GJB][64];
+    __shared__ double Cs[GJR][GJB + 1];
+    const int t = threadIdx.x, j0 = blockIdx.x * 64, i0 = blockIdx.y * GJR;
+    for (int e = t; e < b * 64; e += 256) {
+        const int m = e / 64, j = j0 + e % 64;
+        R[m][e % 64] = j >= n ? 0.0 : (j >= k0 && j < k0 + b) ? Pi[m * GJB + (j - k0)] : A[(size_t)(k0 + m) * n + j];
+    }
+    for (int e = t; e < GJR * GJB; e += 256) {
+        const int r = e / GJB, m = e % GJB, i = i0 + r;
+        Cs[r][m] = (i < n && m < b) ? Cw[(size_t)i * GJB + m] : 0.0;
+    }
+    __syncthreads();
+    const int c = t & 63, j = j0 + c;
+    if (j >= n) return;
+    const bool inK = j >= k0 && j < k0 + b;
+#pragma unroll
+    for (int q = 0; q < GJR / 4; q++) {
+        const int r = (t >> 6) + 4 * q, i = i0 + r;
+        if (i >= n || (i >= k0 && i < k0 + b)) continue;
+        double s = inK ? 0.0 : A[(size_t)i * n + j];
+        for (int m = 0; m < b; m++) s -= Cs[r][m] * R[m][c];
+        A[(size_t)i * n + j] = s;
+    }
 }
 
 // y = Ainv x: one workgroup (4 wavefronts) per row
@@ -754,13 +829,16 @@ int ust_k_dense_assemble(const ust_grid *gr, const double *WK, const unsigned ch
 int ust_k_dense_spd_inverse(double *A, int n, double *work, void *stream) {
     if (!A || !work || n <= 0 || n > 4096) return UST_EINVAL;
     hipStream_t st = ust_stream(stream);
-    double *row[2] = {work, work + 2 * (size_t)n}, *col[2] = {work + n, work + 3 * (size_t)n};
-    hipLaunchKernelGGL(k_gj_snapshot0, dim3(ust_grid_size(n, UST_BLOCK)), dim3(UST_BLOCK), 0, st, A, n, row[0],
-                       col[0]);
-    const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((n + 3) / 4));
-    for (int k = 0; k < n; k++)
-        hipLaunchKernelGGL(k_gj_step, grid, dim3(256), 0, st, A, n, k, row[k & 1], col[k & 1], row[(k + 1) & 1],
-                           col[(k + 1) & 1]);
+    double *Pi = work, *Cw = work + GJB * GJB;
+    const int nrp = (n + 63) / 64;
+    const dim3 ugrid((unsigned)nrp, (unsigned)((n + GJR - 1) / GJR));
+    for (int k0 = 0; k0 < n; k0 += GJB) {
+        const int b = n - k0 < GJB ? n - k0 : GJB;
+        const unsigned ncp = (unsigned)(((size_t)n * b + 255) / 256);
+        hipLaunchKernelGGL(k_gjb_pivot, dim3(1), dim3(256), 0, st, A, n, k0, b, Pi);
+        hipLaunchKernelGGL(k_gjb_panels, dim3((unsigned)nrp + ncp), dim3(256), 0, st, A, n, k0, b, Pi, Cw, nrp);
+        hipLaunchKernelGGL(k_gjb_update, ugrid, dim3(256), 0, st, A, n, k0, b, Pi, Cw);
+    }
     UST_LAUNCH_CHECK();
     return UST_OK;
 }

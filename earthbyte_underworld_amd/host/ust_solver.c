@@ -370,7 +370,7 @@ static int levels_structure(ust_context *ctx, const ust_solve_opts *o, int need3
     if (ctx->coarse_dense) {
         ctx->nc = ctx->lev[nlev - 1].ndof;
         /* the inverse plus the 4 nc pivot snapshots of its Gauss-Jordan construction */
-        DALLOC(ctx, ctx->Ainv, ((size_t)ctx->nc * ctx->nc + 4 * (size_t)ctx->nc) * sizeof(double));
+        DALLOC(ctx, ctx->Ainv, ((size_t)ctx->nc * ctx->nc + 32 * (size_t)ctx->nc + 1024) * sizeof(double));
     }
     ctx->levels_prec32 = need32;
     ctx->levels_struct = 1;

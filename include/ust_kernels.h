@@ -380,8 +380,8 @@ int ust_k_prolong_add_f32(const ust_grid *gf, const unsigned char *maskf, const 
 /* A (n x n, n = n_nodes*dim) from the weights; BC dofs get identity rows/columns */
 int ust_k_dense_assemble(const ust_grid *g, const double *WK, const unsigned char *mask, double *A,
                          void *stream);
-/* in-place inverse of an SPD matrix by Gauss-Jordan without pivoting (one launch per pivot; n <= 4096);
- * work: 4 n doubles */
+/* in-place inverse of an SPD matrix by blocked Gauss-Jordan without pivoting (blocks of 32 pivots, three
+ * launches each; n <= 4096); work: 32 n + 1024 doubles */
 int ust_k_dense_spd_inverse(double *A, int n, double *work, void *stream);
 /* y = Ainv x in FP64; the f32 variant converts at the boundary */
 int ust_k_dense_matvec_f64(const double *Ainv, int n, const double *x, double *y, void *stream);

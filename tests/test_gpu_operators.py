@@ -174,7 +174,7 @@ def test_dense_coarse_operator_and_inverse(oracle, dim, nel):
     g, WK, WG, WF, mask = L.Grid(), C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
     assert lib.ust_get_device_views(s.ctx, C.byref(g), C.byref(WK), C.byref(WG), C.byref(WF), C.byref(mask)) == 0
     n = s.ndof
-    dA = _Dev(lib, np.zeros(n * n + 4 * n))
+    dA = _Dev(lib, np.zeros(n * n + 32 * n + 1024))
     assert lib.ust_k_dense_assemble(C.byref(g), WK, mask, dA.p, None) == 0
     A = dA.get()[:n * n].reshape(n, n)
     free = gs.eq >= 0
