@@ -206,6 +206,9 @@ _SIGS = {
                                         vp, vp]),
     "ust_k_apply_K_cheby_prolong_oop_f32": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp, C.c_float, vp, vp, vp]),
     "ust_k_cheby_first_f32": (I, [vp, vp, vp, vp, C.c_float, C.c_size_t, vp]),
+    "ust_k_dot_f32": (I, [vp, vp, C.c_size_t, vp, vp, vp]),
+    "ust_k_cg_p_guarded_f32": (I, [vp, vp, vp, C.c_size_t, vp]),
+    "ust_k_cg_xr_rr_guarded_f32": (I, [vp, vp, vp, vp, vp, vp, vp, C.c_size_t, vp, vp, vp]),
     "ust_k_apply_K_f32_bf16w": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp]),
     "ust_k_apply_K_residual_f32_bf16w": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, vp]),
     "ust_k_apply_K_cheby_f32_bf16w": (I, [C.POINTER(Grid), vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, I, vp,

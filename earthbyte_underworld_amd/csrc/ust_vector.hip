@@ -274,6 +274,96 @@ __global__ __launch_bounds__(256) void kv_cg_xr_rr_guarded_f32(float *x, float *
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// The FP32 PCG's streaming kernels on 16-byte aligned vectors: four dofs per thread and step (one 16-byte load
+// or store per vector), restrict-qualified so the loads of a step are issued together; the n % 4 tail dofs are
+// taken by the first threads after the grid-stride loop.  The sums run in a fixed order per thread.
+__device__ __forceinline__ size_t gid() { return blockIdx.x * (size_t)blockDim.x + threadIdx.x; }
+__device__ __forceinline__ size_t gstride() { return (size_t)gridDim.x * blockDim.x; }
+
+__global__ __launch_bounds__(256) void kv4_cg_xr_rr_guarded_f32(float *__restrict__ x, float *__restrict__ r,
+                                                                const float *__restrict__ p,
+                                                                const float *__restrict__ q, const double *rz,
+                                                                const double *pq, const double *c, size_t n,
+                                                                double *partial) {
+    __shared__ double sh[256 / 64];
+    double s = 0.0;
+    if (c[1] == 0.0) {
+        const float a = (float)(rz[0] / pq[0]);
+        const size_t n4 = n / 4;
+        float4 *x4 = (float4 *)x, *r4 = (float4 *)r;
+        const float4 *p4 = (const float4 *)p, *q4 = (const float4 *)q;
+        for (size_t i = gid(); i < n4; i += gstride()) {
+            float4 xv = x4[i], rv = r4[i];
+            const float4 pv = p4[i], qv = q4[i];
+            xv.x += a * pv.x;
+            xv.y += a * pv.y;
+            xv.z += a * pv.z;
+            xv.w += a * pv.w;
+            rv.x = rv.x - a * qv.x;
+            rv.y = rv.y - a * qv.y;
+            rv.z = rv.z - a * qv.z;
+            rv.w = rv.w - a * qv.w;
+            x4[i] = xv;
+            r4[i] = rv;
+            s += (double)rv.x * (double)rv.x;
+            s += (double)rv.y * (double)rv.y;
+            s += (double)rv.z * (double)rv.z;
+            s += (double)rv.w * (double)rv.w;
+        }
+        for (size_t i = 4 * n4 + gid(); i < n; i += gstride()) {
+            x[i] += a * p[i];
+            const float ri = r[i] - a * q[i];
+            r[i] = ri;
+            s += (double)ri * (double)ri;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ __launch_bounds__(256) void kv4_dot_f32(const float *__restrict__ x, const float *__restrict__ y, size_t n,
+                                                   double *partial) {
+    __shared__ double sh[256 / 64];
+    double s = 0.0;
+    const size_t n4 = n / 4;
+    const float4 *x4 = (const float4 *)x, *y4 = (const float4 *)y;
+    for (size_t i = gid(); i < n4; i += gstride()) {
+        const float4 a = x4[i], b = y4[i];
+        s += (double)a.x * (double)b.x;
+        s += (double)a.y * (double)b.y;
+        s += (double)a.z * (double)b.z;
+        s += (double)a.w * (double)b.w;
+    }
+    for (size_t i = 4 * n4 + gid(); i < n; i += gstride()) s += (double)x[i] * (double)y[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ __launch_bounds__(256) void kv4_cg_p_guarded_f32(const float *__restrict__ z, float *__restrict__ p,
+                                                            const double *c, size_t n) {
+    if (c[1] != 0.0) return;
+    const float beta = (float)c[3];
+    const size_t n4 = n / 4;
+    const float4 *z4 = (const float4 *)z;
+    float4 *p4 = (float4 *)p;
+    for (size_t i = gid(); i < n4; i += gstride()) {
+        const float4 zv = z4[i];
+        float4 pv = p4[i];
+        pv.x = zv.x + beta * pv.x;
+        pv.y = zv.y + beta * pv.y;
+        pv.z = zv.z + beta * pv.z;
+        pv.w = zv.w + beta * pv.w;
+        p4[i] = pv;
+    }
+    for (size_t i = 4 * n4 + gid(); i < n; i += gstride()) p[i] = z[i] + beta * p[i];
+}
+
+static inline bool al16(const void *a) { return ((uintptr_t)a & 15u) == 0; }
+
 // y = (float)(s x): the FP64 residual scaled to unit norm for the FP32 correction solve
 __global__ void kv_d2f_scaled(const double *x, float *y, double s, size_t n) {
     GRID_STRIDE(i, n) y[i] = (float)(s * x[i]);
@@ -377,7 +467,10 @@ int ust_k_dot_f64(const double *x, const double *y, size_t n, double *partial, d
 
 int ust_k_dot_f32(const float *x, const float *y, size_t n, double *partial, double *out, void *stream) {
     const unsigned g = rgrid(n);
-    hipLaunchKernelGGL((kv_dot_stage1<float, true>), dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
+    if (al16(x) && al16(y))
+        hipLaunchKernelGGL(kv4_dot_f32, dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
+    else
+        hipLaunchKernelGGL((kv_dot_stage1<float, true>), dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
     hipLaunchKernelGGL(kv_reduce_stage2, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, out);
     UST_LAUNCH_CHECK();
     return UST_OK;
@@ -482,13 +575,18 @@ int ust_k_cheby_first_from_f64(const double *b, float *b32, const float *invd, f
     LV(kv_cheby_first_from64, n, b, b32, invd, d, x, c2, n);
 }
 int ust_k_cg_p_guarded_f32(const float *z, float *p, const double *ctl, size_t n, void *stream) {
+    if (al16(z) && al16(p)) LV(kv4_cg_p_guarded_f32, (n + 3) / 4, z, p, ctl, n);
     LV(kv_cg_p_guarded_f32, n, z, p, ctl, n);
 }
 int ust_k_cg_xr_rr_guarded_f32(float *x, float *r, const float *p, const float *q, const double *rz, const double *pq,
                                double *ctl, size_t n, double *partial, double *rr, void *stream) {
     const unsigned g = rgrid(n);
-    hipLaunchKernelGGL(kv_cg_xr_rr_guarded_f32, dim3(g), dim3(256), 0, ust_stream(stream), x, r, p, q, rz, pq, ctl, n,
-                       partial);
+    if (al16(x) && al16(r) && al16(p) && al16(q))
+        hipLaunchKernelGGL(kv4_cg_xr_rr_guarded_f32, dim3(g), dim3(256), 0, ust_stream(stream), x, r, p, q, rz, pq, ctl,
+                           n, partial);
+    else
+        hipLaunchKernelGGL(kv_cg_xr_rr_guarded_f32, dim3(g), dim3(256), 0, ust_stream(stream), x, r, p, q, rz, pq, ctl,
+                           n, partial);
     hipLaunchKernelGGL(kv_reduce_check, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, rr, ctl);
     UST_LAUNCH_CHECK();
     return UST_OK;
@@ -496,7 +594,10 @@ int ust_k_cg_xr_rr_guarded_f32(float *x, float *r, const float *p, const float *
 int ust_k_dot_beta_f32(const float *x, const float *y, size_t n, double *partial, double *rz, double *rz_old,
                        double *ctl, void *stream) {
     const unsigned g = rgrid(n);
-    hipLaunchKernelGGL((kv_dot_stage1<float, true>), dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
+    if (al16(x) && al16(y))
+        hipLaunchKernelGGL(kv4_dot_f32, dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
+    else
+        hipLaunchKernelGGL((kv_dot_stage1<float, true>), dim3(g), dim3(256), 0, ust_stream(stream), x, y, n, partial);
     hipLaunchKernelGGL(kv_reduce_beta, dim3(1), dim3(256), 0, ust_stream(stream), partial, (int)g, rz, rz_old, ctl);
     UST_LAUNCH_CHECK();
     return UST_OK;

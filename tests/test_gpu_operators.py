@@ -347,6 +347,49 @@ def test_out_of_place_chebyshev_passes(nel):
         a.free()
 
 
+def test_fp32_cg_vector_kernels():
+    """The FP32 PCG's vector kernels (x += a p, r -= a q with r . r; p = z + beta p; the FP32 dot) on 16-byte
+    aligned vectors (four dofs per thread, with an n % 4 tail) and on vectors offset by one float (the one-dof
+    path): the same vectors bit for bit on both paths, the sums to FP64 rounding of the numpy ones."""
+    import ctypes as C
+    from earthbyte_underworld_amd import _lib as L
+    lib = L.load()
+    n = 1_000_003
+    rng = np.random.default_rng(41)
+    x, r, p, q, z = (rng.normal(size=n).astype(np.float32) for _ in range(5))
+    part = _Dev(lib, np.zeros(1024))
+    out = {}
+    for off in (0, 1):  # off = 1: every vector starts 4 bytes past a 16-byte boundary
+        pad = [np.zeros(off, np.float32), None, np.zeros(1 - off, np.float32)]
+        dx, dr, dp, dq, dz = (_Dev(lib, np.concatenate([pad[0], a, pad[2]])) for a in (x, r, p, q, z))
+        ptr = [C.c_void_p(d.p.value + 4 * off) for d in (dx, dr, dp, dq, dz)]
+        sc = _Dev(lib, np.array([2.0, 3.0, 0.0, 0.0, 0.0, 0.0]))  # rz, pq, then ctl: tol, state, count, beta
+        scp = [C.c_void_p(sc.p.value + 8 * k) for k in range(6)]
+        rr = _Dev(lib, np.zeros(1))
+        assert lib.ust_k_cg_xr_rr_guarded_f32(ptr[0], ptr[1], ptr[2], ptr[3], scp[0], scp[1], scp[2], n, part.p,
+                                              rr.p, None) == 0
+        d1 = _Dev(lib, np.zeros(1))
+        assert lib.ust_k_dot_f32(ptr[1], ptr[4], n, part.p, d1.p, None) == 0
+        ctl = _Dev(lib, np.array([0.0, 0.0, 0.0, 0.7]))
+        assert lib.ust_k_cg_p_guarded_f32(ptr[4], ptr[2], ctl.p, n, None) == 0
+        out[off] = (dx.get()[off:off + n], dr.get()[off:off + n], dp.get()[off:off + n], rr.get()[0], d1.get()[0])
+        for a in (dx, dr, dp, dq, dz, sc, rr, d1, ctl):
+            a.free()
+    for k in range(3):
+        assert np.array_equal(out[0][k], out[1][k]), k
+    a = np.float32(2.0 / 3.0)
+    xs, rs, ps, qs, zs = x, r, p, q, z
+    r_new = out[0][1].astype(np.float64)
+    assert np.abs(r_new - (rs - a * qs)).max() <= 1e-6 * np.abs(rs).max()
+    assert np.abs(out[0][0] - (xs + a * ps)).max() <= 1e-6 * np.abs(xs).max()
+    ref_rr = float(r_new @ r_new)
+    for off in (0, 1):
+        assert abs(out[off][3] - ref_rr) <= 1e-12 * ref_rr
+        assert abs(out[off][4] - float(r_new @ zs.astype(np.float64))) <= 1e-10 * ref_rr
+    assert np.abs(out[0][2] - (zs + np.float32(0.7) * ps)).max() <= 1e-6 * np.abs(ps).max()
+    part.free()
+
+
 def _restrict_numpy(r, fmask, cmask, nnf, dim):
     """b_c = P^T r over the free fine dofs: fine nodes 2I-1..2I+1 per axis, weights 1/2 off the centre"""
     nnc = tuple((n - 1) // 2 + 1 for n in nnf)
