@@ -458,6 +458,15 @@ def test_restriction_is_adjoint_of_prolongation(dim, nel):
         assert np.array_equal(b2, db.get())
         x0 = (dt(c2) * invd * b2).astype(np.float64)
         assert np.allclose(dd.get(), x0, rtol=tol, atol=0) and np.allclose(dx2.get(), x0, rtol=tol, atol=0)
+        # no fine mask (3D: one thread per coarse node): bitwise the same restriction and first step
+        dr0 = _Dev(lib, np.where(fm, 0.0, r).astype(dt))
+        db3, dd3, dx3 = (_Dev(lib, np.zeros(cm.size, dt)) for _ in range(3))
+        assert getattr(lib, f"ust_k_restrict_cheby_first_{prec}")(C.byref(g), None, dcm.p, dr0.p, db3.p, di.p, dd3.p,
+                                                                  dx3.p, c2, None) == 0
+        assert np.array_equal(db3.get(), b2) and np.array_equal(dd3.get(), dd.get())
+        assert np.array_equal(dx3.get(), dx2.get())
+        for a in (dr0, db3, dd3, dx3):
+            a.free()
         for a in (dr, db, dxc, dxf, di, dd, dx2, db2):
             a.free()
     dcm.free()
