@@ -189,8 +189,8 @@ def fine_level_kernels(lib, ctx, s, stream, reps=50):
             s.nel * 27 * 4 + n * (4 * 4 + 1), True),
         "k_applyK3_col<float,float,EPI_PCHEBY0> + k_gather_tiles (FP32 prolongation + Chebyshev step)": (
             lambda: lib.ust_k_apply_K_cheby_prolong_f32(C.byref(g), B["W32"], mask, B["x"], B["b"], B["invd"],
-                                                        B["d"], C.c_float(0.5), B["xc"], B["work"], stream),
-            s.nel * 27 * 4 + n * (5 * 4 + 1) + nc * 4, True),
+                                                        None, C.c_float(0.5), B["xc"], B["work"], stream),
+            s.nel * 27 * 4 + n * (4 * 4 + 1) + nc * 4, True),
         "k_applyK3_col<double,double,EPI_Y> + k_gather_tiles (FP64 refinement residual product)": (
             lambda: lib.ust_k_apply_K_f64(C.byref(g), WK, mask, B["u64"], B["y64"], B["work"], stream),
             s.nel * 27 * 8 + n * (2 * 8 + 1), False),

@@ -293,7 +293,7 @@ __device__ __forceinline__ void epilogue_store(const Epi<T> &ep, const EpiOps<T>
         if (MODE == EPI_RES0) ep.d[i] = ep.c2 * o.invd * o.b;  // as staged (kv_cheby_first's rounding)
     } else {
         const T dn = (MODE == EPI_CHEBY ? ep.c1 * o.d : (T)0) + ep.c2 * o.invd * (o.b - v);
-        ep.d[i] = dn;
+        if (MODE == EPI_CHEBY || ep.d) ep.d[i] = dn;  // a first step may leave d unwritten (no later step reads it)
         ep.y[i] = o.x + dn;
     }
 }
@@ -1028,7 +1028,7 @@ static int apply_tiled(const ust_grid *gr, const WT *W, const unsigned char *mas
     GridDev g = grid_dev(gr, col_wg_target(sizeof(T), MODE));
     if (!W || !mask || !u || !ep.y || !work) return UST_EINVAL;
     if (MODE != EPI_Y && !ep.b) return UST_EINVAL;
-    if (cheby_mode(MODE) && (!ep.invd || !ep.d || ep.xs != u)) return UST_EINVAL;
+    if (cheby_mode(MODE) && (!ep.invd || (MODE == EPI_CHEBY && !ep.d) || ep.xs != u)) return UST_EINVAL;
     if (MODE == EPI_RES0 && (!ep.invd || !ep.d)) return UST_EINVAL;
     if (MODE == EPI_PCHEBY0 && (g.dim != 3 || !ep.xc || g.nx % 2 || g.ny % 2 || g.nz % 2)) return UST_EINVAL;
     if ((size_t)g.nnx * g.nny * g.nnz * g.dim >= (1ull << 32)) return UST_EINVAL;  // 32-bit dof indices

@@ -116,9 +116,11 @@ static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void **xp
     for (int k = x_zero == 2 ? 1 : 0; k < L->cheb_deg; k++) {
         void *x = *xp;
         if (k == 0 && xc) { /* prolongation of the coarse correction fused into the first step */
+            /* a single post-smoothing step leaves its direction unwritten: no later step reads it */
+            float *d32 = L->cheb_deg == 1 ? NULL : L->d32;
             if (oop)
                 CKD(ctx, ust_k_apply_K_cheby_prolong_oop_f32(&L->g, L->WK32, L->mask, (const float *)x, (float *)*alt,
-                                                             (const float *)b, L->invd32, L->d32, (float)L->c2[0],
+                                                             (const float *)b, L->invd32, d32, (float)L->c2[0],
                                                              (const float *)xc, (float *)ctx->kwork, S(ctx)));
             else if (f32 && L->WK16)
                 CKD(ctx, ust_k_apply_K_cheby_prolong_f32_bf16w(&L->g, L->WK16, L->mask, (float *)x, (const float *)b,
@@ -126,7 +128,7 @@ static int lev_smooth(ust_context *ctx, int l, int f32, const void *b, void **xp
                                                                (float *)ctx->kwork, S(ctx)));
             else if (f32)
                 CKD(ctx, ust_k_apply_K_cheby_prolong_f32(&L->g, L->WK32, L->mask, (float *)x, (const float *)b,
-                                                         L->invd32, L->d32, (float)L->c2[0], (const float *)xc,
+                                                         L->invd32, d32, (float)L->c2[0], (const float *)xc,
                                                          (float *)ctx->kwork, S(ctx)));
             else
                 CKD(ctx, ust_k_apply_K_cheby_prolong_f64(&L->g, L->WK64, L->mask, (double *)x, (const double *)b,

@@ -274,13 +274,14 @@ int ust_k_apply_K_f32(const ust_grid *g, const float *WK, const unsigned char *m
  * residual: r = b - K u on the free dofs, r = 0 on the Dirichlet dofs (the multigrid residual before
  * restriction, PETScMGSolver.c:641-716);
  * cheby: one Jacobi-Chebyshev smoothing step in place, r = b - K x, d = c1 d + c2 invd.*r, x += d
- * (first: c1 = 0 and d is not read). */
+ * (first: c1 = 0 and d is not read; d may then be NULL, and is not written, when no later step needs it --
+ * so also for the prolongation passes below). */
 int ust_k_apply_K_residual_f64(const ust_grid *g, const double *WK, const unsigned char *mask, const double *u,
                                const double *b, double *r, double *work, void *stream);
 int ust_k_apply_K_residual_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *u,
                                const float *b, float *r, float *work, void *stream);
 /* the same Chebyshev passes out of place: the step reads x and writes xo (distinct vectors; d is read and
- * written per dof).  On small 3D grids (at most 65,536 elements: the coarser multigrid levels) the FP32 pass is
+ * written per dof, or left unwritten when NULL as above).  On small 3D grids (at most 65,536 elements: the coarser multigrid levels) the FP32 pass is
  * one halo-tile launch without a workspace gather (work is then not used). */
 int ust_k_apply_K_cheby_oop_f32(const ust_grid *g, const float *WK, const unsigned char *mask, const float *x,
                                 float *xo, const float *b, const float *invd, float *d, float c1, float c2, int first,

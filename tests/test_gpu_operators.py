@@ -250,6 +250,11 @@ def test_prolongation_fused_chebyshev_step(prec):
     tol = 1e-13 if prec == "f64" else 2e-6
     assert rel_err(xb.get().astype(np.float64), xa.get().astype(np.float64)) < tol
     assert rel_err(dd.get().astype(np.float64), da.get().astype(np.float64)) < tol
+    # without d (a last step whose direction no later step reads): the same x, bit for bit
+    xn = _Dev(lib, x0.astype(dt))
+    assert fused(C.byref(g), W, mask, xn.p, db.p, di.p, None, c2, dxc.p, ws.p, None) == 0
+    assert np.array_equal(xn.get(), xb.get())
+    xn.free()
     for a in keep + [ws, dxc, db, di, xa, da, xb, dd]:
         a.free()
     s.close()
