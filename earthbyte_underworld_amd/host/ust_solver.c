@@ -619,16 +619,18 @@ static int pcg32(ust_context *ctx, int step, double rtol, int maxit, int *its) {
 #endif
 #define IR_MAX_STEPS 12
 
-/* x = K^-1 b by iterative refinement: x accumulates in FP64 (vt), r = b - K x in FP64 (vr); each step solves
- * K e = r / ||r|| with the FP32 PCG and adds ||r|| e */
+/* x = K^-1 b by iterative refinement: x accumulates in FP64 (in place, or in vt when x is b), r = b - K x in
+ * FP64 (b itself before the first step, then vr); each step solves K e = r / ||r|| with the FP32 PCG and adds
+ * ||r|| e */
 static int pcg_refined(ust_context *ctx, const double *b, double *x, double rtol, int maxit, int *its,
                        int *steps) {
     size_t n = (size_t)ctx->ndof;
     double *sc = ctx->scal, bb, rr;
+    double *acc = x == b ? ctx->vt : x;
+    const double *r = b;
     int total = 0, k, rc;
     *steps = 0;
-    CKD(ctx, ust_k_fill_f64(ctx->vt, 0.0, n, S(ctx)));
-    CKD(ctx, ust_memcpy_d2d(ctx->vr, b, n * sizeof(double), S(ctx)));
+    CKD(ctx, ust_k_fill_f64(acc, 0.0, n, S(ctx)));
     CKD(ctx, ust_k_dot_f64(b, b, n, ctx->partial, sc + SC_T0, S(ctx)));
     CKD(ctx, ust_read_scalar(ctx, SC_T0, &bb));
     rr = bb;
@@ -637,20 +639,21 @@ static int pcg_refined(ust_context *ctx, const double *b, double *x, double rtol
         const double rn = sqrt(rr), want = rtol * sqrt(bb) / rn;
         const double tau = want > IR_TAU_MIN ? want : IR_TAU_MIN;
         int it = 0;
-        CKD(ctx, ust_k_copy_f64_to_f32_scaled(ctx->vr, ctx->wr, 1.0 / rn, n, S(ctx)));
+        CKD(ctx, ust_k_copy_f64_to_f32_scaled(r, ctx->wr, 1.0 / rn, n, S(ctx)));
         if ((rc = pcg32(ctx, k, tau, maxit - total, &it))) return rc;
         total += it;
-        CKD(ctx, ust_k_axpy_f32_to_f64(rn, ctx->wx, ctx->vt, n, S(ctx)));
+        CKD(ctx, ust_k_axpy_f32_to_f64(rn, ctx->wx, acc, n, S(ctx)));
         (*steps)++;
         if (want >= IR_TRUST) break;  /* a loose request: the FP32 recurrence is trusted */
-        if ((rc = lev_apply(ctx, 0, 0, ctx->vt, ctx->vq))) return rc;
+        if ((rc = lev_apply(ctx, 0, 0, acc, ctx->vq))) return rc;
         CKD(ctx, ust_k_residual_f64(b, ctx->vq, ctx->vr, n, S(ctx)));
+        r = ctx->vr;
         CKD(ctx, ust_k_dot_f64(ctx->vr, ctx->vr, n, ctx->partial, sc + SC_T0, S(ctx)));
         CKD(ctx, ust_read_scalar(ctx, SC_T0, &rr));
         if (!(rr == rr)) UST_FAIL(ctx, UST_ENOTCONV, "A11 refinement produced NaN");
     }
     *its = total;
-    CKD(ctx, ust_memcpy_d2d(x, ctx->vt, n * sizeof(double), S(ctx)));
+    if (acc != x) CKD(ctx, ust_memcpy_d2d(x, acc, n * sizeof(double), S(ctx)));
     return UST_OK;
 }
 
